@@ -1,0 +1,154 @@
+/*
+ * gdpoisson -- MI355X (gfx950) multigrid Poisson integration, C ABI.
+ *
+ * Every entry point takes plain device pointers (row-major, dense, FP64 unless
+ * stated) and a HIP stream passed as `void*` (NULL = the null stream).  Calls
+ * enqueue work on that stream and return without waiting, except where a comment
+ * says "synchronises".  Every function returns a status (GDP_OK == 0); on failure
+ * gdp_last_error() gives a message for the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the gradient-domain package of
+ * Giankaqui/visual-computing, gradient-domain/src/gradient_domain/):
+ *   gdp_laplacian        operators.py:86-107   laplacian(u, spacing)         (+ Neumann variant)
+ *   gdp_gradient         operators.py:30-50    gradient(image)
+ *   gdp_divergence       operators.py:53-83    divergence(gx, gy)
+ *   gdp_fold_boundary    operators.py:139-169  fold_boundary(rhs, boundary, spacing)
+ *   gdp_smooth           multigrid.py:147-166  _smooth(u, b, spacing, sweeps)
+ *   gdp_restrict         multigrid.py:169-174  _restrict(residual)
+ *   gdp_prolong          multigrid.py:177-181  _prolong(correction, shape)
+ *   gdp_plan_*           multigrid.py:143-144,206 level hierarchy (_coarse_shape + coarsest rule)
+ *   gdp_vcycle           multigrid.py:184-215  v_cycle(u, b, spacing, options)
+ *   gdp_mg_solve         multigrid.py:218-255  solve(b, spacing, options, initial)
+ *                        solvers.py:121-131    _solve_multigrid
+ *   gdp_mgcg_solve       solvers.py:134-160    _solve_mgcg  (SOLVERS["mgcg"])
+ *   gdp_cg_solve         solvers.py:93-118     _solve_cg    (SOLVERS["cg"])
+ *   gdp_relative_residual solvers.py:78-82     _relative_residual(u, b, spacing)
+ *   gdp_report           solvers.py:49-75      SolverReport fields
+ */
+#ifndef GDPOISSON_H
+#define GDPOISSON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GDP_OK 0
+#define GDP_EINVAL 1   /* bad argument: maps to KeyError / ValueError in Python   */
+#define GDP_ENOMEM 2   /* workspace too small / HBM budget exceeded                */
+#define GDP_EHIP 3     /* a HIP runtime call failed                                */
+#define GDP_ENOTSUP 4  /* shape outside what the kernels support                   */
+
+#define GDP_DIRICHLET 0 /* unknowns = interior, zero outside (operators.py:86-107)   */
+#define GDP_NEUMANN 1   /* unknowns = all pixels, no flux across the border           */
+
+#define GDP_FP64 0      /* V-cycle in FP64 (plain FP64 MG-PCG)                        */
+#define GDP_MIXED 1     /* V-cycle in FP32; r, x, d, q, dots, coarsest solve in FP64  */
+
+/* One independent grid (one colour channel of one image). */
+typedef struct gdp_problem {
+  int32_t rows;      /* m: unknowns along axis 0 */
+  int32_t cols;      /* n: unknowns along axis 1 */
+  int32_t boundary;  /* GDP_DIRICHLET or GDP_NEUMANN */
+  int32_t reserved;
+  double spacing;    /* h */
+} gdp_problem;
+
+/* MultigridOptions (multigrid.py:46-71) plus the solver knobs of this library. */
+typedef struct gdp_options {
+  int32_t precision;        /* GDP_FP64 / GDP_MIXED                                   */
+  int32_t pre_smoothing;    /* nu1 (mgcg: 1, multigrid: 2)                            */
+  int32_t post_smoothing;   /* nu2                                                    */
+  int32_t coarsest_size;    /* Dirichlet: min side <= this is coarsest (8);
+                               Neumann:   max side <= this is coarsest (16)           */
+  int32_t coarsest_sweeps;  /* Dirichlet coarsest RB-GS sweeps (60)                   */
+  int32_t max_iterations;   /* PCG iterations / V-cycles                              */
+  int32_t flexible;         /* 0: Fletcher-Reeves beta (reference), 1: Polak-Ribiere   */
+  int32_t use_graph;        /* 1: replay PCG iterations from a captured hipGraph      */
+  double tolerance;         /* relative residual target                               */
+} gdp_options;
+
+/* SolverReport (solvers.py:49-75) per problem. */
+typedef struct gdp_report {
+  int32_t iterations;          /* PCG iterations or V-cycles                            */
+  int32_t converged;           /* 1 if the recursive residual met the target            */
+  double relative_residual;    /* true ||b - A u|| / ||b|| (FP64, recomputed at the end) */
+  double recursive_residual;   /* ||r_k|| / ||b|| of the last iteration                 */
+} gdp_report;
+
+typedef struct gdp_plan gdp_plan;
+
+const char* gdp_last_error(void);
+int gdp_version(void);
+
+/* ---------------------------------------------------------------- single grids
+ * `planes` independent (m, n) planes stored back to back (plane stride m*n).   */
+
+/* out = laplacian(u, spacing): Dirichlet zero-padded, or Neumann in-grid stencil. */
+int gdp_laplacian(const double* u, double* out, int m, int n, int planes,
+                  double spacing, int boundary, void* stream);
+/* gx, gy = gradient(img) over (h, w) planes. */
+int gdp_gradient(const double* img, double* gx, double* gy, int h, int w, int planes,
+                 void* stream);
+/* out = divergence(gx, gy). */
+int gdp_divergence(const double* gx, const double* gy, double* out, int h, int w,
+                   int planes, void* stream);
+/* out (h-2, w-2) = fold_boundary(rhs (h-2, w-2), boundary (h, w), spacing); out may alias rhs. */
+int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, int h, int w,
+                      int planes, double spacing, void* stream);
+/* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
+   (order = 0) or colour 1 then 0 (order = 1). */
+int gdp_smooth(double* u, const double* b, int m, int n, double spacing, int sweeps,
+               int boundary, int order, void* stream);
+/* coarse (mc, nc) = 0.25 P_r^T fine P_c with fine (m, n). */
+int gdp_restrict(const double* fine, double* coarse, int m, int n, int mc, int nc,
+                 int boundary, void* stream);
+/* fine (m, n) = P_r coarse P_c^T with coarse (mc, nc). */
+int gdp_prolong(const double* coarse, double* fine, int m, int n, int mc, int nc,
+                int boundary, void* stream);
+/* result[p] = ||b - A u|| / ||b|| per plane (0 when b == 0).  Synchronises. */
+int gdp_relative_residual(const double* u, const double* b, int m, int n, int planes,
+                          double spacing, int boundary, double* result, void* stream);
+
+/* ---------------------------------------------------------------- plans
+ * A plan is a batch of independent problems solved together: every kernel launch
+ * covers all of them.  The caller owns the device workspace (gdp_plan_workspace_bytes),
+ * which holds the level pyramid, the FP64 Krylov vectors and the tile tables.        */
+int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* options,
+                    gdp_plan** plan);
+int gdp_plan_workspace_bytes(const gdp_plan* plan, size_t* bytes);
+/* Lays the workspace out and uploads the tables.  Synchronises. */
+int gdp_plan_bind(gdp_plan* plan, void* workspace, size_t bytes);
+int gdp_plan_destroy(gdp_plan* plan);
+/* Number of levels of problem i and the shape of level l. */
+int gdp_plan_levels(const gdp_plan* plan, int problem, int* levels);
+int gdp_plan_level_shape(const gdp_plan* plan, int problem, int level, int* m, int* n);
+/* Tile shapes: kind 0 = down (coarse tile), 1 = up (fine tile), 2 = Krylov (fine tile).
+   Must be called before gdp_plan_bind; index into the compiled menu (see gdp_tile_menu). */
+int gdp_plan_set_tile(gdp_plan* plan, int kind, int level, int menu_index);
+int gdp_tile_menu(int kind, int index, int* th, int* tw);
+
+/* u_out[i] = v_cycle(u_in[i], b[i]) for every problem (arrays of device pointers, one
+   per problem, each a dense (m, n) FP64 grid).  u_out may alias u_in. */
+int gdp_vcycle(gdp_plan* plan, const double* const* u_in, const double* const* b,
+               double* const* u_out, void* stream);
+/* Repeated V-cycles until ||b - A u|| / ||b|| < tolerance (multigrid.solve).
+   `initial` may be NULL (zeros).  residual_history: max_iterations + 1 doubles per
+   problem (may be NULL).  Synchronises. */
+int gdp_mg_solve(gdp_plan* plan, const double* const* b, const double* const* initial,
+                 double* const* u, gdp_report* reports, double* residual_history,
+                 void* stream);
+/* MG-preconditioned CG on -A u = -b; Neumann problems are solved for the zero-mean
+   compatible part of b and return the zero-mean solution.  Synchronises. */
+int gdp_mgcg_solve(gdp_plan* plan, const double* const* b, double* const* u,
+                   gdp_report* reports, void* stream);
+/* Jacobi-preconditioned CG (the reference's "cg" method).  Synchronises. */
+int gdp_cg_solve(gdp_plan* plan, const double* const* b, double* const* u,
+                 gdp_report* reports, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GDPOISSON_H */

@@ -1,0 +1,177 @@
+"""GPU parity: HIP kernels (through the C ABI) vs the FP64 oracle on seeded inputs.
+
+Tolerances: operators and one V-cycle agree with the oracle to round-off (they do
+the same FP64 arithmetic in a different summation order); solves agree to the
+solver tolerance.  Shapes are the reference tests' (gradient-domain/tests/).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import generators as G
+from oracle import multigrid as OM
+from oracle import operators as OO
+from oracle import solvers as OS
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gd():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gdpoisson
+
+    return gdpoisson
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("shape", [(23, 31), (64, 64), (1, 7), (130, 257)])
+@pytest.mark.parametrize("spacing", [1.0, 0.5, 2.0])
+def test_laplacians_match_oracle(gd, shape, spacing):
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(shape)
+    assert rel(gd.laplacian(u, spacing), OO.laplacian(u, spacing)) < 1e-14
+    assert rel(gd.neumann_laplacian(u, spacing), OO.neumann_laplacian(u, spacing)) < 1e-14
+    u3 = rng.standard_normal(shape + (3,))
+    assert rel(gd.laplacian(u3, spacing), OO.laplacian(u3, spacing)) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(23, 31), (17, 21, 3), (1, 5), (40, 1)])
+def test_gradient_divergence_match_oracle(gd, shape):
+    rng = np.random.default_rng(12)
+    u = rng.standard_normal(shape)
+    gx, gy = gd.gradient(u)
+    ox, oy = OO.gradient(u)
+    assert np.array_equal(gx, ox) and np.array_equal(gy, oy)
+    vx, vy = rng.standard_normal(shape), rng.standard_normal(shape)
+    assert np.array_equal(gd.divergence(vx, vy), OO.divergence(vx, vy))
+
+
+def test_fold_boundary_matches_oracle(gd):
+    rng = np.random.default_rng(13)
+    full = rng.standard_normal((14, 18, 3))
+    rhs = rng.standard_normal((12, 16, 3))
+    for h in (1.0, 0.5):
+        assert np.array_equal(gd.fold_boundary(rhs, full, h), OO.fold_boundary(rhs, full, h))
+    f = gd.fold_boundary(np.zeros((3, 3)), np.ones((5, 5)), spacing=0.5)
+    assert f[0, 0] == pytest.approx(-8.0) and f[0, 1] == pytest.approx(-4.0)
+    assert f[1, 1] == pytest.approx(0.0)
+
+
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("shape", [(31, 25), (64, 64), (33, 40)])
+def test_smoother_matches_oracle(gd, bc, shape):
+    from gdpoisson.multigrid import _smooth
+
+    rng = np.random.default_rng(14)
+    u = rng.standard_normal(shape)
+    b = rng.standard_normal(shape)
+    for sweeps in (1, 3):
+        for order in (0, 1):
+            ord_ = (0, 1) if order == 0 else (1, 0)
+            got = _smooth(u, b, 0.5, sweeps, bc, order)
+            want = OM.smooth(u.copy(), b, 0.5, sweeps, bc, ord_)
+            assert rel(got, want) < 1e-13
+
+
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("shape", [(31, 25), (30, 24), (64, 65), (7, 9)])
+def test_transfers_match_oracle(gd, bc, shape):
+    from gdpoisson.multigrid import _prolong, _restrict
+
+    rng = np.random.default_rng(6)
+    f = rng.standard_normal(shape)
+    cs = OM.coarse_shape(shape, bc)
+    c = rng.standard_normal(cs)
+    assert rel(_restrict(f, bc), OM.restrict(f, bc)) < 1e-14
+    assert rel(_prolong(c, shape, bc), OM.prolong(c, shape, bc)) < 1e-14
+
+
+@pytest.mark.parametrize("shape", [(63, 63), (47, 61), (127, 127), (255, 255), (31, 25)])
+@pytest.mark.parametrize("nu", [1, 2])
+def test_dirichlet_vcycle_matches_oracle(gd, shape, nu):
+    rng = np.random.default_rng(15)
+    b, _ = G.manufactured_problem(shape, seed=5)
+    u = rng.standard_normal(shape)
+    opts = gd.MultigridOptions(pre_smoothing=nu, post_smoothing=nu)
+    got = gd.v_cycle(u, b, 1.0, opts)
+    want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(pre_smoothing=nu, post_smoothing=nu))
+    assert rel(got, want) < 1e-11
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (255, 257), (135, 240), (300, 200)])
+def test_neumann_vcycle_matches_oracle(gd, shape):
+    b, _ = G.neumann_problem(shape, seed=17)
+    u = np.zeros(shape)
+    opts = gd.MultigridOptions(pre_smoothing=1, post_smoothing=1, coarsest_size=16)
+    got = gd.v_cycle(u, b, 1.0, opts, boundary="neumann")
+    want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(pre_smoothing=1, post_smoothing=1,
+                                                    coarsest_size=16), "neumann")
+    assert rel(got, want) < 1e-11
+
+
+@pytest.mark.parametrize("size", [63, 127, 255, 511])
+def test_mgcg_reproduces_published_benchmark(gd, size):
+    """docs/benchmark.json (seed 0, tol 1e-8): mgcg takes 7 iterations at 63..1023."""
+    b, x = G.manufactured_problem((size, size), seed=0)
+    u, rep = gd.solve_system(b, method="mgcg", tolerance=1e-8)
+    ou, orep = OS.solve_system(b, method="mgcg", tolerance=1e-8)
+    assert rep.iterations == orep.iterations == 7
+    assert rep.converged
+    assert rep.relative_residual == pytest.approx(orep.relative_residual, rel=1e-6)
+    assert np.abs(u - ou).max() < 1e-12
+    assert np.abs(u - x).max() < 1e-6
+
+
+@pytest.mark.parametrize("method", ["cg", "multigrid", "mgcg", "mgcg_mixed"])
+def test_every_solver_recovers_the_manufactured_solution(gd, method):
+    b, expected = G.manufactured_problem((63, 63), seed=3)
+    u, rep = gd.solve_system(b, method=method, tolerance=1e-10)
+    assert rep.converged
+    assert rep.relative_residual < 1e-8
+    assert np.abs(u - expected).max() < 1e-6
+
+
+def test_solvers_agree_with_direct(gd):
+    b, _ = G.manufactured_problem((47, 61), seed=4)
+    ref, _ = OS.solve_system(b, method="direct")
+    for method in ("cg", "multigrid", "mgcg", "mgcg_mixed"):
+        u, _ = gd.solve_system(b, method=method, tolerance=1e-11)
+        assert np.abs(u - ref).max() < 1e-6
+
+
+def test_unknown_method_and_zero_rhs(gd):
+    with pytest.raises(KeyError):
+        gd.solve_system(np.zeros((8, 8)), method="jacobi")
+    u, rep = gd.solve_system(np.zeros((16, 16)), method="mgcg")
+    assert np.all(u == 0.0) and rep.relative_residual == 0.0
+
+
+@pytest.mark.parametrize("size", [31, 63, 127, 255])
+def test_multigrid_cycles_match_oracle(gd, size):
+    b, _ = G.manufactured_problem((size, size), seed=5)
+    _, rep = gd.solve(b, options=gd.MultigridOptions(tolerance=1e-10))
+    _, orep = OM.solve(b, options=OM.MultigridOptions(tolerance=1e-10))
+    assert rep.converged and rep.cycles == orep.cycles <= 10
+    assert rep.convergence_factor < 0.1
+    assert np.allclose(rep.residuals, orep.residuals, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(256, 256), (255, 257), (135, 240)])
+def test_neumann_mgcg_matches_closed_form(gd, precision, shape):
+    b, exact = G.neumann_problem(shape, seed=1001)
+    method = "mgcg_mixed" if precision == "mixed" else "mgcg"
+    u, rep = gd.solve_system(b, method=method, tolerance=1e-8, boundary="neumann")
+    ou, oit, _ = OS.mgcg(b, bc="neumann", precision=precision, flexible=precision == "mixed")
+    assert rep.converged and rep.relative_residual <= 1e-8
+    assert abs(rep.iterations - oit) <= 1
+    assert abs(u.mean()) < 1e-10
+    assert np.abs(u - exact).max() < 1e-6
+    assert np.abs(u - ou).max() < 1e-6

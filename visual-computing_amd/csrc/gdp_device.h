@@ -1,0 +1,94 @@
+// Device-side data structures shared by the kernels and the host planner.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gdp {
+
+constexpr int kDirichlet = 0;
+constexpr int kNeumann = 1;
+
+// One level of one problem.  Arrays are row-major with `pitch` elements per row and
+// hold the V-cycle working precision T (float or double).
+struct LevelDesc {
+  void* u;        // final iterate after post-smoothing (level 0: z = M r)
+  void* v;        // iterate after pre-smoothing
+  void* b;        // right-hand side
+  int32_t m, n, pitch, bc;
+  double h2;      // spacing^2 at this level
+  double inv_h2;  // 1 / spacing^2
+  int32_t coarse; // index of the next coarser LevelDesc, -1 at the coarsest
+  int32_t problem;
+  int32_t level;
+  int32_t pad;
+};
+
+// FP64 Krylov vectors of one problem on its fine grid (pitched like level 0).
+struct KrylovDesc {
+  double* x;
+  double* r;
+  double* d[2];   // ping-pong search directions
+  double* q;
+  double* rhs;    // FP64 copy of the (mean-projected) right-hand side
+  int32_t m, n, pitch, bc;
+  double h2, inv_h2;
+  int32_t level0; // LevelDesc index of level 0
+  int32_t pad;
+};
+
+// Caller-owned dense (pitch = n) input / output of one solve.
+struct IoDesc {
+  const double* b;
+  double* u;
+};
+
+// Per-problem scalar state of the PCG / MG iteration (FP64).
+struct ProblemState {
+  double delta;      // <r, z>
+  double dq;         // <d, q>
+  double alpha;
+  double beta;
+  double rr;         // ||r||^2 (recursive)
+  double target2;    // (tol * ||rhs||)^2
+  double bnorm2;     // ||rhs||^2
+  double bmean;      // mean of b (Neumann projection)
+  double true_rr;    // ||rhs - (-A) x||^2 recomputed
+  double sumx;       // sum of x (Neumann zero-mean output)
+  int32_t iter;
+  int32_t converged; // recursive residual met the target
+  int32_t done;      // converged or out of iterations: every kernel skips this problem
+  int32_t restart;   // next preconditioned direction starts a new Krylov sequence
+  int32_t max_iter;
+  int32_t flexible;
+  int32_t pad[2];
+};
+
+// One workgroup's work item.
+struct Tile {
+  int32_t desc;  // LevelDesc (V-cycle kernels) or KrylovDesc index
+  int32_t i0, j0;
+  int32_t slot;  // index of this tile's partial-sum slot (problem-contiguous)
+};
+
+// Slot range of one problem in a tile table (for the last-block reductions).
+struct SlotRange {
+  int32_t begin, count;
+};
+
+// Coarse-tail layout of one problem: levels [first, last] live in LDS.
+struct TailDesc {
+  int32_t first;          // LevelDesc index of the first tail level
+  int32_t nlev;           // number of tail levels (the last one is the coarsest)
+  int32_t problem;
+  int32_t krylov;         // KrylovDesc index if first is level 0, else -1
+  int32_t lds_off[16][2]; // byte offsets of (u, b) of each tail level in LDS
+  int32_t scratch_off;    // residual scratch
+  int32_t coarse_off;     // FP64 coarsest working area
+  const double* dct;      // Neumann coarsest: Cm (m*m), Cn (n*n), 1/lambda (m*n)
+};
+
+constexpr int kTailThreads = 512;
+constexpr int kTailLdsBytes = 98304;
+constexpr int kMaxTailLevels = 16;
+
+}  // namespace gdp

@@ -1,0 +1,1141 @@
+// Host side of libgdpoisson: level hierarchy, workspace layout, tile tables, launch
+// sequences, hipGraph capture of PCG iteration pairs, and the C ABI of gdpoisson.h.
+//
+// Level hierarchy (per problem):
+//   Dirichlet  m -> (m - 1) // 2 until min side <= coarsest_size or a coarse side < 2
+//              (multigrid.py:143-144, 206)
+//   Neumann    m -> (m + 1) // 2 until max side <= coarsest_size or min side < 3
+// Levels whose whole tail (that level and every coarser one) fits in LDS run in one
+// k_tail block per problem; finer levels use the tiled k_down / k_up kernels.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gdp_device.h"
+#include "gdp_internal.h"
+#include "gdp_kernels.h"
+#include "gdpoisson.h"
+
+namespace gdp {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+}  // namespace gdp
+
+using namespace gdp;
+
+#define GDP_HIP(x)                                                              \
+  do {                                                                          \
+    hipError_t e_ = (x);                                                        \
+    if (e_ != hipSuccess) {                                                     \
+      set_error("%s failed: %s", #x, hipGetErrorString(e_));                    \
+      return GDP_EHIP;                                                          \
+    }                                                                           \
+  } while (0)
+
+#define GDP_CHECK_LAUNCH()                                                      \
+  do {                                                                          \
+    hipError_t e_ = hipGetLastError();                                          \
+    if (e_ != hipSuccess) {                                                     \
+      set_error("kernel launch failed: %s", hipGetErrorString(e_));             \
+      return GDP_EHIP;                                                          \
+    }                                                                           \
+  } while (0)
+
+namespace {
+
+struct Shape2 {
+  int h, w;
+};
+// Tile menus: kind 0 = down (coarse points per block), 1 = up and 2 = Krylov (fine points).
+const Shape2 kDownMenu[4] = {{4, 32}, {8, 32}, {16, 32}, {8, 64}};
+const Shape2 kUpMenu[4] = {{8, 64}, {16, 64}, {32, 64}, {16, 128}};
+const Shape2 kKryMenu[4] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}};
+constexpr int kMenu = 4;
+constexpr int kMaxDepth = 32;
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+int round_up(int x, int a) { return (x + a - 1) / a * a; }
+int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ----------------------------------------------------------------- launch dispatch
+template <typename T, int CH, int CW>
+void down_nu(int nu, int blocks, hipStream_t s, const Tile* t, const LevelDesc* L,
+             const ProblemState* st) {
+  switch (nu) {
+    case 0: k_down<T, CH, CW, 0><<<blocks, kThreads, 0, s>>>(t, L, st); break;
+    case 1: k_down<T, CH, CW, 1><<<blocks, kThreads, 0, s>>>(t, L, st); break;
+    case 2: k_down<T, CH, CW, 2><<<blocks, kThreads, 0, s>>>(t, L, st); break;
+    default: k_down<T, CH, CW, 3><<<blocks, kThreads, 0, s>>>(t, L, st); break;
+  }
+}
+
+template <typename T>
+void launch_down(int menu, int nu, int blocks, hipStream_t s, const Tile* t, const LevelDesc* L,
+                 const ProblemState* st) {
+  switch (menu) {
+    case 0: down_nu<T, 4, 32>(nu, blocks, s, t, L, st); break;
+    case 1: down_nu<T, 8, 32>(nu, blocks, s, t, L, st); break;
+    case 2: down_nu<T, 16, 32>(nu, blocks, s, t, L, st); break;
+    default: down_nu<T, 8, 64>(nu, blocks, s, t, L, st); break;
+  }
+}
+
+struct UpArgs {
+  const Tile* tiles;
+  const LevelDesc* levels;
+  ProblemState* st;
+  const KrylovDesc* kds;
+  double2* partial;
+  int* counters;
+  const SlotRange* ranges;
+  int order;
+};
+
+template <typename T, int TH, int TW, bool DOTS>
+void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
+#define GDP_UP(NU)                                                                       \
+  k_up<T, TH, TW, NU, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.levels, a.st, a.kds, \
+                                                        a.partial, a.counters, a.ranges, a.order)
+  switch (nu) {
+    case 0: GDP_UP(0); break;
+    case 1: GDP_UP(1); break;
+    case 2: GDP_UP(2); break;
+    default: GDP_UP(3); break;
+  }
+#undef GDP_UP
+}
+
+template <typename T, bool DOTS>
+void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
+  switch (menu) {
+    case 0: up_nu<T, 8, 64, DOTS>(nu, blocks, s, a); break;
+    case 1: up_nu<T, 16, 64, DOTS>(nu, blocks, s, a); break;
+    case 2: up_nu<T, 32, 64, DOTS>(nu, blocks, s, a); break;
+    default: up_nu<T, 16, 128, DOTS>(nu, blocks, s, a); break;
+  }
+}
+
+template <typename T>
+void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a) {
+  if (dots) launch_up_d<T, true>(menu, nu, blocks, s, a);
+  else launch_up_d<T, false>(menu, nu, blocks, s, a);
+}
+
+#define GDP_KRY_DISPATCH(menu, CALL)                 \
+  switch (menu) {                                    \
+    case 0: { constexpr int TH = 8, TW = 64; CALL; } break;   \
+    case 1: { constexpr int TH = 16, TW = 64; CALL; } break;  \
+    case 2: { constexpr int TH = 32, TW = 64; CALL; } break;  \
+    default: { constexpr int TH = 8, TW = 128; CALL; } break; \
+  }
+
+// ----------------------------------------------------------------- plan
+struct LevelInfo {
+  int m, n, pitch;
+  double h;
+  size_t off_u, off_v, off_b;
+};
+
+struct ProbInfo {
+  gdp_problem p;
+  std::vector<LevelInfo> lv;
+  int tail_first = 0;
+  int tail_bytes = 0;
+  int level_base = 0;
+  size_t off_x = 0, off_r = 0, off_d0 = 0, off_d1 = 0, off_q = 0, off_dct = 0;
+  int tail_off[kMaxTailLevels][2];
+  int scratch_off = 0, coarse_off = 0;
+};
+
+struct Table {
+  size_t off = 0;
+  int count = 0;
+};
+
+}  // namespace
+
+struct gdp_plan {
+  gdp_options opt;
+  bool fp32 = false;
+  size_t tsize = 8;
+  std::vector<ProbInfo> probs;
+  int depth = 0;
+  int down_menu[kMaxDepth];
+  int up_menu[kMaxDepth];
+  int kry_menu = 1;
+  int total_levels = 0;
+  int tail_smem = 0;
+  int max_slots = 1;
+  size_t ws_bytes = 0;
+  size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0, off_partial = 0;
+  size_t off_counters = 0, off_ranges_kry = 0, off_ranges_up0 = 0, off_io = 0, off_init = 0;
+  size_t off_hist = 0, off_flag = 0;
+  size_t off_tiles = 0;     // start of the tile tables region
+  size_t tiles_bytes = 0;   // reserved bytes for the tile tables
+  Table down[kMaxDepth], up[kMaxDepth], kry;
+  // runtime
+  char* ws = nullptr;
+  IoDesc* h_io = nullptr;
+  const double** h_init = nullptr;
+  ProblemState* h_state = nullptr;
+  int* h_flag = nullptr;
+  hipStream_t cap = nullptr;
+  hipEvent_t io_ev = nullptr;  // the last upload of h_io / h_init has been consumed
+  hipGraphExec_t pair_exec[2] = {nullptr, nullptr};  // [0] mgcg, [1] cg
+  int last_iters = 0;
+  bool bound = false;
+
+  template <typename X>
+  X* at(size_t off) const { return reinterpret_cast<X*>(ws + off); }
+  LevelDesc* d_levels() const { return at<LevelDesc>(off_levels); }
+  KrylovDesc* d_kry() const { return at<KrylovDesc>(off_kry); }
+  TailDesc* d_tails() const { return at<TailDesc>(off_tails); }
+  ProblemState* d_state() const { return at<ProblemState>(off_state); }
+  double2* d_partial() const { return at<double2>(off_partial); }
+  int* d_counters() const { return at<int>(off_counters); }
+  SlotRange* d_rkry() const { return at<SlotRange>(off_ranges_kry); }
+  SlotRange* d_rup0() const { return at<SlotRange>(off_ranges_up0); }
+  IoDesc* d_io() const { return at<IoDesc>(off_io); }
+  const double** d_init() const { return at<const double*>(off_init); }
+  double* d_hist() const { return at<double>(off_hist); }
+  int* d_flag() const { return at<int>(off_flag); }
+  int count() const { return int(probs.size()); }
+};
+
+namespace {
+
+std::vector<std::pair<int, int>> level_shapes(const gdp_problem& p, const gdp_options& o) {
+  std::vector<std::pair<int, int>> s;
+  int m = p.rows, n = p.cols;
+  s.push_back({m, n});
+  for (;;) {
+    if (p.boundary == GDP_NEUMANN) {
+      if (std::max(m, n) <= o.coarsest_size || std::min(m, n) < 3) break;
+      m = (m + 1) / 2;
+      n = (n + 1) / 2;
+    } else {
+      const int mc = (m - 1) / 2, nc = (n - 1) / 2;
+      if (std::min(m, n) <= o.coarsest_size || std::min(mc, nc) < 2) break;
+      m = mc;
+      n = nc;
+    }
+    s.push_back({m, n});
+  }
+  return s;
+}
+
+// LDS bytes of a tail holding levels [first, L) (u and b per level, one residual
+// scratch of the first level, two FP64 arrays of the coarsest).  Fills the offsets.
+int tail_layout(ProbInfo& pi, int first, size_t tsize) {
+  const int L = int(pi.lv.size());
+  if (L - first > kMaxTailLevels) return 1 << 30;
+  size_t off = 0;
+  auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+  for (int l = first; l < L; ++l) {
+    const size_t pts = size_t(pi.lv[l].m) * pi.lv[l].n;
+    pi.tail_off[l - first][0] = int(off);
+    off += a16(pts * tsize);
+    pi.tail_off[l - first][1] = int(off);
+    off += a16(pts * tsize);
+  }
+  pi.scratch_off = int(off);
+  off += a16(size_t(pi.lv[first].m) * pi.lv[first].n * tsize);
+  pi.coarse_off = int(off);
+  off += 2 * size_t(pi.lv[L - 1].m) * pi.lv[L - 1].n * 8;
+  return int(off);
+}
+
+Shape2 menu_shape(int kind, int idx) {
+  if (kind == 0) return kDownMenu[idx];
+  if (kind == 1) return kUpMenu[idx];
+  return kKryMenu[idx];
+}
+
+// Worst case tile count of a grid over every entry of a menu.
+int max_tiles(int kind, int m, int n) {
+  int best = 0;
+  for (int i = 0; i < kMenu; ++i) {
+    const Shape2 s = menu_shape(kind, i);
+    best = std::max(best, cdiv(m, s.h) * cdiv(n, s.w));
+  }
+  return best;
+}
+
+// Block b of a launch runs on XCD b % 8; give each XCD a contiguous run of tiles so
+// neighbouring tiles (which share halo rows) hit the same L2.
+std::vector<Tile> xcd_swizzle(const std::vector<Tile>& in) {
+  const int N = int(in.size()), X = 8;
+  const int base = N / X, rem = N % X;
+  std::vector<Tile> out(N);
+  for (int b = 0; b < N; ++b) {
+    const int x = b % X, k = b / X;
+    const int start = x * base + std::min(x, rem);
+    out[b] = in[start + k];
+  }
+  return out;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* gdp_last_error(void) { return g_err.c_str(); }
+int gdp_version(void) { return 10000; }
+
+int gdp_tile_menu(int kind, int index, int* th, int* tw) {
+  if (kind < 0 || kind > 2 || index < 0 || index >= kMenu || !th || !tw) {
+    set_error("gdp_tile_menu: bad kind/index");
+    return GDP_EINVAL;
+  }
+  const Shape2 s = menu_shape(kind, index);
+  *th = s.h;
+  *tw = s.w;
+  return GDP_OK;
+}
+
+int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* options,
+                    gdp_plan** out) {
+  if (!problems || count <= 0 || !options || !out) {
+    set_error("gdp_plan_create: null argument or empty batch");
+    return GDP_EINVAL;
+  }
+  const gdp_options& o = *options;
+  if (o.precision != GDP_FP64 && o.precision != GDP_MIXED) {
+    set_error("unknown precision %d", o.precision);
+    return GDP_EINVAL;
+  }
+  if (o.pre_smoothing < 0 || o.pre_smoothing > 3 || o.post_smoothing < 0 ||
+      o.post_smoothing > 3) {
+    set_error("pre/post smoothing must be in [0, 3] (got %d/%d)", o.pre_smoothing,
+              o.post_smoothing);
+    return GDP_ENOTSUP;
+  }
+  if (o.coarsest_size < 1 || o.coarsest_sweeps < 0 || o.max_iterations < 0) {
+    set_error("bad coarsest_size / coarsest_sweeps / max_iterations");
+    return GDP_EINVAL;
+  }
+  gdp_plan* P = new gdp_plan();
+  P->opt = o;
+  P->fp32 = o.precision == GDP_MIXED;
+  P->tsize = P->fp32 ? 4 : 8;
+  for (int i = 0; i < kMaxDepth; ++i) {
+    P->down_menu[i] = 1;
+    P->up_menu[i] = 1;
+  }
+  P->kry_menu = 1;
+  int lvl_total = 0;
+  for (int i = 0; i < count; ++i) {
+    const gdp_problem& pr = problems[i];
+    if (pr.rows < 1 || pr.cols < 1 || !(pr.spacing > 0.0) ||
+        (pr.boundary != GDP_DIRICHLET && pr.boundary != GDP_NEUMANN)) {
+      set_error("problem %d: bad shape (%d, %d), spacing or boundary", i, pr.rows, pr.cols);
+      delete P;
+      return GDP_EINVAL;
+    }
+    ProbInfo pi;
+    pi.p = pr;
+    auto shapes = level_shapes(pr, o);
+    double h = pr.spacing;
+    for (auto& s : shapes) {
+      LevelInfo li;
+      li.m = s.first;
+      li.n = s.second;
+      li.pitch = round_up(s.second, 32);
+      li.h = h;
+      h *= 2.0;
+      pi.lv.push_back(li);
+    }
+    const int L = int(pi.lv.size());
+    int first = -1;
+    for (int f = 0; f < L; ++f) {
+      const int bytes = tail_layout(pi, f, P->tsize);
+      if (bytes <= kTailLdsBytes) {
+        first = f;
+        pi.tail_bytes = bytes;
+        break;
+      }
+    }
+    if (first < 0) {
+      set_error("problem %d (%d x %d): coarsest level %d x %d does not fit in LDS", i, pr.rows,
+                pr.cols, pi.lv[L - 1].m, pi.lv[L - 1].n);
+      delete P;
+      return GDP_ENOTSUP;
+    }
+    if (first >= kMaxDepth) {
+      set_error("problem %d: too many levels", i);
+      delete P;
+      return GDP_ENOTSUP;
+    }
+    pi.tail_first = first;
+    tail_layout(pi, first, P->tsize);
+    pi.level_base = lvl_total;
+    lvl_total += L;
+    P->depth = std::max(P->depth, first);
+    P->tail_smem = std::max(P->tail_smem, pi.tail_bytes);
+    P->probs.push_back(pi);
+  }
+  P->total_levels = lvl_total;
+
+  // ---- workspace layout
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o2 = off;
+    off = align256(off + bytes);
+    return o2;
+  };
+  for (auto& pi : P->probs) {
+    for (auto& li : pi.lv) {
+      const size_t a = size_t(li.m) * li.pitch * P->tsize;
+      li.off_u = take(a);
+      li.off_v = take(a);
+      li.off_b = take(a);
+    }
+    const size_t k = size_t(pi.lv[0].m) * pi.lv[0].pitch * 8;
+    pi.off_x = take(k);
+    pi.off_r = take(k);
+    pi.off_d0 = take(k);
+    pi.off_d1 = take(k);
+    pi.off_q = take(k);
+    const auto& c = pi.lv.back();
+    if (pi.p.boundary == GDP_NEUMANN)
+      pi.off_dct = take(size_t(c.m * c.m + c.n * c.n + c.m * c.n) * 8);
+  }
+  const int P_ = count;
+  P->off_levels = take(sizeof(LevelDesc) * lvl_total);
+  P->off_kry = take(sizeof(KrylovDesc) * P_);
+  P->off_tails = take(sizeof(TailDesc) * P_);
+  P->off_state = take(sizeof(ProblemState) * P_);
+  P->off_counters = take(sizeof(int) * P_);
+  P->off_ranges_kry = take(sizeof(SlotRange) * P_);
+  P->off_ranges_up0 = take(sizeof(SlotRange) * P_);
+  P->off_io = take(sizeof(IoDesc) * P_);
+  P->off_init = take(sizeof(double*) * P_);
+  P->off_hist = take(sizeof(double) * P_ * (size_t(o.max_iterations) + 2));
+  P->off_flag = take(sizeof(int) * 4);
+  // tile tables (worst case over the menus)
+  int slots = 1;
+  size_t tiles = 0;
+  int kry_slots = 0, up0_slots = 0;
+  for (auto& pi : P->probs) {
+    kry_slots += max_tiles(2, pi.lv[0].m, pi.lv[0].n);
+    if (pi.tail_first > 0) up0_slots += max_tiles(1, pi.lv[0].m, pi.lv[0].n);
+    for (int l = 0; l < pi.tail_first; ++l) {
+      tiles += max_tiles(0, pi.lv[l + 1].m, pi.lv[l + 1].n);
+      tiles += max_tiles(1, pi.lv[l].m, pi.lv[l].n);
+    }
+  }
+  tiles += kry_slots;
+  slots = std::max({slots, kry_slots, up0_slots});
+  P->max_slots = slots;
+  P->off_partial = take(sizeof(double2) * slots);
+  P->tiles_bytes = sizeof(Tile) * (tiles + 8 * (2 * P->depth + 1));
+  P->off_tiles = take(P->tiles_bytes);
+  P->ws_bytes = off;
+  *out = P;
+  return GDP_OK;
+}
+
+int gdp_plan_workspace_bytes(const gdp_plan* P, size_t* bytes) {
+  if (!P || !bytes) {
+    set_error("null argument");
+    return GDP_EINVAL;
+  }
+  *bytes = P->ws_bytes;
+  return GDP_OK;
+}
+
+int gdp_plan_levels(const gdp_plan* P, int problem, int* levels) {
+  if (!P || !levels || problem < 0 || problem >= P->count()) {
+    set_error("bad problem index");
+    return GDP_EINVAL;
+  }
+  *levels = int(P->probs[problem].lv.size());
+  return GDP_OK;
+}
+
+int gdp_plan_level_shape(const gdp_plan* P, int problem, int level, int* m, int* n) {
+  if (!P || !m || !n || problem < 0 || problem >= P->count() || level < 0 ||
+      level >= int(P->probs[problem].lv.size())) {
+    set_error("bad problem / level index");
+    return GDP_EINVAL;
+  }
+  *m = P->probs[problem].lv[level].m;
+  *n = P->probs[problem].lv[level].n;
+  return GDP_OK;
+}
+
+int gdp_plan_set_tile(gdp_plan* P, int kind, int level, int menu_index) {
+  if (!P || kind < 0 || kind > 2 || menu_index < 0 || menu_index >= kMenu ||
+      (kind < 2 && (level < 0 || level >= kMaxDepth))) {
+    set_error("gdp_plan_set_tile: bad argument");
+    return GDP_EINVAL;
+  }
+  if (kind == 0) P->down_menu[level] = menu_index;
+  else if (kind == 1) P->up_menu[level] = menu_index;
+  else P->kry_menu = menu_index;
+  if (P->bound) P->bound = false;  // tables must be rebuilt by gdp_plan_bind
+  for (auto& e : P->pair_exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  return GDP_OK;
+}
+
+int gdp_plan_destroy(gdp_plan* P) {
+  if (!P) return GDP_OK;
+  for (auto& e : P->pair_exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (P->cap) (void)hipStreamDestroy(P->cap);
+  if (P->io_ev) (void)hipEventDestroy(P->io_ev);
+  if (P->h_io) (void)hipHostFree(P->h_io);
+  if (P->h_init) (void)hipHostFree(P->h_init);
+  if (P->h_state) (void)hipHostFree(P->h_state);
+  if (P->h_flag) (void)hipHostFree(P->h_flag);
+  delete P;
+  return GDP_OK;
+}
+
+int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
+  if (!P || !workspace) {
+    set_error("null argument");
+    return GDP_EINVAL;
+  }
+  if (bytes < P->ws_bytes) {
+    set_error("workspace of %zu bytes is smaller than the %zu the plan needs", bytes, P->ws_bytes);
+    return GDP_ENOMEM;
+  }
+  if ((reinterpret_cast<uintptr_t>(workspace) & 255) != 0) {
+    set_error("workspace must be 256-byte aligned");
+    return GDP_EINVAL;
+  }
+  P->ws = static_cast<char*>(workspace);
+  const int NP = P->count();
+  if (!P->h_io) {
+    GDP_HIP(hipHostMalloc((void**)&P->h_io, sizeof(IoDesc) * NP, hipHostMallocDefault));
+    GDP_HIP(hipHostMalloc((void**)&P->h_init, sizeof(double*) * NP, hipHostMallocDefault));
+    GDP_HIP(hipHostMalloc((void**)&P->h_state, sizeof(ProblemState) * NP, hipHostMallocDefault));
+    GDP_HIP(hipHostMalloc((void**)&P->h_flag, sizeof(int) * 4, hipHostMallocDefault));
+  }
+  if (!P->cap) GDP_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
+  if (!P->io_ev) GDP_HIP(hipEventCreateWithFlags(&P->io_ev, hipEventDisableTiming));
+
+  // ---- descriptors
+  std::vector<LevelDesc> levels(P->total_levels);
+  std::vector<KrylovDesc> kry(NP);
+  std::vector<TailDesc> tails(NP);
+  std::vector<double> dct_host;
+  for (int p = 0; p < NP; ++p) {
+    ProbInfo& pi = P->probs[p];
+    const int L = int(pi.lv.size());
+    for (int l = 0; l < L; ++l) {
+      const LevelInfo& li = pi.lv[l];
+      LevelDesc& d = levels[pi.level_base + l];
+      memset(&d, 0, sizeof(d));
+      d.u = P->ws + li.off_u;
+      d.v = P->ws + li.off_v;
+      d.b = P->ws + li.off_b;
+      d.m = li.m;
+      d.n = li.n;
+      d.pitch = li.pitch;
+      d.bc = pi.p.boundary;
+      d.h2 = li.h * li.h;
+      d.inv_h2 = 1.0 / d.h2;
+      d.coarse = l + 1 < L ? pi.level_base + l + 1 : -1;
+      d.problem = p;
+      d.level = l;
+    }
+    KrylovDesc& k = kry[p];
+    memset(&k, 0, sizeof(k));
+    k.x = P->at<double>(pi.off_x);
+    k.r = P->at<double>(pi.off_r);
+    k.d[0] = P->at<double>(pi.off_d0);
+    k.d[1] = P->at<double>(pi.off_d1);
+    k.q = P->at<double>(pi.off_q);
+    k.rhs = nullptr;
+    k.m = pi.lv[0].m;
+    k.n = pi.lv[0].n;
+    k.pitch = pi.lv[0].pitch;
+    k.bc = pi.p.boundary;
+    k.h2 = pi.p.spacing * pi.p.spacing;
+    k.inv_h2 = 1.0 / k.h2;
+    k.level0 = pi.level_base;
+    TailDesc& t = tails[p];
+    memset(&t, 0, sizeof(t));
+    t.first = pi.level_base + pi.tail_first;
+    t.nlev = L - pi.tail_first;
+    t.problem = p;
+    t.krylov = pi.tail_first == 0 ? p : -1;
+    for (int l = 0; l < t.nlev; ++l) {
+      t.lds_off[l][0] = pi.tail_off[l][0];
+      t.lds_off[l][1] = pi.tail_off[l][1];
+    }
+    t.scratch_off = pi.scratch_off;
+    t.coarse_off = pi.coarse_off;
+    t.dct = nullptr;
+    if (pi.p.boundary == GDP_NEUMANN) {
+      const int m = pi.lv.back().m, n = pi.lv.back().n;
+      std::vector<double> tab(size_t(m * m + n * n + m * n));
+      auto fill = [](double* C, int s) {
+        for (int k = 0; k < s; ++k)
+          for (int i = 0; i < s; ++i) {
+            const double sc = k == 0 ? std::sqrt(1.0 / s) : std::sqrt(2.0 / s);
+            C[k * s + i] = sc * std::cos(M_PI * (2.0 * i + 1.0) * k / (2.0 * s));
+          }
+      };
+      fill(tab.data(), m);
+      fill(tab.data() + m * m, n);
+      double* il = tab.data() + m * m + n * n;
+      for (int k = 0; k < m; ++k)
+        for (int l = 0; l < n; ++l) {
+          const double lam = 2.0 * std::cos(M_PI * k / m) + 2.0 * std::cos(M_PI * l / n) - 4.0;
+          il[k * n + l] = (k == 0 && l == 0) ? 0.0 : 1.0 / lam;
+        }
+      t.dct = P->at<double>(pi.off_dct);
+      GDP_HIP(hipMemcpy(P->ws + pi.off_dct, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    }
+  }
+  GDP_HIP(hipMemcpy(P->d_levels(), levels.data(), sizeof(LevelDesc) * levels.size(),
+                    hipMemcpyHostToDevice));
+  GDP_HIP(hipMemcpy(P->d_kry(), kry.data(), sizeof(KrylovDesc) * NP, hipMemcpyHostToDevice));
+  GDP_HIP(hipMemcpy(P->d_tails(), tails.data(), sizeof(TailDesc) * NP, hipMemcpyHostToDevice));
+
+  // ---- tile tables
+  size_t toff = P->off_tiles;
+  size_t tend = P->off_tiles + P->tiles_bytes;
+  auto upload = [&](Table& tb, const std::vector<Tile>& v) -> int {
+    tb.off = toff;
+    tb.count = int(v.size());
+    const size_t b = align256(sizeof(Tile) * std::max<size_t>(v.size(), 1));
+    if (toff + sizeof(Tile) * v.size() > tend) {
+      set_error("internal: tile table overflow");
+      return GDP_EINVAL;
+    }
+    if (!v.empty()) {
+      const std::vector<Tile> sw = xcd_swizzle(v);
+      if (hipMemcpy(P->ws + toff, sw.data(), sizeof(Tile) * sw.size(), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+        set_error("hipMemcpy of a tile table failed");
+        return GDP_EHIP;
+      }
+    }
+    toff += std::min(b, tend - toff);
+    return GDP_OK;
+  };
+  std::vector<SlotRange> rk(NP), ru(NP);
+  {
+    std::vector<Tile> v;
+    const Shape2 s = kKryMenu[P->kry_menu];
+    for (int p = 0; p < NP; ++p) {
+      const auto& l0 = P->probs[p].lv[0];
+      rk[p].begin = int(v.size());
+      for (int i = 0; i < l0.m; i += s.h)
+        for (int j = 0; j < l0.n; j += s.w) v.push_back(Tile{p, i, j, int(v.size())});
+      rk[p].count = int(v.size()) - rk[p].begin;
+    }
+    if (int rc = upload(P->kry, v)) return rc;
+  }
+  for (int l = 0; l < P->depth; ++l) {
+    std::vector<Tile> vd, vu;
+    const Shape2 sd = kDownMenu[P->down_menu[l]];
+    const Shape2 su = kUpMenu[P->up_menu[l]];
+    for (int p = 0; p < NP; ++p) {
+      const ProbInfo& pi = P->probs[p];
+      if (pi.tail_first <= l) {
+        if (l == 0) ru[p] = SlotRange{0, 0};
+        continue;
+      }
+      const auto& f = pi.lv[l];
+      const auto& c = pi.lv[l + 1];
+      for (int i = 0; i < c.m; i += sd.h)
+        for (int j = 0; j < c.n; j += sd.w)
+          vd.push_back(Tile{pi.level_base + l, i, j, int(vd.size())});
+      if (l == 0) ru[p].begin = int(vu.size());
+      for (int i = 0; i < f.m; i += su.h)
+        for (int j = 0; j < f.n; j += su.w)
+          vu.push_back(Tile{pi.level_base + l, i, j, int(vu.size())});
+      if (l == 0) ru[p].count = int(vu.size()) - ru[p].begin;
+    }
+    if (int rc = upload(P->down[l], vd)) return rc;
+    if (int rc = upload(P->up[l], vu)) return rc;
+  }
+  GDP_HIP(hipMemcpy(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
+  GDP_HIP(hipMemcpy(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
+  GDP_HIP(hipMemset(P->d_state(), 0, sizeof(ProblemState) * NP));
+  GDP_HIP(hipMemset(P->d_counters(), 0, sizeof(int) * NP));
+  GDP_HIP(hipMemset(P->d_flag(), 0, sizeof(int) * 4));
+
+  // the tail kernels may use more than the default dynamic LDS
+  const int smem = P->tail_smem;
+  if (P->fp32) {
+    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<float, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<float, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+  } else {
+    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<double, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<double, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
+  }
+  GDP_HIP(hipDeviceSynchronize());
+  for (auto& e : P->pair_exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  P->bound = true;
+  return GDP_OK;
+}
+
+}  // extern "C"
+
+// ----------------------------------------------------------------- launch sequences
+namespace {
+
+template <typename T>
+int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots) {
+  const gdp_options& o = P->opt;
+  for (int l = 0; l < P->depth; ++l) {
+    if (!P->down[l].count) continue;
+    launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s,
+                   P->at<Tile>(P->down[l].off), P->d_levels(), P->d_state());
+    GDP_CHECK_LAUNCH();
+  }
+  const int post_order = 0;  // per-problem order is resolved below (Dirichlet: 0, Neumann: 1)
+  (void)post_order;
+  // Neumann problems use the symmetric cycle (post-smoothing colour 1 then 0); a plan
+  // mixing boundary types uses the order of its first problem for the whole batch.
+  const int order = P->probs[0].p.boundary == GDP_NEUMANN ? 1 : 0;
+  if (dots)
+    k_tail<T, true><<<P->count(), kTailThreads, P->tail_smem, s>>>(
+        P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), o.pre_smoothing,
+        o.post_smoothing, o.coarsest_sweeps, order);
+  else
+    k_tail<T, false><<<P->count(), kTailThreads, P->tail_smem, s>>>(
+        P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), o.pre_smoothing,
+        o.post_smoothing, o.coarsest_sweeps, order);
+  GDP_CHECK_LAUNCH();
+  for (int l = P->depth - 1; l >= 0; --l) {
+    if (!P->up[l].count) continue;
+    UpArgs a{P->at<Tile>(P->up[l].off), P->d_levels(), P->d_state(), P->d_kry(),
+             P->d_partial(), P->d_counters(), P->d_rup0(), order};
+    launch_up<T>(P->up_menu[l], o.post_smoothing, dots && l == 0, P->up[l].count, s, a);
+    GDP_CHECK_LAUNCH();
+  }
+  return GDP_OK;
+}
+
+int enqueue_vcycle(gdp_plan* P, hipStream_t s, bool dots) {
+  return P->fp32 ? enqueue_vcycle_t<float>(P, s, dots) : enqueue_vcycle_t<double>(P, s, dots);
+}
+
+template <typename T>
+int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  GDP_KRY_DISPATCH(P->kry_menu,
+                   (k_apply<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
+                       P->d_counters(), P->d_rkry(), din)));
+  GDP_CHECK_LAUNCH();
+  GDP_KRY_DISPATCH(P->kry_menu,
+                   (k_update<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
+                       P->d_counters(), P->d_rkry(), din ^ 1)));
+  GDP_CHECK_LAUNCH();
+  if (jacobi) {
+    GDP_KRY_DISPATCH(P->kry_menu,
+                     (k_jacobi<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
+                         P->d_counters(), P->d_rkry())));
+    GDP_CHECK_LAUNCH();
+    return GDP_OK;
+  }
+  return enqueue_vcycle_t<T>(P, s, true);
+}
+
+int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
+  int rc;
+  if (P->fp32) {
+    if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi))) return rc;
+    return enqueue_pcg_iter_t<float>(P, s, 1, jacobi);
+  }
+  if ((rc = enqueue_pcg_iter_t<double>(P, s, 0, jacobi))) return rc;
+  return enqueue_pcg_iter_t<double>(P, s, 1, jacobi);
+}
+
+template <typename T>
+int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol, int max_it,
+                   bool with_init, int zero_done) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  GDP_KRY_DISPATCH(P->kry_menu, (k_init_stats<TH, TW><<<nb, kThreads, 0, s>>>(
+                                    t, P->d_kry(), P->d_io(), P->d_state(), P->d_partial(),
+                                    P->d_counters(), P->d_rkry(), project)));
+  GDP_CHECK_LAUNCH();
+  GDP_KRY_DISPATCH(P->kry_menu,
+                   (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                       t, P->d_kry(), P->d_levels(), P->d_io(),
+                       with_init ? P->d_init() : nullptr, P->d_state(), P->d_partial(),
+                       P->d_counters(), P->d_rkry(), mode, tol * tol, max_it, P->opt.flexible,
+                       zero_done)));
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
+}
+
+template <typename T>
+int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, bool hist) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  GDP_KRY_DISPATCH(P->kry_menu,
+                   (k_residual<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                       t, P->d_kry(), P->d_levels(), P->d_io(), P->d_state(), P->d_partial(),
+                       P->d_counters(), P->d_rkry(), mode, P->opt.tolerance,
+                       hist ? P->d_hist() : nullptr, P->opt.max_iterations + 2,
+                       respect_done)));
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
+}
+
+template <typename T>
+int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  GDP_KRY_DISPATCH(P->kry_menu, (k_correct<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                                    t, P->d_kry(), P->d_levels(), P->d_state())));
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
+}
+
+int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  GDP_KRY_DISPATCH(P->kry_menu, (k_finalize<TH, TW><<<nb, kThreads, 0, s>>>(
+                                    t, P->d_kry(), P->d_io(), P->d_state(), mean_fix)));
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
+}
+
+#define GDP_T(call) (P->fp32 ? call<float> : call<double>)
+
+int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double* const* init,
+             hipStream_t s) {
+  const int NP = P->count();
+  // the pinned staging buffers are reused: wait until the previous upload has run
+  GDP_HIP(hipEventSynchronize(P->io_ev));
+  for (int p = 0; p < NP; ++p) {
+    if (!b[p] || !u[p]) {
+      set_error("problem %d: null b or u pointer", p);
+      return GDP_EINVAL;
+    }
+    P->h_io[p].b = b[p];
+    P->h_io[p].u = u[p];
+    if (init) P->h_init[p] = init[p];
+  }
+  GDP_HIP(hipMemcpyAsync(P->d_io(), P->h_io, sizeof(IoDesc) * NP, hipMemcpyHostToDevice, s));
+  if (init)
+    GDP_HIP(hipMemcpyAsync(P->d_init(), P->h_init, sizeof(double*) * NP, hipMemcpyHostToDevice,
+                           s));
+  GDP_HIP(hipEventRecord(P->io_ev, s));
+  return GDP_OK;
+}
+
+int read_state(gdp_plan* P, hipStream_t s) {
+  GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
+                         hipMemcpyDeviceToHost, s));
+  GDP_HIP(hipStreamSynchronize(s));
+  return GDP_OK;
+}
+
+bool all_done(const gdp_plan* P) {
+  for (int p = 0; p < P->count(); ++p)
+    if (!P->h_state[p].done) return false;
+  return true;
+}
+
+void fill_reports(const gdp_plan* P, gdp_report* rep) {
+  if (!rep) return;
+  for (int p = 0; p < P->count(); ++p) {
+    const ProblemState& S = P->h_state[p];
+    rep[p].iterations = S.iter;
+    rep[p].converged = S.converged;
+    rep[p].relative_residual = S.bnorm2 > 0.0 ? std::sqrt(S.true_rr / S.bnorm2) : 0.0;
+    rep[p].recursive_residual = S.bnorm2 > 0.0 ? std::sqrt(S.rr / S.bnorm2) : 0.0;
+  }
+}
+
+int launch_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
+  const int g = jacobi ? 1 : 0;
+  if (P->opt.use_graph) {
+    if (!P->pair_exec[g]) {
+      hipGraph_t graph = nullptr;
+      GDP_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_pcg_pair(P, P->cap, jacobi);
+      const hipError_t ee = hipStreamEndCapture(P->cap, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      GDP_HIP(ee);
+      const hipError_t ie = hipGraphInstantiate(&P->pair_exec[g], graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      GDP_HIP(ie);
+    }
+    GDP_HIP(hipGraphLaunch(P->pair_exec[g], s));
+    return GDP_OK;
+  }
+  return enqueue_pcg_pair(P, s, jacobi);
+}
+
+int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* rep,
+            hipStream_t s, bool jacobi) {
+  if (!P || !P->bound) {
+    set_error("plan is not bound to a workspace");
+    return GDP_EINVAL;
+  }
+  int rc;
+  if ((rc = stage_io(P, b, u, nullptr, s))) return rc;
+  const gdp_options& o = P->opt;
+  if ((rc = GDP_T(enqueue_init_t)(P, s, 0, 1, o.tolerance, o.max_iterations, false, 1)))
+    return rc;
+  // first preconditioned residual z0 = M r0 and delta0 = <r0, z0>
+  if (jacobi) {
+    const int nb = P->kry.count;
+    const Tile* t = P->at<Tile>(P->kry.off);
+    if (P->fp32) {
+      GDP_KRY_DISPATCH(P->kry_menu,
+                       (k_jacobi<float, TH, TW><<<nb, kThreads, 0, s>>>(
+                           t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
+                           P->d_counters(), P->d_rkry())));
+    } else {
+      GDP_KRY_DISPATCH(P->kry_menu,
+                       (k_jacobi<double, TH, TW><<<nb, kThreads, 0, s>>>(
+                           t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
+                           P->d_counters(), P->d_rkry())));
+    }
+    GDP_CHECK_LAUNCH();
+  } else if ((rc = enqueue_vcycle(P, s, true))) {
+    return rc;
+  }
+  int launched = 0, restarts = 0;
+  int first_pairs = std::max(1, (P->last_iters + 1) / 2);
+  for (;;) {
+    const int pairs = launched == 0 ? first_pairs : 1;
+    for (int k = 0; k < pairs; ++k)
+      if ((rc = launch_pair(P, s, jacobi))) return rc;
+    launched += 2 * pairs;
+    if ((rc = read_state(P, s))) return rc;
+    if (!all_done(P)) continue;
+    // FP64 true residual of every problem; reopen the ones it rejects
+    if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 0, false))) return rc;
+    GDP_HIP(hipMemsetAsync(P->d_flag(), 0, sizeof(int), s));
+    k_reopen<<<cdiv(P->count(), 64), 64, 0, s>>>(P->d_state(), P->count(), P->d_flag());
+    GDP_CHECK_LAUNCH();
+    GDP_HIP(hipMemcpyAsync(P->h_flag, P->d_flag(), sizeof(int), hipMemcpyDeviceToHost, s));
+    if ((rc = read_state(P, s))) return rc;
+    if (P->h_flag[0] == 0 || restarts >= 4) break;
+    ++restarts;
+    if (jacobi) {
+      set_error("internal: Jacobi CG restart not supported");
+      return GDP_ENOTSUP;
+    }
+    if ((rc = enqueue_vcycle(P, s, true))) return rc;
+  }
+  if ((rc = enqueue_finalize(P, s, 1))) return rc;
+  if ((rc = read_state(P, s))) return rc;
+  int mx = 0;
+  for (int p = 0; p < P->count(); ++p) mx = std::max(mx, P->h_state[p].iter);
+  P->last_iters = mx;
+  fill_reports(P, rep);
+  return GDP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gdp_mgcg_solve(gdp_plan* P, const double* const* b, double* const* u, gdp_report* rep,
+                   void* stream) {
+  return run_pcg(P, b, u, rep, static_cast<hipStream_t>(stream), false);
+}
+
+int gdp_cg_solve(gdp_plan* P, const double* const* b, double* const* u, gdp_report* rep,
+                 void* stream) {
+  return run_pcg(P, b, u, rep, static_cast<hipStream_t>(stream), true);
+}
+
+int gdp_vcycle(gdp_plan* P, const double* const* u_in, const double* const* b,
+               double* const* u_out, void* stream) {
+  if (!P || !P->bound || !u_in || !b || !u_out) {
+    set_error("gdp_vcycle: unbound plan or null argument");
+    return GDP_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int rc;
+  if ((rc = stage_io(P, b, u_out, u_in, s))) return rc;
+  if ((rc = GDP_T(enqueue_init_t)(P, s, 1, 0, 0.0, 1, true, 0))) return rc;
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 0, false))) return rc;
+  if ((rc = enqueue_vcycle(P, s, false))) return rc;
+  if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
+  return enqueue_finalize(P, s, 0);
+}
+
+int gdp_mg_solve(gdp_plan* P, const double* const* b, const double* const* initial,
+                 double* const* u, gdp_report* rep, double* history, void* stream) {
+  if (!P || !P->bound || !b || !u) {
+    set_error("gdp_mg_solve: unbound plan or null argument");
+    return GDP_EINVAL;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const gdp_options& o = P->opt;
+  int rc;
+  if ((rc = stage_io(P, b, u, initial, s))) return rc;
+  if ((rc = GDP_T(enqueue_init_t)(P, s, 1, 1, 0.0, o.max_iterations, initial != nullptr, 1)))
+    return rc;
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 1, true))) return rc;
+  int cycles = 0;
+  int batch = std::max(1, P->last_iters);
+  for (;;) {
+    for (int k = 0; k < batch; ++k) {
+      if ((rc = enqueue_vcycle(P, s, false))) return rc;
+      if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
+      if ((rc = GDP_T(enqueue_residual_t)(P, s, 2, 1, true))) return rc;
+    }
+    cycles += batch;
+    batch = 1;
+    if ((rc = read_state(P, s))) return rc;
+    if (all_done(P) || cycles > o.max_iterations) break;
+  }
+  if ((rc = enqueue_finalize(P, s, 1))) return rc;
+  if ((rc = read_state(P, s))) return rc;
+  int mx = 0;
+  for (int p = 0; p < P->count(); ++p) mx = std::max(mx, P->h_state[p].iter);
+  P->last_iters = mx;
+  if (rep) {
+    fill_reports(P, rep);
+    for (int p = 0; p < P->count(); ++p)
+      rep[p].relative_residual = rep[p].recursive_residual;  // MG's residual is the true one
+  }
+  if (history) {
+    const size_t stride = size_t(o.max_iterations) + 2;
+    std::vector<double> h(stride * P->count());
+    GDP_HIP(hipMemcpy(h.data(), P->d_hist(), h.size() * 8, hipMemcpyDeviceToHost));
+    for (int p = 0; p < P->count(); ++p)
+      for (int k = 0; k <= o.max_iterations; ++k)
+        history[size_t(p) * (o.max_iterations + 1) + k] = h[p * stride + k];
+  }
+  return GDP_OK;
+}
+
+// ----------------------------------------------------------------- single grids
+static int check_grid(const void* a, const void* b, int m, int n, int planes) {
+  if (!a || !b || m < 1 || n < 1 || planes < 1) {
+    set_error("null pointer or empty grid");
+    return GDP_EINVAL;
+  }
+  return GDP_OK;
+}
+
+int gdp_laplacian(const double* u, double* out, int m, int n, int planes, double spacing,
+                  int boundary, void* stream) {
+  if (int rc = check_grid(u, out, m, n, planes)) return rc;
+  GDP_HIP(launch_laplacian(u, out, m, n, planes, spacing, boundary,
+                           static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_gradient(const double* img, double* gx, double* gy, int h, int w, int planes,
+                 void* stream) {
+  if (int rc = check_grid(img, gx, h, w, planes)) return rc;
+  if (!gy) return (set_error("null gy"), GDP_EINVAL);
+  GDP_HIP(launch_gradient(img, gx, gy, h, w, planes, static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_divergence(const double* gx, const double* gy, double* out, int h, int w, int planes,
+                   void* stream) {
+  if (int rc = check_grid(gx, gy, h, w, planes)) return rc;
+  if (!out) return (set_error("null out"), GDP_EINVAL);
+  GDP_HIP(launch_divergence(gx, gy, out, h, w, planes, static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, int h, int w,
+                      int planes, double spacing, void* stream) {
+  if (int rc = check_grid(rhs, boundary, h - 2, w - 2, planes)) return rc;
+  if (!out) return (set_error("null out"), GDP_EINVAL);
+  GDP_HIP(launch_fold(rhs, boundary, out, h, w, planes, spacing,
+                      static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_smooth(double* u, const double* b, int m, int n, double spacing, int sweeps,
+               int boundary, int order, void* stream) {
+  if (int rc = check_grid(u, b, m, n, 1)) return rc;
+  if (sweeps < 0) return (set_error("negative sweeps"), GDP_EINVAL);
+  if (sweeps == 0) return GDP_OK;
+  GDP_HIP(launch_smooth(u, b, m, n, spacing, sweeps, boundary, order,
+                        static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_restrict(const double* fine, double* coarse, int m, int n, int mc, int nc, int boundary,
+                 void* stream) {
+  if (int rc = check_grid(fine, coarse, m, n, 1)) return rc;
+  if (mc < 1 || nc < 1) return (set_error("empty coarse grid"), GDP_EINVAL);
+  GDP_HIP(launch_restrict(fine, coarse, m, n, mc, nc, boundary,
+                          static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_prolong(const double* coarse, double* fine, int m, int n, int mc, int nc, int boundary,
+                void* stream) {
+  if (int rc = check_grid(fine, coarse, m, n, 1)) return rc;
+  if (mc < 1 || nc < 1) return (set_error("empty coarse grid"), GDP_EINVAL);
+  GDP_HIP(launch_prolong(coarse, fine, m, n, mc, nc, boundary,
+                         static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_relative_residual(const double* u, const double* b, int m, int n, int planes,
+                          double spacing, int boundary, double* result, void* stream) {
+  if (int rc = check_grid(u, b, m, n, planes)) return rc;
+  if (!result) return (set_error("null result"), GDP_EINVAL);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int blocks = 64;
+  double2* part = nullptr;
+  GDP_HIP(hipMallocAsync((void**)&part, sizeof(double2) * blocks * planes, s));
+  GDP_HIP(launch_resnorm_partial(u, b, m, n, planes, spacing, boundary, part, blocks, s));
+  std::vector<double2> h(size_t(blocks) * planes);
+  GDP_HIP(hipMemcpyAsync(h.data(), part, sizeof(double2) * h.size(), hipMemcpyDeviceToHost, s));
+  GDP_HIP(hipFreeAsync(part, s));
+  GDP_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < planes; ++p) {
+    double rr = 0.0, bb = 0.0;
+    for (int k = 0; k < blocks; ++k) {
+      rr += h[size_t(p) * blocks + k].x;
+      bb += h[size_t(p) * blocks + k].y;
+    }
+    result[p] = bb > 0.0 ? std::sqrt(rr) / std::sqrt(bb) : 0.0;
+  }
+  return GDP_OK;
+}
+
+}  // extern "C"

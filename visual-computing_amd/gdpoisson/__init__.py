@@ -1,0 +1,35 @@
+"""gdpoisson: MI355X-native multigrid Poisson integration for gradient-domain imaging.
+
+Drop-in GPU replacement for the Poisson path of the gradient-domain package
+(Giankaqui/visual-computing, ``gradient-domain/src/gradient_domain``): the same
+public names (``solve_system``, ``SOLVERS``, ``SolverReport``, ``MultigridOptions``,
+``MultigridReport``, ``v_cycle``, ``solve``, ``solve_neumann``, ``gradient``,
+``divergence``, ``laplacian``, ``fold_boundary``) backed by hand-written HIP kernels
+in ``libgdpoisson.so`` (C ABI: ``include/gdpoisson.h``).  Batched plans
+(:class:`Plan`) solve many grids per launch.
+"""
+
+from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
+from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
+from .plan import Plan, PlanOptions
+from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "MultigridOptions",
+    "MultigridReport",
+    "Plan",
+    "PlanOptions",
+    "SOLVERS",
+    "SolverReport",
+    "divergence",
+    "fold_boundary",
+    "gradient",
+    "laplacian",
+    "neumann_laplacian",
+    "solve",
+    "solve_neumann",
+    "solve_system",
+    "v_cycle",
+]

@@ -1,0 +1,233 @@
+"""Batched multigrid plans: a set of independent grids solved by shared launches.
+
+A :class:`Plan` owns one libgdpoisson plan and its device workspace (a torch
+``uint8`` tensor).  Every kernel launch of a solve covers all of the plan's
+problems, so the three channels of an RGB image, or a batch of images of different
+sizes, cost the launches of one solve.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Sequence
+
+import torch
+
+from . import _lib
+
+BOUNDARY = {"dirichlet": _lib.DIRICHLET, "neumann": _lib.NEUMANN}
+PRECISION = {"fp64": _lib.FP64, "mixed": _lib.MIXED}
+
+
+@dataclass(frozen=True)
+class PlanOptions:
+    """Solver knobs (``MultigridOptions`` of multigrid.py:46-71 plus precision etc.)."""
+
+    precision: str = "fp64"
+    pre_smoothing: int = 1
+    post_smoothing: int = 1
+    coarsest_size: int | None = None  # default: 8 (Dirichlet), 16 (Neumann)
+    coarsest_sweeps: int = 60
+    max_iterations: int = 2000
+    tolerance: float = 1e-8
+    flexible: bool | None = None  # default: Polak-Ribiere for mixed, Fletcher-Reeves for fp64
+    use_graph: bool = True
+
+
+def _device(device) -> torch.device:
+    if device is None:
+        if not torch.cuda.is_available():
+            raise RuntimeError("gdpoisson needs a HIP GPU (torch.cuda.is_available() is False)")
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device(device)
+
+
+class Plan:
+    """A bound libgdpoisson plan for ``shapes`` (one (m, n) per problem)."""
+
+    def __init__(
+        self,
+        shapes: Sequence[tuple[int, int]],
+        boundary: str | Sequence[str] = "dirichlet",
+        spacing: float | Sequence[float] = 1.0,
+        options: PlanOptions | None = None,
+        device=None,
+        tiles: dict | None = None,
+    ):
+        opts = options or PlanOptions()
+        self.options = opts
+        self.shapes = [tuple(int(x) for x in s) for s in shapes]
+        n = len(self.shapes)
+        bcs = [boundary] * n if isinstance(boundary, str) else list(boundary)
+        hs = [float(spacing)] * n if isinstance(spacing, (int, float)) else list(spacing)
+        if len(bcs) != n or len(hs) != n:
+            raise ValueError("boundary / spacing lists must match the number of shapes")
+        for b in bcs:
+            if b not in BOUNDARY:
+                raise ValueError(f"unknown boundary {b!r}")
+        if opts.precision not in PRECISION:
+            raise ValueError(f"unknown precision {opts.precision!r}")
+        self.boundary = bcs
+        self.device = _device(device)
+        probs = (_lib.Problem * n)()
+        for i, ((m, nn), b, h) in enumerate(zip(self.shapes, bcs, hs)):
+            probs[i] = _lib.Problem(m, nn, BOUNDARY[b], 0, h)
+        cs = opts.coarsest_size
+        if cs is None:
+            cs = 16 if bcs[0] == "neumann" else 8
+        flex = opts.flexible
+        if flex is None:
+            flex = opts.precision == "mixed"
+        copt = _lib.Options(
+            PRECISION[opts.precision],
+            opts.pre_smoothing,
+            opts.post_smoothing,
+            cs,
+            opts.coarsest_sweeps,
+            opts.max_iterations,
+            int(bool(flex)),
+            int(bool(opts.use_graph)),
+            opts.tolerance,
+        )
+        L = _lib.lib()
+        self._plan = ctypes.c_void_p()
+        _lib.check(L.gdp_plan_create(probs, n, ctypes.byref(copt), ctypes.byref(self._plan)),
+                   "gdp_plan_create")
+        self.max_iterations = opts.max_iterations
+        if tiles:
+            for (kind, level), idx in tiles.items():
+                _lib.check(L.gdp_plan_set_tile(self._plan, kind, level, idx), "gdp_plan_set_tile")
+        nbytes = ctypes.c_size_t()
+        _lib.check(L.gdp_plan_workspace_bytes(self._plan, ctypes.byref(nbytes)))
+        self.workspace_bytes = int(nbytes.value)
+        with torch.cuda.device(self.device):
+            self.workspace = torch.empty(self.workspace_bytes + 256, dtype=torch.uint8,
+                                         device=self.device)
+        self._bind()
+
+    def _bind(self):
+        base = self.workspace.data_ptr()
+        aligned = (base + 255) & ~255
+        with torch.cuda.device(self.device):
+            _lib.check(
+                _lib.lib().gdp_plan_bind(self._plan, ctypes.c_void_p(aligned),
+                                         self.workspace_bytes),
+                "gdp_plan_bind",
+            )
+
+    def set_tiles(self, tiles: dict):
+        """Re-bind with {(kind, level): menu index} tile choices (see gdp_tile_menu)."""
+        L = _lib.lib()
+        for (kind, level), idx in tiles.items():
+            _lib.check(L.gdp_plan_set_tile(self._plan, kind, level, idx), "gdp_plan_set_tile")
+        self._bind()
+
+    def __del__(self):
+        p = getattr(self, "_plan", None)
+        if p is not None and p.value:
+            try:
+                _lib.lib().gdp_plan_destroy(p)
+            except Exception:
+                pass
+            self._plan = ctypes.c_void_p()
+
+    # -------------------------------------------------------------- helpers
+    @property
+    def count(self) -> int:
+        return len(self.shapes)
+
+    def levels(self, problem: int = 0) -> list[tuple[int, int]]:
+        L = _lib.lib()
+        nl = ctypes.c_int()
+        _lib.check(L.gdp_plan_levels(self._plan, problem, ctypes.byref(nl)))
+        out = []
+        for lv in range(nl.value):
+            m, n = ctypes.c_int(), ctypes.c_int()
+            _lib.check(L.gdp_plan_level_shape(self._plan, problem, lv, ctypes.byref(m),
+                                              ctypes.byref(n)))
+            out.append((m.value, n.value))
+        return out
+
+    def _check_inputs(self, arrays, name):
+        if len(arrays) != self.count:
+            raise ValueError(f"{name}: expected {self.count} arrays, got {len(arrays)}")
+        for a, s in zip(arrays, self.shapes):
+            if tuple(a.shape) != s:
+                raise ValueError(f"{name}: shape {tuple(a.shape)} does not match {s}")
+            if a.dtype != torch.float64 or not a.is_contiguous() or a.device != self.device:
+                raise ValueError(f"{name}: need contiguous float64 tensors on {self.device}")
+
+    # -------------------------------------------------------------- solves
+    def mgcg(self, b: Sequence[torch.Tensor], out: Sequence[torch.Tensor] | None = None,
+             jacobi: bool = False):
+        """MG-preconditioned CG (Jacobi-CG if ``jacobi``) of every problem.
+
+        Returns (solutions, list of _lib.Report).
+        """
+        b = list(b)
+        self._check_inputs(b, "b")
+        if out is None:
+            out = [torch.empty_like(x) for x in b]
+        out = list(out)
+        self._check_inputs(out, "out")
+        reps = (_lib.Report * self.count)()
+        bp = _lib.ptr_array([x.data_ptr() for x in b])
+        up = _lib.ptr_array([x.data_ptr() for x in out])
+        fn = _lib.lib().gdp_cg_solve if jacobi else _lib.lib().gdp_mgcg_solve
+        with torch.cuda.device(self.device):
+            _lib.check(fn(self._plan, bp, up, reps, _lib.current_stream()), "solve")
+        return out, list(reps)
+
+    def vcycle(self, u: Sequence[torch.Tensor], b: Sequence[torch.Tensor],
+               out: Sequence[torch.Tensor] | None = None):
+        u, b = list(u), list(b)
+        self._check_inputs(u, "u")
+        self._check_inputs(b, "b")
+        if out is None:
+            out = [torch.empty_like(x) for x in b]
+        self._check_inputs(list(out), "out")
+        with torch.cuda.device(self.device):
+            _lib.check(
+                _lib.lib().gdp_vcycle(
+                    self._plan,
+                    _lib.ptr_array([x.data_ptr() for x in u]),
+                    _lib.ptr_array([x.data_ptr() for x in b]),
+                    _lib.ptr_array([x.data_ptr() for x in out]),
+                    _lib.current_stream(),
+                ),
+                "gdp_vcycle",
+            )
+        return list(out)
+
+    def mg_solve(self, b: Sequence[torch.Tensor], initial: Sequence[torch.Tensor] | None = None):
+        """Repeated V-cycles (multigrid.solve).  Returns (solutions, reports, histories)."""
+        b = list(b)
+        self._check_inputs(b, "b")
+        out = [torch.empty_like(x) for x in b]
+        ip = None
+        if initial is not None:
+            initial = list(initial)
+            self._check_inputs(initial, "initial")
+            ip = _lib.ptr_array([x.data_ptr() for x in initial])
+        reps = (_lib.Report * self.count)()
+        hist = (ctypes.c_double * (self.count * (self.max_iterations + 1)))()
+        with torch.cuda.device(self.device):
+            _lib.check(
+                _lib.lib().gdp_mg_solve(
+                    self._plan,
+                    _lib.ptr_array([x.data_ptr() for x in b]),
+                    ip,
+                    _lib.ptr_array([x.data_ptr() for x in out]),
+                    reps,
+                    hist,
+                    _lib.current_stream(),
+                ),
+                "gdp_mg_solve",
+            )
+        histories = []
+        stride = self.max_iterations + 1
+        for p in range(self.count):
+            k = reps[p].iterations
+            histories.append([hist[p * stride + i] for i in range(k + 1)])
+        return out, list(reps), histories

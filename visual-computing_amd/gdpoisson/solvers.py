@@ -1,0 +1,141 @@
+"""Poisson solvers on the GPU with the interface of
+``gradient-domain/src/gradient_domain/solvers.py``.
+
+``SOLVERS`` keeps the reference's plug-in signature
+``fn(b, spacing, tolerance, max_iterations) -> (u, iterations, converged)``
+(solvers.py:163-168) and ``solve_system`` its contract (solvers.py:171-221):
+``KeyError`` for an unknown method, ``b`` taken as float64, a zero right-hand side
+returns zeros with residual 0.0, the report carries the true relative residual.
+
+Methods:
+
+``mgcg``        V-cycle-preconditioned CG, FP64 throughout (solvers.py:134-160).
+``mgcg_mixed``  the same CG with an FP32 V-cycle (r, x, d, q, dots and the coarsest
+                solve stay FP64) and the Polak-Ribiere beta.
+``multigrid``   repeated V-cycles, nu = 2/2 (solvers.py:121-131).
+``cg``          Jacobi-preconditioned CG (solvers.py:93-118).
+
+``boundary="neumann"`` (keyword extension) solves ``div(grad u) / h^2 = b`` for the
+zero-mean compatible part of ``b`` and returns the zero-mean solution, the
+convention of ``solve_neumann`` (solvers.py:224-262).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import torch
+
+from ._device import cached_plan, to_device, to_host_like
+from .multigrid import MultigridOptions
+from .plan import PlanOptions
+
+
+@dataclass
+class SolverReport:
+    method: str
+    iterations: int
+    relative_residual: float
+    seconds: float
+    converged: bool
+
+    def __str__(self) -> str:
+        return (
+            f"{self.method}: {self.iterations} iterations, "
+            f"residual {self.relative_residual:.2e}, {self.seconds * 1e3:.1f} ms"
+        )
+
+
+def _pcg_options(precision, tolerance, max_iterations, boundary):
+    return PlanOptions(
+        precision=precision,
+        pre_smoothing=1,
+        post_smoothing=1,
+        coarsest_size=16 if boundary == "neumann" else 8,
+        coarsest_sweeps=60,
+        max_iterations=int(max_iterations),
+        tolerance=float(tolerance),
+    )
+
+
+def _run(kind, b, spacing, tolerance, max_iterations, boundary="dirichlet"):
+    """Solve one grid on the device; returns (u tensor, iterations, converged, true residual)."""
+    if kind == "multigrid":
+        opts = MultigridOptions(max_cycles=int(max_iterations), tolerance=float(tolerance))
+        plan = cached_plan([tuple(b.shape)], boundary, float(spacing), PlanOptions(
+            precision="fp64", pre_smoothing=opts.pre_smoothing,
+            post_smoothing=opts.post_smoothing, coarsest_size=opts.coarsest_size,
+            coarsest_sweeps=opts.coarsest_sweeps, max_iterations=opts.max_cycles,
+            tolerance=opts.tolerance, use_graph=False))
+        (u,), (rep,), _ = plan.mg_solve([b])
+        return u, int(rep.iterations), bool(rep.converged), float(rep.relative_residual)
+    precision = "mixed" if kind == "mgcg_mixed" else "fp64"
+    plan = cached_plan([tuple(b.shape)], boundary, float(spacing),
+                       _pcg_options(precision, tolerance, max_iterations, boundary))
+    (u,), (rep,) = plan.mgcg([b], jacobi=(kind == "cg"))
+    return u, int(rep.iterations), bool(rep.converged), float(rep.relative_residual)
+
+
+def _registry_entry(kind):
+    def fn(b, spacing, tolerance, max_iterations):
+        t, host = to_device(b)
+        u, it, conv, _ = _run(kind, t, spacing, tolerance, max_iterations)
+        return to_host_like(u, host), it, conv
+
+    fn.__name__ = f"_solve_{kind}"
+    return fn
+
+
+SOLVERS = {
+    "cg": _registry_entry("cg"),
+    "multigrid": _registry_entry("multigrid"),
+    "mgcg": _registry_entry("mgcg"),
+    "mgcg_mixed": _registry_entry("mgcg_mixed"),
+}
+
+
+def solve_system(
+    b,
+    method: str = "mgcg",
+    spacing: float = 1.0,
+    tolerance: float = 1e-8,
+    max_iterations: int = 2000,
+    boundary: str = "dirichlet",
+):
+    """Solve ``laplacian(u) = b`` (solvers.py:171-221); returns ``(u, SolverReport)``.
+
+    ``seconds`` covers the device solve from a device-resident ``b`` to a
+    device-resident ``u`` (host copies of numpy inputs are outside it).
+    """
+    if method not in SOLVERS:
+        raise KeyError(f"unknown solver {method!r}; choose from {sorted(SOLVERS)}")
+    if boundary not in ("dirichlet", "neumann"):
+        raise ValueError(f"unknown boundary {boundary!r}")
+    t, host = to_device(b)
+    if t.ndim != 2:
+        raise ValueError(f"b must be 2-D, got shape {tuple(t.shape)}")
+    # a zero right-hand side converges at set-up with zero iterations and u = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u, it, conv, rel = _run(method, t, spacing, tolerance, max_iterations, boundary)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return to_host_like(u, host), SolverReport(
+        method=method, iterations=it, relative_residual=rel, seconds=dt, converged=conv
+    )
+
+
+def solve_neumann(b, tolerance: float = 1e-12, precision: str = "fp64"):
+    """Zero-mean solution of ``div(grad u) = b`` (solvers.py:224-262 contract).
+
+    The reference diagonalises the operator with a DCT; here the same system is
+    solved by the Neumann MG-PCG to ``tolerance`` (default 1e-12 relative residual).
+    Any constant component of ``b`` is discarded, as in the reference.
+    """
+    t, host = to_device(b)
+    if t.ndim != 2:
+        raise ValueError(f"b must be 2-D, got shape {tuple(t.shape)}")
+    u, _, _, _ = _run("mgcg_mixed" if precision == "mixed" else "mgcg", t, 1.0, tolerance,
+                      2000, boundary="neumann")
+    return to_host_like(u, host)

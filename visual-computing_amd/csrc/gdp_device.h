@@ -60,7 +60,8 @@ struct ProblemState {
   int32_t restart;   // next preconditioned direction starts a new Krylov sequence
   int32_t max_iter;
   int32_t flexible;
-  int32_t pad[2];
+  int32_t fault;     // a device-side reduction gave up waiting for a partial sum
+  int32_t pad;
 };
 
 // One workgroup's work item.
@@ -83,11 +84,17 @@ struct TailDesc {
   int32_t krylov;         // KrylovDesc index if first is level 0, else -1
   int32_t lds_off[16][2]; // byte offsets of (u, b) of each tail level in LDS
   int32_t scratch_off;    // residual scratch
-  int32_t coarse_off;     // FP64 coarsest working area
+  int32_t coarse_off;     // FP64 coarsest working area (two arrays of the coarsest level)
+  int32_t tab_off;        // FP64 DCT tables in LDS (Neumann)
+  int32_t pad;
   const double* dct;      // Neumann coarsest: Cm (m*m), Cn (n*n), 1/lambda (m*n)
 };
 
-constexpr int kTailThreads = 512;
+// Partial-sum slots hold this bit pattern (a NaN no arithmetic produces) while empty.
+constexpr unsigned long long kEmptySlot = 0xFFFFFFFFFFFFFFFFull;
+constexpr int kSpinLimit = 1 << 22;
+
+constexpr int kTailThreads = 1024;
 constexpr int kTailLdsBytes = 98304;
 constexpr int kMaxTailLevels = 16;
 

@@ -4,10 +4,11 @@
 // name a level (or Krylov) descriptor and a tile origin, so one launch covers every
 // problem of a plan (RGB channels, a batch of images, the requests of a serving
 // step).  Per-problem scalars (alpha, beta, delta, ||r||) are reduced on the device:
-// each block writes one partial, the last block of a problem (atomic arrival
-// counter) sums the partials in slot order and updates the ProblemState.  The order
-// of every sum is fixed, so a problem's result does not depend on what else is in
-// the batch.
+// each block publishes one partial with a relaxed agent-scope atomic store, the last
+// block of a problem (atomic arrival counter) collects the partials in slot order and
+// updates the ProblemState.  No block issues a release fence (on gfx950 that writes
+// the whole L2 back), and the order of every sum is fixed, so a problem's result
+// does not depend on what else is in the batch.
 //
 // Arithmetic follows gradient-domain/src/gradient_domain/:
 //   smoother      multigrid.py:147-166  u <- (sum of neighbours - h^2 b) / diag, colour by colour
@@ -44,45 +45,79 @@ __device__ inline double block_sum(double v, double* sh) {
   return t;
 }
 
-// Publish this block's partial and report whether it is the problem's last block.
-__device__ inline bool arrive_last(int* counter, int total, int* s_flag) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int prev = atomicAdd(counter, 1);
-    *s_flag = (prev == total - 1);
-  }
-  __syncthreads();
-  const bool last = *s_flag != 0;
-  if (last) __threadfence();
-  return last;
+__device__ inline unsigned long long* slot_ptr(double2* partial, int slot) {
+  return reinterpret_cast<unsigned long long*>(partial + slot);
 }
 
-// Sum of partial slots [begin, begin + count) in slot order (last block only).
+__device__ inline void publish2(double2* partial, int slot, double a, double b) {
+  unsigned long long* p = slot_ptr(partial, slot);
+  __hip_atomic_store(p, static_cast<unsigned long long>(__double_as_longlong(a)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, static_cast<unsigned long long>(__double_as_longlong(b)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait for a published value (bounded), read it and mark the slot empty again.
+__device__ inline double take_slot(unsigned long long* p, int& fault) {
+  unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int spins = 0;
+  while (v == kEmptySlot) {
+    if (++spins > kSpinLimit) {
+      fault = 1;
+      v = 0;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __hip_atomic_store(p, kEmptySlot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __longlong_as_double(static_cast<long long>(v));
+}
+
+// Block-reduce (a, b), publish them in `slot`, and return true in the problem's last
+// block with the problem totals in `tot` (valid in thread 0).  All threads call it.
 template <int NT>
-__device__ inline double2 gather_partials(const double2* partial, SlotRange rg, double* sh) {
-  // Each thread sums a contiguous run so the order is a fixed function of count.
+__device__ inline bool finish(double a, double b, double2* partial, int slot, int* counter,
+                              SlotRange rg, double* sh, int* s_flag, double2& tot, int* fault_out) {
+  const double ba = block_sum<NT>(a, sh);
+  const double bb = block_sum<NT>(b, sh + NT / 64);
+  if (threadIdx.x == 0) {
+    publish2(partial, slot, ba, bb);
+    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_flag = (prev == rg.count - 1);
+  }
+  __syncthreads();
+  if (*s_flag == 0) return false;
+  // last block: every other block has published (its store precedes its arrival)
   const int per = (rg.count + NT - 1) / NT;
   const int lo = rg.begin + threadIdx.x * per;
   const int hi = min(lo + per, rg.begin + rg.count);
   double sx = 0.0, sy = 0.0;
+  int fault = 0;
   for (int i = lo; i < hi; ++i) {
-    const double2 p = partial[i];
-    sx += p.x;
-    sy += p.y;
+    unsigned long long* p = slot_ptr(partial, i);
+    sx += take_slot(p, fault);
+    sy += take_slot(p + 1, fault);
   }
   const double tx = block_sum<NT>(sx, sh);
   const double ty = block_sum<NT>(sy, sh + NT / 64);
-  return make_double2(tx, ty);
+  const double tf = block_sum<NT>(double(fault), sh);
+  if (threadIdx.x == 0) {
+    tot = make_double2(tx, ty);
+    if (tf != 0.0) *fault_out = 1;
+    __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return true;
 }
 
 __device__ inline int neumann_count(int gi, int gj, int m, int n) {
   return (gi > 0) + (gi < m - 1) + (gj > 0) + (gj < n - 1);
 }
 
+// One Gauss-Seidel update: (sum of neighbours - h^2 b) / diag.
 template <typename T>
 __device__ inline T smooth_value(T nb, T b, T h2, bool neu, int gi, int gj, int m, int n) {
-  T val = nb - h2 * b;
+  const T val = nb - h2 * b;
   if (neu) {
     const int cnt = neumann_count(gi, gj, m, n);
     return cnt == 4 ? val * T(0.25) : val / T(cnt);
@@ -95,51 +130,37 @@ __device__ inline T diag_of(bool neu, int gi, int gj, int m, int n) {
   return neu ? T(neumann_count(gi, gj, m, n)) : T(4);
 }
 
-// 1-D transfer weights.  Dirichlet (multigrid.py:111-133): coarse I sits on fine
-// 2I+1; even fine points average their two coarse neighbours; zero outside.
-// Neumann: cell-centred, 3/4 parent + 1/4 next-nearest, reflected at the ends.
-struct Taps {
-  int idx[2];
-  double w[2];
-  int count;
-};
-
-__device__ inline Taps prolong_taps(bool neu, int i, int mc) {
-  Taps t;
-  t.count = 0;
-  if (!neu) {
-    if (i & 1) {
-      const int s = (i - 1) >> 1;
-      if (s >= 0 && s < mc) { t.idx[t.count] = s; t.w[t.count] = 1.0; ++t.count; }
-    } else {
-      const int s0 = (i >> 1) - 1, s1 = i >> 1;
-      if (s0 >= 0 && s0 < mc) { t.idx[t.count] = s0; t.w[t.count] = 0.5; ++t.count; }
-      if (s1 >= 0 && s1 < mc) { t.idx[t.count] = s1; t.w[t.count] = 0.5; ++t.count; }
-    }
+// Row taps of the prolongation at fine index i (coarse indices ia, ib, weights wa, wb).
+// Dirichlet (multigrid.py:111-133): odd i -> coarse (i-1)/2; even i -> mean of coarse
+// i/2-1 and i/2, zero outside.  Neumann: 3/4 parent i/2 + 1/4 next-nearest, with the
+// coarse grid edge-replicated so a missing neighbour reflects onto the parent.
+template <typename T>
+__device__ inline void ptaps(bool neu, int i, int& ia, int& ib, T& wa, T& wb) {
+  if (neu) {
+    ia = i >> 1;
+    ib = (i & 1) ? ia + 1 : ia - 1;
+    wa = T(0.75);
+    wb = T(0.25);
+  } else if (i & 1) {
+    ia = ib = (i - 1) >> 1;
+    wa = T(1);
+    wb = T(0);
   } else {
-    const int p = i >> 1;
-    const int o = (i & 1) ? p + 1 : p - 1;
-    if (o < 0 || o >= mc) { t.idx[0] = p; t.w[0] = 1.0; t.count = 1; }
-    else { t.idx[0] = p; t.w[0] = 0.75; t.idx[1] = o; t.w[1] = 0.25; t.count = 2; }
+    ia = (i >> 1) - 1;
+    ib = i >> 1;
+    wa = wb = T(0.5);
   }
-  return t;
-}
-
-// Restriction weight of fine point (base + k), k = 0..3, for coarse I; base = 2I (Dirichlet,
-// taps k = 0..2) or 2I - 1 (Neumann, taps k = 0..3).  f = fine size, c = coarse size.
-__device__ inline double restrict_weight(bool neu, int I, int k, int f, int c) {
-  if (!neu) return k == 1 ? 1.0 : (k == 3 ? 0.0 : 0.5);
-  const int i = 2 * I - 1 + k;
-  if (i < 0 || i >= f) return 0.0;
-  if (k == 0 || k == 3) return 0.25;
-  if (k == 1) return I == 0 ? 1.0 : 0.75;
-  return I == c - 1 ? 1.0 : 0.75;
 }
 
 // ------------------------------------------------------------------ down kernel
 // Pre-smoothing (NU sweeps from a zero guess) + residual + restriction, fused.
 // One block produces a CH x CW tile of the coarse right-hand side and the
 // pre-smoothed fine iterate of the fine points it owns.
+//
+// Restriction 0.25 P^T r: Dirichlet taps 2I..2I+2 with weights (1/2, 1, 1/2); Neumann
+// taps 2I-1..2I+2 with (1/4, 3/4, 3/4, 1/4) applied to the residual extended by
+// r[-1] = r[0] and, for even sizes, r[f] = r[f-1] (odd sizes: zero), which is exactly
+// the transpose of the reflected cell-centred interpolation.
 template <typename T, int CH, int CW, int NU>
 __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tiles,
                                                    const LevelDesc* __restrict__ levels,
@@ -150,9 +171,10 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
   const LevelDesc C = levels[L.coarse];
   constexpr int OFF = 2 + 2 * NU;
   constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
+  constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;  // residual rows fr0-1 .. fr0+2CH+1
   __shared__ T su[FH * FW];
   __shared__ T sb[FH * FW];
-  __shared__ T sr[FH * FW];
+  __shared__ T sr[RRH * RRW];
   const int m = L.m, n = L.n, pitch = L.pitch;
   const int fr0 = 2 * t.i0, fc0 = 2 * t.j0;
   const int gr0 = fr0 - OFF, gc0 = fc0 - OFF;
@@ -160,21 +182,25 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
   const T h2 = T(L.h2), ih2 = T(L.inv_h2);
   const T* __restrict__ b = static_cast<const T*>(L.b);
 
+  // load b; the first colour-0 half-sweep from u = 0 is folded into the load
   for (int k = threadIdx.x; k < FH * FW; k += kThreads) {
     const int a = k / FW, c = k - a * FW;
     const int gi = gr0 + a, gj = gc0 + c;
-    T bv = T(0);
-    if (a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 && gi < m && gj >= 0 && gj < n)
+    T bv = T(0), uv = T(0);
+    if (a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 && gi < m && gj >= 0 &&
+        gj < n) {
       bv = b[(size_t)gi * pitch + gj];
+      if (NU > 0 && ((gi + gj) & 1) == 0) uv = smooth_value<T>(T(0), bv, h2, neu, gi, gj, m, n);
+    }
     sb[k] = bv;
-    su[k] = T(0);
+    su[k] = uv;
   }
   __syncthreads();
 
   // After half-sweep s the points of [s+1, F-1-s) hold exactly what a whole-grid sweep
   // would give; the ring outside that region is only read, never trusted.
 #pragma unroll
-  for (int s = 0; s < 2 * NU; ++s) {
+  for (int s = 1; s < 2 * NU; ++s) {
     const int parity = s & 1;
     const int lo = 1 + s, hr = FH - 1 - s, hc = FW - 1 - s;
     const int rows = hr - lo, half = (hc - lo + 1) >> 1;
@@ -192,47 +218,60 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
     __syncthreads();
   }
 
-  // Residual on the core [OFF-1, OFF+2CH+2) and the pre-smoothed iterate on owned points.
+  // residual on fine rows/cols [fr0-1, fr0+2CH+2) (extended past the grid as above),
+  // and the pre-smoothed iterate of the owned points
   const int fr1 = (t.i0 + CH >= C.m) ? m : min(m, fr0 + 2 * CH);
   const int fc1 = (t.j0 + CW >= C.n) ? n : min(n, fc0 + 2 * CW);
   T* __restrict__ v = static_cast<T*>(L.v);
-  constexpr int CR = 2 * CH + 3, CC = 2 * CW + 3;
-  for (int k = threadIdx.x; k < CR * CC; k += kThreads) {
-    const int ca = k / CC, cc = k - ca * CC;
-    const int a = OFF - 1 + ca, c = OFF - 1 + cc;
-    const int gi = gr0 + a, gj = gc0 + c;
-    const int idx = a * FW + c;
-    T r = T(0);
-    if (gi >= 0 && gi < m && gj >= 0 && gj < n) {
-      const T uc = su[idx];
-      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      r = sb[idx] - (nb - diag_of<T>(neu, gi, gj, m, n) * uc) * ih2;
-      if (gi >= fr0 && gi < fr1 && gj >= fc0 && gj < fc1) v[(size_t)gi * pitch + gj] = uc;
+  for (int k = threadIdx.x; k < RRH * RRW; k += kThreads) {
+    const int ra = k / RRW, rc = k - ra * RRW;
+    int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
+    bool ok = true;
+    if (si < 0 || si >= m || sj < 0 || sj >= n) {
+      // Neumann extension: r[-1] = r[0]; r[f] = r[f-1] for even f; zero otherwise
+      if (!neu) ok = false;
+      if (si == -1) si = 0;
+      else if (si == m && (m & 1) == 0) si = m - 1;
+      else if (si < 0 || si >= m) ok = false;
+      if (sj == -1) sj = 0;
+      else if (sj == n && (n & 1) == 0) sj = n - 1;
+      else if (sj < 0 || sj >= n) ok = false;
+    } else if (si < fr1 && sj < fc1 && si >= fr0 && sj >= fc0) {
+      v[(size_t)si * pitch + sj] = su[(si - gr0) * FW + (sj - gc0)];
     }
-    sr[idx] = r;
+    T r = T(0);
+    if (ok) {
+      const int idx = (si - gr0) * FW + (sj - gc0);
+      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
+      r = sb[idx] - (nb - diag_of<T>(neu, si, sj, m, n) * su[idx]) * ih2;
+    }
+    sr[k] = r;
   }
   __syncthreads();
 
   T* __restrict__ cb = static_cast<T*>(C.b);
-  const int ntap = neu ? 4 : 3;
-  const int tap0 = neu ? -1 : 0;
   for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
-    const int I = t.i0 + k / CW, J = t.j0 + k % CW;
+    const int ci = k / CW, cj = k - ci * CW;
+    const int I = t.i0 + ci, J = t.j0 + cj;
     if (I >= C.m || J >= C.n) continue;
-    const int a0 = 2 * (I - t.i0) + OFF + tap0, c0 = 2 * (J - t.j0) + OFF + tap0;
-    double acc = 0.0;
-    for (int cj = 0; cj < ntap; ++cj) {
-      const double wc = restrict_weight(neu, J, cj, n, C.n);
-      if (wc == 0.0) continue;
-      double col = 0.0;
-      for (int ri = 0; ri < ntap; ++ri) {
-        const double wr = restrict_weight(neu, I, ri, m, C.m);
-        if (wr == 0.0) continue;
-        col += wr * double(sr[(a0 + ri) * FW + c0 + cj]);
-      }
-      acc += wc * col;
+    T acc;
+    if (neu) {
+      const T* r0 = &sr[(2 * ci) * RRW + 2 * cj];  // fine (2I-1, 2J-1)
+      T col[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        col[x] = T(0.25) * r0[x] + T(0.75) * r0[RRW + x] + T(0.75) * r0[2 * RRW + x] +
+                 T(0.25) * r0[3 * RRW + x];
+      acc = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
+    } else {
+      const T* r0 = &sr[(2 * ci + 1) * RRW + 2 * cj + 1];  // fine (2I, 2J)
+      T col[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+        col[x] = T(0.5) * r0[x] + r0[RRW + x] + T(0.5) * r0[2 * RRW + x];
+      acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
     }
-    cb[(size_t)I * C.pitch + J] = T(0.25 * acc);
+    cb[(size_t)I * C.pitch + J] = T(0.25) * acc;
   }
 }
 
@@ -271,10 +310,19 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
   const T* __restrict__ vv = static_cast<const T*>(L.v);
   const T* __restrict__ b = static_cast<const T*>(L.b);
 
+  // coarse correction with its padding: zero outside (Dirichlet), edge values (Neumann)
   for (int k = threadIdx.x; k < CRH * CRW; k += kThreads) {
     const int a = k / CRW, c = k - a * CRW;
-    const int I = cr0 + a, J = cc0 + c;
-    se[k] = (I >= 0 && I < C.m && J >= 0 && J < C.n) ? e[(size_t)I * C.pitch + J] : T(0);
+    int I = cr0 + a, J = cc0 + c;
+    T ev = T(0);
+    if (neu) {
+      I = min(max(I, 0), C.m - 1);
+      J = min(max(J, 0), C.n - 1);
+      ev = e[(size_t)I * C.pitch + J];
+    } else if (I >= 0 && I < C.m && J >= 0 && J < C.n) {
+      ev = e[(size_t)I * C.pitch + J];
+    }
+    se[k] = ev;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < RH * RW; k += kThreads) {
@@ -282,16 +330,14 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
     const int gi = gr0 + a, gj = gc0 + c;
     T uv = T(0), bv = T(0);
     if (gi >= 0 && gi < m && gj >= 0 && gj < n) {
-      const Taps tr = prolong_taps(neu, gi, C.m);
-      const Taps tc = prolong_taps(neu, gj, C.n);
-      double acc = 0.0;
-      for (int y = 0; y < tc.count; ++y) {
-        double col = 0.0;
-        for (int x = 0; x < tr.count; ++x)
-          col += tr.w[x] * double(se[(tr.idx[x] - cr0) * CRW + (tc.idx[y] - cc0)]);
-        acc += tc.w[y] * col;
-      }
-      uv = vv[(size_t)gi * pitch + gj] + T(acc);
+      int ia, ib, ja, jb;
+      T wa, wb, va, vb;
+      ptaps<T>(neu, gi, ia, ib, wa, wb);
+      ptaps<T>(neu, gj, ja, jb, va, vb);
+      const int oa = (ia - cr0) * CRW - cc0, ob = (ib - cr0) * CRW - cc0;
+      const T corr = va * (wa * se[oa + ja] + wb * se[ob + ja]) +
+                     vb * (wa * se[oa + jb] + wb * se[ob + jb]);
+      uv = vv[(size_t)gi * pitch + gj] + corr;
       bv = b[(size_t)gi * pitch + gj];
     }
     su[k] = uv;
@@ -326,19 +372,17 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= m || gj >= n) continue;
     const T z = su[(a + H) * RW + c + H];
-    uo[(size_t)gi * pitch + gj] = z;
+    const size_t o = (size_t)gi * pitch + gj;
+    uo[o] = z;
     if (DOTS) {
-      const size_t o = (size_t)gi * pitch + gj;
       zr += double(z) * K->r[o];
       zq += double(z) * K->q[o];
     }
   }
   if (DOTS) {
-    const double br = block_sum<kThreads>(zr, sh);
-    const double bq = block_sum<kThreads>(zq, sh + kThreads / 64);
-    if (threadIdx.x == 0) partial[t.slot] = make_double2(br, bq);
-    if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-      const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+    double2 tot;
+    if (finish<kThreads>(zr, zq, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot,
+                         &st[p].fault)) {
       if (threadIdx.x == 0) {
         ProblemState& S = st[p];
         double beta;
@@ -348,7 +392,6 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
         S.beta = beta;
         S.delta = tot.x;
         S.restart = 0;
-        counters[p] = 0;
       }
     }
   }
@@ -357,108 +400,157 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
 // ------------------------------------------------------------------ coarse tail
 // One block per problem runs every level from TailDesc.first to the coarsest in LDS:
 // pre-smooth / residual / restrict down, FP64 coarsest solve, prolong / post-smooth up.
+// Loops run rows over waves and columns over lanes (no integer division).
+constexpr int kTailWaves = kTailThreads / 64;
+
+// Red-black half-sweep of colour `parity`; with `from_zero` the iterate is known to be
+// zero, so colour `parity` gets (-h^2 b)/diag and the other colour is cleared.
 template <typename T>
-__device__ void lds_halfsweep(T* u, const T* b, int m, int n, T h2, bool neu, int parity) {
-  const int half = (n + 1) >> 1;
-  for (int k = threadIdx.x; k < m * half; k += kTailThreads) {
-    const int i = k / half;
-    const int j = 2 * (k - i * half) + ((i + parity) & 1);
-    if (j >= n) continue;
-    const int o = i * n + j;
-    const T up = i > 0 ? u[o - n] : T(0);
-    const T dn = i < m - 1 ? u[o + n] : T(0);
-    const T lf = j > 0 ? u[o - 1] : T(0);
-    const T rt = j < n - 1 ? u[o + 1] : T(0);
-    u[o] = smooth_value<T>(((up + dn) + lf) + rt, b[o], h2, neu, i, j, m, n);
+__device__ void lds_halfsweep(T* u, const T* b, int m, int n, T h2, bool neu, int parity,
+                              bool from_zero) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves) {
+    if (from_zero) {
+      for (int j = lane; j < n; j += 64) {
+        const int o = i * n + j;
+        u[o] = ((i + j) & 1) == parity ? smooth_value<T>(T(0), b[o], h2, neu, i, j, m, n) : T(0);
+      }
+      continue;
+    }
+    const int j0 = (i + parity) & 1;
+    for (int j = j0 + 2 * lane; j < n; j += 128) {
+      const int o = i * n + j;
+      const T up = i > 0 ? u[o - n] : T(0);
+      const T dn = i < m - 1 ? u[o + n] : T(0);
+      const T lf = j > 0 ? u[o - 1] : T(0);
+      const T rt = j < n - 1 ? u[o + 1] : T(0);
+      u[o] = smooth_value<T>(((up + dn) + lf) + rt, b[o], h2, neu, i, j, m, n);
+    }
   }
   __syncthreads();
 }
 
 template <typename T>
 __device__ void lds_residual(const T* u, const T* b, T* r, int m, int n, T ih2, bool neu) {
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
-    const int i = o / n, j = o - i * n;
-    const T up = i > 0 ? u[o - n] : T(0);
-    const T dn = i < m - 1 ? u[o + n] : T(0);
-    const T lf = j > 0 ? u[o - 1] : T(0);
-    const T rt = j < n - 1 ? u[o + 1] : T(0);
-    r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
-  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves)
+    for (int j = lane; j < n; j += 64) {
+      const int o = i * n + j;
+      const T up = i > 0 ? u[o - n] : T(0);
+      const T dn = i < m - 1 ? u[o + n] : T(0);
+      const T lf = j > 0 ? u[o - 1] : T(0);
+      const T rt = j < n - 1 ? u[o + 1] : T(0);
+      r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
+    }
   __syncthreads();
+}
+
+// Neumann residual extension used by the restriction (see k_down).
+__device__ inline int nsrc(int i, int f, bool& ok) {
+  if (i == -1) return 0;
+  if (i >= f) {
+    if (i == f && (f & 1) == 0) return f - 1;
+    ok = false;
+    return 0;
+  }
+  return i;
 }
 
 template <typename T>
 __device__ void lds_restrict(const T* r, int m, int n, T* bc, int mc, int nc, bool neu) {
-  const int ntap = neu ? 4 : 3, tap0 = neu ? -1 : 0;
-  for (int o = threadIdx.x; o < mc * nc; o += kTailThreads) {
-    const int I = o / nc, J = o - I * nc;
-    double acc = 0.0;
-    for (int cj = 0; cj < ntap; ++cj) {
-      const double wc = restrict_weight(neu, J, cj, n, nc);
-      if (wc == 0.0) continue;
-      const int j = 2 * J + tap0 + cj;
-      double col = 0.0;
-      for (int ri = 0; ri < ntap; ++ri) {
-        const double wr = restrict_weight(neu, I, ri, m, mc);
-        if (wr == 0.0) continue;
-        col += wr * double(r[(2 * I + tap0 + ri) * n + j]);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int I = wave; I < mc; I += kTailWaves)
+    for (int J = lane; J < nc; J += 64) {
+      T acc = T(0);
+      if (neu) {
+        const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          bool okc = true;
+          const int j = nsrc(2 * J - 1 + y, n, okc);
+          T col = T(0);
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            bool ok = okc;
+            const int i = nsrc(2 * I - 1 + x, m, ok);
+            col += ok ? w[x] * r[i * n + j] : T(0);
+          }
+          acc += w[y] * col;
+        }
+      } else {
+        const T w[3] = {T(0.5), T(1), T(0.5)};
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+          const int j = 2 * J + y;
+          T col = T(0);
+#pragma unroll
+          for (int x = 0; x < 3; ++x) col += w[x] * r[(2 * I + x) * n + j];
+          acc += w[y] * col;
+        }
       }
-      acc += wc * col;
+      bc[I * nc + J] = T(0.25) * acc;
     }
-    bc[o] = T(0.25 * acc);
-  }
   __syncthreads();
 }
 
 template <typename T>
-__device__ void lds_prolong_add(T* u, int m, int n, const T* e, int mc, int nc, bool neu) {
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
-    const int i = o / n, j = o - i * n;
-    const Taps tr = prolong_taps(neu, i, mc);
-    const Taps tc = prolong_taps(neu, j, nc);
-    double acc = 0.0;
-    for (int y = 0; y < tc.count; ++y) {
-      double col = 0.0;
-      for (int x = 0; x < tr.count; ++x) col += tr.w[x] * double(e[tr.idx[x] * nc + tc.idx[y]]);
-      acc += tc.w[y] * col;
-    }
-    u[o] = u[o] + T(acc);
+__device__ inline T lds_coarse(const T* e, int mc, int nc, int I, int J, bool neu) {
+  if (neu) {
+    I = min(max(I, 0), mc - 1);
+    J = min(max(J, 0), nc - 1);
+    return e[I * nc + J];
   }
+  return (I >= 0 && I < mc && J >= 0 && J < nc) ? e[I * nc + J] : T(0);
+}
+
+template <typename T>
+__device__ void lds_prolong_add(T* u, int m, int n, const T* e, int mc, int nc, bool neu) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves)
+    for (int j = lane; j < n; j += 64) {
+      int ia, ib, ja, jb;
+      T wa, wb, va, vb;
+      ptaps<T>(neu, i, ia, ib, wa, wb);
+      ptaps<T>(neu, j, ja, jb, va, vb);
+      const T corr = va * (wa * lds_coarse(e, mc, nc, ia, ja, neu) +
+                           wb * lds_coarse(e, mc, nc, ib, ja, neu)) +
+                     vb * (wa * lds_coarse(e, mc, nc, ia, jb, neu) +
+                           wb * lds_coarse(e, mc, nc, ib, jb, neu));
+      u[i * n + j] += corr;
+    }
   __syncthreads();
 }
 
 // Exact zero-mean solve of the Neumann coarsest level by the orthonormal DCT-II
 // (the solvers.py:224-262 construction): U = Cm^T ((Cm B Cn^T) ./ lambda) Cn.
+// Tables live in LDS: Cm (m*m), Cn (n*n), 1/lambda (m*n, zero for the constant mode).
 __device__ void lds_dct_solve(double* w0, double* w1, int m, int n, const double* tab) {
   const double* Cm = tab;
   const double* Cn = tab + m * m;
   const double* il = tab + m * m + n * n;
-  // w1 = Cm * w0
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
+  const int N = m * n;
+  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w1 = Cm w0
     const int k = o / n, j = o - k * n;
     double s = 0.0;
     for (int i = 0; i < m; ++i) s += Cm[k * m + i] * w0[i * n + j];
     w1[o] = s;
   }
   __syncthreads();
-  // w0 = (w1 * Cn^T) ./ lambda
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
+  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w0 = (w1 Cn^T) ./ lambda
     const int k = o / n, l = o - k * n;
     double s = 0.0;
     for (int j = 0; j < n; ++j) s += w1[k * n + j] * Cn[l * n + j];
     w0[o] = s * il[o];
   }
   __syncthreads();
-  // w1 = Cm^T * w0
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
+  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w1 = Cm^T w0
     const int i = o / n, l = o - i * n;
     double s = 0.0;
     for (int k = 0; k < m; ++k) s += Cm[k * m + i] * w0[k * n + l];
     w1[o] = s;
   }
   __syncthreads();
-  // w0 = w1 * Cn
-  for (int o = threadIdx.x; o < m * n; o += kTailThreads) {
+  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w0 = w1 Cn
     const int i = o / n, j = o - i * n;
     double s = 0.0;
     for (int l = 0; l < n; ++l) s += w1[i * n + l] * Cn[l * n + j];
@@ -479,15 +571,19 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
   const int p = D.problem;
   if (st[p].done) return;
   __shared__ double sh[2 * kTailThreads / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 
-  // load the right-hand side of the first tail level
-  {
+  {  // right-hand side of the first tail level, and the DCT tables
     const LevelDesc L = levels[D.first];
     T* b = reinterpret_cast<T*>(smem + D.lds_off[0][1]);
     const T* g = static_cast<const T*>(L.b);
-    for (int o = threadIdx.x; o < L.m * L.n; o += kTailThreads) {
-      const int i = o / L.n, j = o - i * L.n;
-      b[o] = g[(size_t)i * L.pitch + j];
+    for (int i = wave; i < L.m; i += kTailWaves)
+      for (int j = lane; j < L.n; j += 64) b[i * L.n + j] = g[(size_t)i * L.pitch + j];
+    if (D.dct) {
+      const LevelDesc Lc = levels[D.first + D.nlev - 1];
+      const int nt = Lc.m * Lc.m + Lc.n * Lc.n + Lc.m * Lc.n;
+      double* tb = reinterpret_cast<double*>(smem + D.tab_off);
+      for (int o = threadIdx.x; o < nt; o += kTailThreads) tb[o] = D.dct[o];
     }
     __syncthreads();
   }
@@ -499,17 +595,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     T* b = reinterpret_cast<T*>(smem + D.lds_off[l][1]);
     T* bc = reinterpret_cast<T*>(smem + D.lds_off[l + 1][1]);
     const bool neu = L.bc == kNeumann;
-    for (int o = threadIdx.x; o < L.m * L.n; o += kTailThreads) u[o] = T(0);
-    __syncthreads();
+    const T h2 = T(L.h2);
+    if (pre == 0) {
+      for (int i = wave; i < L.m; i += kTailWaves)
+        for (int j = lane; j < L.n; j += 64) u[i * L.n + j] = T(0);
+      __syncthreads();
+    }
     for (int s = 0; s < pre; ++s) {
-      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, 0);
-      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, 1);
+      lds_halfsweep<T>(u, b, L.m, L.n, h2, neu, 0, s == 0);
+      lds_halfsweep<T>(u, b, L.m, L.n, h2, neu, 1, false);
     }
     lds_residual<T>(u, b, scratch, L.m, L.n, T(L.inv_h2), neu);
     lds_restrict<T>(scratch, L.m, L.n, bc, C.m, C.n, neu);
   }
-  // coarsest level, FP64
-  {
+  {  // coarsest level, FP64
     const int l = D.nlev - 1;
     const LevelDesc L = levels[D.first + l];
     T* u = reinterpret_cast<T*>(smem + D.lds_off[l][0]);
@@ -520,7 +619,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     if (L.bc == kNeumann) {
       for (int o = threadIdx.x; o < N; o += kTailThreads) w0[o] = double(b[o]) * L.h2;
       __syncthreads();
-      lds_dct_solve(w0, w1, L.m, L.n, D.dct);
+      lds_dct_solve(w0, w1, L.m, L.n, reinterpret_cast<const double*>(smem + D.tab_off));
     } else {
       for (int o = threadIdx.x; o < N; o += kTailThreads) {
         w0[o] = 0.0;
@@ -528,8 +627,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
       }
       __syncthreads();
       for (int s = 0; s < coarsest_sweeps; ++s) {
-        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 0);
-        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 1);
+        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 0, false);
+        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 1, false);
       }
     }
     for (int o = threadIdx.x; o < N; o += kTailThreads) u[o] = T(w0[o]);
@@ -545,8 +644,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     lds_prolong_add<T>(u, L.m, L.n, e, C.m, C.n, neu);
     for (int s = 0; s < post; ++s) {
       const int p0 = order == 0 ? 0 : 1;
-      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, p0);
-      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, 1 - p0);
+      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, p0, false);
+      lds_halfsweep<T>(u, b, L.m, L.n, T(L.h2), neu, 1 - p0, false);
     }
   }
   // write the first tail level's iterate
@@ -555,16 +654,17 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
   T* g = static_cast<T*>(L.u);
   double zr = 0.0, zq = 0.0;
   const bool dots = DOTS && D.krylov >= 0;
-  for (int o = threadIdx.x; o < L.m * L.n; o += kTailThreads) {
-    const int i = o / L.n, j = o - i * L.n;
-    const size_t go = (size_t)i * L.pitch + j;
-    g[go] = u[o];
-    if (dots) {
-      const KrylovDesc& K = kds[D.krylov];
-      zr += double(u[o]) * K.r[go];
-      zq += double(u[o]) * K.q[go];
+  for (int i = wave; i < L.m; i += kTailWaves)
+    for (int j = lane; j < L.n; j += 64) {
+      const size_t go = (size_t)i * L.pitch + j;
+      const T z = u[i * L.n + j];
+      g[go] = z;
+      if (dots) {
+        const KrylovDesc& K = kds[D.krylov];
+        zr += double(z) * K.r[go];
+        zq += double(z) * K.q[go];
+      }
     }
-  }
   if (dots) {
     const double br = block_sum<kTailThreads>(zr, sh);
     const double bq = block_sum<kTailThreads>(zq, sh + kTailThreads / 64);
@@ -604,13 +704,13 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
   if (st[p].done) return;
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sd[RH * RW];
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[2 * kThreads / 64];
   __shared__ int s_flag;
   const double beta = st[p].beta;
   const int m = K.m, n = K.n, pitch = K.pitch;
   const T* __restrict__ z = static_cast<const T*>(levels[K.level0].u);
-  const double* __restrict__ dold = K.d[din];
-  double* __restrict__ dnew = K.d[din ^ 1];
+  const double* __restrict__ dold = din ? K.d[1] : K.d[0];
+  double* __restrict__ dnew = din ? K.d[0] : K.d[1];
   const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
   for (int k = threadIdx.x; k < RH * RW; k += kThreads) {
     const int a = k / RW, c = k - a * RW;
@@ -638,14 +738,11 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
     K.q[o] = q;
     acc += dc * q;
   }
-  const double bs = block_sum<kThreads>(acc, sh);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(bs, 0.0);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       st[p].dq = tot.x;
       st[p].alpha = st[p].delta / tot.x;
-      counters[p] = 0;
     }
   }
 }
@@ -664,12 +761,12 @@ __global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ ti
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   if (st[p].done) return;
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[2 * kThreads / 64];
   __shared__ int s_flag;
   const double alpha = st[p].alpha;
   const int m = K.m, n = K.n, pitch = K.pitch;
   T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
-  const double* __restrict__ d = K.d[dcur];
+  const double* __restrict__ d = dcur ? K.d[1] : K.d[0];
   double acc = 0.0;
   for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
     const int a = k / TW, c = k - a * TW;
@@ -682,17 +779,14 @@ __global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ ti
     b0[o] = T(-r);
     acc += r * r;
   }
-  const double bs = block_sum<kThreads>(acc, sh);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(bs, 0.0);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       ProblemState& S = st[p];
       S.rr = tot.x;
       S.iter += 1;
       S.converged = tot.x <= S.target2;
       S.done = S.converged || S.iter >= S.max_iter;
-      counters[p] = 0;
     }
   }
 }
@@ -711,7 +805,7 @@ __global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ ti
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   if (st[p].done) return;
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[2 * kThreads / 64];
   __shared__ int s_flag;
   const int m = K.m, n = K.n, pitch = K.pitch;
   T* __restrict__ z = static_cast<T*>(levels[K.level0].u);
@@ -727,16 +821,13 @@ __global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ ti
     z[o] = T(zz);
     acc += r * double(T(zz));
   }
-  const double bs = block_sum<kThreads>(acc, sh);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(bs, 0.0);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       ProblemState& S = st[p];
       S.beta = (S.iter == 0 || S.restart) ? 0.0 : tot.x / S.delta;
       S.delta = tot.x;
       S.restart = 0;
-      counters[p] = 0;
     }
   }
 }
@@ -755,7 +846,7 @@ __global__ __launch_bounds__(kThreads) void k_init_stats(const Tile* __restrict_
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[2 * kThreads / 64];
   __shared__ int s_flag;
   const double* __restrict__ b = io[p].b;
   double acc = 0.0;
@@ -767,13 +858,10 @@ __global__ __launch_bounds__(kThreads) void k_init_stats(const Tile* __restrict_
       acc += b[(size_t)gi * K.n + gj];
     }
   }
-  const double bs = block_sum<kThreads>(acc, sh);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(bs, 0.0);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       st[p].bmean = (project && K.bc == kNeumann) ? tot.x / (double(K.m) * double(K.n)) : 0.0;
-      counters[p] = 0;
     }
   }
 }
@@ -796,7 +884,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tile
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
-  __shared__ double sh[kThreads / 64];
+  __shared__ double sh[2 * kThreads / 64];
   __shared__ int s_flag;
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
@@ -821,10 +909,8 @@ __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tile
       K.x[o] = x0 ? x0[(size_t)gi * K.n + gj] : 0.0;
     }
   }
-  const double bs = block_sum<kThreads>(acc, sh);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(bs, 0.0);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       ProblemState& S = st[p];
       S.bnorm2 = tot.x;
@@ -839,7 +925,6 @@ __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tile
       S.rr = tot.x;
       S.max_iter = max_iter;
       S.flexible = flexible;
-      counters[p] = 0;
     }
   }
 }
@@ -895,11 +980,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
     rr += res * res;
     sum += xc;
   }
-  const double br = block_sum<kThreads>(rr, sh);
-  const double bx = block_sum<kThreads>(sum, sh + kThreads / 64);
-  if (threadIdx.x == 0) partial[t.slot] = make_double2(br, bx);
-  if (arrive_last(&counters[p], ranges[p].count, &s_flag)) {
-    const double2 tot = gather_partials<kThreads>(partial, ranges[p], sh);
+  double2 tot;
+  if (finish<kThreads>(rr, sum, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
     if (threadIdx.x == 0) {
       ProblemState& S = st[p];
       S.true_rr = tot.x;
@@ -914,7 +996,6 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
           S.done = S.converged || S.iter >= S.max_iter;
         }
       }
-      counters[p] = 0;
     }
   }
 }

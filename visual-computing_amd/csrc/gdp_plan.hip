@@ -159,7 +159,7 @@ struct ProbInfo {
   int level_base = 0;
   size_t off_x = 0, off_r = 0, off_d0 = 0, off_d1 = 0, off_q = 0, off_dct = 0;
   int tail_off[kMaxTailLevels][2];
-  int scratch_off = 0, coarse_off = 0;
+  int scratch_off = 0, coarse_off = 0, tab_off = 0;
 };
 
 struct Table {
@@ -241,13 +241,14 @@ std::vector<std::pair<int, int>> level_shapes(const gdp_problem& p, const gdp_op
 
 // LDS bytes of a tail holding levels [first, L) (u and b per level, one residual
 // scratch of the first level, two FP64 arrays of the coarsest).  Fills the offsets.
-int tail_layout(ProbInfo& pi, int first, size_t tsize) {
+size_t tail_layout(ProbInfo& pi, int first, size_t tsize) {
   const int L = int(pi.lv.size());
-  if (L - first > kMaxTailLevels) return 1 << 30;
+  if (L - first > kMaxTailLevels) return size_t(1) << 40;
   size_t off = 0;
   auto a16 = [](size_t x) { return (x + 15) & ~size_t(15); };
   for (int l = first; l < L; ++l) {
     const size_t pts = size_t(pi.lv[l].m) * pi.lv[l].n;
+    if (pts > (size_t(1) << 24)) return size_t(1) << 40;  // far beyond any LDS
     pi.tail_off[l - first][0] = int(off);
     off += a16(pts * tsize);
     pi.tail_off[l - first][1] = int(off);
@@ -256,8 +257,11 @@ int tail_layout(ProbInfo& pi, int first, size_t tsize) {
   pi.scratch_off = int(off);
   off += a16(size_t(pi.lv[first].m) * pi.lv[first].n * tsize);
   pi.coarse_off = int(off);
-  off += 2 * size_t(pi.lv[L - 1].m) * pi.lv[L - 1].n * 8;
-  return int(off);
+  const size_t cm = pi.lv[L - 1].m, cn = pi.lv[L - 1].n;
+  off += 2 * cm * cn * 8;
+  pi.tab_off = int(off);
+  if (pi.p.boundary == GDP_NEUMANN) off += (cm * cm + cn * cn + cm * cn) * 8;
+  return off;
 }
 
 Shape2 menu_shape(int kind, int idx) {
@@ -364,10 +368,10 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     const int L = int(pi.lv.size());
     int first = -1;
     for (int f = 0; f < L; ++f) {
-      const int bytes = tail_layout(pi, f, P->tsize);
-      if (bytes <= kTailLdsBytes) {
+      const size_t bytes = tail_layout(pi, f, P->tsize);
+      if (bytes <= size_t(kTailLdsBytes)) {
         first = f;
-        pi.tail_bytes = bytes;
+        pi.tail_bytes = int(bytes);
         break;
       }
     }
@@ -588,6 +592,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     }
     t.scratch_off = pi.scratch_off;
     t.coarse_off = pi.coarse_off;
+    t.tab_off = pi.tab_off;
     t.dct = nullptr;
     if (pi.p.boundary == GDP_NEUMANN) {
       const int m = pi.lv.back().m, n = pi.lv.back().n;
@@ -679,6 +684,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   GDP_HIP(hipMemcpy(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
   GDP_HIP(hipMemset(P->d_state(), 0, sizeof(ProblemState) * NP));
   GDP_HIP(hipMemset(P->d_counters(), 0, sizeof(int) * NP));
+  GDP_HIP(hipMemset(P->d_partial(), 0xFF, sizeof(double2) * P->max_slots));
   GDP_HIP(hipMemset(P->d_flag(), 0, sizeof(int) * 4));
 
   // the tail kernels may use more than the default dynamic LDS
@@ -861,6 +867,11 @@ int read_state(gdp_plan* P, hipStream_t s) {
   GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
                          hipMemcpyDeviceToHost, s));
   GDP_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < P->count(); ++p)
+    if (P->h_state[p].fault) {
+      set_error("problem %d: a device reduction timed out waiting for a partial sum", p);
+      return GDP_EHIP;
+    }
   return GDP_OK;
 }
 

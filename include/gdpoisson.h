@@ -148,6 +148,17 @@ int gdp_mgcg_solve(gdp_plan* plan, const double* const* b, double* const* u,
 int gdp_cg_solve(gdp_plan* plan, const double* const* b, double* const* u,
                  gdp_report* reports, void* stream);
 
+/* ---------------------------------------------------------------- kernel timing
+ * No counterpart in the reference (which only wall-clocks whole solves,
+ * solvers.py:211-213).  With profiling on, every launch of the plan is bracketed by
+ * HIP events on its launch stream (graphs are bypassed) and aggregated per kernel
+ * kind with its algorithmic HBM bytes.                                           */
+int gdp_plan_profile(gdp_plan* plan, int enable);  /* also clears the aggregates */
+/* Synchronises the device, then reports the number of kernel kinds seen. */
+int gdp_plan_profile_count(gdp_plan* plan, int* count);
+int gdp_plan_profile_entry(gdp_plan* plan, int index, char* name, int name_len, double* ms,
+                           long long* launches, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

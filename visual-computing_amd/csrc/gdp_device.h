@@ -44,7 +44,8 @@ struct IoDesc {
 
 // Per-problem scalar state of the PCG / MG iteration (FP64).
 struct ProblemState {
-  double delta;      // <r, z>
+  double delta;      // <r, z> of the current direction
+  double delta_old;  // <r, z> of the previous direction (for beta)
   double dq;         // <d, q>
   double alpha;
   double beta;
@@ -60,8 +61,7 @@ struct ProblemState {
   int32_t restart;   // next preconditioned direction starts a new Krylov sequence
   int32_t max_iter;
   int32_t flexible;
-  int32_t fault;     // a device-side reduction gave up waiting for a partial sum
-  int32_t pad;
+  int32_t pad[2];
 };
 
 // One workgroup's work item.
@@ -69,9 +69,11 @@ struct Tile {
   int32_t desc;  // LevelDesc (V-cycle kernels) or KrylovDesc index
   int32_t i0, j0;
   int32_t slot;  // index of this tile's partial-sum slot (problem-contiguous)
+  int32_t lead;  // 1 for the one tile per problem that writes the problem's state
+  int32_t pad[3];
 };
 
-// Slot range of one problem in a tile table (for the last-block reductions).
+// Slot range of one problem in a partial-sum buffer.
 struct SlotRange {
   int32_t begin, count;
 };
@@ -89,10 +91,6 @@ struct TailDesc {
   int32_t pad;
   const double* dct;      // Neumann coarsest: Cm (m*m), Cn (n*n), 1/lambda (m*n)
 };
-
-// Partial-sum slots hold this bit pattern (a NaN no arithmetic produces) while empty.
-constexpr unsigned long long kEmptySlot = 0xFFFFFFFFFFFFFFFFull;
-constexpr int kSpinLimit = 1 << 22;
 
 constexpr int kTailThreads = 1024;
 constexpr int kTailLdsBytes = 98304;

@@ -45,69 +45,51 @@ __device__ inline double block_sum(double v, double* sh) {
   return t;
 }
 
-__device__ inline unsigned long long* slot_ptr(double2* partial, int slot) {
-  return reinterpret_cast<unsigned long long*>(partial + slot);
-}
-
-__device__ inline void publish2(double2* partial, int slot, double a, double b) {
-  unsigned long long* p = slot_ptr(partial, slot);
-  __hip_atomic_store(p, static_cast<unsigned long long>(__double_as_longlong(a)),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(p + 1, static_cast<unsigned long long>(__double_as_longlong(b)),
-                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wait for a published value (bounded), read it and mark the slot empty again.
-__device__ inline double take_slot(unsigned long long* p, int& fault) {
-  unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  int spins = 0;
-  while (v == kEmptySlot) {
-    if (++spins > kSpinLimit) {
-      fault = 1;
-      v = 0;
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-    v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __hip_atomic_store(p, kEmptySlot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double(static_cast<long long>(v));
-}
-
-// Block-reduce (a, b), publish them in `slot`, and return true in the problem's last
-// block with the problem totals in `tot` (valid in thread 0).  All threads call it.
+// Deterministic sum of the partial slots [begin, begin + count) written by an earlier
+// kernel, broadcast to every thread of the block (each thread sums a contiguous run,
+// then the block tree): every block that calls it gets bit-identical totals.
 template <int NT>
-__device__ inline bool finish(double a, double b, double2* partial, int slot, int* counter,
-                              SlotRange rg, double* sh, int* s_flag, double2& tot, int* fault_out) {
-  const double ba = block_sum<NT>(a, sh);
-  const double bb = block_sum<NT>(b, sh + NT / 64);
-  if (threadIdx.x == 0) {
-    publish2(partial, slot, ba, bb);
-    const int prev = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *s_flag = (prev == rg.count - 1);
-  }
-  __syncthreads();
-  if (*s_flag == 0) return false;
-  // last block: every other block has published (its store precedes its arrival)
+__device__ inline double2 reduce_slots(const double2* __restrict__ partial, SlotRange rg,
+                                       double* sh, double2* bc) {
   const int per = (rg.count + NT - 1) / NT;
   const int lo = rg.begin + threadIdx.x * per;
   const int hi = min(lo + per, rg.begin + rg.count);
   double sx = 0.0, sy = 0.0;
-  int fault = 0;
   for (int i = lo; i < hi; ++i) {
-    unsigned long long* p = slot_ptr(partial, i);
-    sx += take_slot(p, fault);
-    sy += take_slot(p + 1, fault);
+    const double2 v = partial[i];
+    sx += v.x;
+    sy += v.y;
   }
   const double tx = block_sum<NT>(sx, sh);
   const double ty = block_sum<NT>(sy, sh + NT / 64);
-  const double tf = block_sum<NT>(double(fault), sh);
-  if (threadIdx.x == 0) {
-    tot = make_double2(tx, ty);
-    if (tf != 0.0) *fault_out = 1;
-    __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) *bc = make_double2(tx, ty);
+  __syncthreads();
+  return *bc;
+}
+
+// This block's contribution (a, b) to a per-problem sum, for a later kernel to reduce.
+template <int NT>
+__device__ inline void write_partial(double a, double b, double2* partial, int slot, double* sh) {
+  const double ba = block_sum<NT>(a, sh);
+  const double bb = block_sum<NT>(b, sh + NT / 64);
+  if (threadIdx.x == 0) partial[slot] = make_double2(ba, bb);
+}
+
+// Convergence test after a PCG update (solvers.py:154): every block of the first kernel
+// that touches problem p reaches the same decision from the update's partial sums of
+// ||r||^2; the lead block records it.  Returns true when the problem is done.
+template <int NT>
+__device__ inline bool check_converged(ProblemState* st, int p, const double2* upart,
+                                       SlotRange rg, bool lead, double* sh, double2* bc) {
+  const double2 t = reduce_slots<NT>(upart, rg, sh, bc);
+  const bool conv = t.x <= st[p].target2;
+  const bool done = conv || st[p].iter >= st[p].max_iter;
+  if (lead && threadIdx.x == 0) {
+    st[p].rr = t.x;
+    st[p].converged = conv;
+    st[p].done = done;
   }
-  return true;
+  return done;
 }
 
 __device__ inline int neumann_count(int gi, int gj, int m, int n) {
@@ -161,13 +143,25 @@ __device__ inline void ptaps(bool neu, int i, int& ia, int& ib, T& wa, T& wb) {
 // taps 2I-1..2I+2 with (1/4, 3/4, 3/4, 1/4) applied to the residual extended by
 // r[-1] = r[0] and, for even sizes, r[f] = r[f-1] (odd sizes: zero), which is exactly
 // the transpose of the reflected cell-centred interpolation.
+//
+// With check_conv the finest-level launch of a PCG iteration first applies the
+// convergence test of the preceding update (check_converged).
 template <typename T, int CH, int CW, int NU>
 __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tiles,
                                                    const LevelDesc* __restrict__ levels,
-                                                   const ProblemState* __restrict__ st) {
+                                                   ProblemState* __restrict__ st,
+                                                   const double2* __restrict__ upart,
+                                                   const SlotRange* __restrict__ rk,
+                                                   int check_conv) {
   const Tile t = tiles[blockIdx.x];
   const LevelDesc L = levels[t.desc];
   if (st[L.problem].done) return;
+  if (check_conv && L.level == 0) {
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, L.problem, upart, rk[L.problem], t.lead != 0, shc, &bcc))
+      return;
+  }
   const LevelDesc C = levels[L.coarse];
   constexpr int OFF = 2 + 2 * NU;
   constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
@@ -182,17 +176,30 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
   const T h2 = T(L.h2), ih2 = T(L.inv_h2);
   const T* __restrict__ b = static_cast<const T*>(L.b);
 
-  // load b; the first colour-0 half-sweep from u = 0 is folded into the load
-  for (int k = threadIdx.x; k < FH * FW; k += kThreads) {
+  // load b; the first colour-0 half-sweep from u = 0 is folded into the load.  All of a
+  // thread's loads are issued before any is consumed (fixed trip count, predicated).
+  constexpr int NL = (FH * FW + kThreads - 1) / kThreads;
+  T bl[NL];
+#pragma unroll
+  for (int it = 0; it < NL; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / FW, c = k - a * FW;
     const int gi = gr0 + a, gj = gc0 + c;
-    T bv = T(0), uv = T(0);
-    if (a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 && gi < m && gj >= 0 &&
-        gj < n) {
-      bv = b[(size_t)gi * pitch + gj];
-      if (NU > 0 && ((gi + gj) & 1) == 0) uv = smooth_value<T>(T(0), bv, h2, neu, gi, gj, m, n);
-    }
-    sb[k] = bv;
+    const bool in = k < FH * FW && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 &&
+                    gi < m && gj >= 0 && gj < n;
+    bl[it] = in ? b[(size_t)gi * pitch + gj] : T(0);
+  }
+#pragma unroll
+  for (int it = 0; it < NL; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    if (k >= FH * FW) break;
+    const int a = k / FW, c = k - a * FW;
+    const int gi = gr0 + a, gj = gc0 + c;
+    T uv = T(0);
+    if (NU > 0 && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 && gi < m &&
+        gj >= 0 && gj < n && ((gi + gj) & 1) == 0)
+      uv = smooth_value<T>(T(0), bl[it], h2, neu, gi, gj, m, n);
+    sb[k] = bl[it];
     su[k] = uv;
   }
   __syncthreads();
@@ -282,12 +289,9 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
 template <typename T, int TH, int TW, int NU, bool DOTS>
 __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
                                                  const LevelDesc* __restrict__ levels,
-                                                 ProblemState* __restrict__ st,
+                                                 const ProblemState* __restrict__ st,
                                                  const KrylovDesc* __restrict__ kds,
-                                                 double2* __restrict__ partial,
-                                                 int* __restrict__ counters,
-                                                 const SlotRange* __restrict__ ranges,
-                                                 int order) {
+                                                 double2* __restrict__ zpart, int order) {
   const Tile t = tiles[blockIdx.x];
   const LevelDesc L = levels[t.desc];
   const int p = L.problem;
@@ -300,7 +304,6 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
   __shared__ T sb[RH * RW];
   __shared__ T se[CRH * CRW];
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
   const int m = L.m, n = L.n, pitch = L.pitch;
   const int gr0 = t.i0 - H, gc0 = t.j0 - H;
   const int cr0 = (gr0 >> 1) - 1, cc0 = (gc0 >> 1) - 1;
@@ -310,25 +313,48 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
   const T* __restrict__ vv = static_cast<const T*>(L.v);
   const T* __restrict__ b = static_cast<const T*>(L.b);
 
-  // coarse correction with its padding: zero outside (Dirichlet), edge values (Neumann)
-  for (int k = threadIdx.x; k < CRH * CRW; k += kThreads) {
+  // Issue every global load of the block first: the coarse correction (with its
+  // padding: zero outside for Dirichlet, edge values for Neumann) and the fine v and b.
+  constexpr int NE = (CRH * CRW + kThreads - 1) / kThreads;
+  constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
+  T el[NE], vl[NR], bl[NR];
+#pragma unroll
+  for (int it = 0; it < NE; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / CRW, c = k - a * CRW;
     int I = cr0 + a, J = cc0 + c;
-    T ev = T(0);
+    bool in = k < CRH * CRW;
     if (neu) {
       I = min(max(I, 0), C.m - 1);
       J = min(max(J, 0), C.n - 1);
-      ev = e[(size_t)I * C.pitch + J];
-    } else if (I >= 0 && I < C.m && J >= 0 && J < C.n) {
-      ev = e[(size_t)I * C.pitch + J];
+    } else {
+      in = in && I >= 0 && I < C.m && J >= 0 && J < C.n;
     }
-    se[k] = ev;
+    el[it] = in ? e[(size_t)I * C.pitch + J] : T(0);
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < RH * RW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / RW, c = k - a * RW;
     const int gi = gr0 + a, gj = gc0 + c;
-    T uv = T(0), bv = T(0);
+    const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
+    const size_t o = in ? (size_t)gi * pitch + gj : 0;
+    vl[it] = in ? vv[o] : T(0);
+    bl[it] = in ? b[o] : T(0);
+  }
+#pragma unroll
+  for (int it = 0; it < NE; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    if (k < CRH * CRW) se[k] = el[it];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    if (k >= RH * RW) break;
+    const int a = k / RW, c = k - a * RW;
+    const int gi = gr0 + a, gj = gc0 + c;
+    T uv = T(0);
     if (gi >= 0 && gi < m && gj >= 0 && gj < n) {
       int ia, ib, ja, jb;
       T wa, wb, va, vb;
@@ -337,11 +363,10 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
       const int oa = (ia - cr0) * CRW - cc0, ob = (ib - cr0) * CRW - cc0;
       const T corr = va * (wa * se[oa + ja] + wb * se[ob + ja]) +
                      vb * (wa * se[oa + jb] + wb * se[ob + jb]);
-      uv = vv[(size_t)gi * pitch + gj] + corr;
-      bv = b[(size_t)gi * pitch + gj];
+      uv = vl[it] + corr;
     }
     su[k] = uv;
-    sb[k] = bv;
+    sb[k] = bl[it];
   }
   __syncthreads();
 
@@ -366,35 +391,35 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
 
   T* __restrict__ uo = static_cast<T*>(L.u);
   double zr = 0.0, zq = 0.0;
-  const KrylovDesc* K = DOTS ? &kds[p] : nullptr;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+  constexpr int NO = (TH * TW + kThreads - 1) / kThreads;
+  double rl[DOTS ? NO : 1], ql[DOTS ? NO : 1];
+  if (DOTS) {
+    const KrylovDesc* K = &kds[p];
+#pragma unroll
+    for (int it = 0; it < NO; ++it) {
+      const int k = threadIdx.x + it * kThreads;
+      const int a = k / TW, c = k - a * TW;
+      const int gi = t.i0 + a, gj = t.j0 + c;
+      const bool in = k < TH * TW && gi < m && gj < n;
+      const size_t o = in ? (size_t)gi * pitch + gj : 0;
+      rl[it] = in ? K->r[o] : 0.0;
+      ql[it] = in ? K->q[o] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NO; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= m || gj >= n) continue;
+    if (k >= TH * TW || gi >= m || gj >= n) continue;
     const T z = su[(a + H) * RW + c + H];
-    const size_t o = (size_t)gi * pitch + gj;
-    uo[o] = z;
+    uo[(size_t)gi * pitch + gj] = z;
     if (DOTS) {
-      zr += double(z) * K->r[o];
-      zq += double(z) * K->q[o];
+      zr += double(z) * rl[it];
+      zq += double(z) * ql[it];
     }
   }
-  if (DOTS) {
-    double2 tot;
-    if (finish<kThreads>(zr, zq, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot,
-                         &st[p].fault)) {
-      if (threadIdx.x == 0) {
-        ProblemState& S = st[p];
-        double beta;
-        if (S.iter == 0 || S.restart) beta = 0.0;
-        else if (S.flexible) beta = -S.alpha * tot.y / S.delta;
-        else beta = tot.x / S.delta;
-        S.beta = beta;
-        S.delta = tot.x;
-        S.restart = 0;
-      }
-    }
-  }
+  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
 }
 
 // ------------------------------------------------------------------ coarse tail
@@ -564,14 +589,23 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
                                                        const LevelDesc* __restrict__ levels,
                                                        ProblemState* __restrict__ st,
                                                        const KrylovDesc* __restrict__ kds,
-                                                       int pre, int post, int coarsest_sweeps,
-                                                       int order) {
+                                                       const double2* __restrict__ upart,
+                                                       const SlotRange* __restrict__ rk,
+                                                       double2* __restrict__ zpart,
+                                                       const SlotRange* __restrict__ rz,
+                                                       int check_conv, int pre, int post,
+                                                       int coarsest_sweeps, int order) {
   extern __shared__ __align__(16) unsigned char smem[];
   const TailDesc& D = tails[blockIdx.x];
   const int p = D.problem;
   if (st[p].done) return;
   __shared__ double sh[2 * kTailThreads / 64];
+  __shared__ double2 bcc;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // a problem whose finest level is in the tail gets its convergence test here
+  if (check_conv && D.krylov >= 0 &&
+      check_converged<kTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
+    return;
 
   {  // right-hand side of the first tail level, and the DCT tables
     const LevelDesc L = levels[D.first];
@@ -665,39 +699,28 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
         zq += double(z) * K.q[go];
       }
     }
-  if (dots) {
-    const double br = block_sum<kTailThreads>(zr, sh);
-    const double bq = block_sum<kTailThreads>(zq, sh + kTailThreads / 64);
-    if (threadIdx.x == 0) {
-      ProblemState& S = st[p];
-      double beta;
-      if (S.iter == 0 || S.restart) beta = 0.0;
-      else if (S.flexible) beta = -S.alpha * bq / S.delta;
-      else beta = br / S.delta;
-      S.beta = beta;
-      S.delta = br;
-      S.restart = 0;
-    }
-  }
+  if (dots) write_partial<kTailThreads>(zr, zq, zpart, rz[p].begin, sh);
 }
 
 // ------------------------------------------------------------------ Krylov kernels
-// Tiles of TH x TW fine points; KrylovDesc and level-0 LevelDesc share the pitch.
-template <int TH, int TW>
-struct KTile {
-  static constexpr int N = TH * TW;
-};
+// FP64 vectors of the fine grid, tiles of TH x TW points (TH * TW a multiple of the
+// block size).  Scalar data flow of one PCG iteration (solvers.py:149-159):
+//   k_apply   reads the V-cycle's <r,z>, <q,z> partials -> delta, beta;
+//             d = z + beta d, q = -A d, partial <d, q>
+//   k_update  reads the <d, q> partials -> alpha = delta / <d, q>;
+//             x += alpha d, r -= alpha q, V-cycle RHS -r, partial ||r||^2
+//   first V-cycle kernel: convergence test from the ||r||^2 partials
+// Each kernel reads only state fields written by earlier kernels and its lead block
+// writes only fields that no block of the same kernel reads.
 
-// d_new = z + beta d_old ; q = -A d_new ; <d_new, q>  -> alpha = delta / <d, q>
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ tiles,
                                                     const KrylovDesc* __restrict__ kds,
                                                     const LevelDesc* __restrict__ levels,
                                                     ProblemState* __restrict__ st,
-                                                    double2* __restrict__ partial,
-                                                    int* __restrict__ counters,
-                                                    const SlotRange* __restrict__ ranges,
-                                                    int din) {
+                                                    const double2* __restrict__ zpart,
+                                                    const SlotRange* __restrict__ rz,
+                                                    double2* __restrict__ apart, int din) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
@@ -705,27 +728,45 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sd[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
-  const double beta = st[p].beta;
+  __shared__ double2 bc;
+  const double2 z2 = reduce_slots<kThreads>(zpart, rz[p], sh, &bc);
+  double beta;
+  if (st[p].iter == 0 || st[p].restart) beta = 0.0;
+  else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;
+  else beta = z2.x / st[p].delta_old;
+  if (t.lead && threadIdx.x == 0) {
+    st[p].delta = z2.x;
+    st[p].beta = beta;
+  }
   const int m = K.m, n = K.n, pitch = K.pitch;
   const T* __restrict__ z = static_cast<const T*>(levels[K.level0].u);
   const double* __restrict__ dold = din ? K.d[1] : K.d[0];
   double* __restrict__ dnew = din ? K.d[0] : K.d[1];
   const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
-  for (int k = threadIdx.x; k < RH * RW; k += kThreads) {
+  constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
+  T zl[NR];
+  double dl[NR];
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / RW, c = k - a * RW;
     const int gi = gr0 + a, gj = gc0 + c;
-    double dv = 0.0;
-    if (gi >= 0 && gi < m && gj >= 0 && gj < n) {
-      const size_t o = (size_t)gi * pitch + gj;
-      dv = double(z[o]) + beta * dold[o];
-    }
-    sd[k] = dv;
+    const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
+    const size_t o = in ? (size_t)gi * pitch + gj : 0;
+    zl[it] = in ? z[o] : T(0);
+    dl[it] = in ? dold[o] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    if (k < RH * RW) sd[k] = double(zl[it]) + beta * dl[it];
   }
   __syncthreads();
   const bool neu = K.bc == kNeumann;
   double acc = 0.0;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= m || gj >= n) continue;
@@ -738,80 +779,90 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
     K.q[o] = q;
     acc += dc * q;
   }
-  double2 tot;
-  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      st[p].dq = tot.x;
-      st[p].alpha = st[p].delta / tot.x;
-    }
-  }
+  write_partial<kThreads>(acc, 0.0, apart, t.slot, sh);
 }
 
-// x += alpha d ; r -= alpha q ; V-cycle RHS = -r ; ||r||^2 -> convergence
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ tiles,
                                                      const KrylovDesc* __restrict__ kds,
                                                      const LevelDesc* __restrict__ levels,
                                                      ProblemState* __restrict__ st,
-                                                     double2* __restrict__ partial,
-                                                     int* __restrict__ counters,
-                                                     const SlotRange* __restrict__ ranges,
-                                                     int dcur) {
+                                                     const double2* __restrict__ apart,
+                                                     const SlotRange* __restrict__ rk,
+                                                     double2* __restrict__ upart, int dcur) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   if (st[p].done) return;
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
-  const double alpha = st[p].alpha;
+  __shared__ double2 bc;
+  const double2 a2 = reduce_slots<kThreads>(apart, rk[p], sh, &bc);
+  const double delta = st[p].delta;
+  const double alpha = delta / a2.x;
+  if (t.lead && threadIdx.x == 0) {
+    st[p].alpha = alpha;
+    st[p].dq = a2.x;
+    st[p].delta_old = delta;
+    st[p].iter += 1;
+    st[p].restart = 0;
+  }
   const int m = K.m, n = K.n, pitch = K.pitch;
   T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
   const double* __restrict__ d = dcur ? K.d[1] : K.d[0];
   double acc = 0.0;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+  constexpr int NO = (TH * TW) / kThreads;
+  double xl[NO], dl[NO], rl[NO], ql[NO];
+  size_t ol[NO];
+  bool il[NO];
+#pragma unroll
+  for (int it = 0; it < NO; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= m || gj >= n) continue;
-    const size_t o = (size_t)gi * pitch + gj;
-    K.x[o] += alpha * d[o];
-    const double r = K.r[o] - alpha * K.q[o];
-    K.r[o] = r;
-    b0[o] = T(-r);
+    il[it] = gi < m && gj < n;
+    ol[it] = il[it] ? (size_t)gi * pitch + gj : 0;
+    xl[it] = il[it] ? K.x[ol[it]] : 0.0;
+    dl[it] = il[it] ? d[ol[it]] : 0.0;
+    rl[it] = il[it] ? K.r[ol[it]] : 0.0;
+    ql[it] = il[it] ? K.q[ol[it]] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < NO; ++it) {
+    if (!il[it]) continue;
+    const double r = rl[it] - alpha * ql[it];
+    K.x[ol[it]] = xl[it] + alpha * dl[it];
+    K.r[ol[it]] = r;
+    b0[ol[it]] = T(-r);
     acc += r * r;
   }
-  double2 tot;
-  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      ProblemState& S = st[p];
-      S.rr = tot.x;
-      S.iter += 1;
-      S.converged = tot.x <= S.target2;
-      S.done = S.converged || S.iter >= S.max_iter;
-    }
-  }
+  write_partial<kThreads>(acc, 0.0, upart, t.slot, sh);
 }
 
 // Jacobi preconditioner of the reference "cg" method (solvers.py:114-117):
-// z = r / (4 / h^2), delta' = <r, z>, beta = delta' / delta.
+// z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply with beta = delta'/delta).
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ tiles,
                                                      const KrylovDesc* __restrict__ kds,
                                                      const LevelDesc* __restrict__ levels,
                                                      ProblemState* __restrict__ st,
-                                                     double2* __restrict__ partial,
-                                                     int* __restrict__ counters,
-                                                     const SlotRange* __restrict__ ranges) {
+                                                     const double2* __restrict__ upart,
+                                                     const SlotRange* __restrict__ rk,
+                                                     double2* __restrict__ zpart,
+                                                     int check_conv) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   if (st[p].done) return;
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
+  __shared__ double2 bc;
+  if (check_conv && check_converged<kThreads>(st, p, upart, rk[p], t.lead != 0, sh, &bc)) return;
   const int m = K.m, n = K.n, pitch = K.pitch;
   T* __restrict__ z = static_cast<T*>(levels[K.level0].u);
   const double diag = 4.0 * K.inv_h2;
   double acc = 0.0;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= m || gj >= n) continue;
@@ -821,77 +872,54 @@ __global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ ti
     z[o] = T(zz);
     acc += r * double(T(zz));
   }
-  double2 tot;
-  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      ProblemState& S = st[p];
-      S.beta = (S.iter == 0 || S.restart) ? 0.0 : tot.x / S.delta;
-      S.delta = tot.x;
-      S.restart = 0;
-    }
-  }
+  write_partial<kThreads>(acc, 0.0, zpart, t.slot, sh);
 }
 
-// Pass 1 of a solve's set-up: mean of b for the Neumann projection onto the
-// compatible (zero-mean) right-hand sides (the constant that solvers.py:224-262 drops).
+// Set-up pass 1: partial sums of b (mean for the Neumann projection onto compatible
+// right-hand sides -- the constant that solvers.py:224-262 drops).
 template <int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_init_stats(const Tile* __restrict__ tiles,
                                                          const KrylovDesc* __restrict__ kds,
                                                          const IoDesc* __restrict__ io,
-                                                         ProblemState* __restrict__ st,
-                                                         double2* __restrict__ partial,
-                                                         int* __restrict__ counters,
-                                                         const SlotRange* __restrict__ ranges,
-                                                         int project) {
+                                                         double2* __restrict__ spart) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
   const double* __restrict__ b = io[p].b;
   double acc = 0.0;
-  if (project && K.bc == kNeumann) {
-    for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
-      const int a = k / TW, c = k - a * TW;
-      const int gi = t.i0 + a, gj = t.j0 + c;
-      if (gi >= K.m || gj >= K.n) continue;
-      acc += b[(size_t)gi * K.n + gj];
-    }
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    const int a = k / TW, c = k - a * TW;
+    const int gi = t.i0 + a, gj = t.j0 + c;
+    acc += (gi < K.m && gj < K.n) ? b[(size_t)gi * K.n + gj] : 0.0;
   }
-  double2 tot;
-  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      st[p].bmean = (project && K.bc == kNeumann) ? tot.x / (double(K.m) * double(K.n)) : 0.0;
-    }
-  }
+  write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
 }
 
-// Pass 2 of a solve's set-up.  mode 0 (PCG): r = -(b - mean), x = d = q = 0, V-cycle
-// RHS = b - mean.  mode 1 (MG iteration): x = initial guess (or 0).  Sets ||b||^2 and
-// the per-problem state.
+// Set-up pass 2.  mode 0 (PCG): r = -(b - mean), x = d = q = 0, V-cycle RHS b - mean.
+// mode 1 (MG iteration): x = initial guess (or 0).  Partial sums of (b - mean)^2.
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tiles,
                                                    const KrylovDesc* __restrict__ kds,
                                                    const LevelDesc* __restrict__ levels,
                                                    const IoDesc* __restrict__ io,
                                                    const double* const* __restrict__ initial,
-                                                   ProblemState* __restrict__ st,
-                                                   double2* __restrict__ partial,
-                                                   int* __restrict__ counters,
-                                                   const SlotRange* __restrict__ ranges,
-                                                   int mode, double tol2, int max_iter,
-                                                   int flexible, int zero_done) {
+                                                   const ProblemState* __restrict__ st,
+                                                   double2* __restrict__ spart, int mode) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
   const double* __restrict__ x0 = initial ? initial[p] : nullptr;
   T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
   double acc = 0.0;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= K.m || gj >= K.n) continue;
@@ -909,41 +937,20 @@ __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tile
       K.x[o] = x0 ? x0[(size_t)gi * K.n + gj] : 0.0;
     }
   }
-  double2 tot;
-  if (finish<kThreads>(acc, 0.0, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      ProblemState& S = st[p];
-      S.bnorm2 = tot.x;
-      S.target2 = tol2 * tot.x;
-      S.iter = 0;
-      S.converged = zero_done && tot.x == 0.0;
-      S.done = S.converged || max_iter <= 0;
-      S.restart = 0;
-      S.alpha = 0.0;
-      S.beta = 0.0;
-      S.delta = 0.0;
-      S.rr = tot.x;
-      S.max_iter = max_iter;
-      S.flexible = flexible;
-    }
-  }
+  write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
 }
 
-// r_true = (b - mean) - L x (FP64), ||r_true||^2 and sum(x); the V-cycle RHS becomes
-// r_true.  mode 0 (PCG): also store r = -r_true so an unconverged problem can restart.
-// mode 1 (MG, before the first cycle): record history[0].  mode 2 (MG, after a cycle,
-// multigrid.py:250-254): count the cycle, record it, test rel < tolerance.
+// r_true = (b - mean) - L x (FP64): partial sums of ||r_true||^2 and of x; the V-cycle
+// RHS becomes r_true and, in mode 0 (PCG), r = -r_true so an unconverged problem can
+// restart from it.
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ tiles,
                                                        const KrylovDesc* __restrict__ kds,
                                                        const LevelDesc* __restrict__ levels,
                                                        const IoDesc* __restrict__ io,
-                                                       ProblemState* __restrict__ st,
-                                                       double2* __restrict__ partial,
-                                                       int* __restrict__ counters,
-                                                       const SlotRange* __restrict__ ranges,
-                                                       int mode, double tol, double* history,
-                                                       int hist_stride, int respect_done) {
+                                                       const ProblemState* __restrict__ st,
+                                                       double2* __restrict__ spart, int mode,
+                                                       int respect_done) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
@@ -951,21 +958,32 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sx[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
-  __shared__ int s_flag;
   const int m = K.m, n = K.n, pitch = K.pitch;
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
   T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
   const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
-  for (int k = threadIdx.x; k < RH * RW; k += kThreads) {
+  constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
+  double xl[NR];
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / RW, c = k - a * RW;
     const int gi = gr0 + a, gj = gc0 + c;
-    sx[k] = (gi >= 0 && gi < m && gj >= 0 && gj < n) ? K.x[(size_t)gi * pitch + gj] : 0.0;
+    const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
+    xl[it] = in ? K.x[(size_t)gi * pitch + gj] : 0.0;
+  }
+#pragma unroll
+  for (int it = 0; it < NR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    if (k < RH * RW) sx[k] = xl[it];
   }
   __syncthreads();
   const bool neu = K.bc == kNeumann;
   double rr = 0.0, sum = 0.0;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= m || gj >= n) continue;
@@ -980,38 +998,73 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
     rr += res * res;
     sum += xc;
   }
-  double2 tot;
-  if (finish<kThreads>(rr, sum, partial, t.slot, &counters[p], ranges[p], sh, &s_flag, tot, &st[p].fault)) {
-    if (threadIdx.x == 0) {
-      ProblemState& S = st[p];
-      S.true_rr = tot.x;
-      S.sumx = tot.y;
-      if (mode >= 1) {
-        const double rel = S.bnorm2 > 0.0 ? sqrt(tot.x / S.bnorm2) : 0.0;
-        if (mode == 2) S.iter += 1;
-        if (history) history[(size_t)p * hist_stride + S.iter] = rel;
-        S.rr = tot.x;
-        if (mode == 2) {
-          S.converged = rel < tol;
-          S.done = S.converged || S.iter >= S.max_iter;
-        }
+  write_partial<kThreads>(rr, sum, spart, t.slot, sh);
+}
+
+// One block per problem: reduce a set-up / residual pass and update the state.
+//   kind 0 (after k_init_stats): bmean
+//   kind 1 (after k_init): ||b||^2, target, fresh iteration state
+//   kind 2 (after k_residual): true ||r||^2, sum(x); MG modes also record the history
+//          and test rel < tolerance (multigrid.py:250-254)
+__global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restrict__ kds,
+                                                     ProblemState* __restrict__ st,
+                                                     const double2* __restrict__ spart,
+                                                     const SlotRange* __restrict__ rk, int kind,
+                                                     int mode, int project, double tol,
+                                                     int max_iter, int flexible, int zero_done,
+                                                     int respect_done, double* history,
+                                                     int hist_stride) {
+  const int p = blockIdx.x;
+  __shared__ double sh[2 * kThreads / 64];
+  __shared__ double2 bc;
+  if (kind == 2 && respect_done && st[p].done) return;
+  const double2 tot = reduce_slots<kThreads>(spart, rk[p], sh, &bc);
+  if (threadIdx.x != 0) return;
+  ProblemState& S = st[p];
+  const KrylovDesc& K = kds[p];
+  if (kind == 0) {
+    S.bmean = (project && K.bc == kNeumann) ? tot.x / (double(K.m) * double(K.n)) : 0.0;
+  } else if (kind == 1) {
+    S.bnorm2 = tot.x;
+    S.target2 = tol * tol * tot.x;
+    S.iter = 0;
+    S.converged = zero_done && tot.x == 0.0;
+    S.done = S.converged || max_iter <= 0;
+    S.restart = 0;
+    S.alpha = S.beta = S.delta = S.delta_old = 0.0;
+    S.rr = tot.x;
+    S.max_iter = max_iter;
+    S.flexible = flexible;
+  } else {
+    S.true_rr = tot.x;
+    S.sumx = tot.y;
+    if (mode >= 1) {
+      const double rel = S.bnorm2 > 0.0 ? sqrt(tot.x / S.bnorm2) : 0.0;
+      if (mode == 2) S.iter += 1;
+      if (history) history[(size_t)p * hist_stride + S.iter] = rel;
+      S.rr = tot.x;
+      if (mode == 2) {
+        S.converged = rel < tol;
+        S.done = S.converged || S.iter >= S.max_iter;
       }
     }
   }
 }
 
-// MG iteration: x += z (the V-cycle correction), one more cycle done.
+// MG iteration: x += z (the V-cycle correction).
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_correct(const Tile* __restrict__ tiles,
                                                       const KrylovDesc* __restrict__ kds,
                                                       const LevelDesc* __restrict__ levels,
-                                                      ProblemState* __restrict__ st) {
+                                                      const ProblemState* __restrict__ st) {
   const Tile t = tiles[blockIdx.x];
   const KrylovDesc K = kds[t.desc];
   const int p = t.desc;
   if (st[p].done) return;
   const T* __restrict__ z = static_cast<const T*>(levels[K.level0].u);
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= K.m || gj >= K.n) continue;
@@ -1019,10 +1072,6 @@ __global__ __launch_bounds__(kThreads) void k_correct(const Tile* __restrict__ t
     K.x[o] += double(z[o]);
   }
 }
-
-}  // namespace gdp
-
-namespace gdp {
 
 // Caller output: u = x - mean(x) for Neumann problems when mean_fix, else u = x.
 template <int TH, int TW>
@@ -1038,7 +1087,9 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const Tile* __restrict__ 
                           ? st[p].sumx / (double(K.m) * double(K.n))
                           : 0.0;
   double* __restrict__ u = io[p].u;
-  for (int k = threadIdx.x; k < TH * TW; k += kThreads) {
+#pragma unroll
+  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= K.m || gj >= K.n) continue;
@@ -1047,8 +1098,8 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const Tile* __restrict__ 
 }
 
 // Restart bookkeeping after the FP64 true-residual check: a problem whose recursive
-// residual converged but whose true residual did not is reopened with a fresh
-// Krylov sequence (residual replacement), if it has iterations left.
+// residual converged but whose true residual did not is reopened with a fresh Krylov
+// sequence (residual replacement) if it has iterations left.
 __global__ void k_reopen(ProblemState* __restrict__ st, int count, int* __restrict__ reopened) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= count) return;

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gdp_device.h"
@@ -73,36 +74,45 @@ int round_up(int x, int a) { return (x + a - 1) / a * a; }
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ----------------------------------------------------------------- launch dispatch
+struct DownArgs {
+  const Tile* tiles;
+  const LevelDesc* levels;
+  ProblemState* st;
+  const double2* upart;
+  const SlotRange* rk;
+  int check_conv;
+};
+
 template <typename T, int CH, int CW>
-void down_nu(int nu, int blocks, hipStream_t s, const Tile* t, const LevelDesc* L,
-             const ProblemState* st) {
+void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWN(NU) \
+  k_down<T, CH, CW, NU><<<blocks, kThreads, 0, s>>>(a.tiles, a.levels, a.st, a.upart, a.rk, \
+                                                    a.check_conv)
   switch (nu) {
-    case 0: k_down<T, CH, CW, 0><<<blocks, kThreads, 0, s>>>(t, L, st); break;
-    case 1: k_down<T, CH, CW, 1><<<blocks, kThreads, 0, s>>>(t, L, st); break;
-    case 2: k_down<T, CH, CW, 2><<<blocks, kThreads, 0, s>>>(t, L, st); break;
-    default: k_down<T, CH, CW, 3><<<blocks, kThreads, 0, s>>>(t, L, st); break;
+    case 0: GDP_DOWN(0); break;
+    case 1: GDP_DOWN(1); break;
+    case 2: GDP_DOWN(2); break;
+    default: GDP_DOWN(3); break;
   }
+#undef GDP_DOWN
 }
 
 template <typename T>
-void launch_down(int menu, int nu, int blocks, hipStream_t s, const Tile* t, const LevelDesc* L,
-                 const ProblemState* st) {
+void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
   switch (menu) {
-    case 0: down_nu<T, 4, 32>(nu, blocks, s, t, L, st); break;
-    case 1: down_nu<T, 8, 32>(nu, blocks, s, t, L, st); break;
-    case 2: down_nu<T, 16, 32>(nu, blocks, s, t, L, st); break;
-    default: down_nu<T, 8, 64>(nu, blocks, s, t, L, st); break;
+    case 0: down_nu<T, 4, 32>(nu, blocks, s, a); break;
+    case 1: down_nu<T, 8, 32>(nu, blocks, s, a); break;
+    case 2: down_nu<T, 16, 32>(nu, blocks, s, a); break;
+    default: down_nu<T, 8, 64>(nu, blocks, s, a); break;
   }
 }
 
 struct UpArgs {
   const Tile* tiles;
   const LevelDesc* levels;
-  ProblemState* st;
+  const ProblemState* st;
   const KrylovDesc* kds;
-  double2* partial;
-  int* counters;
-  const SlotRange* ranges;
+  double2* zpart;
   int order;
 };
 
@@ -110,7 +120,7 @@ template <typename T, int TH, int TW, bool DOTS>
 void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU)                                                                       \
   k_up<T, TH, TW, NU, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.levels, a.st, a.kds, \
-                                                        a.partial, a.counters, a.ranges, a.order)
+                                                        a.zpart, a.order)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;
@@ -182,8 +192,9 @@ struct gdp_plan {
   int tail_smem = 0;
   int max_slots = 1;
   size_t ws_bytes = 0;
-  size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0, off_partial = 0;
-  size_t off_counters = 0, off_ranges_kry = 0, off_ranges_up0 = 0, off_io = 0, off_init = 0;
+  size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0;
+  size_t off_apart = 0, off_upart = 0, off_zpart = 0, off_spart = 0;
+  size_t off_ranges_kry = 0, off_ranges_up0 = 0, off_io = 0, off_init = 0;
   size_t off_hist = 0, off_flag = 0;
   size_t off_tiles = 0;     // start of the tile tables region
   size_t tiles_bytes = 0;   // reserved bytes for the tile tables
@@ -199,6 +210,21 @@ struct gdp_plan {
   hipGraphExec_t pair_exec[2] = {nullptr, nullptr};  // [0] mgcg, [1] cg
   int last_iters = 0;
   bool bound = false;
+  // built-in kernel timing (HIP events around every launch on the launch stream)
+  bool prof = false;
+  struct ProfRec {
+    int id;
+    hipEvent_t a, b;
+    double bytes;
+  };
+  struct ProfAgg {
+    std::string name;
+    double ms = 0.0, bytes = 0.0;
+    long launches = 0;
+  };
+  std::vector<ProfRec> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<ProfAgg> prof_agg;
 
   template <typename X>
   X* at(size_t off) const { return reinterpret_cast<X*>(ws + off); }
@@ -206,8 +232,10 @@ struct gdp_plan {
   KrylovDesc* d_kry() const { return at<KrylovDesc>(off_kry); }
   TailDesc* d_tails() const { return at<TailDesc>(off_tails); }
   ProblemState* d_state() const { return at<ProblemState>(off_state); }
-  double2* d_partial() const { return at<double2>(off_partial); }
-  int* d_counters() const { return at<int>(off_counters); }
+  double2* d_apart() const { return at<double2>(off_apart); }  // <d, q> partials
+  double2* d_upart() const { return at<double2>(off_upart); }  // ||r||^2 partials
+  double2* d_zpart() const { return at<double2>(off_zpart); }  // <r, z>, <q, z> partials
+  double2* d_spart() const { return at<double2>(off_spart); }  // set-up / residual partials
   SlotRange* d_rkry() const { return at<SlotRange>(off_ranges_kry); }
   SlotRange* d_rup0() const { return at<SlotRange>(off_ranges_up0); }
   IoDesc* d_io() const { return at<IoDesc>(off_io); }
@@ -425,7 +453,6 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   P->off_kry = take(sizeof(KrylovDesc) * P_);
   P->off_tails = take(sizeof(TailDesc) * P_);
   P->off_state = take(sizeof(ProblemState) * P_);
-  P->off_counters = take(sizeof(int) * P_);
   P->off_ranges_kry = take(sizeof(SlotRange) * P_);
   P->off_ranges_up0 = take(sizeof(SlotRange) * P_);
   P->off_io = take(sizeof(IoDesc) * P_);
@@ -445,9 +472,12 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
   }
   tiles += kry_slots;
-  slots = std::max({slots, kry_slots, up0_slots});
+  slots = std::max({slots, kry_slots, up0_slots + P_});
   P->max_slots = slots;
-  P->off_partial = take(sizeof(double2) * slots);
+  P->off_apart = take(sizeof(double2) * slots);
+  P->off_upart = take(sizeof(double2) * slots);
+  P->off_zpart = take(sizeof(double2) * slots);
+  P->off_spart = take(sizeof(double2) * slots);
   P->tiles_bytes = sizeof(Tile) * (tiles + 8 * (2 * P->depth + 1));
   P->off_tiles = take(P->tiles_bytes);
   P->ws_bytes = off;
@@ -508,6 +538,11 @@ int gdp_plan_destroy(gdp_plan* P) {
     if (e) (void)hipGraphExecDestroy(e);
   if (P->cap) (void)hipStreamDestroy(P->cap);
   if (P->io_ev) (void)hipEventDestroy(P->io_ev);
+  for (auto& r : P->prof_pending) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : P->prof_pool) (void)hipEventDestroy(e);
   if (P->h_io) (void)hipHostFree(P->h_io);
   if (P->h_init) (void)hipHostFree(P->h_init);
   if (P->h_state) (void)hipHostFree(P->h_state);
@@ -651,7 +686,8 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       const auto& l0 = P->probs[p].lv[0];
       rk[p].begin = int(v.size());
       for (int i = 0; i < l0.m; i += s.h)
-        for (int j = 0; j < l0.n; j += s.w) v.push_back(Tile{p, i, j, int(v.size())});
+        for (int j = 0; j < l0.n; j += s.w)
+          v.push_back(Tile{p, i, j, int(v.size()), int(v.size()) == rk[p].begin, {0, 0, 0}});
       rk[p].count = int(v.size()) - rk[p].begin;
     }
     if (int rc = upload(P->kry, v)) return rc;
@@ -668,23 +704,32 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       }
       const auto& f = pi.lv[l];
       const auto& c = pi.lv[l + 1];
+      const int d0 = int(vd.size());
       for (int i = 0; i < c.m; i += sd.h)
         for (int j = 0; j < c.n; j += sd.w)
-          vd.push_back(Tile{pi.level_base + l, i, j, int(vd.size())});
+          vd.push_back(Tile{pi.level_base + l, i, j, int(vd.size()), int(vd.size()) == d0,
+                            {0, 0, 0}});
       if (l == 0) ru[p].begin = int(vu.size());
       for (int i = 0; i < f.m; i += su.h)
         for (int j = 0; j < f.n; j += su.w)
-          vu.push_back(Tile{pi.level_base + l, i, j, int(vu.size())});
+          vu.push_back(Tile{pi.level_base + l, i, j, int(vu.size()), 0, {0, 0, 0}});
       if (l == 0) ru[p].count = int(vu.size()) - ru[p].begin;
     }
     if (int rc = upload(P->down[l], vd)) return rc;
     if (int rc = upload(P->up[l], vu)) return rc;
   }
+  {
+    // problems whose finest level is in the tail get one direct <r,z> slot each,
+    // after the slots of the finest up-kernel table
+    int zb = P->depth > 0 ? P->up[0].count : 0;
+    for (int p = 0; p < NP; ++p)
+      if (P->probs[p].tail_first == 0) ru[p] = SlotRange{zb++, 1};
+  }
   GDP_HIP(hipMemcpy(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
   GDP_HIP(hipMemcpy(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
   GDP_HIP(hipMemset(P->d_state(), 0, sizeof(ProblemState) * NP));
-  GDP_HIP(hipMemset(P->d_counters(), 0, sizeof(int) * NP));
-  GDP_HIP(hipMemset(P->d_partial(), 0xFF, sizeof(double2) * P->max_slots));
+  for (size_t o : {P->off_apart, P->off_upart, P->off_zpart, P->off_spart})
+    GDP_HIP(hipMemset(P->ws + o, 0, sizeof(double2) * P->max_slots));
   GDP_HIP(hipMemset(P->d_flag(), 0, sizeof(int) * 4));
 
   // the tail kernels may use more than the default dynamic LDS
@@ -715,66 +760,166 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
 // ----------------------------------------------------------------- launch sequences
 namespace {
 
+// Algorithmic bytes per fine point of the Krylov grid (p = V-cycle element size).
+double krylov_points(const gdp_plan* P) {
+  double n = 0.0;
+  for (const auto& pi : P->probs) n += double(pi.lv[0].m) * pi.lv[0].n;
+  return n;
+}
+
+double level_points(const gdp_plan* P, int l) {
+  double n = 0.0;
+  for (const auto& pi : P->probs)
+    if (pi.tail_first > l) n += double(pi.lv[l].m) * pi.lv[l].n;
+  return n;
+}
+
+int prof_id(gdp_plan* P, const std::string& name) {
+  for (size_t i = 0; i < P->prof_agg.size(); ++i)
+    if (P->prof_agg[i].name == name) return int(i);
+  P->prof_agg.push_back(gdp_plan::ProfAgg{name});
+  return int(P->prof_agg.size()) - 1;
+}
+
+hipEvent_t prof_event(gdp_plan* P) {
+  if (!P->prof_pool.empty()) {
+    hipEvent_t e = P->prof_pool.back();
+    P->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Wraps one launch: events on the launch stream when profiling, nothing otherwise.
+struct ProfScope {
+  gdp_plan* P;
+  hipStream_t s;
+  int id = -1;
+  double bytes;
+  hipEvent_t a = nullptr;
+  ProfScope(gdp_plan* P_, hipStream_t s_, const char* name, double bytes_)
+      : P(P_), s(s_), bytes(bytes_) {
+    if (!P->prof) return;
+    id = prof_id(P, name);
+    a = prof_event(P);
+    (void)hipEventRecord(a, s);
+  }
+  ~ProfScope() {
+    if (id < 0) return;
+    hipEvent_t b = prof_event(P);
+    (void)hipEventRecord(b, s);
+    P->prof_pending.push_back(gdp_plan::ProfRec{id, a, b, bytes});
+  }
+};
+
+void prof_collect(gdp_plan* P) {
+  for (auto& r : P->prof_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& g = P->prof_agg[r.id];
+      g.ms += ms;
+      g.bytes += r.bytes;
+      g.launches += 1;
+    }
+    P->prof_pool.push_back(r.a);
+    P->prof_pool.push_back(r.b);
+  }
+  P->prof_pending.clear();
+}
+
 template <typename T>
-int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots) {
+int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
   const gdp_options& o = P->opt;
   for (int l = 0; l < P->depth; ++l) {
     if (!P->down[l].count) continue;
-    launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s,
-                   P->at<Tile>(P->down[l].off), P->d_levels(), P->d_state());
+    char nm[32];
+    snprintf(nm, sizeof nm, "down L%d", l);
+    ProfScope ps(P, s, nm, (2.25 * sizeof(T)) * level_points(P, l));
+    DownArgs a{P->at<Tile>(P->down[l].off), P->d_levels(), P->d_state(), P->d_upart(),
+               P->d_rkry(), check_conv ? 1 : 0};
+    launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
-  const int post_order = 0;  // per-problem order is resolved below (Dirichlet: 0, Neumann: 1)
-  (void)post_order;
   // Neumann problems use the symmetric cycle (post-smoothing colour 1 then 0); a plan
   // mixing boundary types uses the order of its first problem for the whole batch.
   const int order = P->probs[0].p.boundary == GDP_NEUMANN ? 1 : 0;
-  if (dots)
-    k_tail<T, true><<<P->count(), kTailThreads, P->tail_smem, s>>>(
-        P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), o.pre_smoothing,
-        o.post_smoothing, o.coarsest_sweeps, order);
-  else
-    k_tail<T, false><<<P->count(), kTailThreads, P->tail_smem, s>>>(
-        P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), o.pre_smoothing,
-        o.post_smoothing, o.coarsest_sweeps, order);
-  GDP_CHECK_LAUNCH();
+  double tail_pts = 0.0;
+  for (const auto& pi : P->probs)
+    tail_pts += double(pi.lv[pi.tail_first].m) * pi.lv[pi.tail_first].n;
+  {
+    ProfScope ps(P, s, "tail", 2.0 * sizeof(T) * tail_pts);
+    const int cc = check_conv ? 1 : 0;
+    if (dots)
+      k_tail<T, true><<<P->count(), kTailThreads, P->tail_smem, s>>>(
+          P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(), P->d_rkry(),
+          P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing, o.coarsest_sweeps,
+          order);
+    else
+      k_tail<T, false><<<P->count(), kTailThreads, P->tail_smem, s>>>(
+          P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(), P->d_rkry(),
+          P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing, o.coarsest_sweeps,
+          order);
+    GDP_CHECK_LAUNCH();
+  }
   for (int l = P->depth - 1; l >= 0; --l) {
     if (!P->up[l].count) continue;
+    char nm[32];
+    const bool dl = dots && l == 0;
+    snprintf(nm, sizeof nm, dl ? "up L%d+dots" : "up L%d", l);
+    ProfScope ps(P, s, nm, (3.25 * sizeof(T) + (dl ? 16.0 : 0.0)) * level_points(P, l));
     UpArgs a{P->at<Tile>(P->up[l].off), P->d_levels(), P->d_state(), P->d_kry(),
-             P->d_partial(), P->d_counters(), P->d_rup0(), order};
-    launch_up<T>(P->up_menu[l], o.post_smoothing, dots && l == 0, P->up[l].count, s, a);
+             P->d_zpart(), order};
+    launch_up<T>(P->up_menu[l], o.post_smoothing, dl, P->up[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
   return GDP_OK;
 }
 
-int enqueue_vcycle(gdp_plan* P, hipStream_t s, bool dots) {
-  return P->fp32 ? enqueue_vcycle_t<float>(P, s, dots) : enqueue_vcycle_t<double>(P, s, dots);
+int enqueue_vcycle(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
+  return P->fp32 ? enqueue_vcycle_t<float>(P, s, dots, check_conv)
+                 : enqueue_vcycle_t<double>(P, s, dots, check_conv);
+}
+
+template <typename T>
+int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
+  const int nb = P->kry.count;
+  const Tile* t = P->at<Tile>(P->kry.off);
+  ProfScope ps(P, s, "jacobi", (8.0 + sizeof(T)) * krylov_points(P));
+  GDP_KRY_DISPATCH(P->kry_menu,
+                   (k_jacobi<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_upart(), P->d_rkry(),
+                       P->d_zpart(), check_conv ? 1 : 0)));
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
 }
 
 template <typename T>
 int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
   const int nb = P->kry.count;
   const Tile* t = P->at<Tile>(P->kry.off);
-  GDP_KRY_DISPATCH(P->kry_menu,
-                   (k_apply<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
-                       P->d_counters(), P->d_rkry(), din)));
-  GDP_CHECK_LAUNCH();
-  GDP_KRY_DISPATCH(P->kry_menu,
-                   (k_update<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
-                       P->d_counters(), P->d_rkry(), din ^ 1)));
-  GDP_CHECK_LAUNCH();
-  if (jacobi) {
+  const double N = krylov_points(P);
+  // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
+  const SlotRange* rz = jacobi ? P->d_rkry() : P->d_rup0();
+  {
+    ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
-                     (k_jacobi<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
-                         P->d_counters(), P->d_rkry())));
+                     (k_apply<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_zpart(), rz,
+                         P->d_apart(), din)));
     GDP_CHECK_LAUNCH();
-    return GDP_OK;
   }
-  return enqueue_vcycle_t<T>(P, s, true);
+  {
+    ProfScope ps(P, s, "update", (48.0 + sizeof(T)) * N);
+    GDP_KRY_DISPATCH(P->kry_menu,
+                     (k_update<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_apart(),
+                         P->d_rkry(), P->d_upart(), din ^ 1)));
+    GDP_CHECK_LAUNCH();
+  }
+  if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
+  return enqueue_vcycle_t<T>(P, s, true, true);
 }
 
 int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
@@ -787,43 +932,82 @@ int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
   return enqueue_pcg_iter_t<double>(P, s, 1, jacobi);
 }
 
+struct ScalarArgs {
+  int kind = 0, mode = 0, project = 0;
+  double tol = 0.0;
+  int max_iter = 0, zero_done = 0, respect_done = 0;
+  bool hist = false;
+};
+
+int enqueue_scalar(gdp_plan* P, hipStream_t s, const ScalarArgs& a) {
+  static const char* names[3] = {"scalar stats", "scalar init", "scalar residual"};
+  ProfScope ps(P, s, names[a.kind], 16.0 * P->kry.count);
+  k_scalar<<<P->count(), kThreads, 0, s>>>(
+      P->d_kry(), P->d_state(), P->d_spart(), P->d_rkry(), a.kind, a.mode, a.project, a.tol,
+      a.max_iter, P->opt.flexible, a.zero_done, a.respect_done, a.hist ? P->d_hist() : nullptr,
+      P->opt.max_iterations + 2);
+  GDP_CHECK_LAUNCH();
+  return GDP_OK;
+}
+
 template <typename T>
 int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol, int max_it,
                    bool with_init, int zero_done) {
   const int nb = P->kry.count;
   const Tile* t = P->at<Tile>(P->kry.off);
-  GDP_KRY_DISPATCH(P->kry_menu, (k_init_stats<TH, TW><<<nb, kThreads, 0, s>>>(
-                                    t, P->d_kry(), P->d_io(), P->d_state(), P->d_partial(),
-                                    P->d_counters(), P->d_rkry(), project)));
-  GDP_CHECK_LAUNCH();
-  GDP_KRY_DISPATCH(P->kry_menu,
-                   (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                       t, P->d_kry(), P->d_levels(), P->d_io(),
-                       with_init ? P->d_init() : nullptr, P->d_state(), P->d_partial(),
-                       P->d_counters(), P->d_rkry(), mode, tol * tol, max_it, P->opt.flexible,
-                       zero_done)));
-  GDP_CHECK_LAUNCH();
-  return GDP_OK;
+  const double N = krylov_points(P);
+  int rc;
+  {
+    ProfScope ps(P, s, "init_stats", 8.0 * N);
+    GDP_KRY_DISPATCH(P->kry_menu, (k_init_stats<TH, TW><<<nb, kThreads, 0, s>>>(
+                                      t, P->d_kry(), P->d_io(), P->d_spart())));
+    GDP_CHECK_LAUNCH();
+  }
+  ScalarArgs a;
+  a.kind = 0;
+  a.project = project;
+  if ((rc = enqueue_scalar(P, s, a))) return rc;
+  {
+    ProfScope ps(P, s, "init", (mode == 0 ? 48.0 + sizeof(T) : 16.0) * N);
+    GDP_KRY_DISPATCH(P->kry_menu,
+                     (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                         t, P->d_kry(), P->d_levels(), P->d_io(),
+                         with_init ? P->d_init() : nullptr, P->d_state(), P->d_spart(), mode)));
+    GDP_CHECK_LAUNCH();
+  }
+  a.kind = 1;
+  a.tol = tol;
+  a.max_iter = max_it;
+  a.zero_done = zero_done;
+  return enqueue_scalar(P, s, a);
 }
 
 template <typename T>
 int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, bool hist) {
   const int nb = P->kry.count;
   const Tile* t = P->at<Tile>(P->kry.off);
-  GDP_KRY_DISPATCH(P->kry_menu,
-                   (k_residual<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                       t, P->d_kry(), P->d_levels(), P->d_io(), P->d_state(), P->d_partial(),
-                       P->d_counters(), P->d_rkry(), mode, P->opt.tolerance,
-                       hist ? P->d_hist() : nullptr, P->opt.max_iterations + 2,
-                       respect_done)));
-  GDP_CHECK_LAUNCH();
-  return GDP_OK;
+  {
+    ProfScope ps(P, s, "residual", (24.0 + sizeof(T)) * krylov_points(P));
+    GDP_KRY_DISPATCH(P->kry_menu,
+                     (k_residual<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                         t, P->d_kry(), P->d_levels(), P->d_io(), P->d_state(), P->d_spart(),
+                         mode, respect_done)));
+    GDP_CHECK_LAUNCH();
+  }
+  ScalarArgs a;
+  a.kind = 2;
+  a.mode = mode;
+  a.tol = P->opt.tolerance;
+  a.respect_done = respect_done;
+  a.hist = hist;
+  return enqueue_scalar(P, s, a);
 }
 
 template <typename T>
 int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
   const int nb = P->kry.count;
   const Tile* t = P->at<Tile>(P->kry.off);
+  ProfScope ps(P, s, "correct", (16.0 + sizeof(T)) * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu, (k_correct<T, TH, TW><<<nb, kThreads, 0, s>>>(
                                     t, P->d_kry(), P->d_levels(), P->d_state())));
   GDP_CHECK_LAUNCH();
@@ -833,6 +1017,7 @@ int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
 int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix) {
   const int nb = P->kry.count;
   const Tile* t = P->at<Tile>(P->kry.off);
+  ProfScope ps(P, s, "finalize", 16.0 * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu, (k_finalize<TH, TW><<<nb, kThreads, 0, s>>>(
                                     t, P->d_kry(), P->d_io(), P->d_state(), mean_fix)));
   GDP_CHECK_LAUNCH();
@@ -867,11 +1052,7 @@ int read_state(gdp_plan* P, hipStream_t s) {
   GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
                          hipMemcpyDeviceToHost, s));
   GDP_HIP(hipStreamSynchronize(s));
-  for (int p = 0; p < P->count(); ++p)
-    if (P->h_state[p].fault) {
-      set_error("problem %d: a device reduction timed out waiting for a partial sum", p);
-      return GDP_EHIP;
-    }
+  if (P->prof) prof_collect(P);
   return GDP_OK;
 }
 
@@ -894,7 +1075,7 @@ void fill_reports(const gdp_plan* P, gdp_report* rep) {
 
 int launch_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
   const int g = jacobi ? 1 : 0;
-  if (P->opt.use_graph) {
+  if (P->opt.use_graph && !P->prof) {
     if (!P->pair_exec[g]) {
       hipGraph_t graph = nullptr;
       GDP_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
@@ -926,27 +1107,12 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   const gdp_options& o = P->opt;
   if ((rc = GDP_T(enqueue_init_t)(P, s, 0, 1, o.tolerance, o.max_iterations, false, 1)))
     return rc;
-  // first preconditioned residual z0 = M r0 and delta0 = <r0, z0>
-  if (jacobi) {
-    const int nb = P->kry.count;
-    const Tile* t = P->at<Tile>(P->kry.off);
-    if (P->fp32) {
-      GDP_KRY_DISPATCH(P->kry_menu,
-                       (k_jacobi<float, TH, TW><<<nb, kThreads, 0, s>>>(
-                           t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
-                           P->d_counters(), P->d_rkry())));
-    } else {
-      GDP_KRY_DISPATCH(P->kry_menu,
-                       (k_jacobi<double, TH, TW><<<nb, kThreads, 0, s>>>(
-                           t, P->d_kry(), P->d_levels(), P->d_state(), P->d_partial(),
-                           P->d_counters(), P->d_rkry())));
-    }
-    GDP_CHECK_LAUNCH();
-  } else if ((rc = enqueue_vcycle(P, s, true))) {
-    return rc;
-  }
+  // first preconditioned residual z0 = M r0 and its <r0, z0>
+  if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
+  else rc = enqueue_vcycle(P, s, true, false);
+  if (rc) return rc;
   int launched = 0, restarts = 0;
-  int first_pairs = std::max(1, (P->last_iters + 1) / 2);
+  const int first_pairs = std::max(1, (P->last_iters + 1) / 2);
   for (;;) {
     const int pairs = launched == 0 ? first_pairs : 1;
     for (int k = 0; k < pairs; ++k)
@@ -963,11 +1129,9 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
     if ((rc = read_state(P, s))) return rc;
     if (P->h_flag[0] == 0 || restarts >= 4) break;
     ++restarts;
-    if (jacobi) {
-      set_error("internal: Jacobi CG restart not supported");
-      return GDP_ENOTSUP;
-    }
-    if ((rc = enqueue_vcycle(P, s, true))) return rc;
+    if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
+    else rc = enqueue_vcycle(P, s, true, false);
+    if (rc) return rc;
   }
   if ((rc = enqueue_finalize(P, s, 1))) return rc;
   if ((rc = read_state(P, s))) return rc;
@@ -987,6 +1151,34 @@ int gdp_mgcg_solve(gdp_plan* P, const double* const* b, double* const* u, gdp_re
   return run_pcg(P, b, u, rep, static_cast<hipStream_t>(stream), false);
 }
 
+int gdp_plan_profile(gdp_plan* P, int enable) {
+  if (!P) return (set_error("null plan"), GDP_EINVAL);
+  P->prof = enable != 0;
+  P->prof_agg.clear();
+  return GDP_OK;
+}
+
+int gdp_plan_profile_count(gdp_plan* P, int* count) {
+  if (!P || !count) return (set_error("null argument"), GDP_EINVAL);
+  GDP_HIP(hipDeviceSynchronize());
+  prof_collect(P);
+  *count = int(P->prof_agg.size());
+  return GDP_OK;
+}
+
+int gdp_plan_profile_entry(gdp_plan* P, int index, char* name, int name_len, double* ms,
+                           long long* launches, double* bytes) {
+  if (!P || index < 0 || index >= int(P->prof_agg.size()) || !name || name_len < 1 || !ms ||
+      !launches || !bytes)
+    return (set_error("bad profile entry request"), GDP_EINVAL);
+  const auto& g = P->prof_agg[index];
+  snprintf(name, size_t(name_len), "%s", g.name.c_str());
+  *ms = g.ms;
+  *launches = g.launches;
+  *bytes = g.bytes;
+  return GDP_OK;
+}
+
 int gdp_cg_solve(gdp_plan* P, const double* const* b, double* const* u, gdp_report* rep,
                  void* stream) {
   return run_pcg(P, b, u, rep, static_cast<hipStream_t>(stream), true);
@@ -1003,7 +1195,7 @@ int gdp_vcycle(gdp_plan* P, const double* const* u_in, const double* const* b,
   if ((rc = stage_io(P, b, u_out, u_in, s))) return rc;
   if ((rc = GDP_T(enqueue_init_t)(P, s, 1, 0, 0.0, 1, true, 0))) return rc;
   if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 0, false))) return rc;
-  if ((rc = enqueue_vcycle(P, s, false))) return rc;
+  if ((rc = enqueue_vcycle(P, s, false, false))) return rc;
   if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
   return enqueue_finalize(P, s, 0);
 }
@@ -1025,7 +1217,7 @@ int gdp_mg_solve(gdp_plan* P, const double* const* b, const double* const* initi
   int batch = std::max(1, P->last_iters);
   for (;;) {
     for (int k = 0; k < batch; ++k) {
-      if ((rc = enqueue_vcycle(P, s, false))) return rc;
+      if ((rc = enqueue_vcycle(P, s, false, false))) return rc;
       if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
       if ((rc = GDP_T(enqueue_residual_t)(P, s, 2, 1, true))) return rc;
     }

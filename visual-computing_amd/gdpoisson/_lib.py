@@ -41,6 +41,9 @@ EXPORTS = (
     "gdp_mg_solve",
     "gdp_mgcg_solve",
     "gdp_cg_solve",
+    "gdp_plan_profile",
+    "gdp_plan_profile_count",
+    "gdp_plan_profile_entry",
 )
 
 
@@ -114,6 +117,13 @@ def _declare(lib):
         "gdp_mg_solve": (c_int, [P, PP, PP, PP, POINTER(Report), POINTER(c_double), P]),
         "gdp_mgcg_solve": (c_int, [P, PP, PP, POINTER(Report), P]),
         "gdp_cg_solve": (c_int, [P, PP, PP, POINTER(Report), P]),
+        "gdp_plan_profile": (c_int, [P, c_int]),
+        "gdp_plan_profile_count": (c_int, [P, POINTER(c_int)]),
+        "gdp_plan_profile_entry": (
+            c_int,
+            [P, c_int, ctypes.c_char_p, c_int, POINTER(c_double), POINTER(ctypes.c_longlong),
+             POINTER(c_double)],
+        ),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -231,3 +231,24 @@ class Plan:
             k = reps[p].iterations
             histories.append([hist[p * stride + i] for i in range(k + 1)])
         return out, list(reps), histories
+
+    # -------------------------------------------------------------- kernel timing
+    def profile(self, enable: bool = True) -> None:
+        """Bracket every launch with HIP events on its stream (graphs are bypassed)."""
+        _lib.check(_lib.lib().gdp_plan_profile(self._plan, int(bool(enable))), "profile")
+
+    def profile_report(self) -> dict:
+        """{kernel kind: {"ms", "launches", "bytes"}} aggregated since profile(True)."""
+        L = _lib.lib()
+        n = ctypes.c_int()
+        with torch.cuda.device(self.device):
+            _lib.check(L.gdp_plan_profile_count(self._plan, ctypes.byref(n)), "profile_count")
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        ms, launches, nbytes = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_double()
+        for i in range(n.value):
+            _lib.check(L.gdp_plan_profile_entry(self._plan, i, buf, 64, ctypes.byref(ms),
+                                                ctypes.byref(launches), ctypes.byref(nbytes)))
+            out[buf.value.decode()] = {"ms": ms.value, "launches": int(launches.value),
+                                       "bytes": nbytes.value}
+        return out

@@ -108,6 +108,8 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--nu", type=int, default=2,
+                    help="pre/post smoothing sweeps of the mixed solver's V-cycle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -126,19 +128,29 @@ def main():
     log("building inputs")
     bs, exact = make_inputs()
     mixed = gdpoisson.Plan([SHAPE] * CHANNELS, boundary="neumann",
-                           options=gdpoisson.PlanOptions(precision="mixed"))
+                           options=gdpoisson.PlanOptions(precision="mixed",
+                                                         pre_smoothing=args.nu,
+                                                         post_smoothing=args.nu))
+    # plain FP64 MG-PCG: the reference's mgcg cycle (nu = 1/1, solvers.py:137) in FP64
     fp64 = gdpoisson.Plan([SHAPE] * CHANNELS, boundary="neumann",
                           options=gdpoisson.PlanOptions(precision="fp64"))
+    # the same cycle as the mixed solver, all in FP64
+    fp64_same = gdpoisson.Plan([SHAPE] * CHANNELS, boundary="neumann",
+                               options=gdpoisson.PlanOptions(precision="fp64",
+                                                             pre_smoothing=args.nu,
+                                                             post_smoothing=args.nu))
 
     log("timing mixed")
     t_mix, lat_mix, u_mix, rep_mix = time_solves(mixed, bs, args.steps, args.warmup)
     log("timing fp64")
     fsteps = max(3, args.steps // 3)
     t_64, lat_64, u_64, rep_64 = time_solves(fp64, bs, fsteps, max(1, args.warmup))
+    t_64s, _, _, rep_64s = time_solves(fp64_same, bs, fsteps, max(1, args.warmup))
 
     mpix = SHAPE[0] * SHAPE[1] / 1e6
     v_mix = mpix * args.steps / t_mix
     v_64 = mpix * fsteps / t_64
+    v_64s = mpix * fsteps / t_64s
 
     # accuracy of every channel
     err_exact, err_fp64, res = [], [], []
@@ -194,8 +206,12 @@ def main():
         "fp64_value": round(v_64, 2),
         "fp64_ms_per_step": round(t_64 / fsteps * 1e3, 4),
         "mixed_vs_fp64": round(v_mix / v_64, 3),
+        "fp64_same_cycle_value": round(v_64s, 2),
+        "mixed_vs_fp64_same_cycle": round(v_mix / v_64s, 3),
+        "smoothing": {"mixed": [args.nu, args.nu], "fp64": [1, 1], "fp64_same_cycle": [args.nu, args.nu]},
         "iterations": {"mixed": [r.iterations for r in rep_mix],
-                       "fp64": [r.iterations for r in rep_64]},
+                       "fp64": [r.iterations for r in rep_64],
+                       "fp64_same_cycle": [r.iterations for r in rep_64s]},
         "relative_residual": [float(f"{r:.3e}") for r in res],
         "converged": ok,
         "max_abs_err_vs_exact": float(f"{max(err_exact):.3e}"),

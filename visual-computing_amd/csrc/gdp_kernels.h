@@ -167,8 +167,7 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
   constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
   constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;  // residual rows fr0-1 .. fr0+2CH+1
   __shared__ T su[FH * FW];
-  __shared__ T sb[FH * FW];
-  __shared__ T sr[RRH * RRW];
+  __shared__ T sb[FH * FW];  // b, then the residual (in place)
   const int m = L.m, n = L.n, pitch = L.pitch;
   const int fr0 = 2 * t.i0, fc0 = 2 * t.j0;
   const int gr0 = fr0 - OFF, gc0 = fc0 - OFF;
@@ -225,36 +224,66 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
     __syncthreads();
   }
 
-  // residual on fine rows/cols [fr0-1, fr0+2CH+2) (extended past the grid as above),
-  // and the pre-smoothed iterate of the owned points
+  // residual on fine rows/cols [fr0-1, fr0+2CH+2), written over b (the residual of a
+  // point reads b only at that point), and the pre-smoothed iterate of the owned points
   const int fr1 = (t.i0 + CH >= C.m) ? m : min(m, fr0 + 2 * CH);
   const int fc1 = (t.j0 + CW >= C.n) ? n : min(n, fc0 + 2 * CW);
   T* __restrict__ v = static_cast<T*>(L.v);
-  for (int k = threadIdx.x; k < RRH * RRW; k += kThreads) {
+  constexpr int NRR = (RRH * RRW + kThreads - 1) / kThreads;
+  T rv[NRR];
+#pragma unroll
+  for (int it = 0; it < NRR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
     const int ra = k / RRW, rc = k - ra * RRW;
-    int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
-    bool ok = true;
-    if (si < 0 || si >= m || sj < 0 || sj >= n) {
-      // Neumann extension: r[-1] = r[0]; r[f] = r[f-1] for even f; zero otherwise
-      if (!neu) ok = false;
-      if (si == -1) si = 0;
-      else if (si == m && (m & 1) == 0) si = m - 1;
-      else if (si < 0 || si >= m) ok = false;
-      if (sj == -1) sj = 0;
-      else if (sj == n && (n & 1) == 0) sj = n - 1;
-      else if (sj < 0 || sj >= n) ok = false;
-    } else if (si < fr1 && sj < fc1 && si >= fr0 && sj >= fc0) {
-      v[(size_t)si * pitch + sj] = su[(si - gr0) * FW + (sj - gc0)];
-    }
+    const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
     T r = T(0);
-    if (ok) {
+    if (k < RRH * RRW && si >= 0 && si < m && sj >= 0 && sj < n) {
       const int idx = (si - gr0) * FW + (sj - gc0);
+      const T uc = su[idx];
       const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      r = sb[idx] - (nb - diag_of<T>(neu, si, sj, m, n) * su[idx]) * ih2;
+      r = sb[idx] - (nb - diag_of<T>(neu, si, sj, m, n) * uc) * ih2;
+      if (si < fr1 && sj < fc1 && si >= fr0 && sj >= fc0) v[(size_t)si * pitch + sj] = uc;
     }
-    sr[k] = r;
+    rv[it] = r;
   }
   __syncthreads();
+#pragma unroll
+  for (int it = 0; it < NRR; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    const int ra = k / RRW, rc = k - ra * RRW;
+    if (k < RRH * RRW) sb[(ra + OFF - 1) * FW + rc + OFF - 1] = rv[it];
+  }
+  __syncthreads();
+  if (neu) {
+    // Neumann extension of the residual past the grid: r[-1] = r[0], r[f] = r[f-1] for
+    // even f, zero otherwise (corners take both rules)
+#pragma unroll
+    for (int it = 0; it < NRR; ++it) {
+      const int k = threadIdx.x + it * kThreads;
+      const int ra = k / RRW, rc = k - ra * RRW;
+      const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
+      if (k >= RRH * RRW || (si >= 0 && si < m && sj >= 0 && sj < n)) continue;
+      int ti = si, tj = sj;
+      bool ok = true;
+      if (ti == -1) ti = 0;
+      else if (ti == m && (m & 1) == 0) ti = m - 1;
+      else if (ti < 0 || ti >= m) ok = false;
+      if (tj == -1) tj = 0;
+      else if (tj == n && (n & 1) == 0) tj = n - 1;
+      else if (tj < 0 || tj >= n) ok = false;
+      rv[it] = ok ? sb[(ti - gr0) * FW + (tj - gc0)] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NRR; ++it) {
+      const int k = threadIdx.x + it * kThreads;
+      const int ra = k / RRW, rc = k - ra * RRW;
+      const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
+      if (k >= RRH * RRW || (si >= 0 && si < m && sj >= 0 && sj < n)) continue;
+      sb[(ra + OFF - 1) * FW + rc + OFF - 1] = rv[it];
+    }
+    __syncthreads();
+  }
 
   T* __restrict__ cb = static_cast<T*>(C.b);
   for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
@@ -263,19 +292,19 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
     if (I >= C.m || J >= C.n) continue;
     T acc;
     if (neu) {
-      const T* r0 = &sr[(2 * ci) * RRW + 2 * cj];  // fine (2I-1, 2J-1)
+      const T* r0 = &sb[(2 * ci + OFF - 1) * FW + 2 * cj + OFF - 1];  // fine (2I-1, 2J-1)
       T col[4];
 #pragma unroll
       for (int x = 0; x < 4; ++x)
-        col[x] = T(0.25) * r0[x] + T(0.75) * r0[RRW + x] + T(0.75) * r0[2 * RRW + x] +
-                 T(0.25) * r0[3 * RRW + x];
+        col[x] = T(0.25) * r0[x] + T(0.75) * r0[FW + x] + T(0.75) * r0[2 * FW + x] +
+                 T(0.25) * r0[3 * FW + x];
       acc = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
     } else {
-      const T* r0 = &sr[(2 * ci + 1) * RRW + 2 * cj + 1];  // fine (2I, 2J)
+      const T* r0 = &sb[(2 * ci + OFF) * FW + 2 * cj + OFF];  // fine (2I, 2J)
       T col[3];
 #pragma unroll
       for (int x = 0; x < 3; ++x)
-        col[x] = T(0.5) * r0[x] + r0[RRW + x] + T(0.5) * r0[2 * RRW + x];
+        col[x] = T(0.5) * r0[x] + r0[FW + x] + T(0.5) * r0[2 * FW + x];
       acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
     }
     cb[(size_t)I * C.pitch + J] = T(0.25) * acc;
@@ -428,45 +457,76 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
 // Loops run rows over waves and columns over lanes (no integer division).
 constexpr int kTailWaves = kTailThreads / 64;
 
+// The tail loops hand each thread up to kTailBatch points per pass and issue all of
+// their LDS loads before any store, so a pass costs about one LDS round trip.
+constexpr int kTailBatch = 4;
+
 // Red-black half-sweep of colour `parity`; with `from_zero` the iterate is known to be
 // zero, so colour `parity` gets (-h^2 b)/diag and the other colour is cleared.
 template <typename T>
 __device__ void lds_halfsweep(T* u, const T* b, int m, int n, T h2, bool neu, int parity,
                               bool from_zero) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = wave; i < m; i += kTailWaves) {
-    if (from_zero) {
-      for (int j = lane; j < n; j += 64) {
-        const int o = i * n + j;
-        u[o] = ((i + j) & 1) == parity ? smooth_value<T>(T(0), b[o], h2, neu, i, j, m, n) : T(0);
+  if (from_zero) {
+    const int total = m * n;
+    for (int base = 0; base < total; base += kTailBatch * kTailThreads) {
+      T val[kTailBatch];
+#pragma unroll
+      for (int r = 0; r < kTailBatch; ++r) {
+        const int o = base + threadIdx.x + r * kTailThreads;
+        const int i = o / n, j = o - i * n;
+        val[r] = (o < total && ((i + j) & 1) == parity)
+                     ? smooth_value<T>(T(0), b[min(o, total - 1)], h2, neu, i, j, m, n)
+                     : T(0);
       }
-      continue;
+#pragma unroll
+      for (int r = 0; r < kTailBatch; ++r) {
+        const int o = base + threadIdx.x + r * kTailThreads;
+        if (o < total) u[o] = val[r];
+      }
     }
-    const int j0 = (i + parity) & 1;
-    for (int j = j0 + 2 * lane; j < n; j += 128) {
-      const int o = i * n + j;
-      const T up = i > 0 ? u[o - n] : T(0);
-      const T dn = i < m - 1 ? u[o + n] : T(0);
-      const T lf = j > 0 ? u[o - 1] : T(0);
-      const T rt = j < n - 1 ? u[o + 1] : T(0);
-      u[o] = smooth_value<T>(((up + dn) + lf) + rt, b[o], h2, neu, i, j, m, n);
+    __syncthreads();
+    return;
+  }
+  const int half = (n + 1) >> 1;
+  const int total = m * half;
+  for (int base = 0; base < total; base += kTailBatch * kTailThreads) {
+    T val[kTailBatch];
+    int oo[kTailBatch];
+#pragma unroll
+    for (int r = 0; r < kTailBatch; ++r) {
+      const int k = base + threadIdx.x + r * kTailThreads;
+      const int i = k / half;
+      const int j = 2 * (k - i * half) + ((i + parity) & 1);
+      const bool ok = k < total && j < n;
+      const int o = ok ? i * n + j : -1;
+      oo[r] = o;
+      if (ok) {
+        const T up = i > 0 ? u[o - n] : T(0);
+        const T dn = i < m - 1 ? u[o + n] : T(0);
+        const T lf = j > 0 ? u[o - 1] : T(0);
+        const T rt = j < n - 1 ? u[o + 1] : T(0);
+        val[r] = smooth_value<T>(((up + dn) + lf) + rt, b[o], h2, neu, i, j, m, n);
+      }
     }
+#pragma unroll
+    for (int r = 0; r < kTailBatch; ++r)
+      if (oo[r] >= 0) u[oo[r]] = val[r];
   }
   __syncthreads();
 }
 
+// r = b - A u (r may not alias u or b)
 template <typename T>
 __device__ void lds_residual(const T* u, const T* b, T* r, int m, int n, T ih2, bool neu) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = wave; i < m; i += kTailWaves)
-    for (int j = lane; j < n; j += 64) {
-      const int o = i * n + j;
-      const T up = i > 0 ? u[o - n] : T(0);
-      const T dn = i < m - 1 ? u[o + n] : T(0);
-      const T lf = j > 0 ? u[o - 1] : T(0);
-      const T rt = j < n - 1 ? u[o + 1] : T(0);
-      r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
-    }
+  const int total = m * n;
+  for (int o = threadIdx.x; o < total; o += kTailThreads) {
+    const int i = o / n, j = o - i * n;
+    const T up = i > 0 ? u[o - n] : T(0);
+    const T dn = i < m - 1 ? u[o + n] : T(0);
+    const T lf = j > 0 ? u[o - 1] : T(0);
+    const T rt = j < n - 1 ? u[o + 1] : T(0);
+    r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
+  }
   __syncthreads();
 }
 
@@ -483,38 +543,41 @@ __device__ inline int nsrc(int i, int f, bool& ok) {
 
 template <typename T>
 __device__ void lds_restrict(const T* r, int m, int n, T* bc, int mc, int nc, bool neu) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int I = wave; I < mc; I += kTailWaves)
-    for (int J = lane; J < nc; J += 64) {
-      T acc = T(0);
-      if (neu) {
-        const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+  const int total = mc * nc;
+  for (int o = threadIdx.x; o < total; o += kTailThreads) {
+    const int I = o / nc, J = o - I * nc;
+    T acc = T(0);
+    if (neu) {
+      const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+      int ii[4], jj[4];
+      bool oi[4], oj[4];
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          bool okc = true;
-          const int j = nsrc(2 * J - 1 + y, n, okc);
-          T col = T(0);
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            bool ok = okc;
-            const int i = nsrc(2 * I - 1 + x, m, ok);
-            col += ok ? w[x] * r[i * n + j] : T(0);
-          }
-          acc += w[y] * col;
-        }
-      } else {
-        const T w[3] = {T(0.5), T(1), T(0.5)};
-#pragma unroll
-        for (int y = 0; y < 3; ++y) {
-          const int j = 2 * J + y;
-          T col = T(0);
-#pragma unroll
-          for (int x = 0; x < 3; ++x) col += w[x] * r[(2 * I + x) * n + j];
-          acc += w[y] * col;
-        }
+      for (int x = 0; x < 4; ++x) {
+        oi[x] = true;
+        oj[x] = true;
+        ii[x] = nsrc(2 * I - 1 + x, m, oi[x]);
+        jj[x] = nsrc(2 * J - 1 + x, n, oj[x]);
       }
-      bc[I * nc + J] = T(0.25) * acc;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        T col = T(0);
+#pragma unroll
+        for (int x = 0; x < 4; ++x) col += (oi[x] && oj[y]) ? w[x] * r[ii[x] * n + jj[y]] : T(0);
+        acc += w[y] * col;
+      }
+    } else {
+      const T w[3] = {T(0.5), T(1), T(0.5)};
+#pragma unroll
+      for (int y = 0; y < 3; ++y) {
+        const int j = 2 * J + y;
+        T col = T(0);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) col += w[x] * r[(2 * I + x) * n + j];
+        acc += w[y] * col;
+      }
     }
+    bc[o] = T(0.25) * acc;
+  }
   __syncthreads();
 }
 
@@ -530,19 +593,38 @@ __device__ inline T lds_coarse(const T* e, int mc, int nc, int I, int J, bool ne
 
 template <typename T>
 __device__ void lds_prolong_add(T* u, int m, int n, const T* e, int mc, int nc, bool neu) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = wave; i < m; i += kTailWaves)
-    for (int j = lane; j < n; j += 64) {
-      int ia, ib, ja, jb;
-      T wa, wb, va, vb;
-      ptaps<T>(neu, i, ia, ib, wa, wb);
-      ptaps<T>(neu, j, ja, jb, va, vb);
-      const T corr = va * (wa * lds_coarse(e, mc, nc, ia, ja, neu) +
-                           wb * lds_coarse(e, mc, nc, ib, ja, neu)) +
-                     vb * (wa * lds_coarse(e, mc, nc, ia, jb, neu) +
-                           wb * lds_coarse(e, mc, nc, ib, jb, neu));
-      u[i * n + j] += corr;
-    }
+  const int total = m * n;
+  for (int o = threadIdx.x; o < total; o += kTailThreads) {
+    const int i = o / n, j = o - i * n;
+    int ia, ib, ja, jb;
+    T wa, wb, va, vb;
+    ptaps<T>(neu, i, ia, ib, wa, wb);
+    ptaps<T>(neu, j, ja, jb, va, vb);
+    const T corr = va * (wa * lds_coarse(e, mc, nc, ia, ja, neu) +
+                         wb * lds_coarse(e, mc, nc, ib, ja, neu)) +
+                   vb * (wa * lds_coarse(e, mc, nc, ia, jb, neu) +
+                         wb * lds_coarse(e, mc, nc, ib, jb, neu));
+    u[o] += corr;
+  }
+  __syncthreads();
+}
+
+// One pass of the DCT solve: out[a][c] = sum_k A(a, c, k) with G lanes per output
+// splitting k (interleaved) and combined by shuffles, so the four small matrix
+// products of a 16 x 16 solve use every thread of the block.
+template <int G, typename F>
+__device__ inline void dct_pass(int N, int K, F term, double* out, const double* scale) {
+  const int lane = threadIdx.x % G;
+  for (int base = 0; base < N * G; base += kTailThreads) {
+    const int t = base + threadIdx.x;
+    const int o = t / G;
+    double s = 0.0;
+    if (o < N)
+      for (int k = lane; k < K; k += G) s += term(o, k);
+#pragma unroll
+    for (int w = G / 2; w > 0; w >>= 1) s += __shfl_xor(s, w, 64);
+    if (o < N && lane == 0) out[o] = scale ? s * scale[o] : s;
+  }
   __syncthreads();
 }
 
@@ -554,34 +636,15 @@ __device__ void lds_dct_solve(double* w0, double* w1, int m, int n, const double
   const double* Cn = tab + m * m;
   const double* il = tab + m * m + n * n;
   const int N = m * n;
-  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w1 = Cm w0
-    const int k = o / n, j = o - k * n;
-    double s = 0.0;
-    for (int i = 0; i < m; ++i) s += Cm[k * m + i] * w0[i * n + j];
-    w1[o] = s;
-  }
-  __syncthreads();
-  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w0 = (w1 Cn^T) ./ lambda
-    const int k = o / n, l = o - k * n;
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) s += w1[k * n + j] * Cn[l * n + j];
-    w0[o] = s * il[o];
-  }
-  __syncthreads();
-  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w1 = Cm^T w0
-    const int i = o / n, l = o - i * n;
-    double s = 0.0;
-    for (int k = 0; k < m; ++k) s += Cm[k * m + i] * w0[k * n + l];
-    w1[o] = s;
-  }
-  __syncthreads();
-  for (int o = threadIdx.x; o < N; o += kTailThreads) {  // w0 = w1 Cn
-    const int i = o / n, j = o - i * n;
-    double s = 0.0;
-    for (int l = 0; l < n; ++l) s += w1[i * n + l] * Cn[l * n + j];
-    w0[o] = s;
-  }
-  __syncthreads();
+  // w1 = Cm w0 ; w0 = (w1 Cn^T) ./ lambda ; w1 = Cm^T w0 ; w0 = w1 Cn
+  dct_pass<4>(N, m, [&](int o, int i) { const int k = o / n, j = o - k * n;
+                                        return Cm[k * m + i] * w0[i * n + j]; }, w1, nullptr);
+  dct_pass<4>(N, n, [&](int o, int j) { const int k = o / n, l = o - k * n;
+                                        return w1[k * n + j] * Cn[l * n + j]; }, w0, il);
+  dct_pass<4>(N, m, [&](int o, int k) { const int i = o / n, l = o - i * n;
+                                        return Cm[k * m + i] * w0[k * n + l]; }, w1, nullptr);
+  dct_pass<4>(N, n, [&](int o, int l) { const int i = o / n, j = o - i * n;
+                                        return w1[i * n + l] * Cn[l * n + j]; }, w0, nullptr);
 }
 
 template <typename T, bool DOTS>

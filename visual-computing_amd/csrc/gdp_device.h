@@ -64,14 +64,44 @@ struct ProblemState {
   int32_t pad[2];
 };
 
-// One workgroup's work item.
-struct Tile {
-  int32_t desc;  // LevelDesc (V-cycle kernels) or KrylovDesc index
-  int32_t i0, j0;
-  int32_t slot;  // index of this tile's partial-sum slot (problem-contiguous)
-  int32_t lead;  // 1 for the one tile per problem that writes the problem's state
+// One workgroup's work item of a V-cycle kernel: everything the block needs about its
+// level and the next coarser one, so a block starts with a single descriptor load.
+struct VTile {
+  void* u;        // level arrays (T)
+  void* v;
+  void* b;
+  void* cu;       // next coarser level: iterate and right-hand side (T)
+  void* cb;
+  double* kr;     // finest level only: the FP64 residual and q = -A d of the problem
+  double* kq;
+  double h2, inv_h2;
+  int32_t m, n, pitch, bc;
+  int32_t cm, cn, cpitch, level;
+  int32_t problem, i0, j0, slot;  // slot: this tile's partial-sum slot (finest up kernel)
+  int32_t lead;                   // the one tile per problem that writes its state
+  int32_t rbeg, rcnt;             // ||r||^2 partials to test convergence from (level 0)
   int32_t pad[3];
 };
+static_assert(sizeof(VTile) == 144, "VTile layout");
+
+// One workgroup's work item of a Krylov (finest-grid FP64) kernel.
+struct KTile {
+  double* x;
+  double* r;
+  double* d0;
+  double* d1;
+  double* q;
+  void* z;        // finest V-cycle level: iterate (the preconditioned residual), T
+  void* b0;       // finest V-cycle level: right-hand side, T
+  double inv_h2;
+  int32_t m, n, pitch, bc;
+  int32_t problem, i0, j0, slot;
+  int32_t lead;
+  int32_t kbeg, kcnt;  // slots of this problem's Krylov-tile partials
+  int32_t zbeg, zcnt;  // slots of this problem's <r, z> partials (finest up kernel / tail)
+  int32_t pad[3];
+};
+static_assert(sizeof(KTile) == 128, "KTile layout");
 
 // Slot range of one problem in a partial-sum buffer.
 struct SlotRange {

@@ -147,33 +147,22 @@ __device__ inline void ptaps(bool neu, int i, int& ia, int& ib, T& wa, T& wb) {
 // With check_conv the finest-level launch of a PCG iteration first applies the
 // convergence test of the preceding update (check_converged).
 template <typename T, int CH, int CW, int NU>
-__global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tiles,
-                                                   const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_down(const VTile* __restrict__ tiles,
                                                    ProblemState* __restrict__ st,
                                                    const double2* __restrict__ upart,
-                                                   const SlotRange* __restrict__ rk,
                                                    int check_conv) {
-  const Tile t = tiles[blockIdx.x];
-  const LevelDesc L = levels[t.desc];
-  if (st[L.problem].done) return;
-  if (check_conv && L.level == 0) {
-    __shared__ double shc[2 * kThreads / 64];
-    __shared__ double2 bcc;
-    if (check_converged<kThreads>(st, L.problem, upart, rk[L.problem], t.lead != 0, shc, &bcc))
-      return;
-  }
-  const LevelDesc C = levels[L.coarse];
+  const VTile t = tiles[blockIdx.x];
   constexpr int OFF = 2 + 2 * NU;
   constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
   constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;  // residual rows fr0-1 .. fr0+2CH+1
   __shared__ T su[FH * FW];
   __shared__ T sb[FH * FW];  // b, then the residual (in place)
-  const int m = L.m, n = L.n, pitch = L.pitch;
+  const int m = t.m, n = t.n, pitch = t.pitch;
   const int fr0 = 2 * t.i0, fc0 = 2 * t.j0;
   const int gr0 = fr0 - OFF, gc0 = fc0 - OFF;
-  const bool neu = L.bc == kNeumann;
-  const T h2 = T(L.h2), ih2 = T(L.inv_h2);
-  const T* __restrict__ b = static_cast<const T*>(L.b);
+  const bool neu = t.bc == kNeumann;
+  const T h2 = T(t.h2), ih2 = T(t.inv_h2);
+  const T* __restrict__ b = static_cast<const T*>(t.b);
 
   // load b; the first colour-0 half-sweep from u = 0 is folded into the load.  All of a
   // thread's loads are issued before any is consumed (fixed trip count, predicated).
@@ -187,6 +176,15 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
     const bool in = k < FH * FW && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 &&
                     gi < m && gj >= 0 && gj < n;
     bl[it] = in ? b[(size_t)gi * pitch + gj] : T(0);
+  }
+  if (st[t.problem].done) return;
+  if (check_conv && t.level == 0) {
+    // convergence test of the preceding PCG update (the loads above are in flight)
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
+                                  shc, &bcc))
+      return;
   }
 #pragma unroll
   for (int it = 0; it < NL; ++it) {
@@ -226,9 +224,9 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
 
   // residual on fine rows/cols [fr0-1, fr0+2CH+2), written over b (the residual of a
   // point reads b only at that point), and the pre-smoothed iterate of the owned points
-  const int fr1 = (t.i0 + CH >= C.m) ? m : min(m, fr0 + 2 * CH);
-  const int fc1 = (t.j0 + CW >= C.n) ? n : min(n, fc0 + 2 * CW);
-  T* __restrict__ v = static_cast<T*>(L.v);
+  const int fr1 = (t.i0 + CH >= t.cm) ? m : min(m, fr0 + 2 * CH);
+  const int fc1 = (t.j0 + CW >= t.cn) ? n : min(n, fc0 + 2 * CW);
+  T* __restrict__ v = static_cast<T*>(t.v);
   constexpr int NRR = (RRH * RRW + kThreads - 1) / kThreads;
   T rv[NRR];
 #pragma unroll
@@ -285,11 +283,11 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
     __syncthreads();
   }
 
-  T* __restrict__ cb = static_cast<T*>(C.b);
+  T* __restrict__ cb = static_cast<T*>(t.cb);
   for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
     const int ci = k / CW, cj = k - ci * CW;
     const int I = t.i0 + ci, J = t.j0 + cj;
-    if (I >= C.m || J >= C.n) continue;
+    if (I >= t.cm || J >= t.cn) continue;
     T acc;
     if (neu) {
       const T* r0 = &sb[(2 * ci + OFF - 1) * FW + 2 * cj + OFF - 1];  // fine (2I-1, 2J-1)
@@ -307,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
         col[x] = T(0.5) * r0[x] + r0[FW + x] + T(0.5) * r0[2 * FW + x];
       acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
     }
-    cb[(size_t)I * C.pitch + J] = T(0.25) * acc;
+    cb[(size_t)I * t.cpitch + J] = T(0.25) * acc;
   }
 }
 
@@ -316,16 +314,11 @@ __global__ __launch_bounds__(kThreads) void k_down(const Tile* __restrict__ tile
 // DOTS (finest level of a PCG iteration) the block also accumulates <r, z> and <q, z>
 // and the problem's last block turns them into delta and beta.
 template <typename T, int TH, int TW, int NU, bool DOTS>
-__global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
-                                                 const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_up(const VTile* __restrict__ tiles,
                                                  const ProblemState* __restrict__ st,
-                                                 const KrylovDesc* __restrict__ kds,
                                                  double2* __restrict__ zpart, int order) {
-  const Tile t = tiles[blockIdx.x];
-  const LevelDesc L = levels[t.desc];
-  const int p = L.problem;
-  if (st[p].done) return;
-  const LevelDesc C = levels[L.coarse];
+  const VTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   constexpr int H = 2 * NU;
   constexpr int RH = TH + 2 * H, RW = TW + 2 * H;
   constexpr int CRH = RH / 2 + 4, CRW = RW / 2 + 4;
@@ -333,15 +326,15 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
   __shared__ T sb[RH * RW];
   __shared__ T se[CRH * CRW];
   __shared__ double sh[2 * kThreads / 64];
-  const int m = L.m, n = L.n, pitch = L.pitch;
+  const int m = t.m, n = t.n, pitch = t.pitch;
   const int gr0 = t.i0 - H, gc0 = t.j0 - H;
   const int cr0 = (gr0 >> 1) - 1, cc0 = (gc0 >> 1) - 1;
-  const bool neu = L.bc == kNeumann;
-  const T h2 = T(L.h2);
-  const T* __restrict__ e = static_cast<const T*>(C.u);
-  const T* __restrict__ vv = static_cast<const T*>(L.v);
-  const T* __restrict__ b = static_cast<const T*>(L.b);
-
+  const bool neu = t.bc == kNeumann;
+  const T h2 = T(t.h2);
+  const T* __restrict__ e = static_cast<const T*>(t.cu);
+  const T* __restrict__ vv = static_cast<const T*>(t.v);
+  const T* __restrict__ b = static_cast<const T*>(t.b);
+  const int Cm = t.cm, Cn = t.cn, Cp = t.cpitch;
   // Issue every global load of the block first: the coarse correction (with its
   // padding: zero outside for Dirichlet, edge values for Neumann) and the fine v and b.
   constexpr int NE = (CRH * CRW + kThreads - 1) / kThreads;
@@ -354,12 +347,12 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
     int I = cr0 + a, J = cc0 + c;
     bool in = k < CRH * CRW;
     if (neu) {
-      I = min(max(I, 0), C.m - 1);
-      J = min(max(J, 0), C.n - 1);
+      I = min(max(I, 0), Cm - 1);
+      J = min(max(J, 0), Cn - 1);
     } else {
-      in = in && I >= 0 && I < C.m && J >= 0 && J < C.n;
+      in = in && I >= 0 && I < Cm && J >= 0 && J < Cn;
     }
-    el[it] = in ? e[(size_t)I * C.pitch + J] : T(0);
+    el[it] = in ? e[(size_t)I * Cp + J] : T(0);
   }
 #pragma unroll
   for (int it = 0; it < NR; ++it) {
@@ -371,6 +364,7 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
     vl[it] = in ? vv[o] : T(0);
     bl[it] = in ? b[o] : T(0);
   }
+  if (st[p].done) return;
 #pragma unroll
   for (int it = 0; it < NE; ++it) {
     const int k = threadIdx.x + it * kThreads;
@@ -418,12 +412,12 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
     __syncthreads();
   }
 
-  T* __restrict__ uo = static_cast<T*>(L.u);
+  T* __restrict__ uo = static_cast<T*>(t.u);
   double zr = 0.0, zq = 0.0;
   constexpr int NO = (TH * TW + kThreads - 1) / kThreads;
   double rl[DOTS ? NO : 1], ql[DOTS ? NO : 1];
   if (DOTS) {
-    const KrylovDesc* K = &kds[p];
+    // the finest level shares the Krylov grid's pitch
 #pragma unroll
     for (int it = 0; it < NO; ++it) {
       const int k = threadIdx.x + it * kThreads;
@@ -431,8 +425,8 @@ __global__ __launch_bounds__(kThreads) void k_up(const Tile* __restrict__ tiles,
       const int gi = t.i0 + a, gj = t.j0 + c;
       const bool in = k < TH * TW && gi < m && gj < n;
       const size_t o = in ? (size_t)gi * pitch + gj : 0;
-      rl[it] = in ? K->r[o] : 0.0;
-      ql[it] = in ? K->q[o] : 0.0;
+      rl[it] = in ? t.kr[o] : 0.0;
+      ql[it] = in ? t.kq[o] : 0.0;
     }
   }
 #pragma unroll
@@ -659,11 +653,24 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
                                                        int check_conv, int pre, int post,
                                                        int coarsest_sweeps, int order) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const TailDesc& D = tails[blockIdx.x];
-  const int p = D.problem;
-  if (st[p].done) return;
+  __shared__ TailDesc D;
+  __shared__ LevelDesc LV[kMaxTailLevels];
   __shared__ double sh[2 * kTailThreads / 64];
   __shared__ double2 bcc;
+  // the descriptors are read once, into LDS
+  {
+    const int* src = reinterpret_cast<const int*>(&tails[blockIdx.x]);
+    int* dst = reinterpret_cast<int*>(&D);
+    for (int w = threadIdx.x; w < int(sizeof(TailDesc) / 4); w += kTailThreads) dst[w] = src[w];
+    __syncthreads();
+    const int* ls = reinterpret_cast<const int*>(&levels[D.first]);
+    int* ld = reinterpret_cast<int*>(LV);
+    const int nw = D.nlev * int(sizeof(LevelDesc) / 4);
+    for (int w = threadIdx.x; w < nw; w += kTailThreads) ld[w] = ls[w];
+    __syncthreads();
+  }
+  const int p = D.problem;
+  if (st[p].done) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // a problem whose finest level is in the tail gets its convergence test here
   if (check_conv && D.krylov >= 0 &&
@@ -671,13 +678,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     return;
 
   {  // right-hand side of the first tail level, and the DCT tables
-    const LevelDesc L = levels[D.first];
+    const LevelDesc& L = LV[0];
     T* b = reinterpret_cast<T*>(smem + D.lds_off[0][1]);
     const T* g = static_cast<const T*>(L.b);
     for (int i = wave; i < L.m; i += kTailWaves)
       for (int j = lane; j < L.n; j += 64) b[i * L.n + j] = g[(size_t)i * L.pitch + j];
     if (D.dct) {
-      const LevelDesc Lc = levels[D.first + D.nlev - 1];
+      const LevelDesc& Lc = LV[D.nlev - 1];
       const int nt = Lc.m * Lc.m + Lc.n * Lc.n + Lc.m * Lc.n;
       double* tb = reinterpret_cast<double*>(smem + D.tab_off);
       for (int o = threadIdx.x; o < nt; o += kTailThreads) tb[o] = D.dct[o];
@@ -686,8 +693,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
   }
   T* scratch = reinterpret_cast<T*>(smem + D.scratch_off);
   for (int l = 0; l + 1 < D.nlev; ++l) {
-    const LevelDesc L = levels[D.first + l];
-    const LevelDesc C = levels[D.first + l + 1];
+    const LevelDesc& L = LV[l];
+    const LevelDesc& C = LV[l + 1];
     T* u = reinterpret_cast<T*>(smem + D.lds_off[l][0]);
     T* b = reinterpret_cast<T*>(smem + D.lds_off[l][1]);
     T* bc = reinterpret_cast<T*>(smem + D.lds_off[l + 1][1]);
@@ -707,7 +714,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
   }
   {  // coarsest level, FP64
     const int l = D.nlev - 1;
-    const LevelDesc L = levels[D.first + l];
+    const LevelDesc& L = LV[l];
     T* u = reinterpret_cast<T*>(smem + D.lds_off[l][0]);
     const T* b = reinterpret_cast<const T*>(smem + D.lds_off[l][1]);
     double* w0 = reinterpret_cast<double*>(smem + D.coarse_off);
@@ -732,8 +739,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     __syncthreads();
   }
   for (int l = D.nlev - 2; l >= 0; --l) {
-    const LevelDesc L = levels[D.first + l];
-    const LevelDesc C = levels[D.first + l + 1];
+    const LevelDesc& L = LV[l];
+    const LevelDesc& C = LV[l + 1];
     T* u = reinterpret_cast<T*>(smem + D.lds_off[l][0]);
     const T* b = reinterpret_cast<const T*>(smem + D.lds_off[l][1]);
     const T* e = reinterpret_cast<const T*>(smem + D.lds_off[l + 1][0]);
@@ -746,20 +753,21 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     }
   }
   // write the first tail level's iterate
-  const LevelDesc L = levels[D.first];
+  const LevelDesc& L = LV[0];
   const T* u = reinterpret_cast<const T*>(smem + D.lds_off[0][0]);
   T* g = static_cast<T*>(L.u);
   double zr = 0.0, zq = 0.0;
   const bool dots = DOTS && D.krylov >= 0;
+  const double* kr = dots ? kds[D.krylov].r : nullptr;
+  const double* kq = dots ? kds[D.krylov].q : nullptr;
   for (int i = wave; i < L.m; i += kTailWaves)
     for (int j = lane; j < L.n; j += 64) {
       const size_t go = (size_t)i * L.pitch + j;
       const T z = u[i * L.n + j];
       g[go] = z;
       if (dots) {
-        const KrylovDesc& K = kds[D.krylov];
-        zr += double(z) * K.r[go];
-        zq += double(z) * K.q[go];
+        zr += double(z) * kr[go];
+        zq += double(z) * kq[go];
       }
     }
   if (dots) write_partial<kTailThreads>(zr, zq, zpart, rz[p].begin, sh);
@@ -777,35 +785,23 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
 // writes only fields that no block of the same kernel reads.
 
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ tiles,
-                                                    const KrylovDesc* __restrict__ kds,
-                                                    const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ tiles,
                                                     ProblemState* __restrict__ st,
                                                     const double2* __restrict__ zpart,
-                                                    const SlotRange* __restrict__ rz,
-                                                    double2* __restrict__ apart, int din) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
-  if (st[p].done) return;
+                                                    double2* __restrict__ apart, int din,
+                                                    int jacobi) {
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sd[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
-  const double2 z2 = reduce_slots<kThreads>(zpart, rz[p], sh, &bc);
-  double beta;
-  if (st[p].iter == 0 || st[p].restart) beta = 0.0;
-  else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;
-  else beta = z2.x / st[p].delta_old;
-  if (t.lead && threadIdx.x == 0) {
-    st[p].delta = z2.x;
-    st[p].beta = beta;
-  }
-  const int m = K.m, n = K.n, pitch = K.pitch;
-  const T* __restrict__ z = static_cast<const T*>(levels[K.level0].u);
-  const double* __restrict__ dold = din ? K.d[1] : K.d[0];
-  double* __restrict__ dnew = din ? K.d[0] : K.d[1];
+  const int m = t.m, n = t.n, pitch = t.pitch;
+  const T* __restrict__ z = static_cast<const T*>(t.z);
+  const double* __restrict__ dold = din ? t.d1 : t.d0;
+  double* __restrict__ dnew = din ? t.d0 : t.d1;
   const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
+  // vector loads first; the scalar reduction below overlaps them
   constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
   T zl[NR];
   double dl[NR];
@@ -819,13 +815,24 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
     zl[it] = in ? z[o] : T(0);
     dl[it] = in ? dold[o] : 0.0;
   }
+  if (st[p].done) return;
+  const double2 z2 = reduce_slots<kThreads>(
+      zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
+  double beta;
+  if (st[p].iter == 0 || st[p].restart) beta = 0.0;
+  else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;
+  else beta = z2.x / st[p].delta_old;
+  if (t.lead && threadIdx.x == 0) {
+    st[p].delta = z2.x;
+    st[p].beta = beta;
+  }
 #pragma unroll
   for (int it = 0; it < NR; ++it) {
     const int k = threadIdx.x + it * kThreads;
     if (k < RH * RW) sd[k] = double(zl[it]) + beta * dl[it];
   }
   __syncthreads();
-  const bool neu = K.bc == kNeumann;
+  const bool neu = t.bc == kNeumann;
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
@@ -836,43 +843,27 @@ __global__ __launch_bounds__(kThreads) void k_apply(const Tile* __restrict__ til
     const int idx = (a + 1) * RW + c + 1;
     const double dc = sd[idx];
     const double nb = ((sd[idx - RW] + sd[idx + RW]) + sd[idx - 1]) + sd[idx + 1];
-    const double q = -(nb - diag_of<double>(neu, gi, gj, m, n) * dc) * K.inv_h2;
+    const double q = -(nb - diag_of<double>(neu, gi, gj, m, n) * dc) * t.inv_h2;
     const size_t o = (size_t)gi * pitch + gj;
     dnew[o] = dc;
-    K.q[o] = q;
+    t.q[o] = q;
     acc += dc * q;
   }
   write_partial<kThreads>(acc, 0.0, apart, t.slot, sh);
 }
 
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ tiles,
-                                                     const KrylovDesc* __restrict__ kds,
-                                                     const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ tiles,
                                                      ProblemState* __restrict__ st,
                                                      const double2* __restrict__ apart,
-                                                     const SlotRange* __restrict__ rk,
                                                      double2* __restrict__ upart, int dcur) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
-  if (st[p].done) return;
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
-  const double2 a2 = reduce_slots<kThreads>(apart, rk[p], sh, &bc);
-  const double delta = st[p].delta;
-  const double alpha = delta / a2.x;
-  if (t.lead && threadIdx.x == 0) {
-    st[p].alpha = alpha;
-    st[p].dq = a2.x;
-    st[p].delta_old = delta;
-    st[p].iter += 1;
-    st[p].restart = 0;
-  }
-  const int m = K.m, n = K.n, pitch = K.pitch;
-  T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
-  const double* __restrict__ d = dcur ? K.d[1] : K.d[0];
-  double acc = 0.0;
+  const int m = t.m, n = t.n, pitch = t.pitch;
+  T* __restrict__ b0 = static_cast<T*>(t.b0);
+  const double* __restrict__ d = dcur ? t.d1 : t.d0;
   constexpr int NO = (TH * TW) / kThreads;
   double xl[NO], dl[NO], rl[NO], ql[NO];
   size_t ol[NO];
@@ -884,17 +875,29 @@ __global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ ti
     const int gi = t.i0 + a, gj = t.j0 + c;
     il[it] = gi < m && gj < n;
     ol[it] = il[it] ? (size_t)gi * pitch + gj : 0;
-    xl[it] = il[it] ? K.x[ol[it]] : 0.0;
+    xl[it] = il[it] ? t.x[ol[it]] : 0.0;
     dl[it] = il[it] ? d[ol[it]] : 0.0;
-    rl[it] = il[it] ? K.r[ol[it]] : 0.0;
-    ql[it] = il[it] ? K.q[ol[it]] : 0.0;
+    rl[it] = il[it] ? t.r[ol[it]] : 0.0;
+    ql[it] = il[it] ? t.q[ol[it]] : 0.0;
   }
+  if (st[p].done) return;
+  const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.kbeg, t.kcnt}, sh, &bc);
+  const double delta = st[p].delta;
+  const double alpha = delta / a2.x;
+  if (t.lead && threadIdx.x == 0) {
+    st[p].alpha = alpha;
+    st[p].dq = a2.x;
+    st[p].delta_old = delta;
+    st[p].iter += 1;
+    st[p].restart = 0;
+  }
+  double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < NO; ++it) {
     if (!il[it]) continue;
     const double r = rl[it] - alpha * ql[it];
-    K.x[ol[it]] = xl[it] + alpha * dl[it];
-    K.r[ol[it]] = r;
+    t.x[ol[it]] = xl[it] + alpha * dl[it];
+    t.r[ol[it]] = r;
     b0[ol[it]] = T(-r);
     acc += r * r;
   }
@@ -904,33 +907,30 @@ __global__ __launch_bounds__(kThreads) void k_update(const Tile* __restrict__ ti
 // Jacobi preconditioner of the reference "cg" method (solvers.py:114-117):
 // z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply with beta = delta'/delta).
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ tiles,
-                                                     const KrylovDesc* __restrict__ kds,
-                                                     const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_jacobi(const KTile* __restrict__ tiles,
                                                      ProblemState* __restrict__ st,
                                                      const double2* __restrict__ upart,
-                                                     const SlotRange* __restrict__ rk,
                                                      double2* __restrict__ zpart,
                                                      int check_conv) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   if (st[p].done) return;
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
-  if (check_conv && check_converged<kThreads>(st, p, upart, rk[p], t.lead != 0, sh, &bc)) return;
-  const int m = K.m, n = K.n, pitch = K.pitch;
-  T* __restrict__ z = static_cast<T*>(levels[K.level0].u);
-  const double diag = 4.0 * K.inv_h2;
+  if (check_conv && check_converged<kThreads>(st, p, upart, SlotRange{t.kbeg, t.kcnt},
+                                              t.lead != 0, sh, &bc))
+    return;
+  T* __restrict__ z = static_cast<T*>(t.z);
+  const double diag = 4.0 * t.inv_h2;
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= m || gj >= n) continue;
-    const size_t o = (size_t)gi * pitch + gj;
-    const double r = K.r[o];
+    if (gi >= t.m || gj >= t.n) continue;
+    const size_t o = (size_t)gi * t.pitch + gj;
+    const double r = t.r[o];
     const double zz = r / diag;
     z[o] = T(zz);
     acc += r * double(T(zz));
@@ -941,22 +941,19 @@ __global__ __launch_bounds__(kThreads) void k_jacobi(const Tile* __restrict__ ti
 // Set-up pass 1: partial sums of b (mean for the Neumann projection onto compatible
 // right-hand sides -- the constant that solvers.py:224-262 drops).
 template <int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_init_stats(const Tile* __restrict__ tiles,
-                                                         const KrylovDesc* __restrict__ kds,
+__global__ __launch_bounds__(kThreads) void k_init_stats(const KTile* __restrict__ tiles,
                                                          const IoDesc* __restrict__ io,
                                                          double2* __restrict__ spart) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
+  const KTile t = tiles[blockIdx.x];
   __shared__ double sh[2 * kThreads / 64];
-  const double* __restrict__ b = io[p].b;
+  const double* __restrict__ b = io[t.problem].b;
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    acc += (gi < K.m && gj < K.n) ? b[(size_t)gi * K.n + gj] : 0.0;
+    acc += (gi < t.m && gj < t.n) ? b[(size_t)gi * t.n + gj] : 0.0;
   }
   write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
 }
@@ -964,40 +961,37 @@ __global__ __launch_bounds__(kThreads) void k_init_stats(const Tile* __restrict_
 // Set-up pass 2.  mode 0 (PCG): r = -(b - mean), x = d = q = 0, V-cycle RHS b - mean.
 // mode 1 (MG iteration): x = initial guess (or 0).  Partial sums of (b - mean)^2.
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tiles,
-                                                   const KrylovDesc* __restrict__ kds,
-                                                   const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ tiles,
                                                    const IoDesc* __restrict__ io,
                                                    const double* const* __restrict__ initial,
                                                    const ProblemState* __restrict__ st,
                                                    double2* __restrict__ spart, int mode) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   __shared__ double sh[2 * kThreads / 64];
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
   const double* __restrict__ x0 = initial ? initial[p] : nullptr;
-  T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
+  T* __restrict__ b0 = static_cast<T*>(t.b0);
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= K.m || gj >= K.n) continue;
-    const size_t o = (size_t)gi * K.pitch + gj;
-    const double bb = b[(size_t)gi * K.n + gj] - mean;
+    if (gi >= t.m || gj >= t.n) continue;
+    const size_t o = (size_t)gi * t.pitch + gj;
+    const double bb = b[(size_t)gi * t.n + gj] - mean;
     acc += bb * bb;
     if (mode == 0) {
-      K.r[o] = -bb;
-      K.x[o] = 0.0;
-      K.d[0][o] = 0.0;
-      K.d[1][o] = 0.0;
-      K.q[o] = 0.0;
+      t.r[o] = -bb;
+      t.x[o] = 0.0;
+      t.d0[o] = 0.0;
+      t.d1[o] = 0.0;
+      t.q[o] = 0.0;
       b0[o] = T(bb);
     } else {
-      K.x[o] = x0 ? x0[(size_t)gi * K.n + gj] : 0.0;
+      t.x[o] = x0 ? x0[(size_t)gi * t.n + gj] : 0.0;
     }
   }
   write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
@@ -1007,24 +1001,21 @@ __global__ __launch_bounds__(kThreads) void k_init(const Tile* __restrict__ tile
 // RHS becomes r_true and, in mode 0 (PCG), r = -r_true so an unconverged problem can
 // restart from it.
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ tiles,
-                                                       const KrylovDesc* __restrict__ kds,
-                                                       const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__ tiles,
                                                        const IoDesc* __restrict__ io,
                                                        const ProblemState* __restrict__ st,
                                                        double2* __restrict__ spart, int mode,
                                                        int respect_done) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
   if (respect_done && st[p].done) return;
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sx[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
-  const int m = K.m, n = K.n, pitch = K.pitch;
+  const int m = t.m, n = t.n, pitch = t.pitch;
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
-  T* __restrict__ b0 = static_cast<T*>(levels[K.level0].b);
+  T* __restrict__ b0 = static_cast<T*>(t.b0);
   const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
   constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
   double xl[NR];
@@ -1034,7 +1025,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
     const int a = k / RW, c = k - a * RW;
     const int gi = gr0 + a, gj = gc0 + c;
     const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
-    xl[it] = in ? K.x[(size_t)gi * pitch + gj] : 0.0;
+    xl[it] = in ? t.x[(size_t)gi * pitch + gj] : 0.0;
   }
 #pragma unroll
   for (int it = 0; it < NR; ++it) {
@@ -1042,7 +1033,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
     if (k < RH * RW) sx[k] = xl[it];
   }
   __syncthreads();
-  const bool neu = K.bc == kNeumann;
+  const bool neu = t.bc == kNeumann;
   double rr = 0.0, sum = 0.0;
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
@@ -1053,10 +1044,10 @@ __global__ __launch_bounds__(kThreads) void k_residual(const Tile* __restrict__ 
     const int idx = (a + 1) * RW + c + 1;
     const double xc = sx[idx];
     const double nb = ((sx[idx - RW] + sx[idx + RW]) + sx[idx - 1]) + sx[idx + 1];
-    const double lx = (nb - diag_of<double>(neu, gi, gj, m, n) * xc) * K.inv_h2;
+    const double lx = (nb - diag_of<double>(neu, gi, gj, m, n) * xc) * t.inv_h2;
     const double res = (b[(size_t)gi * n + gj] - mean) - lx;
     const size_t o = (size_t)gi * pitch + gj;
-    if (mode == 0) K.r[o] = -res;
+    if (mode == 0) t.r[o] = -res;
     b0[o] = T(res);
     rr += res * res;
     sum += xc;
@@ -1116,38 +1107,32 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
 
 // MG iteration: x += z (the V-cycle correction).
 template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_correct(const Tile* __restrict__ tiles,
-                                                      const KrylovDesc* __restrict__ kds,
-                                                      const LevelDesc* __restrict__ levels,
+__global__ __launch_bounds__(kThreads) void k_correct(const KTile* __restrict__ tiles,
                                                       const ProblemState* __restrict__ st) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
-  if (st[p].done) return;
-  const T* __restrict__ z = static_cast<const T*>(levels[K.level0].u);
+  const KTile t = tiles[blockIdx.x];
+  if (st[t.problem].done) return;
+  const T* __restrict__ z = static_cast<const T*>(t.z);
 #pragma unroll
   for (int it = 0; it < (TH * TW) / kThreads; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= K.m || gj >= K.n) continue;
-    const size_t o = (size_t)gi * K.pitch + gj;
-    K.x[o] += double(z[o]);
+    if (gi >= t.m || gj >= t.n) continue;
+    const size_t o = (size_t)gi * t.pitch + gj;
+    t.x[o] += double(z[o]);
   }
 }
 
 // Caller output: u = x - mean(x) for Neumann problems when mean_fix, else u = x.
 template <int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_finalize(const Tile* __restrict__ tiles,
-                                                       const KrylovDesc* __restrict__ kds,
+__global__ __launch_bounds__(kThreads) void k_finalize(const KTile* __restrict__ tiles,
                                                        const IoDesc* __restrict__ io,
                                                        const ProblemState* __restrict__ st,
                                                        int mean_fix) {
-  const Tile t = tiles[blockIdx.x];
-  const KrylovDesc K = kds[t.desc];
-  const int p = t.desc;
-  const double mean = (mean_fix && K.bc == kNeumann)
-                          ? st[p].sumx / (double(K.m) * double(K.n))
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
+  const double mean = (mean_fix && t.bc == kNeumann)
+                          ? st[p].sumx / (double(t.m) * double(t.n))
                           : 0.0;
   double* __restrict__ u = io[p].u;
 #pragma unroll
@@ -1155,8 +1140,8 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const Tile* __restrict__ 
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= K.m || gj >= K.n) continue;
-    u[(size_t)gi * K.n + gj] = K.x[(size_t)gi * K.pitch + gj] - mean;
+    if (gi >= t.m || gj >= t.n) continue;
+    u[(size_t)gi * t.n + gj] = t.x[(size_t)gi * t.pitch + gj] - mean;
   }
 }
 

@@ -75,19 +75,16 @@ int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ----------------------------------------------------------------- launch dispatch
 struct DownArgs {
-  const Tile* tiles;
-  const LevelDesc* levels;
+  const VTile* tiles;
   ProblemState* st;
   const double2* upart;
-  const SlotRange* rk;
   int check_conv;
 };
 
 template <typename T, int CH, int CW>
 void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWN(NU) \
-  k_down<T, CH, CW, NU><<<blocks, kThreads, 0, s>>>(a.tiles, a.levels, a.st, a.upart, a.rk, \
-                                                    a.check_conv)
+  k_down<T, CH, CW, NU><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
   switch (nu) {
     case 0: GDP_DOWN(0); break;
     case 1: GDP_DOWN(1); break;
@@ -108,10 +105,8 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
 }
 
 struct UpArgs {
-  const Tile* tiles;
-  const LevelDesc* levels;
+  const VTile* tiles;
   const ProblemState* st;
-  const KrylovDesc* kds;
   double2* zpart;
   int order;
 };
@@ -119,8 +114,7 @@ struct UpArgs {
 template <typename T, int TH, int TW, bool DOTS>
 void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU)                                                                       \
-  k_up<T, TH, TW, NU, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.levels, a.st, a.kds, \
-                                                        a.zpart, a.order)
+  k_up<T, TH, TW, NU, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;
@@ -310,10 +304,11 @@ int max_tiles(int kind, int m, int n) {
 
 // Block b of a launch runs on XCD b % 8; give each XCD a contiguous run of tiles so
 // neighbouring tiles (which share halo rows) hit the same L2.
-std::vector<Tile> xcd_swizzle(const std::vector<Tile>& in) {
+template <typename TT>
+std::vector<TT> xcd_swizzle(const std::vector<TT>& in) {
   const int N = int(in.size()), X = 8;
   const int base = N / X, rem = N % X;
-  std::vector<Tile> out(N);
+  std::vector<TT> out(N);
   for (int b = 0; b < N; ++b) {
     const int x = b % X, k = b / X;
     const int start = x * base + std::min(x, rem);
@@ -478,7 +473,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   P->off_upart = take(sizeof(double2) * slots);
   P->off_zpart = take(sizeof(double2) * slots);
   P->off_spart = take(sizeof(double2) * slots);
-  P->tiles_bytes = sizeof(Tile) * (tiles + 8 * (2 * P->depth + 1));
+  P->tiles_bytes = sizeof(VTile) * (tiles + 8 * (2 * P->depth + 1)) + 256 * (2 * P->depth + 2);
   P->off_tiles = take(P->tiles_bytes);
   P->ws_bytes = off;
   *out = P;
@@ -656,67 +651,92 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   GDP_HIP(hipMemcpy(P->d_kry(), kry.data(), sizeof(KrylovDesc) * NP, hipMemcpyHostToDevice));
   GDP_HIP(hipMemcpy(P->d_tails(), tails.data(), sizeof(TailDesc) * NP, hipMemcpyHostToDevice));
 
-  // ---- tile tables
+  // ---- tile tables (self-contained records; see VTile / KTile)
   size_t toff = P->off_tiles;
-  size_t tend = P->off_tiles + P->tiles_bytes;
-  auto upload = [&](Table& tb, const std::vector<Tile>& v) -> int {
+  const size_t tend = P->off_tiles + P->tiles_bytes;
+  auto upload = [&](Table& tb, const void* data, size_t rec, size_t count) -> int {
     tb.off = toff;
-    tb.count = int(v.size());
-    const size_t b = align256(sizeof(Tile) * std::max<size_t>(v.size(), 1));
-    if (toff + sizeof(Tile) * v.size() > tend) {
+    tb.count = int(count);
+    const size_t b = align256(rec * std::max<size_t>(count, 1));
+    if (toff + rec * count > tend) {
       set_error("internal: tile table overflow");
       return GDP_EINVAL;
     }
-    if (!v.empty()) {
-      const std::vector<Tile> sw = xcd_swizzle(v);
-      if (hipMemcpy(P->ws + toff, sw.data(), sizeof(Tile) * sw.size(), hipMemcpyHostToDevice) !=
-          hipSuccess) {
-        set_error("hipMemcpy of a tile table failed");
-        return GDP_EHIP;
-      }
+    if (count && hipMemcpy(P->ws + toff, data, rec * count, hipMemcpyHostToDevice) != hipSuccess) {
+      set_error("hipMemcpy of a tile table failed");
+      return GDP_EHIP;
     }
     toff += std::min(b, tend - toff);
     return GDP_OK;
   };
+  // Krylov slot ranges (fixed by the Krylov tile shape)
   std::vector<SlotRange> rk(NP), ru(NP);
   {
-    std::vector<Tile> v;
     const Shape2 s = kKryMenu[P->kry_menu];
+    int c = 0;
     for (int p = 0; p < NP; ++p) {
       const auto& l0 = P->probs[p].lv[0];
-      rk[p].begin = int(v.size());
-      for (int i = 0; i < l0.m; i += s.h)
-        for (int j = 0; j < l0.n; j += s.w)
-          v.push_back(Tile{p, i, j, int(v.size()), int(v.size()) == rk[p].begin, {0, 0, 0}});
-      rk[p].count = int(v.size()) - rk[p].begin;
+      rk[p].begin = c;
+      c += cdiv(l0.m, s.h) * cdiv(l0.n, s.w);
+      rk[p].count = c - rk[p].begin;
     }
-    if (int rc = upload(P->kry, v)) return rc;
   }
+  auto vtile = [&](int p, int l, int i0, int j0, int slot, int lead) {
+    const ProbInfo& pi = P->probs[p];
+    const LevelInfo& f = pi.lv[l];
+    VTile t;
+    memset(&t, 0, sizeof(t));
+    t.u = P->ws + f.off_u;
+    t.v = P->ws + f.off_v;
+    t.b = P->ws + f.off_b;
+    if (l + 1 < int(pi.lv.size())) {
+      const LevelInfo& c = pi.lv[l + 1];
+      t.cu = P->ws + c.off_u;
+      t.cb = P->ws + c.off_b;
+      t.cm = c.m;
+      t.cn = c.n;
+      t.cpitch = c.pitch;
+    }
+    t.kr = P->at<double>(pi.off_r);
+    t.kq = P->at<double>(pi.off_q);
+    t.h2 = f.h * f.h;
+    t.inv_h2 = 1.0 / t.h2;
+    t.m = f.m;
+    t.n = f.n;
+    t.pitch = f.pitch;
+    t.bc = pi.p.boundary;
+    t.level = l;
+    t.problem = p;
+    t.i0 = i0;
+    t.j0 = j0;
+    t.slot = slot;
+    t.lead = lead;
+    t.rbeg = rk[p].begin;
+    t.rcnt = rk[p].count;
+    return t;
+  };
   for (int l = 0; l < P->depth; ++l) {
-    std::vector<Tile> vd, vu;
+    std::vector<VTile> vd, vu;
     const Shape2 sd = kDownMenu[P->down_menu[l]];
     const Shape2 su = kUpMenu[P->up_menu[l]];
     for (int p = 0; p < NP; ++p) {
       const ProbInfo& pi = P->probs[p];
-      if (pi.tail_first <= l) {
-        if (l == 0) ru[p] = SlotRange{0, 0};
-        continue;
-      }
+      if (pi.tail_first <= l) continue;
       const auto& f = pi.lv[l];
       const auto& c = pi.lv[l + 1];
-      const int d0 = int(vd.size());
+      const size_t d0 = vd.size();
       for (int i = 0; i < c.m; i += sd.h)
         for (int j = 0; j < c.n; j += sd.w)
-          vd.push_back(Tile{pi.level_base + l, i, j, int(vd.size()), int(vd.size()) == d0,
-                            {0, 0, 0}});
+          vd.push_back(vtile(p, l, i, j, int(vd.size()), vd.size() == d0));
       if (l == 0) ru[p].begin = int(vu.size());
       for (int i = 0; i < f.m; i += su.h)
-        for (int j = 0; j < f.n; j += su.w)
-          vu.push_back(Tile{pi.level_base + l, i, j, int(vu.size()), 0, {0, 0, 0}});
+        for (int j = 0; j < f.n; j += su.w) vu.push_back(vtile(p, l, i, j, int(vu.size()), 0));
       if (l == 0) ru[p].count = int(vu.size()) - ru[p].begin;
     }
-    if (int rc = upload(P->down[l], vd)) return rc;
-    if (int rc = upload(P->up[l], vu)) return rc;
+    const auto swd = xcd_swizzle(vd);
+    const auto swu = xcd_swizzle(vu);
+    if (int rc = upload(P->down[l], swd.data(), sizeof(VTile), swd.size())) return rc;
+    if (int rc = upload(P->up[l], swu.data(), sizeof(VTile), swu.size())) return rc;
   }
   {
     // problems whose finest level is in the tail get one direct <r,z> slot each,
@@ -724,6 +744,43 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     int zb = P->depth > 0 ? P->up[0].count : 0;
     for (int p = 0; p < NP; ++p)
       if (P->probs[p].tail_first == 0) ru[p] = SlotRange{zb++, 1};
+  }
+  {
+    std::vector<KTile> v;
+    const Shape2 s = kKryMenu[P->kry_menu];
+    for (int p = 0; p < NP; ++p) {
+      const ProbInfo& pi = P->probs[p];
+      const auto& l0 = pi.lv[0];
+      for (int i = 0; i < l0.m; i += s.h)
+        for (int j = 0; j < l0.n; j += s.w) {
+          KTile t;
+          memset(&t, 0, sizeof(t));
+          t.x = P->at<double>(pi.off_x);
+          t.r = P->at<double>(pi.off_r);
+          t.d0 = P->at<double>(pi.off_d0);
+          t.d1 = P->at<double>(pi.off_d1);
+          t.q = P->at<double>(pi.off_q);
+          t.z = P->ws + l0.off_u;
+          t.b0 = P->ws + l0.off_b;
+          t.inv_h2 = 1.0 / (pi.p.spacing * pi.p.spacing);
+          t.m = l0.m;
+          t.n = l0.n;
+          t.pitch = l0.pitch;
+          t.bc = pi.p.boundary;
+          t.problem = p;
+          t.i0 = i;
+          t.j0 = j;
+          t.slot = int(v.size());
+          t.lead = int(v.size()) == rk[p].begin;
+          t.kbeg = rk[p].begin;
+          t.kcnt = rk[p].count;
+          t.zbeg = ru[p].begin;
+          t.zcnt = ru[p].count;
+          v.push_back(t);
+        }
+    }
+    const auto sw = xcd_swizzle(v);
+    if (int rc = upload(P->kry, sw.data(), sizeof(KTile), sw.size())) return rc;
   }
   GDP_HIP(hipMemcpy(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
   GDP_HIP(hipMemcpy(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
@@ -837,8 +894,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     char nm[32];
     snprintf(nm, sizeof nm, "down L%d", l);
     ProfScope ps(P, s, nm, (2.25 * sizeof(T)) * level_points(P, l));
-    DownArgs a{P->at<Tile>(P->down[l].off), P->d_levels(), P->d_state(), P->d_upart(),
-               P->d_rkry(), check_conv ? 1 : 0};
+    DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), check_conv ? 1 : 0};
     launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
@@ -869,8 +925,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     const bool dl = dots && l == 0;
     snprintf(nm, sizeof nm, dl ? "up L%d+dots" : "up L%d", l);
     ProfScope ps(P, s, nm, (3.25 * sizeof(T) + (dl ? 16.0 : 0.0)) * level_points(P, l));
-    UpArgs a{P->at<Tile>(P->up[l].off), P->d_levels(), P->d_state(), P->d_kry(),
-             P->d_zpart(), order};
+    UpArgs a{P->at<VTile>(P->up[l].off), P->d_state(), P->d_zpart(), order};
     launch_up<T>(P->up_menu[l], o.post_smoothing, dl, P->up[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
@@ -885,12 +940,11 @@ int enqueue_vcycle(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
 template <typename T>
 int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "jacobi", (8.0 + sizeof(T)) * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu,
                    (k_jacobi<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                       t, P->d_kry(), P->d_levels(), P->d_state(), P->d_upart(), P->d_rkry(),
-                       P->d_zpart(), check_conv ? 1 : 0)));
+                       t, P->d_state(), P->d_upart(), P->d_zpart(), check_conv ? 1 : 0)));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
 }
@@ -898,24 +952,21 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
 template <typename T>
 int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
   // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
-  const SlotRange* rz = jacobi ? P->d_rkry() : P->d_rup0();
   {
     ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
                      (k_apply<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_zpart(), rz,
-                         P->d_apart(), din)));
+                         t, P->d_state(), P->d_zpart(), P->d_apart(), din, jacobi ? 1 : 0)));
     GDP_CHECK_LAUNCH();
   }
   {
     ProfScope ps(P, s, "update", (48.0 + sizeof(T)) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
                      (k_update<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_kry(), P->d_levels(), P->d_state(), P->d_apart(),
-                         P->d_rkry(), P->d_upart(), din ^ 1)));
+                         t, P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
     GDP_CHECK_LAUNCH();
   }
   if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
@@ -954,13 +1005,13 @@ template <typename T>
 int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol, int max_it,
                    bool with_init, int zero_done) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
   int rc;
   {
     ProfScope ps(P, s, "init_stats", 8.0 * N);
     GDP_KRY_DISPATCH(P->kry_menu, (k_init_stats<TH, TW><<<nb, kThreads, 0, s>>>(
-                                      t, P->d_kry(), P->d_io(), P->d_spart())));
+                                      t, P->d_io(), P->d_spart())));
     GDP_CHECK_LAUNCH();
   }
   ScalarArgs a;
@@ -971,8 +1022,8 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
     ProfScope ps(P, s, "init", (mode == 0 ? 48.0 + sizeof(T) : 16.0) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
                      (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_kry(), P->d_levels(), P->d_io(),
-                         with_init ? P->d_init() : nullptr, P->d_state(), P->d_spart(), mode)));
+                         t, P->d_io(), with_init ? P->d_init() : nullptr, P->d_state(),
+                         P->d_spart(), mode)));
     GDP_CHECK_LAUNCH();
   }
   a.kind = 1;
@@ -985,13 +1036,12 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
 template <typename T>
 int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, bool hist) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   {
     ProfScope ps(P, s, "residual", (24.0 + sizeof(T)) * krylov_points(P));
     GDP_KRY_DISPATCH(P->kry_menu,
                      (k_residual<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_kry(), P->d_levels(), P->d_io(), P->d_state(), P->d_spart(),
-                         mode, respect_done)));
+                         t, P->d_io(), P->d_state(), P->d_spart(), mode, respect_done)));
     GDP_CHECK_LAUNCH();
   }
   ScalarArgs a;
@@ -1006,20 +1056,20 @@ int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, b
 template <typename T>
 int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "correct", (16.0 + sizeof(T)) * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu, (k_correct<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                                    t, P->d_kry(), P->d_levels(), P->d_state())));
+                                    t, P->d_state())));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
 }
 
 int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix) {
   const int nb = P->kry.count;
-  const Tile* t = P->at<Tile>(P->kry.off);
+  const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "finalize", 16.0 * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu, (k_finalize<TH, TW><<<nb, kThreads, 0, s>>>(
-                                    t, P->d_kry(), P->d_io(), P->d_state(), mean_fix)));
+                                    t, P->d_io(), P->d_state(), mean_fix)));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
 }

@@ -110,6 +110,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only run a few mixed solves and a plain device copy of 512 MiB "
+                    "(the FETCH_SIZE / WRITE_SIZE calibration) for rocprofv3; no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -139,6 +142,17 @@ def main():
                                options=gdpoisson.PlanOptions(precision="fp64",
                                                              pre_smoothing=args.nu,
                                                              post_smoothing=args.nu))
+
+    if args.profile_run:
+        for _ in range(2 + args.steps):
+            mixed.mgcg(bs)
+        src = torch.empty(64 << 20, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # plain copy: reads and writes 512 MiB each
+        torch.cuda.synchronize()
+        log("profile run done")
+        return
 
     log("timing mixed")
     t_mix, lat_mix, u_mix, rep_mix = time_solves(mixed, bs, args.steps, args.warmup)
@@ -181,6 +195,13 @@ def main():
         }
         for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])
     }
+
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_1024x1024_rgb_mixed.json"
+    if tfile.exists():
+        t = json.loads(tfile.read_text()).get("kernels", {}).get(dom_name)
+        if t:
+            traffic = t["hbm_bytes_per_launch"]
 
     line = {
         "metric": METRIC,
@@ -226,7 +247,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_us": round(avg_s * 1e6, 2),
         },

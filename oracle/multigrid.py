@@ -24,6 +24,10 @@ problems with the DCT in ``solvers.py:224-262``):
 * operator ``neumann_laplacian`` (in-grid neighbours only), rediscretised at 2h;
 * coarsest level (``max side <= coarsest_size`` or ``min side < 3``): exact FP64 solve by
   the DCT-II diagonalisation of ``solvers.solve_neumann`` (zero-mean solution);
+
+Dirichlet extension: ``coarsest_sweeps == 0`` replaces the coarsest sweeps by the exact
+FP64 solve of the coarsest Dirichlet system, diagonalised by the DST-I (eigenvalues
+``2 cos(pi k/(m+1)) + 2 cos(pi l/(n+1)) - 4``).
 * symmetric cycle: pre-smoothing colour 0 then 1, post-smoothing colour 1 then 0.
 
 Every function takes a ``dtype`` so the same code also describes the FP32 V-cycle of
@@ -37,7 +41,7 @@ from functools import lru_cache
 
 import numpy as np
 import scipy.sparse as sp
-from scipy.fft import dctn, idctn
+from scipy.fft import dctn, dstn, idctn, idstn
 
 from .operators import laplacian, neumann_laplacian
 
@@ -212,10 +216,25 @@ def dct_neumann_solve(b: np.ndarray, spacing: float = 1.0) -> np.ndarray:
     return idctn(t, type=2, norm="ortho")
 
 
+def dst_dirichlet_solve(b: np.ndarray, spacing: float = 1.0) -> np.ndarray:
+    """Exact solution of ``laplacian(u, h) = b`` (zero outside) by DST-I."""
+    b = np.asarray(b, dtype=np.float64) * (spacing * spacing)
+    m, n = b.shape
+    lam = (
+        2.0 * np.cos(np.pi * np.arange(1, m + 1) / (m + 1))[:, None]
+        + 2.0 * np.cos(np.pi * np.arange(1, n + 1) / (n + 1))[None, :]
+        - 4.0
+    )
+    return idstn(dstn(b, type=1, norm="ortho") / lam, type=1, norm="ortho")
+
+
 def coarsest_solve(u, b, spacing, options, bc):
     if bc == NEUMANN:
         res = b.astype(np.float64) - neumann_laplacian(u.astype(np.float64), spacing)
         return (u.astype(np.float64) + dct_neumann_solve(res, spacing)).astype(u.dtype)
+    if options.coarsest_sweeps == 0:
+        res = b.astype(np.float64) - laplacian(u.astype(np.float64), spacing)
+        return (u.astype(np.float64) + dst_dirichlet_solve(res, spacing)).astype(u.dtype)
     # Dirichlet: coarsest_sweeps RB-GS sweeps (multigrid.py:206-207), done in FP64.
     out = smooth(u.astype(np.float64), b.astype(np.float64), spacing, options.coarsest_sweeps)
     return out.astype(u.dtype)

@@ -95,10 +95,20 @@ def solve_multigrid(b, spacing, tolerance, max_iterations):
     return u, max_iterations, False
 
 
-def default_pcg_options(bc: str) -> MultigridOptions:
-    """nu = 1/1 as solvers.py:137; Neumann stops coarsening at 16 cells per side."""
+def default_pcg_options(bc: str, precision: str = "fp64") -> MultigridOptions:
+    """The cycle of each solver configuration.
+
+    Dirichlet FP64: the reference mgcg cycle (nu = 1/1, solvers.py:137; coarsest <= 8
+    with 60 sweeps).  Neumann: nu = 1/1 (FP64) or 2/2 (mixed), coarsening to 16 cells
+    per side with the exact DCT coarsest solve.  Dirichlet mixed: nu = 2/2, coarsening
+    to 16 with the exact DST coarsest solve.
+    """
+    nu = 2 if precision == "mixed" else 1
     if bc == NEUMANN:
-        return MultigridOptions(pre_smoothing=1, post_smoothing=1, coarsest_size=16)
+        return MultigridOptions(pre_smoothing=nu, post_smoothing=nu, coarsest_size=16)
+    if precision == "mixed":
+        return MultigridOptions(pre_smoothing=2, post_smoothing=2, coarsest_size=16,
+                                coarsest_sweeps=0)
     return MultigridOptions(pre_smoothing=1, post_smoothing=1)
 
 
@@ -114,7 +124,7 @@ def mgcg(
     history=None,
 ):
     """MG-preconditioned CG on ``-A u = -b`` (solvers.py:134-160 structure)."""
-    options = options or default_pcg_options(bc)
+    options = options or default_pcg_options(bc, precision)
     vdt = np.float32 if precision == "mixed" else np.float64
     b = np.asarray(b, dtype=np.float64)
     if bc == NEUMANN:

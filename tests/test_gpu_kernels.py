@@ -105,6 +105,21 @@ def test_dirichlet_vcycle_matches_oracle(gd, shape, nu):
     assert rel(got, want) < 1e-11
 
 
+@pytest.mark.parametrize("shape", [(63, 63), (254, 256), (127, 200)])
+@pytest.mark.parametrize("nu", [1, 2])
+def test_dirichlet_exact_coarsest_vcycle_matches_oracle(gd, shape, nu):
+    """coarsest_sweeps = 0: exact DST-I coarsest solve (the mixed solver's cycle)."""
+    rng = np.random.default_rng(16)
+    b, _ = G.manufactured_problem(shape, seed=6)
+    u = rng.standard_normal(shape)
+    kw = dict(pre_smoothing=nu, post_smoothing=nu, coarsest_size=32, coarsest_sweeps=0)
+    got = gd.v_cycle(u, b, 1.0, gd.MultigridOptions(**kw))
+    want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(**kw))
+    assert rel(got, want) < 1e-11
+    got32 = gd.v_cycle(u, b, 1.0, gd.MultigridOptions(**kw), precision="mixed")
+    assert rel(got32, want) < 1e-4
+
+
 @pytest.mark.parametrize("shape", [(64, 64), (255, 257), (135, 240), (300, 200)])
 def test_neumann_vcycle_matches_oracle(gd, shape):
     b, _ = G.neumann_problem(shape, seed=17)

@@ -209,6 +209,18 @@ def test_one_cycle_reduces_the_residual_by_an_order_of_magnitude():
     assert rep.residuals[1] < 0.1 * rep.residuals[0]
 
 
+def test_dst_coarsest_solve_is_exact():
+    rng = np.random.default_rng(31)
+    for shape, h in (((13, 17), 0.5), ((30, 30), 4.0), ((1, 5), 1.0)):
+        u = rng.standard_normal(shape)
+        assert np.abs(M.dst_dirichlet_solve(O.laplacian(u, h), h) - u).max() < 1e-11
+    # the exact-coarsest Dirichlet cycle: one cycle on a coarsest-only grid is exact
+    u = rng.standard_normal((20, 24))
+    b = O.laplacian(u)
+    opts = M.MultigridOptions(coarsest_size=32, coarsest_sweeps=0)
+    assert np.abs(M.v_cycle(np.zeros_like(b), b, 1.0, opts) - u).max() < 1e-11
+
+
 def test_neumann_solver_is_exact():
     rng = np.random.default_rng(8)
     for shape in ((64, 64), (97, 131)):
@@ -261,6 +273,8 @@ def test_neumann_level_chain():
     assert M.level_shapes((1024, 1024), opts, "neumann")[-1] == (16, 16)
     assert len(M.level_shapes((1024, 1024), opts, "neumann")) == 7
     assert M.level_shapes((2160, 3840), opts, "neumann")[-1] == (9, 15)
+    o32 = M.MultigridOptions(coarsest_size=32)
+    assert M.level_shapes((1024, 1024), o32, "neumann")[-1] == (32, 32)
     d = M.level_shapes((1022, 1022), M.MultigridOptions(), "dirichlet")
     assert [s[0] for s in d] == [1022, 510, 254, 126, 62, 30, 14, 6]
 
@@ -276,9 +290,11 @@ def test_neumann_mgcg_matches_dct_and_closed_form(shape):
 
 
 def test_mixed_precision_oracle_tracks_fp64():
+    """The same cycle in FP32 (FP64 outer PCG) needs at most one more iteration."""
     b, exact = G.neumann_problem((128, 160), seed=1002)
-    u64, it64, _ = S.mgcg(b, bc="neumann")
-    u32, it32, c32 = S.mgcg(b, bc="neumann", precision="mixed", flexible=True)
+    opts = S.default_pcg_options("neumann", "mixed")
+    u64, it64, _ = S.mgcg(b, bc="neumann", options=opts)
+    u32, it32, c32 = S.mgcg(b, bc="neumann", precision="mixed", flexible=True, options=opts)
     assert c32 and abs(it32 - it64) <= 1
     assert np.abs(u32 - exact).max() < 1e-6
 

@@ -119,11 +119,11 @@ struct TailDesc {
   int32_t coarse_off;     // FP64 coarsest working area (two arrays of the coarsest level)
   int32_t tab_off;        // FP64 DCT tables in LDS (Neumann)
   int32_t pad;
-  const double* dct;      // Neumann coarsest: Cm (m*m), Cn (n*n), 1/lambda (m*n)
+  const double* dct;      // exact coarsest solve: Cm (m*m), Cn (n*n), 1/lambda (m*n)
 };
 
 constexpr int kTailThreads = 1024;
-constexpr int kTailLdsBytes = 98304;
+constexpr int kTailLdsBytes = 155648;  // of the 160 KB of LDS a gfx950 workgroup may use
 constexpr int kMaxTailLevels = 16;
 
 }  // namespace gdp

@@ -134,6 +134,251 @@ __device__ inline void ptaps(bool neu, int i, int& ia, int& ib, T& wa, T& wb) {
   }
 }
 
+// ------------------------------------------------------------------ interior fast paths
+// A tile whose whole LDS region (halo included) lies strictly inside the grid needs no
+// bounds tests: every point has four in-grid neighbours (diag = 4 for Dirichlet and
+// Neumann alike) and the transfers never reach a boundary.  Threads map to the region
+// as (row = wave + 4 i, column = lane + 64 j), so no index arithmetic is divided.
+
+template <typename T, int FH, int FW>
+__device__ inline void interior_halfsweep(T* su, const T* sb, int s, int parity, int par0,
+                                          T h2) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lo = 1 + s, hr = FH - 1 - s, hc = FW - 1 - s;
+  constexpr int NRR = (FH + 3) / 4;
+  constexpr int NCC = (FW + 127) / 128;
+  T val[NRR][NCC];
+  int at[NRR][NCC];
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr) {
+    const int a = lo + wv + 4 * rr;
+    const int c0 = lo + ((parity + a + par0 + lo) & 1);
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int c = c0 + 2 * lane + 128 * cc;
+      const bool ok = a < hr && c < hc;
+      const int idx = ok ? a * FW + c : FW + 1;
+      at[rr][cc] = ok ? idx : -1;
+      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
+      val[rr][cc] = (nb - h2 * sb[idx]) * T(0.25);
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc)
+      if (at[rr][cc] >= 0) su[at[rr][cc]] = val[rr][cc];
+  __syncthreads();
+}
+
+template <typename T, int CH, int CW, int NU>
+__device__ void down_interior(const VTile& t, ProblemState* st, const double2* upart,
+                              int check_conv, T* su, T* sb) {
+  constexpr int OFF = 2 + 2 * NU;
+  constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
+  constexpr int NRR = (FH + 3) / 4, NCC = (FW + 63) / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gr0 = 2 * t.i0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const int pitch = t.pitch;
+  const int par0 = (gr0 + gc0) & 1;
+  const T h2 = T(t.h2), ih2 = T(t.inv_h2);
+  const T* __restrict__ b = static_cast<const T*>(t.b) + (size_t)gr0 * pitch + gc0;
+  T bl[NRR][NCC];
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr) {
+    const int a = wv + 4 * rr;
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int c = lane + 64 * cc;
+      const bool in = a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1;
+      bl[rr][cc] = in ? b[(size_t)a * pitch + c] : T(0);
+    }
+  }
+  if (st[t.problem].done) return;
+  if (check_conv && t.level == 0) {
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
+                                  shc, &bcc))
+      return;
+  }
+  // b into LDS; the first colour-0 half-sweep from u = 0 folded in
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr) {
+    const int a = wv + 4 * rr;
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int c = lane + 64 * cc;
+      if (a < FH && c < FW) {
+        const int k = a * FW + c;
+        const T bv = bl[rr][cc];
+        sb[k] = bv;
+        const bool upd = NU > 0 && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 &&
+                         ((a + c + par0) & 1) == 0;
+        su[k] = upd ? -h2 * bv * T(0.25) : T(0);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 1; s < 2 * NU; ++s) interior_halfsweep<T, FH, FW>(su, sb, s, s & 1, par0, h2);
+  // residual over fine rows / cols [fr0-1, fr0+2CH+2) in place of b; owned iterate out
+  constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;
+  constexpr int NR2 = (RRH + 3) / 4, NC2 = (RRW + 63) / 64;
+  T* __restrict__ v = static_cast<T*>(t.v) + (size_t)gr0 * pitch + gc0;
+  T rv[NR2][NC2];
+#pragma unroll
+  for (int rr = 0; rr < NR2; ++rr) {
+    const int a = OFF - 1 + wv + 4 * rr;
+#pragma unroll
+    for (int cc = 0; cc < NC2; ++cc) {
+      const int c = OFF - 1 + lane + 64 * cc;
+      const bool ok = a < OFF - 1 + RRH && c < OFF - 1 + RRW;
+      const int idx = ok ? a * FW + c : FW + 1;
+      const T uc = su[idx];
+      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
+      rv[rr][cc] = sb[idx] - (nb - T(4) * uc) * ih2;
+      if (ok && a >= OFF && a < OFF + 2 * CH && c >= OFF && c < OFF + 2 * CW)
+        v[(size_t)a * pitch + c] = uc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < NR2; ++rr) {
+    const int a = OFF - 1 + wv + 4 * rr;
+#pragma unroll
+    for (int cc = 0; cc < NC2; ++cc) {
+      const int c = OFF - 1 + lane + 64 * cc;
+      if (a < OFF - 1 + RRH && c < OFF - 1 + RRW) sb[a * FW + c] = rv[rr][cc];
+    }
+  }
+  __syncthreads();
+  const bool neu = t.bc == kNeumann;
+  T* __restrict__ cb = static_cast<T*>(t.cb);
+  for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
+    const int ci = k / CW, cj = k - ci * CW;
+    T acc;
+    if (neu) {
+      const T* r0 = &sb[(2 * ci + OFF - 1) * FW + 2 * cj + OFF - 1];
+      T col[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+        col[x] = T(0.25) * r0[x] + T(0.75) * r0[FW + x] + T(0.75) * r0[2 * FW + x] +
+                 T(0.25) * r0[3 * FW + x];
+      acc = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
+    } else {
+      const T* r0 = &sb[(2 * ci + OFF) * FW + 2 * cj + OFF];
+      T col[3];
+#pragma unroll
+      for (int x = 0; x < 3; ++x)
+        col[x] = T(0.5) * r0[x] + r0[FW + x] + T(0.5) * r0[2 * FW + x];
+      acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
+    }
+    cb[(size_t)(t.i0 + ci) * t.cpitch + t.j0 + cj] = T(0.25) * acc;
+  }
+}
+
+template <typename T, int TH, int TW, int NU, bool DOTS>
+__device__ void up_interior(const VTile& t, const ProblemState* st, double2* zpart, int order,
+                            T* su, T* sb, T* se, double* sh) {
+  constexpr int H = 2 * NU;
+  constexpr int RH = TH + 2 * H, RW = TW + 2 * H;
+  constexpr int CRH = RH / 2 + 4, CRW = RW / 2 + 4;
+  constexpr int NRR = (RH + 3) / 4, NCC = (RW + 63) / 64;
+  constexpr int NER = (CRH + 3) / 4, NEC = (CRW + 63) / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int gr0 = t.i0 - H, gc0 = t.j0 - H;
+  const int cr0 = (gr0 >> 1) - 1, cc0 = (gc0 >> 1) - 1;
+  const int pitch = t.pitch;
+  const int par0 = (gr0 + gc0) & 1;
+  const T h2 = T(t.h2);
+  const T* __restrict__ e = static_cast<const T*>(t.cu) + (size_t)cr0 * t.cpitch + cc0;
+  const T* __restrict__ vv = static_cast<const T*>(t.v) + (size_t)gr0 * pitch + gc0;
+  const T* __restrict__ b = static_cast<const T*>(t.b) + (size_t)gr0 * pitch + gc0;
+  T el[NER][NEC], vl[NRR][NCC], bl[NRR][NCC];
+#pragma unroll
+  for (int rr = 0; rr < NER; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NEC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      el[rr][cc] = (a < CRH && c < CRW) ? e[(size_t)a * t.cpitch + c] : T(0);
+    }
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      const bool in = a < RH && c < RW;
+      vl[rr][cc] = in ? vv[(size_t)a * pitch + c] : T(0);
+      bl[rr][cc] = in ? b[(size_t)a * pitch + c] : T(0);
+    }
+  if (st[t.problem].done) return;
+#pragma unroll
+  for (int rr = 0; rr < NER; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NEC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      if (a < CRH && c < CRW) se[a * CRW + c] = el[rr][cc];
+    }
+  __syncthreads();
+  const bool neu = t.bc == kNeumann;
+#pragma unroll
+  for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      if (a < RH && c < RW) {
+        int ia, ib, ja, jb;
+        T wa, wb, va, vb;
+        ptaps<T>(neu, gr0 + a, ia, ib, wa, wb);
+        ptaps<T>(neu, gc0 + c, ja, jb, va, vb);
+        const int oa = (ia - cr0) * CRW - cc0, ob = (ib - cr0) * CRW - cc0;
+        const T corr = va * (wa * se[oa + ja] + wb * se[ob + ja]) +
+                       vb * (wa * se[oa + jb] + wb * se[ob + jb]);
+        su[a * RW + c] = vl[rr][cc] + corr;
+        sb[a * RW + c] = bl[rr][cc];
+      }
+    }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 2 * NU; ++s) {
+    const int parity = order == 0 ? (s & 1) : 1 - (s & 1);
+    interior_halfsweep<T, RH, RW>(su, sb, s, parity, par0, h2);
+  }
+  T* __restrict__ uo = static_cast<T*>(t.u) + (size_t)t.i0 * pitch + t.j0;
+  constexpr int NOR = (TH + 3) / 4, NOC = (TW + 63) / 64;
+  double zr = 0.0, zq = 0.0;
+  double rl[DOTS ? NOR : 1][DOTS ? NOC : 1], ql[DOTS ? NOR : 1][DOTS ? NOC : 1];
+  if (DOTS) {
+    const double* kr = t.kr + (size_t)t.i0 * pitch + t.j0;
+    const double* kq = t.kq + (size_t)t.i0 * pitch + t.j0;
+#pragma unroll
+    for (int rr = 0; rr < NOR; ++rr)
+#pragma unroll
+      for (int cc = 0; cc < NOC; ++cc) {
+        const int a = wv + 4 * rr, c = lane + 64 * cc;
+        const bool in = a < TH && c < TW;
+        rl[rr][cc] = in ? kr[(size_t)a * pitch + c] : 0.0;
+        ql[rr][cc] = in ? kq[(size_t)a * pitch + c] : 0.0;
+      }
+  }
+#pragma unroll
+  for (int rr = 0; rr < NOR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NOC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      if (a < TH && c < TW) {
+        const T z = su[(a + H) * RW + c + H];
+        uo[(size_t)a * pitch + c] = z;
+        if (DOTS) {
+          zr += double(z) * rl[rr][cc];
+          zq += double(z) * ql[rr][cc];
+        }
+      }
+    }
+  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
+}
+
 // ------------------------------------------------------------------ down kernel
 // Pre-smoothing (NU sweeps from a zero guess) + residual + restriction, fused.
 // One block produces a CH x CW tile of the coarse right-hand side and the
@@ -163,6 +408,11 @@ __global__ __launch_bounds__(kThreads) void k_down(const VTile* __restrict__ til
   const bool neu = t.bc == kNeumann;
   const T h2 = T(t.h2), ih2 = T(t.inv_h2);
   const T* __restrict__ b = static_cast<const T*>(t.b);
+  if (gr0 >= 1 && gr0 + FH <= m - 1 && gc0 >= 1 && gc0 + FW <= n - 1 && t.i0 + CH < t.cm &&
+      t.j0 + CW < t.cn) {
+    down_interior<T, CH, CW, NU>(t, st, upart, check_conv, su, sb);
+    return;
+  }
 
   // load b; the first colour-0 half-sweep from u = 0 is folded into the load.  All of a
   // thread's loads are issued before any is consumed (fixed trip count, predicated).
@@ -335,6 +585,11 @@ __global__ __launch_bounds__(kThreads) void k_up(const VTile* __restrict__ tiles
   const T* __restrict__ vv = static_cast<const T*>(t.v);
   const T* __restrict__ b = static_cast<const T*>(t.b);
   const int Cm = t.cm, Cn = t.cn, Cp = t.cpitch;
+  if (gr0 >= 1 && gr0 + RH <= m - 1 && gc0 >= 1 && gc0 + RW <= n - 1 && cr0 >= 0 &&
+      cr0 + CRH <= Cm && cc0 >= 0 && cc0 + CRW <= Cn) {
+    up_interior<T, TH, TW, NU, DOTS>(t, st, zpart, order, su, sb, se, sh);
+    return;
+  }
   // Issue every global load of the block first: the coarse correction (with its
   // padding: zero outside for Dirichlet, edge values for Neumann) and the fine v and b.
   constexpr int NE = (CRH * CRW + kThreads - 1) / kThreads;
@@ -451,60 +706,33 @@ __global__ __launch_bounds__(kThreads) void k_up(const VTile* __restrict__ tiles
 // Loops run rows over waves and columns over lanes (no integer division).
 constexpr int kTailWaves = kTailThreads / 64;
 
-// The tail loops hand each thread up to kTailBatch points per pass and issue all of
-// their LDS loads before any store, so a pass costs about one LDS round trip.
-constexpr int kTailBatch = 4;
+// Tail loops put rows on waves and columns on lanes (no division), and a wave with no
+// row of the level branches over the whole pass, so a small level costs only the
+// waves it needs: on one CU every executed wave instruction costs the SIMD 4 cycles
+// whether or not its lanes are active.
 
 // Red-black half-sweep of colour `parity`; with `from_zero` the iterate is known to be
 // zero, so colour `parity` gets (-h^2 b)/diag and the other colour is cleared.
 template <typename T>
 __device__ void lds_halfsweep(T* u, const T* b, int m, int n, T h2, bool neu, int parity,
                               bool from_zero) {
-  if (from_zero) {
-    const int total = m * n;
-    for (int base = 0; base < total; base += kTailBatch * kTailThreads) {
-      T val[kTailBatch];
-#pragma unroll
-      for (int r = 0; r < kTailBatch; ++r) {
-        const int o = base + threadIdx.x + r * kTailThreads;
-        const int i = o / n, j = o - i * n;
-        val[r] = (o < total && ((i + j) & 1) == parity)
-                     ? smooth_value<T>(T(0), b[min(o, total - 1)], h2, neu, i, j, m, n)
-                     : T(0);
-      }
-#pragma unroll
-      for (int r = 0; r < kTailBatch; ++r) {
-        const int o = base + threadIdx.x + r * kTailThreads;
-        if (o < total) u[o] = val[r];
-      }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves) {
+    T* ur = u + i * n;
+    const T* br = b + i * n;
+    if (from_zero) {
+      for (int j = lane; j < n; j += 64)
+        ur[j] = ((i + j) & 1) == parity ? smooth_value<T>(T(0), br[j], h2, neu, i, j, m, n)
+                                        : T(0);
+      continue;
     }
-    __syncthreads();
-    return;
-  }
-  const int half = (n + 1) >> 1;
-  const int total = m * half;
-  for (int base = 0; base < total; base += kTailBatch * kTailThreads) {
-    T val[kTailBatch];
-    int oo[kTailBatch];
-#pragma unroll
-    for (int r = 0; r < kTailBatch; ++r) {
-      const int k = base + threadIdx.x + r * kTailThreads;
-      const int i = k / half;
-      const int j = 2 * (k - i * half) + ((i + parity) & 1);
-      const bool ok = k < total && j < n;
-      const int o = ok ? i * n + j : -1;
-      oo[r] = o;
-      if (ok) {
-        const T up = i > 0 ? u[o - n] : T(0);
-        const T dn = i < m - 1 ? u[o + n] : T(0);
-        const T lf = j > 0 ? u[o - 1] : T(0);
-        const T rt = j < n - 1 ? u[o + 1] : T(0);
-        val[r] = smooth_value<T>(((up + dn) + lf) + rt, b[o], h2, neu, i, j, m, n);
-      }
+    for (int j = ((i + parity) & 1) + 2 * lane; j < n; j += 128) {
+      const T up = i > 0 ? ur[j - n] : T(0);
+      const T dn = i < m - 1 ? ur[j + n] : T(0);
+      const T lf = j > 0 ? ur[j - 1] : T(0);
+      const T rt = j < n - 1 ? ur[j + 1] : T(0);
+      ur[j] = smooth_value<T>(((up + dn) + lf) + rt, br[j], h2, neu, i, j, m, n);
     }
-#pragma unroll
-    for (int r = 0; r < kTailBatch; ++r)
-      if (oo[r] >= 0) u[oo[r]] = val[r];
   }
   __syncthreads();
 }
@@ -512,15 +740,16 @@ __device__ void lds_halfsweep(T* u, const T* b, int m, int n, T h2, bool neu, in
 // r = b - A u (r may not alias u or b)
 template <typename T>
 __device__ void lds_residual(const T* u, const T* b, T* r, int m, int n, T ih2, bool neu) {
-  const int total = m * n;
-  for (int o = threadIdx.x; o < total; o += kTailThreads) {
-    const int i = o / n, j = o - i * n;
-    const T up = i > 0 ? u[o - n] : T(0);
-    const T dn = i < m - 1 ? u[o + n] : T(0);
-    const T lf = j > 0 ? u[o - 1] : T(0);
-    const T rt = j < n - 1 ? u[o + 1] : T(0);
-    r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
-  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves)
+    for (int j = lane; j < n; j += 64) {
+      const int o = i * n + j;
+      const T up = i > 0 ? u[o - n] : T(0);
+      const T dn = i < m - 1 ? u[o + n] : T(0);
+      const T lf = j > 0 ? u[o - 1] : T(0);
+      const T rt = j < n - 1 ? u[o + 1] : T(0);
+      r[o] = b[o] - ((((up + dn) + lf) + rt) - diag_of<T>(neu, i, j, m, n) * u[o]) * ih2;
+    }
   __syncthreads();
 }
 
@@ -537,41 +766,42 @@ __device__ inline int nsrc(int i, int f, bool& ok) {
 
 template <typename T>
 __device__ void lds_restrict(const T* r, int m, int n, T* bc, int mc, int nc, bool neu) {
-  const int total = mc * nc;
-  for (int o = threadIdx.x; o < total; o += kTailThreads) {
-    const int I = o / nc, J = o - I * nc;
-    T acc = T(0);
-    if (neu) {
-      const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
-      int ii[4], jj[4];
-      bool oi[4], oj[4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int I = wave; I < mc; I += kTailWaves)
+    for (int J = lane; J < nc; J += 64) {
+      T acc = T(0);
+      if (neu) {
+        const T w[4] = {T(0.25), T(0.75), T(0.75), T(0.25)};
+        int ii[4], jj[4];
+        bool oi[4], oj[4];
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        oi[x] = true;
-        oj[x] = true;
-        ii[x] = nsrc(2 * I - 1 + x, m, oi[x]);
-        jj[x] = nsrc(2 * J - 1 + x, n, oj[x]);
+        for (int x = 0; x < 4; ++x) {
+          oi[x] = true;
+          oj[x] = true;
+          ii[x] = nsrc(2 * I - 1 + x, m, oi[x]);
+          jj[x] = nsrc(2 * J - 1 + x, n, oj[x]);
+        }
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          T col = T(0);
+#pragma unroll
+          for (int x = 0; x < 4; ++x)
+            col += (oi[x] && oj[y]) ? w[x] * r[ii[x] * n + jj[y]] : T(0);
+          acc += w[y] * col;
+        }
+      } else {
+        const T w[3] = {T(0.5), T(1), T(0.5)};
+#pragma unroll
+        for (int y = 0; y < 3; ++y) {
+          const int j = 2 * J + y;
+          T col = T(0);
+#pragma unroll
+          for (int x = 0; x < 3; ++x) col += w[x] * r[(2 * I + x) * n + j];
+          acc += w[y] * col;
+        }
       }
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        T col = T(0);
-#pragma unroll
-        for (int x = 0; x < 4; ++x) col += (oi[x] && oj[y]) ? w[x] * r[ii[x] * n + jj[y]] : T(0);
-        acc += w[y] * col;
-      }
-    } else {
-      const T w[3] = {T(0.5), T(1), T(0.5)};
-#pragma unroll
-      for (int y = 0; y < 3; ++y) {
-        const int j = 2 * J + y;
-        T col = T(0);
-#pragma unroll
-        for (int x = 0; x < 3; ++x) col += w[x] * r[(2 * I + x) * n + j];
-        acc += w[y] * col;
-      }
+      bc[I * nc + J] = T(0.25) * acc;
     }
-    bc[o] = T(0.25) * acc;
-  }
   __syncthreads();
 }
 
@@ -587,58 +817,88 @@ __device__ inline T lds_coarse(const T* e, int mc, int nc, int I, int J, bool ne
 
 template <typename T>
 __device__ void lds_prolong_add(T* u, int m, int n, const T* e, int mc, int nc, bool neu) {
-  const int total = m * n;
-  for (int o = threadIdx.x; o < total; o += kTailThreads) {
-    const int i = o / n, j = o - i * n;
-    int ia, ib, ja, jb;
-    T wa, wb, va, vb;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = wave; i < m; i += kTailWaves) {
+    int ia, ib;
+    T wa, wb;
     ptaps<T>(neu, i, ia, ib, wa, wb);
-    ptaps<T>(neu, j, ja, jb, va, vb);
-    const T corr = va * (wa * lds_coarse(e, mc, nc, ia, ja, neu) +
-                         wb * lds_coarse(e, mc, nc, ib, ja, neu)) +
-                   vb * (wa * lds_coarse(e, mc, nc, ia, jb, neu) +
-                         wb * lds_coarse(e, mc, nc, ib, jb, neu));
-    u[o] += corr;
+    for (int j = lane; j < n; j += 64) {
+      int ja, jb;
+      T va, vb;
+      ptaps<T>(neu, j, ja, jb, va, vb);
+      const T corr = va * (wa * lds_coarse(e, mc, nc, ia, ja, neu) +
+                           wb * lds_coarse(e, mc, nc, ib, ja, neu)) +
+                     vb * (wa * lds_coarse(e, mc, nc, ia, jb, neu) +
+                           wb * lds_coarse(e, mc, nc, ib, jb, neu));
+      u[i * n + j] += corr;
+    }
   }
   __syncthreads();
 }
 
-// One pass of the DCT solve: out[a][c] = sum_k A(a, c, k) with G lanes per output
-// splitting k (interleaved) and combined by shuffles, so the four small matrix
-// products of a 16 x 16 solve use every thread of the block.
-template <int G, typename F>
-__device__ inline void dct_pass(int N, int K, F term, double* out, const double* scale) {
-  const int lane = threadIdx.x % G;
-  for (int base = 0; base < N * G; base += kTailThreads) {
-    const int t = base + threadIdx.x;
-    const int o = t / G;
-    double s = 0.0;
-    if (o < N)
-      for (int k = lane; k < K; k += G) s += term(o, k);
-#pragma unroll
-    for (int w = G / 2; w > 0; w >>= 1) s += __shfl_xor(s, w, 64);
-    if (o < N && lane == 0) out[o] = scale ? s * scale[o] : s;
-  }
-  __syncthreads();
-}
-
-// Exact zero-mean solve of the Neumann coarsest level by the orthonormal DCT-II
-// (the solvers.py:224-262 construction): U = Cm^T ((Cm B Cn^T) ./ lambda) Cn.
-// Tables live in LDS: Cm (m*m), Cn (n*n), 1/lambda (m*n, zero for the constant mode).
+// Exact solve of the coarsest level by an orthonormal transform that diagonalises its
+// operator: U = Cm^T ((Cm B Cn^T) ./ lambda) Cn.  Neumann: DCT-II (the solvers.py:224-262
+// construction, zero-mean solution: 1/lambda of the constant mode is 0).  Dirichlet:
+// DST-I.  Tables live in LDS: Cm (m*m), Cn (n*n), 1/lambda (m*n).  Each pass puts
+// output rows on waves and output columns on lanes.
 __device__ void lds_dct_solve(double* w0, double* w1, int m, int n, const double* tab) {
   const double* Cm = tab;
   const double* Cn = tab + m * m;
   const double* il = tab + m * m + n * n;
-  const int N = m * n;
-  // w1 = Cm w0 ; w0 = (w1 Cn^T) ./ lambda ; w1 = Cm^T w0 ; w0 = w1 Cn
-  dct_pass<4>(N, m, [&](int o, int i) { const int k = o / n, j = o - k * n;
-                                        return Cm[k * m + i] * w0[i * n + j]; }, w1, nullptr);
-  dct_pass<4>(N, n, [&](int o, int j) { const int k = o / n, l = o - k * n;
-                                        return w1[k * n + j] * Cn[l * n + j]; }, w0, il);
-  dct_pass<4>(N, m, [&](int o, int k) { const int i = o / n, l = o - i * n;
-                                        return Cm[k * m + i] * w0[k * n + l]; }, w1, nullptr);
-  dct_pass<4>(N, n, [&](int o, int l) { const int i = o / n, j = o - i * n;
-                                        return w1[i * n + l] * Cn[l * n + j]; }, w0, nullptr);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = wave; k < m; k += kTailWaves)  // w1 = Cm w0
+    for (int j = lane; j < n; j += 64) {
+      double s = 0.0;
+      for (int i = 0; i < m; ++i) s += Cm[k * m + i] * w0[i * n + j];
+      w1[k * n + j] = s;
+    }
+  __syncthreads();
+  for (int k = wave; k < m; k += kTailWaves)  // w0 = (w1 Cn^T) ./ lambda
+    for (int l = lane; l < n; l += 64) {
+      double s = 0.0;
+      for (int j = 0; j < n; ++j) s += w1[k * n + j] * Cn[l * n + j];
+      w0[k * n + l] = s * il[k * n + l];
+    }
+  __syncthreads();
+  for (int i = wave; i < m; i += kTailWaves)  // w1 = Cm^T w0
+    for (int l = lane; l < n; l += 64) {
+      double s = 0.0;
+      for (int k = 0; k < m; ++k) s += Cm[k * m + i] * w0[k * n + l];
+      w1[i * n + l] = s;
+    }
+  __syncthreads();
+  for (int i = wave; i < m; i += kTailWaves)  // w0 = w1 Cn
+    for (int j = lane; j < n; j += 64) {
+      double s = 0.0;
+      for (int l = 0; l < n; ++l) s += w1[i * n + l] * Cn[l * n + j];
+      w0[i * n + j] = s;
+    }
+  __syncthreads();
+}
+
+// Many sweeps on a level of at most 64 x 8 points: one wave does them all, keeping its
+// points in registers and exchanging through LDS with wave-level ordering only (LDS
+// operations of a wave complete in order), then the block synchronises once.
+__device__ void wave_sweeps(double* u, const double* b, int m, int n, double h2, int sweeps) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    for (int s = 0; s < sweeps; ++s)
+      for (int parity = 0; parity < 2; ++parity) {
+        for (int o = lane; o < m * n; o += 64) {
+          const int i = o / n, j = o - i * n;
+          if (((i + j) & 1) != parity) continue;
+          const double up = i > 0 ? u[o - n] : 0.0;
+          const double dn = i < m - 1 ? u[o + n] : 0.0;
+          const double lf = j > 0 ? u[o - 1] : 0.0;
+          const double rt = j < n - 1 ? u[o + 1] : 0.0;
+          u[o] = ((((up + dn) + lf) + rt) - h2 * b[o]) * 0.25;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+  }
+  __syncthreads();
 }
 
 template <typename T, bool DOTS>
@@ -720,7 +980,8 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
     double* w0 = reinterpret_cast<double*>(smem + D.coarse_off);
     double* w1 = w0 + L.m * L.n;
     const int N = L.m * L.n;
-    if (L.bc == kNeumann) {
+    if (D.dct) {
+      // exact: DCT-II (Neumann) or DST-I (Dirichlet, coarsest_sweeps == 0) diagonalisation
       for (int o = threadIdx.x; o < N; o += kTailThreads) w0[o] = double(b[o]) * L.h2;
       __syncthreads();
       lds_dct_solve(w0, w1, L.m, L.n, reinterpret_cast<const double*>(smem + D.tab_off));
@@ -730,9 +991,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
         w1[o] = double(b[o]);
       }
       __syncthreads();
-      for (int s = 0; s < coarsest_sweeps; ++s) {
-        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 0, false);
-        lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 1, false);
+      if (N <= 512) {
+        wave_sweeps(w0, w1, L.m, L.n, L.h2, coarsest_sweeps);
+      } else {
+        for (int s = 0; s < coarsest_sweeps; ++s) {
+          lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 0, false);
+          lds_halfsweep<double>(w0, w1, L.m, L.n, L.h2, false, 1, false);
+        }
       }
     }
     for (int o = threadIdx.x; o < N; o += kTailThreads) u[o] = T(w0[o]);
@@ -793,28 +1058,32 @@ __global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ ti
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
   constexpr int RH = TH + 2, RW = TW + 2;
+  constexpr int NRR = (RH + 3) / 4, NCC = (RW + 63) / 64;
   __shared__ double sd[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int m = t.m, n = t.n, pitch = t.pitch;
+  const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
+  // the region (halo included) inside the grid: no tests, every diagonal is 4
+  const bool interior = gr0 >= 0 && gr0 + RH <= m && gc0 >= 0 && gc0 + RW <= n;
   const T* __restrict__ z = static_cast<const T*>(t.z);
   const double* __restrict__ dold = din ? t.d1 : t.d0;
   double* __restrict__ dnew = din ? t.d0 : t.d1;
-  const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
   // vector loads first; the scalar reduction below overlaps them
-  constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
-  T zl[NR];
-  double dl[NR];
+  T zl[NRR][NCC];
+  double dl[NRR][NCC];
 #pragma unroll
-  for (int it = 0; it < NR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / RW, c = k - a * RW;
-    const int gi = gr0 + a, gj = gc0 + c;
-    const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
-    const size_t o = in ? (size_t)gi * pitch + gj : 0;
-    zl[it] = in ? z[o] : T(0);
-    dl[it] = in ? dold[o] : 0.0;
-  }
+  for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      const int gi = gr0 + a, gj = gc0 + c;
+      const bool in = a < RH && c < RW && (interior || (gi >= 0 && gi < m && gj >= 0 && gj < n));
+      const size_t o = in ? (size_t)gi * pitch + gj : 0;
+      zl[rr][cc] = in ? z[o] : T(0);
+      dl[rr][cc] = in ? dold[o] : 0.0;
+    }
   if (st[p].done) return;
   const double2 z2 = reduce_slots<kThreads>(
       zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
@@ -827,28 +1096,33 @@ __global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ ti
     st[p].beta = beta;
   }
 #pragma unroll
-  for (int it = 0; it < NR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    if (k < RH * RW) sd[k] = double(zl[it]) + beta * dl[it];
-  }
+  for (int rr = 0; rr < NRR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NCC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      if (a < RH && c < RW) sd[a * RW + c] = double(zl[rr][cc]) + beta * dl[rr][cc];
+    }
   __syncthreads();
   const bool neu = t.bc == kNeumann;
+  constexpr int NOR = (TH + 3) / 4, NOC = (TW + 63) / 64;
   double acc = 0.0;
 #pragma unroll
-  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / TW, c = k - a * TW;
-    const int gi = t.i0 + a, gj = t.j0 + c;
-    if (gi >= m || gj >= n) continue;
-    const int idx = (a + 1) * RW + c + 1;
-    const double dc = sd[idx];
-    const double nb = ((sd[idx - RW] + sd[idx + RW]) + sd[idx - 1]) + sd[idx + 1];
-    const double q = -(nb - diag_of<double>(neu, gi, gj, m, n) * dc) * t.inv_h2;
-    const size_t o = (size_t)gi * pitch + gj;
-    dnew[o] = dc;
-    t.q[o] = q;
-    acc += dc * q;
-  }
+  for (int rr = 0; rr < NOR; ++rr)
+#pragma unroll
+    for (int cc = 0; cc < NOC; ++cc) {
+      const int a = wv + 4 * rr, c = lane + 64 * cc;
+      const int gi = t.i0 + a, gj = t.j0 + c;
+      if (a >= TH || c >= TW || gi >= m || gj >= n) continue;
+      const int idx = (a + 1) * RW + c + 1;
+      const double dc = sd[idx];
+      const double nb = ((sd[idx - RW] + sd[idx + RW]) + sd[idx - 1]) + sd[idx + 1];
+      const double dg = interior ? 4.0 : diag_of<double>(neu, gi, gj, m, n);
+      const double q = -(nb - dg * dc) * t.inv_h2;
+      const size_t o = (size_t)gi * pitch + gj;
+      dnew[o] = dc;
+      t.q[o] = q;
+      acc += dc * q;
+    }
   write_partial<kThreads>(acc, 0.0, apart, t.slot, sh);
 }
 

@@ -157,6 +157,7 @@ struct LevelInfo {
 
 struct ProbInfo {
   gdp_problem p;
+  bool exact = false;  // exact transform solve on the coarsest level
   std::vector<LevelInfo> lv;
   int tail_first = 0;
   int tail_bytes = 0;
@@ -282,7 +283,7 @@ size_t tail_layout(ProbInfo& pi, int first, size_t tsize) {
   const size_t cm = pi.lv[L - 1].m, cn = pi.lv[L - 1].n;
   off += 2 * cm * cn * 8;
   pi.tab_off = int(off);
-  if (pi.p.boundary == GDP_NEUMANN) off += (cm * cm + cn * cn + cm * cn) * 8;
+  if (pi.exact) off += (cm * cm + cn * cn + cm * cn) * 8;
   return off;
 }
 
@@ -377,6 +378,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
     ProbInfo pi;
     pi.p = pr;
+    pi.exact = pr.boundary == GDP_NEUMANN || o.coarsest_sweeps == 0;
     auto shapes = level_shapes(pr, o);
     double h = pr.spacing;
     for (auto& s : shapes) {
@@ -440,8 +442,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     pi.off_d1 = take(k);
     pi.off_q = take(k);
     const auto& c = pi.lv.back();
-    if (pi.p.boundary == GDP_NEUMANN)
-      pi.off_dct = take(size_t(c.m * c.m + c.n * c.n + c.m * c.n) * 8);
+    if (pi.exact) pi.off_dct = take(size_t(c.m * c.m + c.n * c.n + c.m * c.n) * 8);
   }
   const int P_ = count;
   P->off_levels = take(sizeof(LevelDesc) * lvl_total);
@@ -624,14 +625,21 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     t.coarse_off = pi.coarse_off;
     t.tab_off = pi.tab_off;
     t.dct = nullptr;
-    if (pi.p.boundary == GDP_NEUMANN) {
+    if (pi.exact) {
       const int m = pi.lv.back().m, n = pi.lv.back().n;
       std::vector<double> tab(size_t(m * m + n * n + m * n));
-      auto fill = [](double* C, int s) {
+      const bool neu = pi.p.boundary == GDP_NEUMANN;
+      // orthonormal DCT-II (Neumann) or DST-I (Dirichlet) matrices, row k = mode k
+      auto fill = [neu](double* C, int s) {
         for (int k = 0; k < s; ++k)
           for (int i = 0; i < s; ++i) {
-            const double sc = k == 0 ? std::sqrt(1.0 / s) : std::sqrt(2.0 / s);
-            C[k * s + i] = sc * std::cos(M_PI * (2.0 * i + 1.0) * k / (2.0 * s));
+            if (neu) {
+              const double sc = k == 0 ? std::sqrt(1.0 / s) : std::sqrt(2.0 / s);
+              C[k * s + i] = sc * std::cos(M_PI * (2.0 * i + 1.0) * k / (2.0 * s));
+            } else {
+              C[k * s + i] = std::sqrt(2.0 / (s + 1)) *
+                             std::sin(M_PI * (i + 1.0) * (k + 1.0) / (s + 1.0));
+            }
           }
       };
       fill(tab.data(), m);
@@ -639,8 +647,13 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       double* il = tab.data() + m * m + n * n;
       for (int k = 0; k < m; ++k)
         for (int l = 0; l < n; ++l) {
-          const double lam = 2.0 * std::cos(M_PI * k / m) + 2.0 * std::cos(M_PI * l / n) - 4.0;
-          il[k * n + l] = (k == 0 && l == 0) ? 0.0 : 1.0 / lam;
+          double lam;
+          if (neu)
+            lam = 2.0 * std::cos(M_PI * k / m) + 2.0 * std::cos(M_PI * l / n) - 4.0;
+          else
+            lam = 2.0 * std::cos(M_PI * (k + 1.0) / (m + 1.0)) +
+                  2.0 * std::cos(M_PI * (l + 1.0) / (n + 1.0)) - 4.0;
+          il[k * n + l] = (neu && k == 0 && l == 0) ? 0.0 : 1.0 / lam;
         }
       t.dct = P->at<double>(pi.off_dct);
       GDP_HIP(hipMemcpy(P->ws + pi.off_dct, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));

@@ -25,10 +25,15 @@ class PlanOptions:
     """Solver knobs (``MultigridOptions`` of multigrid.py:46-71 plus precision etc.)."""
 
     precision: str = "fp64"
-    pre_smoothing: int = 1
-    post_smoothing: int = 1
-    coarsest_size: int | None = None  # default: 8 (Dirichlet), 16 (Neumann)
-    coarsest_sweeps: int = 60
+    # default: 1/1 for FP64 (the reference mgcg cycle, solvers.py:137); 2/2 for mixed,
+    # where the extra sweeps run in LDS and cut the PCG iterations from 9 to 6
+    pre_smoothing: int | None = None
+    post_smoothing: int | None = None
+    # defaults: Neumann coarsens to 16 cells with the exact DCT coarsest solve; Dirichlet
+    # FP64 keeps the reference (coarsest <= 8, 60 sweeps, multigrid.py:66-69); Dirichlet
+    # mixed coarsens to 16 with the exact DST coarsest solve (coarsest_sweeps = 0)
+    coarsest_size: int | None = None
+    coarsest_sweeps: int | None = None
     max_iterations: int = 2000
     tolerance: float = 1e-8
     flexible: bool | None = None  # default: Polak-Ribiere for mixed, Fletcher-Reeves for fp64
@@ -73,18 +78,26 @@ class Plan:
         probs = (_lib.Problem * n)()
         for i, ((m, nn), b, h) in enumerate(zip(self.shapes, bcs, hs)):
             probs[i] = _lib.Problem(m, nn, BOUNDARY[b], 0, h)
+        mixed = opts.precision == "mixed"
         cs = opts.coarsest_size
         if cs is None:
-            cs = 16 if bcs[0] == "neumann" else 8
+            cs = 16 if (bcs[0] == "neumann" or mixed) else 8
+        sweeps = opts.coarsest_sweeps
+        if sweeps is None:
+            sweeps = 0 if (mixed and bcs[0] == "dirichlet") else 60
         flex = opts.flexible
         if flex is None:
             flex = opts.precision == "mixed"
+        nu_default = 2 if opts.precision == "mixed" else 1
+        pre = nu_default if opts.pre_smoothing is None else opts.pre_smoothing
+        post = nu_default if opts.post_smoothing is None else opts.post_smoothing
+        self.smoothing = (pre, post)
         copt = _lib.Options(
             PRECISION[opts.precision],
-            opts.pre_smoothing,
-            opts.post_smoothing,
+            pre,
+            post,
             cs,
-            opts.coarsest_sweeps,
+            sweeps,
             opts.max_iterations,
             int(bool(flex)),
             int(bool(opts.use_graph)),

@@ -50,10 +50,6 @@ class SolverReport:
 def _pcg_options(precision, tolerance, max_iterations, boundary):
     return PlanOptions(
         precision=precision,
-        pre_smoothing=1,
-        post_smoothing=1,
-        coarsest_size=16 if boundary == "neumann" else 8,
-        coarsest_sweeps=60,
         max_iterations=int(max_iterations),
         tolerance=float(tolerance),
     )

@@ -102,6 +102,71 @@ def cpu_baseline():
     }
 
 
+COPY_BYTES = 64 << 23  # the --profile-run calibration copy: 512 MiB read + 512 MiB written
+
+
+def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
+    """The plan profiler's kind name of a rocprofv3 dispatch of this workload.
+
+    Levels are ranked by grid size per kernel (largest grid = finest level).
+    """
+    base = name.split("(")[0]
+    for key, kind in (("k_down<", "down"), ("k_up<", "up"), ("k_tail<", "tail"),
+                      ("k_apply<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
+                      ("k_residual<", "residual"), ("k_finalize<", "finalize"),
+                      ("k_init_stats<", "init_stats")):
+        if key in base:
+            if kind in ("down", "up"):
+                dots = kind == "up" and ", true>" in base
+                lv = sorted(grids[key], reverse=True).index(grid)
+                return f"{kind} L{lv}" + ("+dots" if dots else "")
+            return kind
+    return None
+
+
+def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
+    """Per-kernel HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch).
+
+    Each counter is scaled by the ratio of the known bytes of the plain device copy in
+    the same run (__amd_rocclr_copyBuffer of 512 MiB) to what the counter reads for it.
+    Dispatches that exited early (converged problems) are left out: the mean is over
+    dispatches whose count is at least half the kernel's maximum.
+    """
+    import csv
+    import collections
+
+    out = {"calibration": {}, "kernels": {}}
+    per = collections.defaultdict(dict)
+    for counter, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+        copy = [float(r["Counter_Value"]) * 1024 for r in rows
+                if "copyBuffer" in r["Kernel_Name"] and int(r["Grid_Size"]) >= 1 << 16]
+        if not copy:
+            raise SystemExit(f"{path}: no calibration copy dispatch found")
+        scale = COPY_BYTES / max(copy)
+        out["calibration"][counter] = {"copy_bytes": COPY_BYTES, "counter_bytes": max(copy),
+                                       "scale": round(scale, 4)}
+        grids = collections.defaultdict(set)
+        for r in rows:
+            base = r["Kernel_Name"].split("(")[0]
+            for key in ("k_down<", "k_up<"):
+                if key in base:
+                    grids[key].add(int(r["Grid_Size"]))
+        vals = collections.defaultdict(list)
+        for r in rows:
+            kind = kernel_kind(r["Kernel_Name"], int(r["Grid_Size"]), grids)
+            if kind:
+                vals[kind].append(float(r["Counter_Value"]) * 1024 * scale)
+        for kind, v in vals.items():
+            full = [x for x in v if x >= 0.5 * max(v)] or v
+            per[kind][counter] = sum(full) / len(full)
+    for kind, d in per.items():
+        f, w = d.get("FETCH_SIZE", 0.0), d.get("WRITE_SIZE", 0.0)
+        out["kernels"][kind] = {"fetch_bytes": round(f), "write_bytes": round(w),
+                                "hbm_bytes_per_launch": round(f + w)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,10 +175,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="reduce two rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE counter_collection "
+                    "CSVs of a --profile-run into per-kernel HBM bytes per launch (JSON on stdout)")
     ap.add_argument("--profile-run", action="store_true",
                     help="only run a few mixed solves and a plain device copy of 512 MiB "
                     "(the FETCH_SIZE / WRITE_SIZE calibration) for rocprofv3; no JSON line")
     args = ap.parse_args()
+    if args.traffic:
+        print(json.dumps(reduce_traffic(*args.traffic), indent=1))
+        return
     if args.gpus != 1:
         print(
             f"bench.py measures one GPU only (got --gpus {args.gpus}): the Poisson path has "

@@ -101,8 +101,10 @@ template <typename T>
 __device__ inline T smooth_value(T nb, T b, T h2, bool neu, int gi, int gj, int m, int n) {
   const T val = nb - h2 * b;
   if (neu) {
+    // 1/count by selection: a true division expands to a long scaled-division sequence
     const int cnt = neumann_count(gi, gj, m, n);
-    return cnt == 4 ? val * T(0.25) : val / T(cnt);
+    const T inv = cnt == 4 ? T(0.25) : (cnt == 3 ? T(1.0 / 3.0) : (cnt == 2 ? T(0.5) : T(1)));
+    return val * inv;
   }
   return val * T(0.25);
 }

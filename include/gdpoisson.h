@@ -128,7 +128,9 @@ int gdp_plan_level_shape(const gdp_plan* plan, int problem, int level, int* m, i
 /* Tile shapes: kind 0 = down (coarse tile), 1 = up (fine tile), 2 = Krylov (fine tile).
    Must be called before gdp_plan_bind; index into the compiled menu (see gdp_tile_menu). */
 int gdp_plan_set_tile(gdp_plan* plan, int kind, int level, int menu_index);
-int gdp_tile_menu(int kind, int index, int* th, int* tw);
+/* Block tile (rows th x columns tw) of a menu entry for a kernel with nu sweeps (the
+   down and up strips get narrower as nu grows; the Krylov tiles ignore nu). */
+int gdp_tile_menu(int kind, int index, int nu, int* th, int* tw);
 
 /* u_out[i] = v_cycle(u_in[i], b[i]) for every problem (arrays of device pointers, one
    per problem, each a dense (m, n) FP64 grid).  u_out may alias u_in. */

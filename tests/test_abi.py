@@ -128,6 +128,14 @@ def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
     for kind in range(3):
         for idx in range(4):
-            assert L.lib().gdp_tile_menu(kind, idx, ctypes.byref(th), ctypes.byref(tw)) == 0
-            assert th.value > 0 and tw.value % 32 == 0
-    assert L.lib().gdp_tile_menu(3, 0, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
+            for nu in range(4):
+                assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
+                                             ctypes.byref(tw)) == 0
+                assert th.value > 0 and th.value % 4 == 0 and tw.value > 0
+    # strips are 128 fine columns wide including their margins
+    L.lib().gdp_tile_menu(0, 0, 2, ctypes.byref(th), ctypes.byref(tw))
+    assert tw.value == 58
+    L.lib().gdp_tile_menu(1, 0, 2, ctypes.byref(th), ctypes.byref(tw))
+    assert tw.value == 116
+    for bad in ((3, 0, 1), (0, 4, 1), (0, 0, 4)):
+        assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL

@@ -120,6 +120,22 @@ def test_dirichlet_exact_coarsest_vcycle_matches_oracle(gd, shape, nu):
     assert rel(got32, want) < 1e-4
 
 
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("nu", [0, 1, 2, 3])
+def test_vcycle_across_many_strips_matches_oracle(gd, bc, nu):
+    """A grid several strips wide and tall on every strip level, for each sweep count."""
+    shape = (263, 391)
+    rng = np.random.default_rng(40 + nu)
+    b = rng.standard_normal(shape)
+    u = rng.standard_normal(shape)
+    kw = dict(pre_smoothing=nu, post_smoothing=nu, coarsest_size=16, coarsest_sweeps=0)
+    if bc == "dirichlet":
+        kw = dict(pre_smoothing=nu, post_smoothing=nu)
+    got = gd.v_cycle(u, b, 1.0, gd.MultigridOptions(**kw), boundary=bc)
+    want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(**kw), bc)
+    assert rel(got, want) < 1e-11
+
+
 @pytest.mark.parametrize("shape", [(64, 64), (255, 257), (135, 240), (300, 200)])
 def test_neumann_vcycle_matches_oracle(gd, shape):
     b, _ = G.neumann_problem(shape, seed=17)
@@ -190,3 +206,35 @@ def test_neumann_mgcg_matches_closed_form(gd, precision, shape):
     assert abs(u.mean()) < 1e-10
     assert np.abs(u - exact).max() < 1e-6
     assert np.abs(u - ou).max() < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(96, 130), (64, 64)])
+def test_solve_neumann_matches_the_dct_oracle(gd, shape):
+    """solvers.py:224-262 contract: zero-mean solution, constant part of b discarded."""
+    b, _ = G.neumann_problem(shape, seed=31)
+    b = b + 0.5  # a constant component, which the solve must discard
+    u = gd.solve_neumann(b)
+    assert abs(float(u.mean())) < 1e-12
+    assert np.abs(u - OS.solve_neumann(b)).max() < 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_uneven_problems_match_each_alone(gd, precision):
+    """Uneven shapes in one plan (the batched configuration at test sizes): every problem
+    gives bit-for-bit the result it gives alone, and meets the closed form."""
+    import torch
+
+    shapes = G.batch_shapes(count=9, seed=4000, low=24, high=300)
+    bs, exact = [], []
+    for k, s in enumerate(shapes):
+        b, x = G.neumann_problem(s, seed=4100 + k)
+        bs.append(torch.from_numpy(b).cuda())
+        exact.append(x)
+    opts = gd.PlanOptions(precision=precision)
+    together, reps = gd.Plan(shapes, boundary="neumann", options=opts).mgcg(bs)
+    for k, s in enumerate(shapes):
+        alone, rep1 = gd.Plan([s], boundary="neumann", options=opts).mgcg([bs[k]])
+        assert reps[k].converged and reps[k].relative_residual <= 1e-8
+        assert reps[k].iterations == rep1[0].iterations
+        assert torch.equal(together[k], alone[0]), (k, s)
+        assert np.abs(together[k].cpu().numpy() - exact[k]).max() < 1e-6

@@ -2,7 +2,52 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "gdp_device.h"
+
 namespace gdp {
+
+// Launches of the strip kernels of the tiled multigrid levels (gdp_sdown_*.hip,
+// gdp_sup_*.hip).  menu: strip rows (kDownRows / kUpRows index), nu: sweeps.
+struct DownArgs {
+  const VTile* tiles;
+  ProblemState* st;
+  const double2* upart;
+  int check_conv;
+};
+struct UpArgs {
+  const VTile* tiles;
+  const ProblemState* st;
+  double2* zpart;
+  int order;
+};
+template <typename T>
+void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a);
+template <typename T>
+void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a);
+
+// The register-strip coarse tail (gdp_stail_*.hip): one 1024-thread block per problem.
+struct TailArgs {
+  const TailDesc* tails;
+  const LevelDesc* levels;
+  ProblemState* st;
+  const KrylovDesc* kds;
+  const double2* upart;
+  const SlotRange* rk;
+  double2* zpart;
+  const SlotRange* rz;
+  int check_conv, pre, post, coarsest_sweeps, order;
+};
+template <typename T>
+void launch_stail(bool dots, int blocks, int smem, hipStream_t s, const TailArgs& a);
+// Allow smem bytes of dynamic LDS for the strip-tail kernels of element type T.
+template <typename T>
+hipError_t stail_set_smem(int smem);
+// Dynamic LDS bytes of a strip tail whose coarsest level is cm x cn.
+int stail_lds_bytes(int cm, int cn, bool exact);
+// Largest first level a strip tail takes: rows / columns.
+int stail_max_rows(bool fp32);
+constexpr int kStailMaxCols = 128;
+constexpr int kStailMaxLevels = 5;
 
 hipError_t launch_laplacian(const double* u, double* out, int m, int n, int planes,
                             double spacing, int bc, hipStream_t s);

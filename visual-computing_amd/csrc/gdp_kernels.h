@@ -1,14 +1,13 @@
-// MI355X (gfx950) kernels of the multigrid-preconditioned CG Poisson solver.
+// MI355X (gfx950) kernels of the multigrid-preconditioned CG Poisson solver: the
+// coarse tail (every level at or below 64 x 64 of one problem in one block, in LDS) and
+// the FP64 Krylov kernels.  The smoothing kernels of the larger levels are in
+// gdp_strip.h.
 //
-// Every kernel is batched: blockIdx.x indexes a host-built Tile table whose entries
-// name a level (or Krylov) descriptor and a tile origin, so one launch covers every
-// problem of a plan (RGB channels, a batch of images, the requests of a serving
-// step).  Per-problem scalars (alpha, beta, delta, ||r||) are reduced on the device:
-// each block publishes one partial with a relaxed agent-scope atomic store, the last
-// block of a problem (atomic arrival counter) collects the partials in slot order and
-// updates the ProblemState.  No block issues a release fence (on gfx950 that writes
-// the whole L2 back), and the order of every sum is fixed, so a problem's result
-// does not depend on what else is in the batch.
+// Every kernel is batched: blockIdx.x indexes a host-built table whose records carry
+// everything a block needs (pointers, sizes, tile origin, partial-sum slot), so one
+// launch covers every problem of a plan (RGB channels, a batch of images).  Per-problem
+// scalars (alpha, beta, delta, ||r||) are reduced on the device without atomics
+// (gdp_reduce.h): a problem's result does not depend on what else is in the batch.
 //
 // Arithmetic follows gradient-domain/src/gradient_domain/:
 //   smoother      multigrid.py:147-166  u <- (sum of neighbours - h^2 b) / diag, colour by colour
@@ -21,76 +20,10 @@
 // exact DCT coarsest solve) have no counterpart in the reference.
 #pragma once
 #include "gdp_device.h"
+#include "gdp_dct.h"
+#include "gdp_reduce.h"
 
 namespace gdp {
-
-constexpr int kThreads = 256;
-
-// ------------------------------------------------------------------ reductions
-// Sum over the block; the result is valid in thread 0.  Fixed order: shuffle tree
-// inside each wave, then waves in index order.
-template <int NT>
-__device__ inline double block_sum(double v, double* sh) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NT / 64; ++i) t += sh[i];
-  }
-  return t;
-}
-
-// Deterministic sum of the partial slots [begin, begin + count) written by an earlier
-// kernel, broadcast to every thread of the block (each thread sums a contiguous run,
-// then the block tree): every block that calls it gets bit-identical totals.
-template <int NT>
-__device__ inline double2 reduce_slots(const double2* __restrict__ partial, SlotRange rg,
-                                       double* sh, double2* bc) {
-  const int per = (rg.count + NT - 1) / NT;
-  const int lo = rg.begin + threadIdx.x * per;
-  const int hi = min(lo + per, rg.begin + rg.count);
-  double sx = 0.0, sy = 0.0;
-  for (int i = lo; i < hi; ++i) {
-    const double2 v = partial[i];
-    sx += v.x;
-    sy += v.y;
-  }
-  const double tx = block_sum<NT>(sx, sh);
-  const double ty = block_sum<NT>(sy, sh + NT / 64);
-  if (threadIdx.x == 0) *bc = make_double2(tx, ty);
-  __syncthreads();
-  return *bc;
-}
-
-// This block's contribution (a, b) to a per-problem sum, for a later kernel to reduce.
-template <int NT>
-__device__ inline void write_partial(double a, double b, double2* partial, int slot, double* sh) {
-  const double ba = block_sum<NT>(a, sh);
-  const double bb = block_sum<NT>(b, sh + NT / 64);
-  if (threadIdx.x == 0) partial[slot] = make_double2(ba, bb);
-}
-
-// Convergence test after a PCG update (solvers.py:154): every block of the first kernel
-// that touches problem p reaches the same decision from the update's partial sums of
-// ||r||^2; the lead block records it.  Returns true when the problem is done.
-template <int NT>
-__device__ inline bool check_converged(ProblemState* st, int p, const double2* upart,
-                                       SlotRange rg, bool lead, double* sh, double2* bc) {
-  const double2 t = reduce_slots<NT>(upart, rg, sh, bc);
-  const bool conv = t.x <= st[p].target2;
-  const bool done = conv || st[p].iter >= st[p].max_iter;
-  if (lead && threadIdx.x == 0) {
-    st[p].rr = t.x;
-    st[p].converged = conv;
-    st[p].done = done;
-  }
-  return done;
-}
 
 __device__ inline int neumann_count(int gi, int gj, int m, int n) {
   return (gi > 0) + (gi < m - 1) + (gj > 0) + (gj < n - 1);
@@ -136,577 +69,10 @@ __device__ inline void ptaps(bool neu, int i, int& ia, int& ib, T& wa, T& wb) {
   }
 }
 
-// ------------------------------------------------------------------ interior fast paths
-// A tile whose whole LDS region (halo included) lies strictly inside the grid needs no
-// bounds tests: every point has four in-grid neighbours (diag = 4 for Dirichlet and
-// Neumann alike) and the transfers never reach a boundary.  Threads map to the region
-// as (row = wave + 4 i, column = lane + 64 j), so no index arithmetic is divided.
-
-template <typename T, int FH, int FW>
-__device__ inline void interior_halfsweep(T* su, const T* sb, int s, int parity, int par0,
-                                          T h2) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int lo = 1 + s, hr = FH - 1 - s, hc = FW - 1 - s;
-  constexpr int NRR = (FH + 3) / 4;
-  constexpr int NCC = (FW + 127) / 128;
-  T val[NRR][NCC];
-  int at[NRR][NCC];
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr) {
-    const int a = lo + wv + 4 * rr;
-    const int c0 = lo + ((parity + a + par0 + lo) & 1);
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int c = c0 + 2 * lane + 128 * cc;
-      const bool ok = a < hr && c < hc;
-      const int idx = ok ? a * FW + c : FW + 1;
-      at[rr][cc] = ok ? idx : -1;
-      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      val[rr][cc] = (nb - h2 * sb[idx]) * T(0.25);
-    }
-  }
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc)
-      if (at[rr][cc] >= 0) su[at[rr][cc]] = val[rr][cc];
-  __syncthreads();
-}
-
-template <typename T, int CH, int CW, int NU>
-__device__ void down_interior(const VTile& t, ProblemState* st, const double2* upart,
-                              int check_conv, T* su, T* sb) {
-  constexpr int OFF = 2 + 2 * NU;
-  constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
-  constexpr int NRR = (FH + 3) / 4, NCC = (FW + 63) / 64;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int gr0 = 2 * t.i0 - OFF, gc0 = 2 * t.j0 - OFF;
-  const int pitch = t.pitch;
-  const int par0 = (gr0 + gc0) & 1;
-  const T h2 = T(t.h2), ih2 = T(t.inv_h2);
-  const T* __restrict__ b = static_cast<const T*>(t.b) + (size_t)gr0 * pitch + gc0;
-  T bl[NRR][NCC];
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr) {
-    const int a = wv + 4 * rr;
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int c = lane + 64 * cc;
-      const bool in = a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1;
-      bl[rr][cc] = in ? b[(size_t)a * pitch + c] : T(0);
-    }
-  }
-  if (st[t.problem].done) return;
-  if (check_conv && t.level == 0) {
-    __shared__ double shc[2 * kThreads / 64];
-    __shared__ double2 bcc;
-    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
-                                  shc, &bcc))
-      return;
-  }
-  // b into LDS; the first colour-0 half-sweep from u = 0 folded in
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr) {
-    const int a = wv + 4 * rr;
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int c = lane + 64 * cc;
-      if (a < FH && c < FW) {
-        const int k = a * FW + c;
-        const T bv = bl[rr][cc];
-        sb[k] = bv;
-        const bool upd = NU > 0 && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 &&
-                         ((a + c + par0) & 1) == 0;
-        su[k] = upd ? -h2 * bv * T(0.25) : T(0);
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int s = 1; s < 2 * NU; ++s) interior_halfsweep<T, FH, FW>(su, sb, s, s & 1, par0, h2);
-  // residual over fine rows / cols [fr0-1, fr0+2CH+2) in place of b; owned iterate out
-  constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;
-  constexpr int NR2 = (RRH + 3) / 4, NC2 = (RRW + 63) / 64;
-  T* __restrict__ v = static_cast<T*>(t.v) + (size_t)gr0 * pitch + gc0;
-  T rv[NR2][NC2];
-#pragma unroll
-  for (int rr = 0; rr < NR2; ++rr) {
-    const int a = OFF - 1 + wv + 4 * rr;
-#pragma unroll
-    for (int cc = 0; cc < NC2; ++cc) {
-      const int c = OFF - 1 + lane + 64 * cc;
-      const bool ok = a < OFF - 1 + RRH && c < OFF - 1 + RRW;
-      const int idx = ok ? a * FW + c : FW + 1;
-      const T uc = su[idx];
-      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      rv[rr][cc] = sb[idx] - (nb - T(4) * uc) * ih2;
-      if (ok && a >= OFF && a < OFF + 2 * CH && c >= OFF && c < OFF + 2 * CW)
-        v[(size_t)a * pitch + c] = uc;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int rr = 0; rr < NR2; ++rr) {
-    const int a = OFF - 1 + wv + 4 * rr;
-#pragma unroll
-    for (int cc = 0; cc < NC2; ++cc) {
-      const int c = OFF - 1 + lane + 64 * cc;
-      if (a < OFF - 1 + RRH && c < OFF - 1 + RRW) sb[a * FW + c] = rv[rr][cc];
-    }
-  }
-  __syncthreads();
-  const bool neu = t.bc == kNeumann;
-  T* __restrict__ cb = static_cast<T*>(t.cb);
-  for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
-    const int ci = k / CW, cj = k - ci * CW;
-    T acc;
-    if (neu) {
-      const T* r0 = &sb[(2 * ci + OFF - 1) * FW + 2 * cj + OFF - 1];
-      T col[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-        col[x] = T(0.25) * r0[x] + T(0.75) * r0[FW + x] + T(0.75) * r0[2 * FW + x] +
-                 T(0.25) * r0[3 * FW + x];
-      acc = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
-    } else {
-      const T* r0 = &sb[(2 * ci + OFF) * FW + 2 * cj + OFF];
-      T col[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x)
-        col[x] = T(0.5) * r0[x] + r0[FW + x] + T(0.5) * r0[2 * FW + x];
-      acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
-    }
-    cb[(size_t)(t.i0 + ci) * t.cpitch + t.j0 + cj] = T(0.25) * acc;
-  }
-}
-
-template <typename T, int TH, int TW, int NU, bool DOTS>
-__device__ void up_interior(const VTile& t, const ProblemState* st, double2* zpart, int order,
-                            T* su, T* sb, T* se, double* sh) {
-  constexpr int H = 2 * NU;
-  constexpr int RH = TH + 2 * H, RW = TW + 2 * H;
-  constexpr int CRH = RH / 2 + 4, CRW = RW / 2 + 4;
-  constexpr int NRR = (RH + 3) / 4, NCC = (RW + 63) / 64;
-  constexpr int NER = (CRH + 3) / 4, NEC = (CRW + 63) / 64;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int gr0 = t.i0 - H, gc0 = t.j0 - H;
-  const int cr0 = (gr0 >> 1) - 1, cc0 = (gc0 >> 1) - 1;
-  const int pitch = t.pitch;
-  const int par0 = (gr0 + gc0) & 1;
-  const T h2 = T(t.h2);
-  const T* __restrict__ e = static_cast<const T*>(t.cu) + (size_t)cr0 * t.cpitch + cc0;
-  const T* __restrict__ vv = static_cast<const T*>(t.v) + (size_t)gr0 * pitch + gc0;
-  const T* __restrict__ b = static_cast<const T*>(t.b) + (size_t)gr0 * pitch + gc0;
-  T el[NER][NEC], vl[NRR][NCC], bl[NRR][NCC];
-#pragma unroll
-  for (int rr = 0; rr < NER; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NEC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      el[rr][cc] = (a < CRH && c < CRW) ? e[(size_t)a * t.cpitch + c] : T(0);
-    }
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      const bool in = a < RH && c < RW;
-      vl[rr][cc] = in ? vv[(size_t)a * pitch + c] : T(0);
-      bl[rr][cc] = in ? b[(size_t)a * pitch + c] : T(0);
-    }
-  if (st[t.problem].done) return;
-#pragma unroll
-  for (int rr = 0; rr < NER; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NEC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      if (a < CRH && c < CRW) se[a * CRW + c] = el[rr][cc];
-    }
-  __syncthreads();
-  const bool neu = t.bc == kNeumann;
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      if (a < RH && c < RW) {
-        int ia, ib, ja, jb;
-        T wa, wb, va, vb;
-        ptaps<T>(neu, gr0 + a, ia, ib, wa, wb);
-        ptaps<T>(neu, gc0 + c, ja, jb, va, vb);
-        const int oa = (ia - cr0) * CRW - cc0, ob = (ib - cr0) * CRW - cc0;
-        const T corr = va * (wa * se[oa + ja] + wb * se[ob + ja]) +
-                       vb * (wa * se[oa + jb] + wb * se[ob + jb]);
-        su[a * RW + c] = vl[rr][cc] + corr;
-        sb[a * RW + c] = bl[rr][cc];
-      }
-    }
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < 2 * NU; ++s) {
-    const int parity = order == 0 ? (s & 1) : 1 - (s & 1);
-    interior_halfsweep<T, RH, RW>(su, sb, s, parity, par0, h2);
-  }
-  T* __restrict__ uo = static_cast<T*>(t.u) + (size_t)t.i0 * pitch + t.j0;
-  constexpr int NOR = (TH + 3) / 4, NOC = (TW + 63) / 64;
-  double zr = 0.0, zq = 0.0;
-  double rl[DOTS ? NOR : 1][DOTS ? NOC : 1], ql[DOTS ? NOR : 1][DOTS ? NOC : 1];
-  if (DOTS) {
-    const double* kr = t.kr + (size_t)t.i0 * pitch + t.j0;
-    const double* kq = t.kq + (size_t)t.i0 * pitch + t.j0;
-#pragma unroll
-    for (int rr = 0; rr < NOR; ++rr)
-#pragma unroll
-      for (int cc = 0; cc < NOC; ++cc) {
-        const int a = wv + 4 * rr, c = lane + 64 * cc;
-        const bool in = a < TH && c < TW;
-        rl[rr][cc] = in ? kr[(size_t)a * pitch + c] : 0.0;
-        ql[rr][cc] = in ? kq[(size_t)a * pitch + c] : 0.0;
-      }
-  }
-#pragma unroll
-  for (int rr = 0; rr < NOR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NOC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      if (a < TH && c < TW) {
-        const T z = su[(a + H) * RW + c + H];
-        uo[(size_t)a * pitch + c] = z;
-        if (DOTS) {
-          zr += double(z) * rl[rr][cc];
-          zq += double(z) * ql[rr][cc];
-        }
-      }
-    }
-  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
-}
-
-// ------------------------------------------------------------------ down kernel
-// Pre-smoothing (NU sweeps from a zero guess) + residual + restriction, fused.
-// One block produces a CH x CW tile of the coarse right-hand side and the
-// pre-smoothed fine iterate of the fine points it owns.
-//
-// Restriction 0.25 P^T r: Dirichlet taps 2I..2I+2 with weights (1/2, 1, 1/2); Neumann
-// taps 2I-1..2I+2 with (1/4, 3/4, 3/4, 1/4) applied to the residual extended by
-// r[-1] = r[0] and, for even sizes, r[f] = r[f-1] (odd sizes: zero), which is exactly
-// the transpose of the reflected cell-centred interpolation.
-//
-// With check_conv the finest-level launch of a PCG iteration first applies the
-// convergence test of the preceding update (check_converged).
-template <typename T, int CH, int CW, int NU>
-__global__ __launch_bounds__(kThreads) void k_down(const VTile* __restrict__ tiles,
-                                                   ProblemState* __restrict__ st,
-                                                   const double2* __restrict__ upart,
-                                                   int check_conv) {
-  const VTile t = tiles[blockIdx.x];
-  constexpr int OFF = 2 + 2 * NU;
-  constexpr int FH = 2 * CH + 2 + 2 * OFF, FW = 2 * CW + 2 + 2 * OFF;
-  constexpr int RRH = 2 * CH + 3, RRW = 2 * CW + 3;  // residual rows fr0-1 .. fr0+2CH+1
-  __shared__ T su[FH * FW];
-  __shared__ T sb[FH * FW];  // b, then the residual (in place)
-  const int m = t.m, n = t.n, pitch = t.pitch;
-  const int fr0 = 2 * t.i0, fc0 = 2 * t.j0;
-  const int gr0 = fr0 - OFF, gc0 = fc0 - OFF;
-  const bool neu = t.bc == kNeumann;
-  const T h2 = T(t.h2), ih2 = T(t.inv_h2);
-  const T* __restrict__ b = static_cast<const T*>(t.b);
-  if (gr0 >= 1 && gr0 + FH <= m - 1 && gc0 >= 1 && gc0 + FW <= n - 1 && t.i0 + CH < t.cm &&
-      t.j0 + CW < t.cn) {
-    down_interior<T, CH, CW, NU>(t, st, upart, check_conv, su, sb);
-    return;
-  }
-
-  // load b; the first colour-0 half-sweep from u = 0 is folded into the load.  All of a
-  // thread's loads are issued before any is consumed (fixed trip count, predicated).
-  constexpr int NL = (FH * FW + kThreads - 1) / kThreads;
-  T bl[NL];
-#pragma unroll
-  for (int it = 0; it < NL; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / FW, c = k - a * FW;
-    const int gi = gr0 + a, gj = gc0 + c;
-    const bool in = k < FH * FW && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 &&
-                    gi < m && gj >= 0 && gj < n;
-    bl[it] = in ? b[(size_t)gi * pitch + gj] : T(0);
-  }
-  if (st[t.problem].done) return;
-  if (check_conv && t.level == 0) {
-    // convergence test of the preceding PCG update (the loads above are in flight)
-    __shared__ double shc[2 * kThreads / 64];
-    __shared__ double2 bcc;
-    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
-                                  shc, &bcc))
-      return;
-  }
-#pragma unroll
-  for (int it = 0; it < NL; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    if (k >= FH * FW) break;
-    const int a = k / FW, c = k - a * FW;
-    const int gi = gr0 + a, gj = gc0 + c;
-    T uv = T(0);
-    if (NU > 0 && a >= 1 && a < FH - 1 && c >= 1 && c < FW - 1 && gi >= 0 && gi < m &&
-        gj >= 0 && gj < n && ((gi + gj) & 1) == 0)
-      uv = smooth_value<T>(T(0), bl[it], h2, neu, gi, gj, m, n);
-    sb[k] = bl[it];
-    su[k] = uv;
-  }
-  __syncthreads();
-
-  // After half-sweep s the points of [s+1, F-1-s) hold exactly what a whole-grid sweep
-  // would give; the ring outside that region is only read, never trusted.
-#pragma unroll
-  for (int s = 1; s < 2 * NU; ++s) {
-    const int parity = s & 1;
-    const int lo = 1 + s, hr = FH - 1 - s, hc = FW - 1 - s;
-    const int rows = hr - lo, half = (hc - lo + 1) >> 1;
-    for (int k = threadIdx.x; k < rows * half; k += kThreads) {
-      const int a = lo + k / half;
-      int c = lo + 2 * (k % half);
-      const int gi = gr0 + a;
-      int gj = gc0 + c;
-      if (((gi + gj) & 1) != parity) { ++c; ++gj; }
-      if (c >= hc || gi < 0 || gi >= m || gj < 0 || gj >= n) continue;
-      const int idx = a * FW + c;
-      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      su[idx] = smooth_value<T>(nb, sb[idx], h2, neu, gi, gj, m, n);
-    }
-    __syncthreads();
-  }
-
-  // residual on fine rows/cols [fr0-1, fr0+2CH+2), written over b (the residual of a
-  // point reads b only at that point), and the pre-smoothed iterate of the owned points
-  const int fr1 = (t.i0 + CH >= t.cm) ? m : min(m, fr0 + 2 * CH);
-  const int fc1 = (t.j0 + CW >= t.cn) ? n : min(n, fc0 + 2 * CW);
-  T* __restrict__ v = static_cast<T*>(t.v);
-  constexpr int NRR = (RRH * RRW + kThreads - 1) / kThreads;
-  T rv[NRR];
-#pragma unroll
-  for (int it = 0; it < NRR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int ra = k / RRW, rc = k - ra * RRW;
-    const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
-    T r = T(0);
-    if (k < RRH * RRW && si >= 0 && si < m && sj >= 0 && sj < n) {
-      const int idx = (si - gr0) * FW + (sj - gc0);
-      const T uc = su[idx];
-      const T nb = ((su[idx - FW] + su[idx + FW]) + su[idx - 1]) + su[idx + 1];
-      r = sb[idx] - (nb - diag_of<T>(neu, si, sj, m, n) * uc) * ih2;
-      if (si < fr1 && sj < fc1 && si >= fr0 && sj >= fc0) v[(size_t)si * pitch + sj] = uc;
-    }
-    rv[it] = r;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < NRR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int ra = k / RRW, rc = k - ra * RRW;
-    if (k < RRH * RRW) sb[(ra + OFF - 1) * FW + rc + OFF - 1] = rv[it];
-  }
-  __syncthreads();
-  if (neu) {
-    // Neumann extension of the residual past the grid: r[-1] = r[0], r[f] = r[f-1] for
-    // even f, zero otherwise (corners take both rules)
-#pragma unroll
-    for (int it = 0; it < NRR; ++it) {
-      const int k = threadIdx.x + it * kThreads;
-      const int ra = k / RRW, rc = k - ra * RRW;
-      const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
-      if (k >= RRH * RRW || (si >= 0 && si < m && sj >= 0 && sj < n)) continue;
-      int ti = si, tj = sj;
-      bool ok = true;
-      if (ti == -1) ti = 0;
-      else if (ti == m && (m & 1) == 0) ti = m - 1;
-      else if (ti < 0 || ti >= m) ok = false;
-      if (tj == -1) tj = 0;
-      else if (tj == n && (n & 1) == 0) tj = n - 1;
-      else if (tj < 0 || tj >= n) ok = false;
-      rv[it] = ok ? sb[(ti - gr0) * FW + (tj - gc0)] : T(0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < NRR; ++it) {
-      const int k = threadIdx.x + it * kThreads;
-      const int ra = k / RRW, rc = k - ra * RRW;
-      const int si = fr0 - 1 + ra, sj = fc0 - 1 + rc;
-      if (k >= RRH * RRW || (si >= 0 && si < m && sj >= 0 && sj < n)) continue;
-      sb[(ra + OFF - 1) * FW + rc + OFF - 1] = rv[it];
-    }
-    __syncthreads();
-  }
-
-  T* __restrict__ cb = static_cast<T*>(t.cb);
-  for (int k = threadIdx.x; k < CH * CW; k += kThreads) {
-    const int ci = k / CW, cj = k - ci * CW;
-    const int I = t.i0 + ci, J = t.j0 + cj;
-    if (I >= t.cm || J >= t.cn) continue;
-    T acc;
-    if (neu) {
-      const T* r0 = &sb[(2 * ci + OFF - 1) * FW + 2 * cj + OFF - 1];  // fine (2I-1, 2J-1)
-      T col[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-        col[x] = T(0.25) * r0[x] + T(0.75) * r0[FW + x] + T(0.75) * r0[2 * FW + x] +
-                 T(0.25) * r0[3 * FW + x];
-      acc = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
-    } else {
-      const T* r0 = &sb[(2 * ci + OFF) * FW + 2 * cj + OFF];  // fine (2I, 2J)
-      T col[3];
-#pragma unroll
-      for (int x = 0; x < 3; ++x)
-        col[x] = T(0.5) * r0[x] + r0[FW + x] + T(0.5) * r0[2 * FW + x];
-      acc = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
-    }
-    cb[(size_t)I * t.cpitch + J] = T(0.25) * acc;
-  }
-}
-
-// ------------------------------------------------------------------ up kernel
-// Prolongation of the coarse correction + post-smoothing (NU sweeps), fused.  With
-// DOTS (finest level of a PCG iteration) the block also accumulates <r, z> and <q, z>
-// and the problem's last block turns them into delta and beta.
-template <typename T, int TH, int TW, int NU, bool DOTS>
-__global__ __launch_bounds__(kThreads) void k_up(const VTile* __restrict__ tiles,
-                                                 const ProblemState* __restrict__ st,
-                                                 double2* __restrict__ zpart, int order) {
-  const VTile t = tiles[blockIdx.x];
-  const int p = t.problem;
-  constexpr int H = 2 * NU;
-  constexpr int RH = TH + 2 * H, RW = TW + 2 * H;
-  constexpr int CRH = RH / 2 + 4, CRW = RW / 2 + 4;
-  __shared__ T su[RH * RW];
-  __shared__ T sb[RH * RW];
-  __shared__ T se[CRH * CRW];
-  __shared__ double sh[2 * kThreads / 64];
-  const int m = t.m, n = t.n, pitch = t.pitch;
-  const int gr0 = t.i0 - H, gc0 = t.j0 - H;
-  const int cr0 = (gr0 >> 1) - 1, cc0 = (gc0 >> 1) - 1;
-  const bool neu = t.bc == kNeumann;
-  const T h2 = T(t.h2);
-  const T* __restrict__ e = static_cast<const T*>(t.cu);
-  const T* __restrict__ vv = static_cast<const T*>(t.v);
-  const T* __restrict__ b = static_cast<const T*>(t.b);
-  const int Cm = t.cm, Cn = t.cn, Cp = t.cpitch;
-  if (gr0 >= 1 && gr0 + RH <= m - 1 && gc0 >= 1 && gc0 + RW <= n - 1 && cr0 >= 0 &&
-      cr0 + CRH <= Cm && cc0 >= 0 && cc0 + CRW <= Cn) {
-    up_interior<T, TH, TW, NU, DOTS>(t, st, zpart, order, su, sb, se, sh);
-    return;
-  }
-  // Issue every global load of the block first: the coarse correction (with its
-  // padding: zero outside for Dirichlet, edge values for Neumann) and the fine v and b.
-  constexpr int NE = (CRH * CRW + kThreads - 1) / kThreads;
-  constexpr int NR = (RH * RW + kThreads - 1) / kThreads;
-  T el[NE], vl[NR], bl[NR];
-#pragma unroll
-  for (int it = 0; it < NE; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / CRW, c = k - a * CRW;
-    int I = cr0 + a, J = cc0 + c;
-    bool in = k < CRH * CRW;
-    if (neu) {
-      I = min(max(I, 0), Cm - 1);
-      J = min(max(J, 0), Cn - 1);
-    } else {
-      in = in && I >= 0 && I < Cm && J >= 0 && J < Cn;
-    }
-    el[it] = in ? e[(size_t)I * Cp + J] : T(0);
-  }
-#pragma unroll
-  for (int it = 0; it < NR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / RW, c = k - a * RW;
-    const int gi = gr0 + a, gj = gc0 + c;
-    const bool in = k < RH * RW && gi >= 0 && gi < m && gj >= 0 && gj < n;
-    const size_t o = in ? (size_t)gi * pitch + gj : 0;
-    vl[it] = in ? vv[o] : T(0);
-    bl[it] = in ? b[o] : T(0);
-  }
-  if (st[p].done) return;
-#pragma unroll
-  for (int it = 0; it < NE; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    if (k < CRH * CRW) se[k] = el[it];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int it = 0; it < NR; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    if (k >= RH * RW) break;
-    const int a = k / RW, c = k - a * RW;
-    const int gi = gr0 + a, gj = gc0 + c;
-    T uv = T(0);
-    if (gi >= 0 && gi < m && gj >= 0 && gj < n) {
-      int ia, ib, ja, jb;
-      T wa, wb, va, vb;
-      ptaps<T>(neu, gi, ia, ib, wa, wb);
-      ptaps<T>(neu, gj, ja, jb, va, vb);
-      const int oa = (ia - cr0) * CRW - cc0, ob = (ib - cr0) * CRW - cc0;
-      const T corr = va * (wa * se[oa + ja] + wb * se[ob + ja]) +
-                     vb * (wa * se[oa + jb] + wb * se[ob + jb]);
-      uv = vl[it] + corr;
-    }
-    su[k] = uv;
-    sb[k] = bl[it];
-  }
-  __syncthreads();
-
-#pragma unroll
-  for (int s = 0; s < 2 * NU; ++s) {
-    const int parity = order == 0 ? (s & 1) : 1 - (s & 1);
-    const int lo = 1 + s, hr = RH - 1 - s, hc = RW - 1 - s;
-    const int rows = hr - lo, half = (hc - lo + 1) >> 1;
-    for (int k = threadIdx.x; k < rows * half; k += kThreads) {
-      const int a = lo + k / half;
-      int c = lo + 2 * (k % half);
-      const int gi = gr0 + a;
-      int gj = gc0 + c;
-      if (((gi + gj) & 1) != parity) { ++c; ++gj; }
-      if (c >= hc || gi < 0 || gi >= m || gj < 0 || gj >= n) continue;
-      const int idx = a * RW + c;
-      const T nb = ((su[idx - RW] + su[idx + RW]) + su[idx - 1]) + su[idx + 1];
-      su[idx] = smooth_value<T>(nb, sb[idx], h2, neu, gi, gj, m, n);
-    }
-    __syncthreads();
-  }
-
-  T* __restrict__ uo = static_cast<T*>(t.u);
-  double zr = 0.0, zq = 0.0;
-  constexpr int NO = (TH * TW + kThreads - 1) / kThreads;
-  double rl[DOTS ? NO : 1], ql[DOTS ? NO : 1];
-  if (DOTS) {
-    // the finest level shares the Krylov grid's pitch
-#pragma unroll
-    for (int it = 0; it < NO; ++it) {
-      const int k = threadIdx.x + it * kThreads;
-      const int a = k / TW, c = k - a * TW;
-      const int gi = t.i0 + a, gj = t.j0 + c;
-      const bool in = k < TH * TW && gi < m && gj < n;
-      const size_t o = in ? (size_t)gi * pitch + gj : 0;
-      rl[it] = in ? t.kr[o] : 0.0;
-      ql[it] = in ? t.kq[o] : 0.0;
-    }
-  }
-#pragma unroll
-  for (int it = 0; it < NO; ++it) {
-    const int k = threadIdx.x + it * kThreads;
-    const int a = k / TW, c = k - a * TW;
-    const int gi = t.i0 + a, gj = t.j0 + c;
-    if (k >= TH * TW || gi >= m || gj >= n) continue;
-    const T z = su[(a + H) * RW + c + H];
-    uo[(size_t)gi * pitch + gj] = z;
-    if (DOTS) {
-      zr += double(z) * rl[it];
-      zq += double(z) * ql[it];
-    }
-  }
-  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
-}
-
 // ------------------------------------------------------------------ coarse tail
 // One block per problem runs every level from TailDesc.first to the coarsest in LDS:
 // pre-smooth / residual / restrict down, FP64 coarsest solve, prolong / post-smooth up.
 // Loops run rows over waves and columns over lanes (no integer division).
-constexpr int kTailWaves = kTailThreads / 64;
 
 // Tail loops put rows on waves and columns on lanes (no division), and a wave with no
 // row of the level branches over the whole pass, so a small level costs only the
@@ -835,46 +201,6 @@ __device__ void lds_prolong_add(T* u, int m, int n, const T* e, int mc, int nc, 
       u[i * n + j] += corr;
     }
   }
-  __syncthreads();
-}
-
-// Exact solve of the coarsest level by an orthonormal transform that diagonalises its
-// operator: U = Cm^T ((Cm B Cn^T) ./ lambda) Cn.  Neumann: DCT-II (the solvers.py:224-262
-// construction, zero-mean solution: 1/lambda of the constant mode is 0).  Dirichlet:
-// DST-I.  Tables live in LDS: Cm (m*m), Cn (n*n), 1/lambda (m*n).  Each pass puts
-// output rows on waves and output columns on lanes.
-__device__ void lds_dct_solve(double* w0, double* w1, int m, int n, const double* tab) {
-  const double* Cm = tab;
-  const double* Cn = tab + m * m;
-  const double* il = tab + m * m + n * n;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int k = wave; k < m; k += kTailWaves)  // w1 = Cm w0
-    for (int j = lane; j < n; j += 64) {
-      double s = 0.0;
-      for (int i = 0; i < m; ++i) s += Cm[k * m + i] * w0[i * n + j];
-      w1[k * n + j] = s;
-    }
-  __syncthreads();
-  for (int k = wave; k < m; k += kTailWaves)  // w0 = (w1 Cn^T) ./ lambda
-    for (int l = lane; l < n; l += 64) {
-      double s = 0.0;
-      for (int j = 0; j < n; ++j) s += w1[k * n + j] * Cn[l * n + j];
-      w0[k * n + l] = s * il[k * n + l];
-    }
-  __syncthreads();
-  for (int i = wave; i < m; i += kTailWaves)  // w1 = Cm^T w0
-    for (int l = lane; l < n; l += 64) {
-      double s = 0.0;
-      for (int k = 0; k < m; ++k) s += Cm[k * m + i] * w0[k * n + l];
-      w1[i * n + l] = s;
-    }
-  __syncthreads();
-  for (int i = wave; i < m; i += kTailWaves)  // w0 = w1 Cn
-    for (int j = lane; j < n; j += 64) {
-      double s = 0.0;
-      for (int l = 0; l < n; ++l) s += w1[i * n + l] * Cn[l * n + j];
-      w0[i * n + j] = s;
-    }
   __syncthreads();
 }
 

@@ -5,8 +5,10 @@
 //   Dirichlet  m -> (m - 1) // 2 until min side <= coarsest_size or a coarse side < 2
 //              (multigrid.py:143-144, 206)
 //   Neumann    m -> (m + 1) // 2 until max side <= coarsest_size or min side < 3
-// Levels whose whole tail (that level and every coarser one) fits in LDS run in one
-// k_tail block per problem; finer levels use the tiled k_down / k_up kernels.
+// The coarse tail of a problem (a level and every coarser one) runs in one block per
+// problem: k_stail (registers, gdp_stail.h) from the first level at most 128 columns
+// wide that it can hold, else k_tail (LDS) from the first level whose tail fits in LDS.
+// Finer levels use the strip kernels k_sdown / k_sup (gdp_strip.h).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -62,11 +64,16 @@ namespace {
 struct Shape2 {
   int h, w;
 };
-// Tile menus: kind 0 = down (coarse points per block), 1 = up and 2 = Krylov (fine points).
-const Shape2 kDownMenu[4] = {{4, 32}, {8, 32}, {16, 32}, {8, 64}};
-const Shape2 kUpMenu[4] = {{8, 64}, {16, 64}, {32, 64}, {16, 128}};
+// Tile menus.  Kind 0 = down: coarse rows per strip (a block stacks four strips; a
+// strip is StripGeo<nu>::kDownCols coarse columns wide).  Kind 1 = up: fine rows per
+// strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per block.
+const int kDownRows[4] = {1, 2, 4, 8};
+const int kUpRows[4] = {2, 4, 8, 16};
 const Shape2 kKryMenu[4] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}};
 constexpr int kMenu = 4;
+constexpr int kStrips = kThreads / 64;  // strips per block
+int down_cols(int nu) { return (125 - 4 * nu) / 2; }
+int up_cols(int nu) { return 124 - 4 * nu; }
 constexpr int kMaxDepth = 32;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -74,72 +81,6 @@ int round_up(int x, int a) { return (x + a - 1) / a * a; }
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ----------------------------------------------------------------- launch dispatch
-struct DownArgs {
-  const VTile* tiles;
-  ProblemState* st;
-  const double2* upart;
-  int check_conv;
-};
-
-template <typename T, int CH, int CW>
-void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
-#define GDP_DOWN(NU) \
-  k_down<T, CH, CW, NU><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
-  switch (nu) {
-    case 0: GDP_DOWN(0); break;
-    case 1: GDP_DOWN(1); break;
-    case 2: GDP_DOWN(2); break;
-    default: GDP_DOWN(3); break;
-  }
-#undef GDP_DOWN
-}
-
-template <typename T>
-void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
-  switch (menu) {
-    case 0: down_nu<T, 4, 32>(nu, blocks, s, a); break;
-    case 1: down_nu<T, 8, 32>(nu, blocks, s, a); break;
-    case 2: down_nu<T, 16, 32>(nu, blocks, s, a); break;
-    default: down_nu<T, 8, 64>(nu, blocks, s, a); break;
-  }
-}
-
-struct UpArgs {
-  const VTile* tiles;
-  const ProblemState* st;
-  double2* zpart;
-  int order;
-};
-
-template <typename T, int TH, int TW, bool DOTS>
-void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
-#define GDP_UP(NU)                                                                       \
-  k_up<T, TH, TW, NU, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
-  switch (nu) {
-    case 0: GDP_UP(0); break;
-    case 1: GDP_UP(1); break;
-    case 2: GDP_UP(2); break;
-    default: GDP_UP(3); break;
-  }
-#undef GDP_UP
-}
-
-template <typename T, bool DOTS>
-void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
-  switch (menu) {
-    case 0: up_nu<T, 8, 64, DOTS>(nu, blocks, s, a); break;
-    case 1: up_nu<T, 16, 64, DOTS>(nu, blocks, s, a); break;
-    case 2: up_nu<T, 32, 64, DOTS>(nu, blocks, s, a); break;
-    default: up_nu<T, 16, 128, DOTS>(nu, blocks, s, a); break;
-  }
-}
-
-template <typename T>
-void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a) {
-  if (dots) launch_up_d<T, true>(menu, nu, blocks, s, a);
-  else launch_up_d<T, false>(menu, nu, blocks, s, a);
-}
-
 #define GDP_KRY_DISPATCH(menu, CALL)                 \
   switch (menu) {                                    \
     case 0: { constexpr int TH = 8, TW = 64; CALL; } break;   \
@@ -161,6 +102,7 @@ struct ProbInfo {
   std::vector<LevelInfo> lv;
   int tail_first = 0;
   int tail_bytes = 0;
+  bool stail = false;  // register-strip tail (k_stail) rather than the LDS tail (k_tail)
   int level_base = 0;
   size_t off_x = 0, off_r = 0, off_d0 = 0, off_d1 = 0, off_q = 0, off_dct = 0;
   int tail_off[kMaxTailLevels][2];
@@ -184,7 +126,9 @@ struct gdp_plan {
   int up_menu[kMaxDepth];
   int kry_menu = 1;
   int total_levels = 0;
-  int tail_smem = 0;
+  int tail_smem = 0;   // dynamic LDS of the LDS tail (k_tail)
+  int stail_smem = 0;  // dynamic LDS of the register-strip tail (k_stail)
+  int n_stail = 0;     // strip-tail problems: the first n_stail entries of the tail table
   int max_slots = 1;
   size_t ws_bytes = 0;
   size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0;
@@ -287,17 +231,19 @@ size_t tail_layout(ProbInfo& pi, int first, size_t tsize) {
   return off;
 }
 
-Shape2 menu_shape(int kind, int idx) {
-  if (kind == 0) return kDownMenu[idx];
-  if (kind == 1) return kUpMenu[idx];
+// Block tile of a menu entry: down in coarse points, up and Krylov in fine points; the
+// strip widths depend on the number of sweeps nu of the kernel.
+Shape2 menu_shape(int kind, int idx, int nu) {
+  if (kind == 0) return Shape2{kStrips * kDownRows[idx], down_cols(nu)};
+  if (kind == 1) return Shape2{kStrips * kUpRows[idx], up_cols(nu)};
   return kKryMenu[idx];
 }
 
 // Worst case tile count of a grid over every entry of a menu.
-int max_tiles(int kind, int m, int n) {
+int max_tiles(int kind, int m, int n, int nu) {
   int best = 0;
   for (int i = 0; i < kMenu; ++i) {
-    const Shape2 s = menu_shape(kind, i);
+    const Shape2 s = menu_shape(kind, i, nu);
     best = std::max(best, cdiv(m, s.h) * cdiv(n, s.w));
   }
   return best;
@@ -326,12 +272,12 @@ extern "C" {
 const char* gdp_last_error(void) { return g_err.c_str(); }
 int gdp_version(void) { return 10000; }
 
-int gdp_tile_menu(int kind, int index, int* th, int* tw) {
-  if (kind < 0 || kind > 2 || index < 0 || index >= kMenu || !th || !tw) {
-    set_error("gdp_tile_menu: bad kind/index");
+int gdp_tile_menu(int kind, int index, int nu, int* th, int* tw) {
+  if (kind < 0 || kind > 2 || index < 0 || index >= kMenu || nu < 0 || nu > 3 || !th || !tw) {
+    set_error("gdp_tile_menu: bad kind/index/nu");
     return GDP_EINVAL;
   }
-  const Shape2 s = menu_shape(kind, index);
+  const Shape2 s = menu_shape(kind, index, nu);
   *th = s.h;
   *tw = s.w;
   return GDP_OK;
@@ -363,8 +309,8 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   P->fp32 = o.precision == GDP_MIXED;
   P->tsize = P->fp32 ? 4 : 8;
   for (int i = 0; i < kMaxDepth; ++i) {
-    P->down_menu[i] = 1;
-    P->up_menu[i] = 1;
+    P->down_menu[i] = i == 0 ? 2 : 1;
+    P->up_menu[i] = i == 0 ? 2 : 1;
   }
   P->kry_menu = 1;
   int lvl_total = 0;
@@ -392,7 +338,20 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
     const int L = int(pi.lv.size());
     int first = -1;
+    // the register-strip tail takes the first level at most 128 columns wide and
+    // stail_max_rows rows tall whose chain to the coarsest fits its level count and LDS
     for (int f = 0; f < L; ++f) {
+      const auto& c = pi.lv[L - 1];
+      const int bytes = stail_lds_bytes(c.m, c.n, pi.exact);
+      if (pi.lv[f].m <= stail_max_rows(P->fp32) && pi.lv[f].n <= kStailMaxCols &&
+          L - f <= kStailMaxLevels && bytes <= kTailLdsBytes) {
+        first = f;
+        pi.stail = true;
+        pi.tail_bytes = bytes;
+        break;
+      }
+    }
+    for (int f = 0; f < L && first < 0; ++f) {
       const size_t bytes = tail_layout(pi, f, P->tsize);
       if (bytes <= size_t(kTailLdsBytes)) {
         first = f;
@@ -412,11 +371,16 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
       return GDP_ENOTSUP;
     }
     pi.tail_first = first;
-    tail_layout(pi, first, P->tsize);
+    if (!pi.stail) tail_layout(pi, first, P->tsize);
     pi.level_base = lvl_total;
     lvl_total += L;
     P->depth = std::max(P->depth, first);
-    P->tail_smem = std::max(P->tail_smem, pi.tail_bytes);
+    if (pi.stail) {
+      P->stail_smem = std::max(P->stail_smem, pi.tail_bytes);
+      P->n_stail++;
+    } else {
+      P->tail_smem = std::max(P->tail_smem, pi.tail_bytes);
+    }
     P->probs.push_back(pi);
   }
   P->total_levels = lvl_total;
@@ -460,11 +424,12 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   size_t tiles = 0;
   int kry_slots = 0, up0_slots = 0;
   for (auto& pi : P->probs) {
-    kry_slots += max_tiles(2, pi.lv[0].m, pi.lv[0].n);
-    if (pi.tail_first > 0) up0_slots += max_tiles(1, pi.lv[0].m, pi.lv[0].n);
+    kry_slots += max_tiles(2, pi.lv[0].m, pi.lv[0].n, 0);
+    if (pi.tail_first > 0)
+      up0_slots += max_tiles(1, pi.lv[0].m, pi.lv[0].n, o.post_smoothing);
     for (int l = 0; l < pi.tail_first; ++l) {
-      tiles += max_tiles(0, pi.lv[l + 1].m, pi.lv[l + 1].n);
-      tiles += max_tiles(1, pi.lv[l].m, pi.lv[l].n);
+      tiles += max_tiles(0, pi.lv[l + 1].m, pi.lv[l + 1].n, o.pre_smoothing);
+      tiles += max_tiles(1, pi.lv[l].m, pi.lv[l].n, o.post_smoothing);
     }
   }
   tiles += kry_slots;
@@ -662,7 +627,16 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   GDP_HIP(hipMemcpy(P->d_levels(), levels.data(), sizeof(LevelDesc) * levels.size(),
                     hipMemcpyHostToDevice));
   GDP_HIP(hipMemcpy(P->d_kry(), kry.data(), sizeof(KrylovDesc) * NP, hipMemcpyHostToDevice));
-  GDP_HIP(hipMemcpy(P->d_tails(), tails.data(), sizeof(TailDesc) * NP, hipMemcpyHostToDevice));
+  {
+    // strip-tail problems first (k_stail takes entries [0, n_stail), k_tail the rest)
+    std::vector<TailDesc> ordered;
+    for (int p = 0; p < NP; ++p)
+      if (P->probs[p].stail) ordered.push_back(tails[p]);
+    for (int p = 0; p < NP; ++p)
+      if (!P->probs[p].stail) ordered.push_back(tails[p]);
+    GDP_HIP(hipMemcpy(P->d_tails(), ordered.data(), sizeof(TailDesc) * NP,
+                      hipMemcpyHostToDevice));
+  }
 
   // ---- tile tables (self-contained records; see VTile / KTile)
   size_t toff = P->off_tiles;
@@ -730,8 +704,8 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   };
   for (int l = 0; l < P->depth; ++l) {
     std::vector<VTile> vd, vu;
-    const Shape2 sd = kDownMenu[P->down_menu[l]];
-    const Shape2 su = kUpMenu[P->up_menu[l]];
+    const Shape2 sd = menu_shape(0, P->down_menu[l], P->opt.pre_smoothing);
+    const Shape2 su = menu_shape(1, P->up_menu[l], P->opt.post_smoothing);
     for (int p = 0; p < NP; ++p) {
       const ProbInfo& pi = P->probs[p];
       if (pi.tail_first <= l) continue;
@@ -803,6 +777,9 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   GDP_HIP(hipMemset(P->d_flag(), 0, sizeof(int) * 4));
 
   // the tail kernels may use more than the default dynamic LDS
+  if (P->n_stail) {
+    GDP_HIP(P->fp32 ? stail_set_smem<float>(P->stail_smem) : stail_set_smem<double>(P->stail_smem));
+  }
   const int smem = P->tail_smem;
   if (P->fp32) {
     GDP_HIP(hipFuncSetAttribute((const void*)k_tail<float, true>,
@@ -920,17 +897,27 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
   {
     ProfScope ps(P, s, "tail", 2.0 * sizeof(T) * tail_pts);
     const int cc = check_conv ? 1 : 0;
-    if (dots)
-      k_tail<T, true><<<P->count(), kTailThreads, P->tail_smem, s>>>(
-          P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(), P->d_rkry(),
-          P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing, o.coarsest_sweeps,
-          order);
-    else
-      k_tail<T, false><<<P->count(), kTailThreads, P->tail_smem, s>>>(
-          P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(), P->d_rkry(),
-          P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing, o.coarsest_sweeps,
-          order);
-    GDP_CHECK_LAUNCH();
+    const int ns = P->n_stail, no = P->count() - P->n_stail;
+    if (ns) {
+      TailArgs a{P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
+                 P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
+                 o.post_smoothing, o.coarsest_sweeps, order};
+      launch_stail<T>(dots, ns, P->stail_smem, s, a);
+      GDP_CHECK_LAUNCH();
+    }
+    if (no) {
+      if (dots)
+        k_tail<T, true><<<no, kTailThreads, P->tail_smem, s>>>(
+            P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
+            P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
+            o.coarsest_sweeps, order);
+      else
+        k_tail<T, false><<<no, kTailThreads, P->tail_smem, s>>>(
+            P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
+            P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
+            o.coarsest_sweeps, order);
+      GDP_CHECK_LAUNCH();
+    }
   }
   for (int l = P->depth - 1; l >= 0; --l) {
     if (!P->up[l].count) continue;

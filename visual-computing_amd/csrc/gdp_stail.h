@@ -1,0 +1,511 @@
+// The coarse tail in registers: every level of one problem from the first one that is at
+// most 128 columns wide and 64 (FP32) or 32 (FP64) rows tall down to the coarsest, in
+// one block of 8 waves
+// (512 threads, so each wave may hold up to 256 registers: the tail keeps every level's
+// iterate and right-hand side in registers across the whole V-cycle).
+//
+// Each level is laid out like a strip kernel's (gdp_strip.h): wave w holds rows
+// [w R, w R + R) of the level (R = kRows0 >> depth, at least 1) and lane l the column
+// pair (2l, 2l+1) of each row, in registers.  Vertical neighbours across wave
+// boundaries come through a small LDS halo slot, double-buffered so each red-black
+// half-sweep costs one barrier; horizontal neighbours come from DPP wave shifts.
+// Levels hand data to each other through one LDS buffer of rows 128 wide: the fine
+// residual on the way down (each wave reads the fine rows its coarse rows restrict
+// from), the coarse iterate on the way up.  The coarsest level is solved in FP64: by
+// the exact DCT / DST transform in LDS, or by coarsest_sweeps red-black sweeps.
+//
+// Arithmetic follows gradient-domain/src/gradient_domain/multigrid.py (v_cycle
+// :184-215) exactly as the strip kernels do; the exact coarsest solves follow
+// solvers.py:224-262 (Neumann) and its DST-I analogue (Dirichlet).
+#pragma once
+#include "gdp_dct.h"
+#include "gdp_strip.h"
+
+namespace gdp {
+
+constexpr int kSTailThreads = 512;
+constexpr int kSTailWaves = kSTailThreads / 64;  // 8
+constexpr int kSTailMaxLevels = 5;
+constexpr int kSTailCols = 128;                 // columns of a level row (64 lanes x 2)
+constexpr int kSTailXbBytes = 65536;            // level exchange buffer
+constexpr int kSTailXhBytes = 2 * kSTailWaves * 2 * 64 * 16;  // halo slots (FP64 pairs)
+
+template <typename T>
+struct STailRows {
+  // rows per wave of the first level: 64 rows in all (FP32) or 32 (FP64)
+  static constexpr int kRows0 = sizeof(T) == 4 ? 8 : 4;
+};
+
+template <int RD0, int D>
+struct RowsAt {
+  static constexpr int value = (RD0 >> D) > 0 ? (RD0 >> D) : 1;
+};
+
+struct STailCtx {
+  unsigned char* xb;   // level exchange rows (kSTailCols elements of T per row)
+  unsigned char* xh;   // halo slots [2][16 waves][2 rows][64 lanes] pairs
+  double* tab;         // exact coarsest: Cm, Cn, 1/lambda
+  double* w0;          // exact coarsest work arrays
+  double* w1;
+  const LevelDesc* LV;
+  int nlev, w, lane, pre, post, csweeps, order, hbuf;
+  bool exact;
+};
+
+// Halo exchange: publish this wave's first and last rows, return the row above the
+// wave's first row and the row below its last (zero at the ends of the block).
+template <typename V2>
+__device__ __attribute__((always_inline)) void st_halo(STailCtx& c, const V2& top, const V2& bot, V2& above, V2& below) {
+  V2* h = reinterpret_cast<V2*>(c.xh) + c.hbuf * (kSTailWaves * 2 * 64);
+  h[(c.w * 2 + 0) * 64 + c.lane] = top;
+  h[(c.w * 2 + 1) * 64 + c.lane] = bot;
+  __syncthreads();
+  above = c.w > 0 ? h[((c.w - 1) * 2 + 1) * 64 + c.lane] : V2{0, 0};
+  below = c.w < kSTailWaves - 1 ? h[((c.w + 1) * 2 + 0) * 64 + c.lane] : V2{0, 0};
+  c.hbuf ^= 1;
+}
+
+// Red-black half-sweep of colour Q over a level's rows held by this wave.  wi holds
+// each row's inverse diagonals (zero outside the grid, so rows past the grid stay zero
+// without a test).  With an even row count per wave a row's parity is its index's, so
+// the choice of x or y is fixed at compile time.
+template <typename T, int RD, int Q>
+__device__ __attribute__((always_inline)) void st_halfsweep_q(STailCtx& c, typename Pair<T>::type* u,
+                                                            const typename Pair<T>::type* hb,
+                                                            const typename Pair<T>::type* wi) {
+  using V2 = typename Pair<T>::type;
+  V2 above, below;
+  st_halo(c, u[0], u[RD - 1], above, below);
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const V2 up = r > 0 ? u[r - 1] : above;
+    const V2 dn = r < RD - 1 ? u[r + 1] : below;
+    const bool ux = (RD % 2 == 0) ? (((r + Q) & 1) == 0) : (((c.w * RD + r + Q) & 1) == 0);
+    if (ux) {
+      const T nb = ((up.x + dn.x) + dpp_shr1(u[r].y)) + u[r].y;
+      u[r].x = wi[r].x * nb - hb[r].x;
+    } else {
+      const T nb = ((up.y + dn.y) + u[r].x) + dpp_shl1(u[r].x);
+      u[r].y = wi[r].y * nb - hb[r].y;
+    }
+  }
+}
+
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_halfsweep(STailCtx& c, typename Pair<T>::type* u,
+                                                          const typename Pair<T>::type* hb,
+                                                          const typename Pair<T>::type* wi, int q) {
+  if (q) st_halfsweep_q<T, RD, 1>(c, u, hb, wi);
+  else st_halfsweep_q<T, RD, 0>(c, u, hb, wi);
+}
+
+// Inverse diagonals wi of this wave's rows and hb = h^2 b / diag (zero outside the
+// grid).
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_scale_rhs(const STailCtx& c, typename Pair<T>::type* hb,
+                                                          typename Pair<T>::type* wi,
+                                                          const LaneCols<T>& lc, int m, bool neu,
+                                                          T h2) {
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const int gi = c.w * RD + r;
+    const int vc = row_count(gi, m, neu);
+    const bool rin = gi < m;
+    wi[r].x = (rin && lc.inx) ? lc.wx(vc) : T(0);
+    wi[r].y = (rin && lc.iny) ? lc.wy(vc) : T(0);
+    hb[r].x = wi[r].x * h2 * hb[r].x;
+    hb[r].y = wi[r].y * h2 * hb[r].y;
+  }
+}
+
+// Publish this wave's rows of a level (row-major, kSTailCols per row) to the exchange
+// buffer, then synchronise.
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, const typename Pair<T>::type* v, int m) {
+  using V2 = typename Pair<T>::type;
+  V2* xb = reinterpret_cast<V2*>(c.xb);
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const int gi = c.w * RD + r;
+    if (gi < m) xb[gi * (kSTailCols / 2) + c.lane] = v[r];
+  }
+  __syncthreads();
+}
+
+// Coarsest level: u from the right-hand side b (plain) of this wave's rows; the
+// result is published to the exchange buffer.
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const LevelDesc& L, typename Pair<T>::type* b) {
+  using V2 = typename Pair<T>::type;
+  using D2 = typename Pair<double>::type;
+  const int m = L.m, n = L.n;
+  const bool neu = L.bc == kNeumann;
+  V2 u[RD];
+  if (c.exact) {
+    const int gj = 2 * c.lane;
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int gi = c.w * RD + r;
+      if (gi < m) {
+        if (gj < n) c.w0[gi * n + gj] = double(b[r].x) * L.h2;
+        if (gj + 1 < n) c.w0[gi * n + gj + 1] = double(b[r].y) * L.h2;
+      }
+    }
+    __syncthreads();
+    lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailWaves);
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int gi = c.w * RD + r;
+      u[r] = V2{T(0), T(0)};
+      if (gi < m) {
+        if (gj < n) u[r].x = T(c.w0[gi * n + gj]);
+        if (gj + 1 < n) u[r].y = T(c.w0[gi * n + gj + 1]);
+      }
+    }
+  } else {
+    // coarsest_sweeps red-black sweeps in FP64 from zero (multigrid.py:206-207)
+    LaneCols<double> lc;
+    lc.init(0, c.lane, n, neu);
+    D2 ud[RD], hd[RD];
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      ud[r] = D2{0.0, 0.0};
+      hd[r] = D2{double(b[r].x), double(b[r].y)};
+    }
+    D2 wd[RD];
+    st_scale_rhs<double, RD>(c, hd, wd, lc, m, neu, L.h2);
+    for (int s = 0; s < c.csweeps; ++s) {
+      st_halfsweep_q<double, RD, 0>(c, ud, hd, wd);
+      st_halfsweep_q<double, RD, 1>(c, ud, hd, wd);
+    }
+#pragma unroll
+    for (int r = 0; r < RD; ++r) u[r] = V2{T(ud[r].x), T(ud[r].y)};
+  }
+  st_publish<T, RD>(c, u, m);
+}
+
+// Fine residual (scaled by h^2) of this wave's rows to the exchange buffer.
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const typename Pair<T>::type* u,
+                            const typename Pair<T>::type* hb, const LaneCols<T>& lc, int m,
+                            bool neu) {
+  using V2 = typename Pair<T>::type;
+  V2 above, below;
+  st_halo(c, u[0], u[RD - 1], above, below);
+  V2 r2[RD];
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const int gi = c.w * RD + r;
+    const int vc = row_count(gi, m, neu);
+    const V2 up = r > 0 ? u[r - 1] : above;
+    const V2 dn = r < RD - 1 ? u[r + 1] : below;
+    const T nx = ((up.x + dn.x) + dpp_shr1(u[r].y)) + u[r].y;
+    const T ny = ((up.y + dn.y) + u[r].x) + dpp_shl1(u[r].x);
+    const T rx = (T(vc) + lc.hx) * (hb[r].x + u[r].x) - nx;
+    const T ry = (T(vc) + lc.hy) * (hb[r].y + u[r].y) - ny;
+    r2[r].x = (gi < m && lc.inx) ? rx : T(0);
+    r2[r].y = (gi < m && lc.iny) ? ry : T(0);
+  }
+  st_publish<T, RD>(c, r2, m);
+}
+
+// Restriction of the published fine residual to this wave's coarse rows (plain right-
+// hand side of the coarse level): 0.25 P^T r P / h^2, Neumann with the residual
+// extension r[-1] = r[0] and r[f] = r[f-1] for even f (zero for odd f).  Published rows
+// are zero past the grid's last column, so only the column extension and the rows need
+// mapping; column indices are fixed per lane, row offsets per coarse row.
+template <typename T, int RC>
+__device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const LevelDesc& F, const LevelDesc& C,
+                            typename Pair<T>::type* cb) {
+  const T* R = reinterpret_cast<const T*>(c.xb);
+  const int m = F.m, n = F.n, mc = C.m, nc = C.n;
+  const bool neu = F.bc == kNeumann;
+  const T s = T(0.25) * T(F.inv_h2);
+  const int l = c.lane;
+  // fine columns 4l-1 .. 4l+4 (Neumann) or 4l .. 4l+4 (Dirichlet)
+  int cj[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int j = 4 * l - (neu ? 1 : 0) + k;
+    if (neu && j == -1) j = 0;
+    if (neu && j == n && (n & 1) == 0) j = n - 1;
+    cj[k] = min(max(j, 0), kSTailCols - 1);
+  }
+  const bool v0ok = 2 * l < nc, v1ok = 2 * l + 1 < nc;
+#pragma unroll
+  for (int r = 0; r < RC; ++r) {
+    const int I = c.w * RC + r;
+    T v0 = T(0), v1 = T(0);
+    if (I < mc) {
+      if (neu) {
+        int ro[4];
+        T wr[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          int i = 2 * I - 1 + k;
+          T wk = (k == 0 || k == 3) ? T(0.25) : T(0.75);
+          if (i == -1) i = 0;
+          if (i == m && (m & 1) == 0) i = m - 1;
+          if (i >= m) {
+            i = 0;
+            wk = T(0);
+          }
+          ro[k] = i * kSTailCols;
+          wr[k] = wk;
+        }
+        T col[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          col[k] = wr[0] * R[ro[0] + cj[k]] + wr[1] * R[ro[1] + cj[k]] +
+                   wr[2] * R[ro[2] + cj[k]] + wr[3] * R[ro[3] + cj[k]];
+        v0 = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
+        v1 = T(0.25) * col[2] + T(0.75) * col[3] + T(0.75) * col[4] + T(0.25) * col[5];
+      } else {
+        const int r0 = 2 * I * kSTailCols;
+        T col[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+          col[k] = T(0.5) * R[r0 + cj[k]] + R[r0 + kSTailCols + cj[k]] +
+                   T(0.5) * R[r0 + 2 * kSTailCols + cj[k]];
+        v0 = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
+        v1 = T(0.5) * col[2] + col[3] + T(0.5) * col[4];
+      }
+    }
+    cb[r].x = (I < mc && v0ok) ? s * v0 : T(0);
+    cb[r].y = (I < mc && v1ok) ? s * v1 : T(0);
+  }
+}
+
+// u += P e for this wave's rows, e = the published coarse iterate (zero past the coarse
+// grid's last column; column 127 is always such a column).
+template <typename T, int RD>
+__device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const LevelDesc& F, const LevelDesc& C,
+                           typename Pair<T>::type* u, const LaneCols<T>& lc) {
+  const T* E = reinterpret_cast<const T*>(c.xb);
+  const int m = F.m, mc = C.m, nc = C.n;
+  const bool neu = F.bc == kNeumann;
+  const int l = c.lane;
+  int cj[3];  // coarse columns l-1, l, l+1
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    int J = l - 1 + k;
+    if (neu) J = min(max(J, 0), nc - 1);
+    else if (J < 0) J = kSTailCols - 1;
+    cj[k] = J;
+  }
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const int gi = c.w * RD + r;
+    if (gi >= m) continue;
+    int ra, rb;  // coarse rows and weights of the vertical interpolation
+    T wa, wb;
+    if (neu) {
+      const int P = gi >> 1, O = (gi & 1) ? P + 1 : P - 1;
+      ra = min(max(P, 0), mc - 1);
+      rb = min(max(O, 0), mc - 1);
+      wa = T(0.75);
+      wb = T(0.25);
+    } else if (gi & 1) {
+      // the last fine row of an even-sized grid has no coarse parent
+      ra = rb = (gi - 1) >> 1;
+      wa = ra < mc ? T(1) : T(0);
+      wb = T(0);
+      if (ra >= mc) ra = rb = 0;
+    } else {
+      ra = (gi >> 1) - 1;
+      rb = gi >> 1;
+      wa = wb = T(0.5);
+      if (ra < 0) {
+        ra = 0;
+        wa = T(0);
+      }
+      if (rb >= mc) {
+        rb = 0;
+        wb = T(0);
+      }
+    }
+    const int oa = ra * kSTailCols, ob = rb * kSTailCols;
+    T t[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[k] = wa * E[oa + cj[k]] + wb * E[ob + cj[k]];
+    T cx, cy;
+    if (neu) {
+      cx = T(0.75) * t[1] + T(0.25) * t[0];
+      cy = T(0.75) * t[1] + T(0.25) * t[2];
+    } else {
+      cx = T(0.5) * (t[0] + t[1]);
+      cy = t[1];
+    }
+    if (lc.inx) u[r].x += cx;
+    if (lc.iny) u[r].y += cy;
+  }
+  __syncthreads();  // every wave has read the exchange buffer
+}
+
+template <typename T, int D>
+__device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b);
+
+template <typename T, int D>
+struct STailLevel {
+  static __device__ __attribute__((always_inline)) void run(STailCtx& c, typename Pair<T>::type* b) {
+    st_level<T, D>(c, b);
+  }
+};
+template <typename T>
+struct STailLevel<T, kSTailMaxLevels> {
+  static __device__ void run(STailCtx&, typename Pair<T>::type*) {}
+};
+
+// Level D of the tail: b holds the plain right-hand side of this wave's rows.  On
+// return the level's iterate is published in the exchange buffer.
+template <typename T, int D>
+__device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b) {
+  using V2 = typename Pair<T>::type;
+  constexpr int RD = RowsAt<STailRows<T>::kRows0, D>::value;
+  constexpr int RC = RowsAt<STailRows<T>::kRows0, D + 1>::value;
+  const LevelDesc& L = c.LV[D];
+  if (D == c.nlev - 1) {
+    st_coarsest<T, RD>(c, L, b);
+    return;
+  }
+  const LevelDesc& C = c.LV[D + 1];
+  const int m = L.m;
+  const bool neu = L.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(0, c.lane, L.n, neu);
+  V2* hb = b;  // scaled in place
+  V2 wi[RD];
+  st_scale_rhs<T, RD>(c, hb, wi, lc, m, neu, T(L.h2));
+  // pre-smoothing from zero: the first colour-0 half-sweep is u = -hb on colour 0
+  V2 u[RD];
+#pragma unroll
+  for (int r = 0; r < RD; ++r) {
+    const int gi = c.w * RD + r;
+    u[r] = V2{T(0), T(0)};
+    if (c.pre > 0) {
+      if ((gi & 1) == 0) u[r].x = -hb[r].x;
+      else u[r].y = -hb[r].y;
+    }
+  }
+  for (int s = 1; s < 2 * c.pre; ++s) st_halfsweep<T, RD>(c, u, hb, wi, s & 1);
+  st_residual<T, RD>(c, u, hb, lc, m, neu);
+  V2 cb[RC];
+  st_restrict<T, RC>(c, L, C, cb);
+  STailLevel<T, D + 1>::run(c, cb);
+  {
+    // recompute the lane's column data rather than keep it live across the recursion
+    int lane = c.lane;
+    asm volatile("" : "+v"(lane));
+    lc.init(0, lane, L.n, neu);
+  }
+  st_prolong<T, RD>(c, L, C, u, lc);
+  for (int s = 0; s < 2 * c.post; ++s)
+    st_halfsweep<T, RD>(c, u, hb, wi, (s + c.order) & 1);
+  st_publish<T, RD>(c, u, m);
+}
+
+template <typename T, bool DOTS>
+__global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restrict__ tails,
+                                                        const LevelDesc* __restrict__ levels,
+                                                        ProblemState* __restrict__ st,
+                                                        const KrylovDesc* __restrict__ kds,
+                                                        const double2* __restrict__ upart,
+                                                        const SlotRange* __restrict__ rk,
+                                                        double2* __restrict__ zpart,
+                                                        const SlotRange* __restrict__ rz,
+                                                        int check_conv, int pre, int post,
+                                                        int coarsest_sweeps, int order) {
+  using V2 = typename Pair<T>::type;
+  constexpr int R0 = STailRows<T>::kRows0;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ TailDesc D;
+  __shared__ LevelDesc LV[kMaxTailLevels];
+  __shared__ double sh[2 * kSTailThreads / 64];
+  __shared__ double2 bcc;
+  {
+    const int* src = reinterpret_cast<const int*>(&tails[blockIdx.x]);
+    int* dst = reinterpret_cast<int*>(&D);
+    for (int w = threadIdx.x; w < int(sizeof(TailDesc) / 4); w += kSTailThreads) dst[w] = src[w];
+    __syncthreads();
+    const int* ls = reinterpret_cast<const int*>(&levels[D.first]);
+    int* ld = reinterpret_cast<int*>(LV);
+    const int nw = D.nlev * int(sizeof(LevelDesc) / 4);
+    for (int w = threadIdx.x; w < nw; w += kSTailThreads) ld[w] = ls[w];
+    __syncthreads();
+  }
+  const int p = D.problem;
+  STailCtx c;
+  c.xb = smem;
+  c.xh = smem + kSTailXbBytes;
+  c.tab = reinterpret_cast<double*>(smem + kSTailXbBytes + kSTailXhBytes);
+  c.LV = LV;
+  c.nlev = D.nlev;
+  c.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  c.lane = threadIdx.x & 63;
+  c.pre = pre;
+  c.post = post;
+  c.csweeps = coarsest_sweeps;
+  c.order = order;
+  c.hbuf = 0;
+  c.exact = D.dct != nullptr;
+  const LevelDesc& L0 = LV[0];
+  const LevelDesc& Lc = LV[D.nlev - 1];
+  const int ntab = c.exact ? Lc.m * Lc.m + Lc.n * Lc.n + Lc.m * Lc.n : 0;
+  c.w0 = c.tab + ntab;
+  c.w1 = c.w0 + Lc.m * Lc.n;
+
+  // right-hand side of the first tail level (issued before the convergence test)
+  V2 b[R0];
+  {
+    const GDP_GLOBAL T* g = (const GDP_GLOBAL T*)L0.b;
+    const int gj = 2 * c.lane;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const int gi = c.w * R0 + r;
+      b[r] = V2{T(0), T(0)};
+      if (gi < L0.m && gj < L0.n) b[r] = *(const GDP_GLOBAL V2*)(g + (size_t)gi * L0.pitch + gj);
+    }
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+      if (gj + 1 >= L0.n) b[r].y = T(0);
+  }
+  if (st[p].done) return;
+  // a problem whose finest level is in the tail gets its convergence test here
+  if (check_conv && D.krylov >= 0 &&
+      check_converged<kSTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
+    return;
+  if (c.exact)
+    for (int o = threadIdx.x; o < ntab; o += kSTailThreads) c.tab[o] = D.dct[o];
+  // (the first barrier inside the level recursion orders the table loads)
+
+  st_level<T, 0>(c, b);
+
+  // the first tail level's iterate, from the exchange buffer, to global memory
+  const V2* xb = reinterpret_cast<const V2*>(c.xb);
+  GDP_GLOBAL T* g = (GDP_GLOBAL T*)L0.u;
+  const bool dots = DOTS && D.krylov >= 0;
+  const GDP_GLOBAL double* kr = dots ? (const GDP_GLOBAL double*)kds[D.krylov].r : nullptr;
+  const GDP_GLOBAL double* kq = dots ? (const GDP_GLOBAL double*)kds[D.krylov].q : nullptr;
+  double zr = 0.0, zq = 0.0;
+  const int gj = 2 * c.lane;
+#pragma unroll
+  for (int r = 0; r < R0; ++r) {
+    const int gi = c.w * R0 + r;
+    if (gi >= L0.m || gj >= L0.n) continue;
+    const V2 z = xb[gi * (kSTailCols / 2) + c.lane];
+    const size_t o = (size_t)gi * L0.pitch + gj;
+    g[o] = z.x;
+    if (gj + 1 < L0.n) g[o + 1] = z.y;
+    if (dots) {
+      zr += double(z.x) * kr[o];
+      zq += double(z.x) * kq[o];
+      if (gj + 1 < L0.n) {
+        zr += double(z.y) * kr[o + 1];
+        zq += double(z.y) * kq[o + 1];
+      }
+    }
+  }
+  if (dots) write_partial<kSTailThreads>(zr, zq, zpart, rz[p].begin, sh);
+}
+
+}  // namespace gdp

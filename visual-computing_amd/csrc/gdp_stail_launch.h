@@ -1,0 +1,29 @@
+// Host dispatch of the register-strip coarse tail.
+#pragma once
+#include "gdp_internal.h"
+#include "gdp_stail.h"
+
+namespace gdp {
+
+template <typename T>
+void launch_stail(bool dots, int blocks, int smem, hipStream_t s, const TailArgs& a) {
+  if (dots)
+    k_stail<T, true><<<blocks, kSTailThreads, smem, s>>>(a.tails, a.levels, a.st, a.kds, a.upart,
+                                                        a.rk, a.zpart, a.rz, a.check_conv, a.pre,
+                                                        a.post, a.coarsest_sweeps, a.order);
+  else
+    k_stail<T, false><<<blocks, kSTailThreads, smem, s>>>(a.tails, a.levels, a.st, a.kds, a.upart,
+                                                         a.rk, a.zpart, a.rz, a.check_conv, a.pre,
+                                                         a.post, a.coarsest_sweeps, a.order);
+}
+
+template <typename T>
+hipError_t stail_set_smem(int smem) {
+  hipError_t e = hipFuncSetAttribute((const void*)k_stail<T, true>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void*)k_stail<T, false>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+}
+
+}  // namespace gdp

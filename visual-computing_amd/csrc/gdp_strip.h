@@ -1,0 +1,514 @@
+// Register-resident wave strips: the smoothing kernels of the tiled multigrid levels.
+//
+// A wave owns a strip 128 fine columns wide and FH rows tall.  Lane l keeps the two
+// adjacent columns 2l (x) and 2l+1 (y) of every row of the strip in registers, so in a
+// red-black half-sweep a point finds its vertical neighbours and one horizontal
+// neighbour in its own lane and the other horizontal neighbour one lane away, fetched
+// with a DPP wave shift.  A half-sweep of a row costs one update per lane (x or y,
+// fixed by the row's colour) of five VALU operations; there is no LDS traffic and no
+// barrier.  Each block stacks four independent strips (its tile record covers four
+// strips' worth of rows) so the block-level reductions of the PCG scalars keep their
+// one-slot-per-block layout.
+//
+// Halo: a strip is loaded with a margin around the points it owns.  After k
+// half-sweeps only the points at least k rows/columns inside the strip agree with a
+// whole-grid sweep (the strip's outer ring is never updated, or is updated from the
+// zeros a wave shift brings in), so the margins below are sized for the number of
+// sweeps; everything the kernels write lies inside the exact region.
+//
+// Boundaries: points outside the grid stay zero (inverse diagonal 0, right-hand side
+// 0), so the sum of the four neighbours is the sum over in-grid neighbours for both
+// boundary conditions; only the inverse diagonal differs (1/4 Dirichlet, 1/#in-grid
+// neighbours Neumann).  It is kept per lane for the three possible vertical counts of
+// a row, and the row picks one.  A strip whose rows are all strictly inside the grid
+// runs a variant with no per-row tests.
+//
+// Arithmetic follows gradient-domain/src/gradient_domain/multigrid.py: smoother
+// :147-166 (colour 0 then 1, or the reverse for the symmetric Neumann post-smoother),
+// residual :210, restriction 0.25 P^T r P :169-174, prolongation P e P^T :177-181.
+#pragma once
+#include "gdp_reduce.h"
+
+namespace gdp {
+
+#define GDP_GLOBAL __attribute__((address_space(1)))
+
+template <typename T>
+struct Pair;
+template <>
+struct Pair<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// Wave shifts: lane l receives lane l-1's value (shr) or lane l+1's (shl); the lane
+// with no source receives 0.  Every lane of the wave must be active.
+__device__ inline int dpp_shr1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ inline int dpp_shl1_i(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ inline float dpp_shr1(float v) { return __int_as_float(dpp_shr1_i(__float_as_int(v))); }
+__device__ inline float dpp_shl1(float v) { return __int_as_float(dpp_shl1_i(__float_as_int(v))); }
+__device__ inline double dpp_shr1(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_shr1_i(int(unsigned(b)));
+  const unsigned hi = (unsigned)dpp_shr1_i(int(unsigned(b >> 32)));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline double dpp_shl1(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_shl1_i(int(unsigned(b)));
+  const unsigned hi = (unsigned)dpp_shl1_i(int(unsigned(b >> 32)));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 1 / k for the neighbour counts k = 1..4 (0 for k <= 0).
+template <typename T>
+__device__ inline T inv_count(int k) {
+  return k == 4 ? T(0.25) : (k == 3 ? T(1.0 / 3.0) : (k == 2 ? T(0.5) : (k == 1 ? T(1) : T(0))));
+}
+
+// Geometry of the two strip kernels for NU pre/post sweeps.
+template <int NU>
+struct StripGeo {
+  // down: coarse tile columns per strip and the fine offset of the tile's first point
+  static constexpr int kOff = 2 * NU + 2;
+  static constexpr int kDownCols = (125 - 4 * NU) / 2;
+  // up: margins (rows / columns) and owned fine columns per strip.  The columns get
+  // one more pair: the interpolation of a strip's first and last column needs coarse
+  // values from outside the wave, so those two columns start out inexact.
+  static constexpr int kHalo = 2 * NU;
+  static constexpr int kColHalo = 2 * NU + 2;
+  static constexpr int kUpCols = 128 - 2 * kColHalo;
+};
+
+// Per-lane column data of a strip whose first fine column is gc0 (even).
+template <typename T>
+struct LaneCols {
+  int gjx;       // global column of x (y is gjx + 1)
+  int inx, iny;  // columns inside the grid (0 / 1)
+  int cx, cy;    // in-grid horizontal neighbours (Neumann) or 2 (Dirichlet)
+  T hx, hy;      // the same as T
+  T ix2, iy2;    // inverse diagonal of a row with two in-grid vertical neighbours
+
+  __device__ __attribute__((always_inline)) void init(int gc0, int lane, int n, bool neu) {
+    gjx = gc0 + 2 * lane;
+    inx = gjx >= 0 && gjx < n;
+    iny = gjx + 1 >= 0 && gjx + 1 < n;
+    cx = neu ? (gjx > 0) + (gjx < n - 1) : 2;
+    cy = neu ? (gjx + 1 > 0) + (gjx + 1 < n - 1) : 2;
+    hx = T(cx);
+    hy = T(cy);
+    ix2 = inx ? inv_count<T>(cx + 2) : T(0);
+    iy2 = iny ? inv_count<T>(cy + 2) : T(0);
+  }
+  // Inverse diagonal for a row with vc in-grid vertical neighbours.  Rows with fewer
+  // than two (the grid's first and last) compute it; a select between two stored
+  // fields would be merged into one load from a computed address and force the
+  // record into memory.
+  __device__ __attribute__((always_inline)) T wx(int vc) const {
+    return vc == 2 ? ix2 : (inx ? inv_count<T>(cx + vc) : T(0));
+  }
+  __device__ __attribute__((always_inline)) T wy(int vc) const {
+    return vc == 2 ? iy2 : (iny ? inv_count<T>(cy + vc) : T(0));
+  }
+};
+
+// Vertical in-grid neighbour count of global row gi (2 for Dirichlet: diag is 4).
+__device__ inline int row_count(int gi, int m, bool neu) {
+  return neu ? (gi > 0) + (gi < m - 1) : 2;
+}
+
+// One red-black half-sweep over rows 1 .. FH-2 of a strip.  Row a updates x when
+// (a + S + FLIP) is even, else y.  EDGE: rows may lie outside the grid or on its
+// first/last row.
+template <typename T, int FH, int S, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
+                                       const LaneCols<T>& lc, int gr0, int m, bool neu) {
+#pragma unroll
+  for (int a = 1; a < FH - 1; ++a) {
+    int vc = 2;
+    if (EDGE) {
+      const int gi = gr0 + a;
+      if (gi < 0 || gi >= m) continue;
+      vc = row_count(gi, m, neu);
+    }
+    if (((a + S + FLIP) & 1) == 0) {
+      const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
+      const T w = EDGE ? lc.wx(vc) : lc.ix2;
+      u[a].x = w * nb - hb[a].x;
+    } else {
+      const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
+      const T w = EDGE ? lc.wy(vc) : lc.iy2;
+      u[a].y = w * nb - hb[a].y;
+    }
+  }
+}
+
+template <typename T, int FH, int S0, int NS, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void strip_sweeps(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
+                                    const LaneCols<T>& lc, int gr0, int m, bool neu) {
+  if constexpr (NS > 0) {
+    strip_halfsweep<T, FH, S0, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+    strip_sweeps<T, FH, S0 + 1, NS - 1, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+  }
+}
+
+// ------------------------------------------------------------------ down
+// Pre-smoothing (NU sweeps from a zero guess) + residual + restriction.  A strip
+// produces CR coarse rows x kDownCols coarse columns of the next level's right-hand
+// side and the pre-smoothed iterate v of the fine points those coarse points own.
+//
+// Fine rows of the strip: gr0 = 2 I0 - kOff .. gr0 + FH - 1 (likewise columns, 128 of
+// them).  The residual is kept scaled by h^2: r' = h^2 b - (sum - diag u) =
+// diag (hb' + u) - sum, with hb' = h^2 b / diag, and the restriction applies
+// 0.25 / h^2.  Neumann restriction taps fine 2I-1 .. 2I+2 (1/4, 3/4, 3/4, 1/4) of the
+// residual extended by r[-1] = r[0] and, for even sizes, r[f] = r[f-1]; Dirichlet taps
+// 2I .. 2I+2 (1/2, 1, 1/2).
+template <typename T, int NU, int CR, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0, int lane, typename Pair<T>::type* hb) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int OFF = G::kOff, CC = G::kDownCols;
+  constexpr int FH = 2 * CR + (NU > 0 ? 4 * NU + 3 : 4);
+  const int m = t.m, n = t.n;
+  const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const bool neu = t.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(gc0, lane, n, neu);
+  const T h2 = T(t.h2);
+
+  // hb' = h^2 b / diag (zero outside the grid) and the first colour-0 half-sweep from
+  // the zero guess, u = -hb' on colour 0
+  V2 u[FH];
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const int gi = gr0 + a;
+    int vc = 2;
+    bool rin = true;
+    if (EDGE) {
+      rin = gi >= 0 && gi < m;
+      vc = row_count(gi, m, neu);
+    }
+    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+    hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
+    hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
+    if (NU > 0 && ((a + FLIP) & 1) == 0) {
+      u[a].x = -hb[a].x;
+      u[a].y = T(0);
+    } else if (NU > 0) {
+      u[a].x = T(0);
+      u[a].y = -hb[a].y;
+    } else {
+      u[a] = V2{T(0), T(0)};
+    }
+  }
+  if constexpr (NU > 0) strip_sweeps<T, FH, 1, 2 * NU - 1, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+
+  // pre-smoothed iterate of the owned fine points (the last strip row / column of the
+  // grid owns every remaining fine row / column)
+  {
+    const int fr0 = 2 * I0, fc0 = 2 * t.j0;
+    const int fr1 = (I0 + CR >= t.cm) ? m : min(m, fr0 + 2 * CR);
+    const int fc1 = (t.j0 + CC >= t.cn) ? n : min(n, fc0 + 2 * CC);
+    GDP_GLOBAL T* v = (GDP_GLOBAL T*)t.v;
+    const bool wx = lc.gjx >= fc0 && lc.gjx < fc1;
+    const bool wy = lc.gjx + 1 >= fc0 && lc.gjx + 1 < fc1;
+#pragma unroll
+    for (int a = OFF; a < OFF + 2 * CR + 2 && a < FH; ++a) {
+      const int gi = gr0 + a;
+      if (gi >= fr1) break;
+      GDP_GLOBAL T* row = v + (size_t)gi * t.pitch + lc.gjx;
+      if (wx && wy) {
+        *(GDP_GLOBAL V2*)row = u[a];
+      } else if (wx) {
+        row[0] = u[a].x;
+      }
+    }
+  }
+
+  // scaled residual over the rows the restriction reads, in place of hb'
+  constexpr int R0 = OFF - 1, R1 = OFF + 2 * CR;  // inclusive
+#pragma unroll
+  for (int a = R0; a <= R1; ++a) {
+    int vc = 2;
+    bool rin = true;
+    if (EDGE) {
+      const int gi = gr0 + a;
+      rin = gi >= 0 && gi < m;
+      vc = row_count(gi, m, neu);
+    }
+    const T cx = T(vc) + lc.hx, cy = T(vc) + lc.hy;
+    const T nx = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
+    const T ny = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
+    const T rx = cx * (hb[a].x + u[a].x) - nx;
+    const T ry = cy * (hb[a].y + u[a].y) - ny;
+    hb[a].x = (rin && lc.inx) ? rx : T(0);
+    hb[a].y = (rin && lc.iny) ? ry : T(0);
+  }
+  if (neu) {
+    // residual extension past the grid: columns first, then rows (corners take both)
+    if (gc0 < 0 || gc0 + 128 > n) {
+      const bool gl = lc.gjx + 1 == -1;
+      const bool gr = lc.gjx == n && (n & 1) == 0;
+#pragma unroll
+      for (int a = R0; a <= R1; ++a) {
+        const T sx = dpp_shl1(hb[a].x);
+        const T sy = dpp_shr1(hb[a].y);
+        if (gl) hb[a].y = sx;
+        if (gr) hb[a].x = sy;
+      }
+    }
+    if (EDGE) {
+#pragma unroll
+      for (int a = R0; a <= R1; ++a) {
+        const int gi = gr0 + a;
+        if (gi == -1 && a < R1) hb[a] = hb[a + 1];
+        if (gi == m && (m & 1) == 0 && a > R0) hb[a] = hb[a - 1];
+      }
+    }
+  }
+
+  // restriction to the coarse right-hand side
+  GDP_GLOBAL T* cb = (GDP_GLOBAL T*)t.cb;
+  const T s = T(0.25) * T(t.inv_h2);
+  const int jc = lane - OFF / 2;
+  const int J = t.j0 + jc;
+  const bool wj = jc >= 0 && jc < CC && J < t.cn;
+#pragma unroll
+  for (int i = 0; i < CR; ++i) {
+    const int I = I0 + i;
+    T c;
+    if (neu) {
+      const int a0 = 2 * i + OFF - 1;
+      const T vx = (T(0.25) * s) * hb[a0].x + (T(0.75) * s) * hb[a0 + 1].x +
+                   (T(0.75) * s) * hb[a0 + 2].x + (T(0.25) * s) * hb[a0 + 3].x;
+      const T vy = (T(0.25) * s) * hb[a0].y + (T(0.75) * s) * hb[a0 + 1].y +
+                   (T(0.75) * s) * hb[a0 + 2].y + (T(0.25) * s) * hb[a0 + 3].y;
+      c = T(0.25) * dpp_shr1(vy) + T(0.75) * vx + T(0.75) * vy + T(0.25) * dpp_shl1(vx);
+    } else {
+      const int a0 = 2 * i + OFF;
+      const T vx = (T(0.5) * s) * hb[a0].x + s * hb[a0 + 1].x + (T(0.5) * s) * hb[a0 + 2].x;
+      const T vy = (T(0.5) * s) * hb[a0].y + s * hb[a0 + 1].y + (T(0.5) * s) * hb[a0 + 2].y;
+      c = T(0.5) * vx + vy + T(0.5) * dpp_shl1(vx);
+    }
+    if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
+  }
+}
+
+template <typename T, int NU, int CR>
+__global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ tiles,
+                                                    ProblemState* __restrict__ st,
+                                                    const double2* __restrict__ upart,
+                                                    int check_conv) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int OFF = G::kOff;
+  constexpr int FH = 2 * CR + (NU > 0 ? 4 * NU + 3 : 4);
+  const VTile t = tiles[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int I0 = t.i0 + w * CR;
+  const bool work = I0 < t.cm;
+  const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const int m = t.m, n = t.n;
+
+  // every load of the strip is issued before anything waits on one
+  V2 hb[FH];
+  {
+    const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const int gjx = gc0 + 2 * lane;
+    const bool lin = gjx >= 0 && gjx < n;
+#pragma unroll
+    for (int a = 0; a < FH; ++a) {
+      const int gi = gr0 + a;
+      hb[a] = V2{T(0), T(0)};
+      if (work && lin && gi >= 0 && gi < m)
+        hb[a] = *(const GDP_GLOBAL V2*)(b + (size_t)gi * t.pitch + gjx);
+    }
+  }
+  if (st[t.problem].done) return;
+  if (check_conv && t.level == 0) {
+    // convergence test of the preceding PCG update (the loads above are in flight)
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
+                                  shc, &bcc))
+      return;
+  }
+  if (!work) return;
+  const bool edge = gr0 < 1 || gr0 + FH > m - 1;
+  const int flip = (gr0 + gc0) & 1;
+  if (edge) {
+    if (flip) sdown_body<T, NU, CR, 1, true>(t, I0, lane, hb);
+    else sdown_body<T, NU, CR, 0, true>(t, I0, lane, hb);
+  } else {
+    if (flip) sdown_body<T, NU, CR, 1, false>(t, I0, lane, hb);
+    else sdown_body<T, NU, CR, 0, false>(t, I0, lane, hb);
+  }
+}
+
+// ------------------------------------------------------------------ up
+// Prolongation of the coarse correction + post-smoothing.  A strip owns TR fine rows x
+// kUpCols fine columns starting at (i0, j0) and loads a margin of 2 NU rows and
+// 2 NU + 2 columns around them.  With DOTS (finest level of a PCG iteration) the block also sums
+// <r, z> and <q, z> over its points.
+template <typename T, int NU, int TR, int FLIP, bool EDGE, bool DOTS>
+__device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, int lane, typename Pair<T>::type* u,
+                         typename Pair<T>::type* hb, const T* E, double& zr, double& zq) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int H = G::kHalo, TC = G::kUpCols;
+  constexpr int FH = TR + 2 * H;
+  const int m = t.m, n = t.n;
+  const int gr0 = i0 - H, gc0 = t.j0 - G::kColHalo;
+  const bool neu = t.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(gc0, lane, n, neu);
+  const T h2 = T(t.h2);
+
+  // u = v + P e (zero outside the grid); hb' = h^2 b / diag
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const int gi = gr0 + a;
+    int vc = 2;
+    bool rin = true;
+    if (EDGE) {
+      rin = gi >= 0 && gi < m;
+      vc = row_count(gi, m, neu);
+    }
+    T cx, cy;
+    if (neu) {
+      const int P = a / 2 + 1;
+      const int O = (a & 1) ? P + 1 : P - 1;
+      const T tv = T(0.75) * E[P] + T(0.25) * E[O];
+      cx = T(0.75) * tv + T(0.25) * dpp_shr1(tv);
+      cy = T(0.75) * tv + T(0.25) * dpp_shl1(tv);
+    } else {
+      const T tv = (a & 1) ? E[(a + 1) / 2] : T(0.5) * (E[a / 2] + E[a / 2 + 1]);
+      cx = T(0.5) * (dpp_shr1(tv) + tv);
+      cy = tv;
+    }
+    u[a].x = (rin && lc.inx) ? u[a].x + cx : T(0);
+    u[a].y = (rin && lc.iny) ? u[a].y + cy : T(0);
+    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+    hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
+    hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
+  }
+  strip_sweeps<T, FH, 0, 2 * NU, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+
+  GDP_GLOBAL T* uo = (GDP_GLOBAL T*)t.u;
+  const int fc1 = min(n, t.j0 + TC);
+  const bool wx = lc.gjx >= t.j0 && lc.gjx < fc1;
+  const bool wy = lc.gjx + 1 >= t.j0 && lc.gjx + 1 < fc1;
+  const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
+  const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
+#pragma unroll
+  for (int a = H; a < H + TR; ++a) {
+    const int gi = gr0 + a;
+    if (gi >= m) break;
+    const size_t o = (size_t)gi * t.pitch + lc.gjx;
+    if (wx && wy) {
+      *(GDP_GLOBAL V2*)(uo + o) = u[a];
+    } else if (wx) {
+      uo[o] = u[a].x;
+    }
+    if (DOTS) {
+      // the finest level shares the Krylov grid's pitch
+      double rx = 0.0, ry = 0.0, qx = 0.0, qy = 0.0;
+      if (wx && wy) {
+        using D2 = typename Pair<double>::type;
+        const D2 r2 = *(const GDP_GLOBAL D2*)(kr + o);
+        const D2 q2 = *(const GDP_GLOBAL D2*)(kq + o);
+        rx = r2.x;
+        ry = r2.y;
+        qx = q2.x;
+        qy = q2.y;
+      } else if (wx) {
+        rx = kr[o];
+        qx = kq[o];
+      }
+      zr += double(u[a].x) * rx + double(u[a].y) * ry;
+      zq += double(u[a].x) * qx + double(u[a].y) * qy;
+    }
+  }
+}
+
+template <typename T, int NU, int TR, bool DOTS>
+__global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tiles,
+                                                  const ProblemState* __restrict__ st,
+                                                  double2* __restrict__ zpart, int order) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int H = G::kHalo;
+  constexpr int FH = TR + 2 * H;
+  constexpr int NE = FH / 2 + 2;
+  __shared__ double sh[2 * kThreads / 64];
+  const VTile t = tiles[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int i0 = t.i0 + w * TR;
+  const bool work = i0 < t.m;
+  const int gr0 = i0 - H, gc0 = t.j0 - G::kColHalo;
+  const int m = t.m, n = t.n;
+  const bool neu = t.bc == kNeumann;
+
+  // loads: v and b of the strip, and the coarse correction rows it interpolates from
+  // (edge-replicated for Neumann, zero outside for Dirichlet)
+  V2 u[FH], hb[FH];
+  T E[NE];
+  {
+    const GDP_GLOBAL T* vv = (const GDP_GLOBAL T*)t.v;
+    const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const GDP_GLOBAL T* e = (const GDP_GLOBAL T*)t.cu;
+    const int gjx = gc0 + 2 * lane;
+    const bool lin = gjx >= 0 && gjx < n;
+    const int cr0 = gr0 / 2 - 1;
+    int K = gc0 / 2 + lane;
+    bool kin = K >= 0 && K < t.cn;
+    if (neu) {
+      K = min(max(K, 0), t.cn - 1);
+      kin = true;
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      int I = cr0 + k;
+      bool in = work && kin && I >= 0 && I < t.cm;
+      if (neu) {
+        I = min(max(I, 0), t.cm - 1);
+        in = work;
+      }
+      E[k] = in ? e[(size_t)I * t.cpitch + K] : T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < FH; ++a) {
+      const int gi = gr0 + a;
+      u[a] = V2{T(0), T(0)};
+      hb[a] = V2{T(0), T(0)};
+      if (work && lin && gi >= 0 && gi < m) {
+        const size_t o = (size_t)gi * t.pitch + gjx;
+        u[a] = *(const GDP_GLOBAL V2*)(vv + o);
+        hb[a] = *(const GDP_GLOBAL V2*)(b + o);
+      }
+    }
+  }
+  if (st[t.problem].done) return;
+  double zr = 0.0, zq = 0.0;
+  if (work) {
+    const bool edge = gr0 < 1 || gr0 + FH > m - 1;
+    const int flip = ((gr0 + gc0) + order) & 1;
+    if (edge) {
+      if (flip) sup_body<T, NU, TR, 1, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+      else sup_body<T, NU, TR, 0, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+    } else {
+      if (flip) sup_body<T, NU, TR, 1, false, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+      else sup_body<T, NU, TR, 0, false, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+    }
+  }
+  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
+}
+
+}  // namespace gdp

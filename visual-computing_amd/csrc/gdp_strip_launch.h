@@ -1,0 +1,61 @@
+// Host dispatch of the strip kernels: (menu, nu) -> template instance.  Included by
+// one small translation unit per kernel and element type so they compile in parallel.
+#pragma once
+#include "gdp_internal.h"
+#include "gdp_strip.h"
+
+namespace gdp {
+
+template <typename T, int CR>
+void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWN(NU) \
+  k_sdown<T, NU, CR><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
+  switch (nu) {
+    case 0: GDP_DOWN(0); break;
+    case 1: GDP_DOWN(1); break;
+    case 2: GDP_DOWN(2); break;
+    default: GDP_DOWN(3); break;
+  }
+#undef GDP_DOWN
+}
+
+template <typename T>
+void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
+  switch (menu) {
+    case 0: down_nu<T, 1>(nu, blocks, s, a); break;
+    case 1: down_nu<T, 2>(nu, blocks, s, a); break;
+    case 2: down_nu<T, 4>(nu, blocks, s, a); break;
+    default: down_nu<T, 8>(nu, blocks, s, a); break;
+  }
+}
+
+template <typename T, int TR, bool DOTS>
+void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
+#define GDP_UP(NU) \
+  k_sup<T, NU, TR, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
+  switch (nu) {
+    case 0: GDP_UP(0); break;
+    case 1: GDP_UP(1); break;
+    case 2: GDP_UP(2); break;
+    default: GDP_UP(3); break;
+  }
+#undef GDP_UP
+}
+
+template <typename T, bool DOTS>
+void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
+  switch (menu) {
+    case 0: up_nu<T, 2, DOTS>(nu, blocks, s, a); break;
+    case 1: up_nu<T, 4, DOTS>(nu, blocks, s, a); break;
+    case 2: up_nu<T, 8, DOTS>(nu, blocks, s, a); break;
+    default: up_nu<T, 16, DOTS>(nu, blocks, s, a); break;
+  }
+}
+
+template <typename T>
+void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a) {
+  if (dots) launch_up_d<T, true>(menu, nu, blocks, s, a);
+  else launch_up_d<T, false>(menu, nu, blocks, s, a);
+}
+
+}  // namespace gdp

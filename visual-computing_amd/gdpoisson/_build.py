@@ -1,8 +1,9 @@
 """Build libgdpoisson.so (HIP, gfx950) in-tree with hipcc.
 
 ``python -m gdpoisson._build`` (with ``visual-computing_amd`` on ``sys.path``) or
-``build()`` from Python.  The shared library is written next to this file so it
-travels with the source tree.
+``build()`` from Python.  Every source compiles to its own object in parallel (the
+strip kernels are split per kernel and element type for that), then one link makes
+the shared library next to this file, so it travels with the source tree.
 """
 
 from __future__ import annotations
@@ -11,6 +12,8 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -18,7 +21,16 @@ CSRC = PKG.parent / "csrc"
 INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libgdpoisson.so"
 
-SOURCES = ["gdp_plan.hip", "gdp_ops.hip"]
+SOURCES = [
+    "gdp_plan.hip",
+    "gdp_ops.hip",
+    "gdp_sdown_f32.hip",
+    "gdp_sdown_f64.hip",
+    "gdp_sup_f32.hip",
+    "gdp_sup_f64.hip",
+    "gdp_stail_f32.hip",
+    "gdp_stail_f64.hip",
+]
 ARCH = os.environ.get("GDP_OFFLOAD_ARCH", "gfx950")
 
 
@@ -29,35 +41,50 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgdpoisson")
 
 
-def command(out: Path) -> list[str]:
+def compile_command(src: Path, obj: Path) -> list[str]:
     return [
         hipcc(),
         f"--offload-arch={ARCH}",
         "-O3",
         "-std=c++17",
         "-fPIC",
-        "-shared",
+        "-c",
         # no packed-FP32 instructions in device code
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wno-unused-command-line-argument",
         f"-I{INCLUDE}",
         f"-I{CSRC}",
-        *[str(CSRC / s) for s in SOURCES],
+        str(src),
         "-o",
-        str(out),
+        str(obj),
     ]
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = LIB.with_suffix(".so.tmp")
-    cmd = command(tmp)
+def link_command(objs: list[Path], out: Path) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(out)]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
-    os.replace(tmp, LIB)
+        raise RuntimeError(f"hipcc failed with status {res.returncode}: {cmd[-3]}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    jobs = jobs or min(len(SOURCES), max(1, (os.cpu_count() or 1)))
+    with tempfile.TemporaryDirectory(prefix="gdp_build_") as tmpd:
+        objs = [Path(tmpd) / (Path(s).stem + ".o") for s in SOURCES]
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
+            futs = [ex.submit(_run, compile_command(CSRC / s, o), verbose)
+                    for s, o in zip(SOURCES, objs)]
+            for f in futs:
+                f.result()
+        tmp = LIB.with_suffix(".so.tmp")
+        _run(link_command(objs, tmp), verbose)
+        os.replace(tmp, LIB)
     return LIB
 
 

@@ -112,7 +112,7 @@ def _declare(lib):
         "gdp_plan_levels": (c_int, [P, c_int, POINTER(c_int)]),
         "gdp_plan_level_shape": (c_int, [P, c_int, c_int, POINTER(c_int), POINTER(c_int)]),
         "gdp_plan_set_tile": (c_int, [P, c_int, c_int, c_int]),
-        "gdp_tile_menu": (c_int, [c_int, c_int, POINTER(c_int), POINTER(c_int)]),
+        "gdp_tile_menu": (c_int, [c_int, c_int, c_int, POINTER(c_int), POINTER(c_int)]),
         "gdp_vcycle": (c_int, [P, PP, PP, PP, P]),
         "gdp_mg_solve": (c_int, [P, PP, PP, PP, POINTER(Report), POINTER(c_double), P]),
         "gdp_mgcg_solve": (c_int, [P, PP, PP, POINTER(Report), P]),

@@ -238,3 +238,46 @@ def test_batched_uneven_problems_match_each_alone(gd, precision):
         assert reps[k].iterations == rep1[0].iterations
         assert torch.equal(together[k], alone[0]), (k, s)
         assert np.abs(together[k].cpu().numpy() - exact[k]).max() < 1e-6
+
+
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("menu", [0, 1, 2, 3])
+def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):
+    """Each compiled strip height (down and up, every tiled level) gives the oracle's cycle."""
+    import torch
+
+    shape = (300, 260)
+    rng = np.random.default_rng(50 + menu)
+    b = rng.standard_normal(shape)
+    u = rng.standard_normal(shape)
+    kw = dict(pre_smoothing=2, post_smoothing=2)
+    if bc == "neumann":
+        kw.update(coarsest_size=16)
+    tiles = {(k, l): menu for k in (0, 1) for l in range(4)}
+    tiles[(2, 0)] = menu
+    plan = gd.Plan([shape], boundary=bc, options=gd.PlanOptions(**kw), tiles=tiles)
+    (got,) = plan.vcycle([torch.from_numpy(u).cuda()], [torch.from_numpy(b).cuda()])
+    want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(**kw), bc)
+    assert rel(got.cpu().numpy(), want) < 1e-11
+
+
+def test_autotuned_plan_solves_the_same_problem(gd):
+    import torch
+
+    shape = (512, 384)
+    bs, exact = [], []
+    for c in range(3):
+        b, x = G.neumann_problem(shape, seed=60 + c)
+        bs.append(torch.from_numpy(b).cuda())
+        exact.append(x)
+    opts = gd.PlanOptions(precision="mixed")
+    ref, rep0 = gd.Plan([shape] * 3, boundary="neumann", options=opts).mgcg(bs)
+    plan = gd.Plan([shape] * 3, boundary="neumann", options=opts)
+    choice = gd.autotune(plan, use_cache=False)
+    assert set(k for k, _ in choice) <= {0, 1, 2} and all(0 <= v < 4 for v in choice.values())
+    got, rep = plan.mgcg(bs)
+    for c in range(3):
+        assert rep[c].converged and rep[c].relative_residual <= 1e-8
+        assert abs(rep[c].iterations - rep0[c].iterations) <= 1
+        assert np.abs(got[c].cpu().numpy() - exact[c]).max() < 1e-6
+        assert torch.allclose(got[c], ref[c], rtol=0, atol=1e-7)

@@ -61,7 +61,8 @@ struct ProblemState {
   int32_t restart;   // next preconditioned direction starts a new Krylov sequence
   int32_t max_iter;
   int32_t flexible;
-  int32_t pad[2];
+  int32_t checked;   // done, true residual checked, output written (PCG)
+  int32_t pad;
 };
 
 // One workgroup's work item of a V-cycle kernel: everything the block needs about its

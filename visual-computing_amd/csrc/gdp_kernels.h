@@ -599,6 +599,12 @@ __global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ til
   write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
 }
 
+// Which problems a set-up / check kernel processes: respect 0 every problem, 1 those
+// not done, 2 those done whose true residual has not been checked yet.
+__device__ inline bool skip_problem(const ProblemState& S, int respect) {
+  return respect == 1 ? S.done != 0 : (respect == 2 ? (!S.done || S.checked) : false);
+}
+
 // r_true = (b - mean) - L x (FP64): partial sums of ||r_true||^2 and of x; the V-cycle
 // RHS becomes r_true and, in mode 0 (PCG), r = -r_true so an unconverged problem can
 // restart from it.
@@ -610,7 +616,7 @@ __global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__
                                                        int respect_done) {
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
-  if (respect_done && st[p].done) return;
+  if (skip_problem(st[p], respect_done)) return;
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sx[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
@@ -673,7 +679,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
   const int p = blockIdx.x;
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
-  if (kind == 2 && respect_done && st[p].done) return;
+  if (kind == 2 && skip_problem(st[p], respect_done)) return;
   const double2 tot = reduce_slots<kThreads>(spart, rk[p], sh, &bc);
   if (threadIdx.x != 0) return;
   ProblemState& S = st[p];
@@ -687,6 +693,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
     S.converged = zero_done && tot.x == 0.0;
     S.done = S.converged || max_iter <= 0;
     S.restart = 0;
+    S.checked = 0;
     S.alpha = S.beta = S.delta = S.delta_old = 0.0;
     S.rr = tot.x;
     S.max_iter = max_iter;
@@ -725,14 +732,16 @@ __global__ __launch_bounds__(kThreads) void k_correct(const KTile* __restrict__ 
   }
 }
 
-// Caller output: u = x - mean(x) for Neumann problems when mean_fix, else u = x.
+// Caller output: u = x - mean(x) for Neumann problems when mean_fix, else u = x.  With
+// only_checked, only problems whose true residual has been checked are written.
 template <int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_finalize(const KTile* __restrict__ tiles,
                                                        const IoDesc* __restrict__ io,
                                                        const ProblemState* __restrict__ st,
-                                                       int mean_fix) {
+                                                       int mean_fix, int only_checked) {
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
+  if (only_checked && !st[p].checked) return;
   const double mean = (mean_fix && t.bc == kNeumann)
                           ? st[p].sumx / (double(t.m) * double(t.n))
                           : 0.0;
@@ -747,22 +756,31 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const KTile* __restrict__
   }
 }
 
-// Restart bookkeeping after the FP64 true-residual check: a problem whose recursive
-// residual converged but whose true residual did not is reopened with a fresh Krylov
-// sequence (residual replacement) if it has iterations left.
-__global__ void k_reopen(ProblemState* __restrict__ st, int count, int* __restrict__ reopened) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  ProblemState& S = st[p];
-  const bool bad = S.true_rr > S.target2 && S.bnorm2 > 0.0;
-  if (bad && S.iter < S.max_iter) {
-    S.converged = 0;
-    S.done = 0;
-    S.restart = 1;
-    atomicAdd(reopened, 1);
-  } else if (bad) {
-    S.converged = 0;
+// Restart bookkeeping after the FP64 true-residual check, for the problems that are
+// done and not yet checked: one whose recursive residual converged but whose true
+// residual did not is reopened with a fresh Krylov sequence (residual replacement) if
+// it has iterations left and allow is set; every other one is marked checked.  One
+// block; *reopened = number of problems reopened (a plain store, no atomics).
+__global__ __launch_bounds__(kThreads) void k_reopen(ProblemState* __restrict__ st, int count,
+                                                     int allow, int* __restrict__ reopened) {
+  __shared__ double sh[kThreads / 64];
+  int mine = 0;
+  for (int p = threadIdx.x; p < count; p += kThreads) {
+    ProblemState& S = st[p];
+    if (!S.done || S.checked) continue;
+    const bool bad = S.true_rr > S.target2 && S.bnorm2 > 0.0;
+    if (bad && allow && S.iter < S.max_iter) {
+      S.converged = 0;
+      S.done = 0;
+      S.restart = 1;
+      ++mine;
+    } else {
+      if (bad) S.converged = 0;
+      S.checked = 1;
+    }
   }
+  const double tot = block_sum<kThreads>(double(mine), sh);
+  if (threadIdx.x == 0) *reopened = int(tot);
 }
 
 }  // namespace gdp

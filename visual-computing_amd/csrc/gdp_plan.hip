@@ -1064,12 +1064,12 @@ int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
   return GDP_OK;
 }
 
-int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix) {
+int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix, int only_checked = 0) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "finalize", 16.0 * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu, (k_finalize<TH, TW><<<nb, kThreads, 0, s>>>(
-                                    t, P->d_io(), P->d_state(), mean_fix)));
+                                    t, P->d_io(), P->d_state(), mean_fix, only_checked)));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
 }
@@ -1098,12 +1098,20 @@ int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double
   return GDP_OK;
 }
 
-int read_state(gdp_plan* P, hipStream_t s) {
+int read_state(gdp_plan* P, hipStream_t s, bool with_flag = false) {
   GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
                          hipMemcpyDeviceToHost, s));
+  if (with_flag)
+    GDP_HIP(hipMemcpyAsync(P->h_flag, P->d_flag(), sizeof(int), hipMemcpyDeviceToHost, s));
   GDP_HIP(hipStreamSynchronize(s));
   if (P->prof) prof_collect(P);
   return GDP_OK;
+}
+
+bool all_checked(const gdp_plan* P) {
+  for (int p = 0; p < P->count(); ++p)
+    if (!P->h_state[p].checked) return false;
+  return true;
 }
 
 bool all_done(const gdp_plan* P) {
@@ -1161,6 +1169,10 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
   else rc = enqueue_vcycle(P, s, true, false);
   if (rc) return rc;
+  // Iterate in batches of graph-captured iteration pairs.  After each batch the problems
+  // that have finished are checked on the device without a host round trip: FP64 true
+  // residual, reopen (residual replacement) or accept, and the accepted ones' output is
+  // written; the host then reads the states once.
   int launched = 0, restarts = 0;
   const int first_pairs = std::max(1, (P->last_iters + 1) / 2);
   for (;;) {
@@ -1168,23 +1180,21 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
     for (int k = 0; k < pairs; ++k)
       if ((rc = launch_pair(P, s, jacobi))) return rc;
     launched += 2 * pairs;
-    if ((rc = read_state(P, s))) return rc;
-    if (!all_done(P)) continue;
-    // FP64 true residual of every problem; reopen the ones it rejects
-    if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 0, false))) return rc;
-    GDP_HIP(hipMemsetAsync(P->d_flag(), 0, sizeof(int), s));
-    k_reopen<<<cdiv(P->count(), 64), 64, 0, s>>>(P->d_state(), P->count(), P->d_flag());
+    if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false))) return rc;
+    k_reopen<<<1, kThreads, 0, s>>>(P->d_state(), P->count(), restarts < 4 ? 1 : 0,
+                                    P->d_flag());
     GDP_CHECK_LAUNCH();
-    GDP_HIP(hipMemcpyAsync(P->h_flag, P->d_flag(), sizeof(int), hipMemcpyDeviceToHost, s));
-    if ((rc = read_state(P, s))) return rc;
-    if (P->h_flag[0] == 0 || restarts >= 4) break;
-    ++restarts;
-    if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
-    else rc = enqueue_vcycle(P, s, true, false);
-    if (rc) return rc;
+    if ((rc = enqueue_finalize(P, s, 1, 1))) return rc;
+    if ((rc = read_state(P, s, true))) return rc;
+    if (all_checked(P)) break;
+    if (P->h_flag[0] > 0) {
+      // reopened problems restart from their true residual
+      ++restarts;
+      if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
+      else rc = enqueue_vcycle(P, s, true, false);
+      if (rc) return rc;
+    }
   }
-  if ((rc = enqueue_finalize(P, s, 1))) return rc;
-  if ((rc = read_state(P, s))) return rc;
   int mx = 0;
   for (int p = 0; p < P->count(); ++p) mx = std::max(mx, P->h_state[p].iter);
   P->last_iters = mx;

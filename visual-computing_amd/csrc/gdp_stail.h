@@ -26,7 +26,11 @@ namespace gdp {
 constexpr int kSTailThreads = 512;
 constexpr int kSTailWaves = kSTailThreads / 64;  // 8
 constexpr int kSTailMaxLevels = 5;
-constexpr int kSTailCols = 128;                 // columns of a level row (64 lanes x 2)
+constexpr int kSTailCols = 128;                 // columns of a level (64 lanes x 2)
+// Exchange-buffer rows: 132 elements, column j of the level at j + 1, so the ghost
+// column -1 sits at 0 and the ghost column n at n + 1; 132 elements keep every row
+// start 16-byte aligned for the quad reads of the restriction.
+constexpr int kXbRow = 132;
 constexpr int kSTailXbBytes = 65536;            // level exchange buffer
 constexpr int kSTailXhBytes = 2 * kSTailWaves * 2 * 64 * 16;  // halo slots (FP64 pairs)
 
@@ -42,7 +46,7 @@ struct RowsAt {
 };
 
 struct STailCtx {
-  unsigned char* xb;   // level exchange rows (kSTailCols elements of T per row)
+  unsigned char* xb;   // level exchange rows (kXbRow elements of T per row)
   unsigned char* xh;   // halo slots [2][16 waves][2 rows][64 lanes] pairs
   double* tab;         // exact coarsest: Cm, Cn, 1/lambda
   double* w0;          // exact coarsest work arrays
@@ -118,16 +122,37 @@ __device__ __attribute__((always_inline)) void st_scale_rhs(const STailCtx& c, t
   }
 }
 
-// Publish this wave's rows of a level (row-major, kSTailCols per row) to the exchange
-// buffer, then synchronise.
+// Ghost columns written with a published level (column -1 and column n):
+enum : int {
+  kGhostZero = 0,     // zero (Dirichlet iterate; residuals never read them)
+  kGhostIter = 1,     // replicate the edge columns (Neumann iterate: the prolongation
+                      // reads edge-replicated coarse values)
+  kGhostResidual = 2  // the Neumann residual extension: r[-1] = r[0], r[n] = r[n-1] for
+                      // even n (zero for odd n)
+};
+
+// Publish this wave's rows of a level (n columns) to the exchange buffer with their
+// ghost columns, then synchronise.
 template <typename T, int RD>
-__device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, const typename Pair<T>::type* v, int m) {
+__device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, const typename Pair<T>::type* v,
+                                                        int m, int n, int ghost) {
   using V2 = typename Pair<T>::type;
-  V2* xb = reinterpret_cast<V2*>(c.xb);
+  T* xb = reinterpret_cast<T*>(c.xb);
+  const int gjx = 2 * c.lane;
 #pragma unroll
   for (int r = 0; r < RD; ++r) {
     const int gi = c.w * RD + r;
-    if (gi < m) xb[gi * (kSTailCols / 2) + c.lane] = v[r];
+    if (gi >= m) continue;
+    V2 val = v[r];
+    const T sy = dpp_shr1(val.y);  // column 2l - 1
+    if (ghost != kGhostZero) {
+      if (gjx == n && (ghost == kGhostIter || (n & 1) == 0)) val.x = sy;
+      if (gjx + 1 == n && ghost == kGhostIter) val.y = val.x;
+    }
+    T* row = xb + gi * kXbRow;
+    row[gjx + 1] = val.x;
+    row[gjx + 2] = val.y;
+    if (c.lane == 0) row[0] = ghost == kGhostZero ? T(0) : val.x;
   }
   __syncthreads();
 }
@@ -152,7 +177,7 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
       }
     }
     __syncthreads();
-    lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailWaves);
+    lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailThreads);
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
       const int gi = c.w * RD + r;
@@ -181,14 +206,14 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
 #pragma unroll
     for (int r = 0; r < RD; ++r) u[r] = V2{T(ud[r].x), T(ud[r].y)};
   }
-  st_publish<T, RD>(c, u, m);
+  st_publish<T, RD>(c, u, m, n, neu ? kGhostIter : kGhostZero);
 }
 
 // Fine residual (scaled by h^2) of this wave's rows to the exchange buffer.
 template <typename T, int RD>
 __device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const typename Pair<T>::type* u,
                             const typename Pair<T>::type* hb, const LaneCols<T>& lc, int m,
-                            bool neu) {
+                            int lc_n, bool neu) {
   using V2 = typename Pair<T>::type;
   V2 above, below;
   st_halo(c, u[0], u[RD - 1], above, below);
@@ -206,128 +231,128 @@ __device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const ty
     r2[r].x = (gi < m && lc.inx) ? rx : T(0);
     r2[r].y = (gi < m && lc.iny) ? ry : T(0);
   }
-  st_publish<T, RD>(c, r2, m);
+  st_publish<T, RD>(c, r2, m, lc_n, neu ? kGhostResidual : kGhostZero);
 }
 
-// Restriction of the published fine residual to this wave's coarse rows (plain right-
-// hand side of the coarse level): 0.25 P^T r P / h^2, Neumann with the residual
-// extension r[-1] = r[0] and r[f] = r[f-1] for even f (zero for odd f).  Published rows
-// are zero past the grid's last column, so only the column extension and the rows need
-// mapping; column indices are fixed per lane, row offsets per coarse row.
+// Four consecutive elements of an exchange row at a 16-byte aligned position.
+template <typename T>
+struct Quad {
+  T a, b, c, d;
+};
+__device__ inline Quad<float> ld_quad(const float* p) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  return Quad<float>{q.x, q.y, q.z, q.w};
+}
+__device__ inline Quad<double> ld_quad(const double* p) {
+  const double2 q0 = *reinterpret_cast<const double2*>(p);
+  const double2 q1 = *reinterpret_cast<const double2*>(p + 2);
+  return Quad<double>{q0.x, q0.y, q1.x, q1.y};
+}
+
+// Restriction of the published fine residual (scaled by h^2, with its ghost columns) to
+// this wave's coarse rows: the plain right-hand side 0.25 P^T r P / h^2 of the coarse
+// level.  Coarse lane l (columns 2l, 2l+1) reads the quad of fine columns 4l-1 .. 4l+2
+// (exchange columns 4l .. 4l+3) and the next lane's first two.
 template <typename T, int RC>
-__device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const LevelDesc& F, const LevelDesc& C,
-                            typename Pair<T>::type* cb) {
+__device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const LevelDesc& F,
+                                                         const LevelDesc& C,
+                                                         typename Pair<T>::type* cb) {
   const T* R = reinterpret_cast<const T*>(c.xb);
-  const int m = F.m, n = F.n, mc = C.m, nc = C.n;
+  const int m = F.m, mc = C.m, nc = C.n;
   const bool neu = F.bc == kNeumann;
   const T s = T(0.25) * T(F.inv_h2);
   const int l = c.lane;
-  // fine columns 4l-1 .. 4l+4 (Neumann) or 4l .. 4l+4 (Dirichlet)
-  int cj[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) {
-    int j = 4 * l - (neu ? 1 : 0) + k;
-    if (neu && j == -1) j = 0;
-    if (neu && j == n && (n & 1) == 0) j = n - 1;
-    cj[k] = min(max(j, 0), kSTailCols - 1);
-  }
   const bool v0ok = 2 * l < nc, v1ok = 2 * l + 1 < nc;
 #pragma unroll
   for (int r = 0; r < RC; ++r) {
     const int I = c.w * RC + r;
-    T v0 = T(0), v1 = T(0);
-    if (I < mc) {
-      if (neu) {
-        int ro[4];
-        T wr[4];
+    // fine rows and weights of the vertical taps (Neumann rows extended like columns)
+    int ro[4];
+    T wr[4];
+    int ntap;
+    if (neu) {
+      ntap = 4;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          int i = 2 * I - 1 + k;
-          T wk = (k == 0 || k == 3) ? T(0.25) : T(0.75);
-          if (i == -1) i = 0;
-          if (i == m && (m & 1) == 0) i = m - 1;
-          if (i >= m) {
-            i = 0;
-            wk = T(0);
-          }
-          ro[k] = i * kSTailCols;
-          wr[k] = wk;
+      for (int k = 0; k < 4; ++k) {
+        int i = 2 * I - 1 + k;
+        T wk = (k == 0 || k == 3) ? T(0.25) : T(0.75);
+        if (i == -1) i = 0;
+        if (i == m && (m & 1) == 0) i = m - 1;
+        if (i >= m || I >= mc) {
+          i = 0;
+          wk = T(0);
         }
-        T col[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-          col[k] = wr[0] * R[ro[0] + cj[k]] + wr[1] * R[ro[1] + cj[k]] +
-                   wr[2] * R[ro[2] + cj[k]] + wr[3] * R[ro[3] + cj[k]];
-        v0 = T(0.25) * col[0] + T(0.75) * col[1] + T(0.75) * col[2] + T(0.25) * col[3];
-        v1 = T(0.25) * col[2] + T(0.75) * col[3] + T(0.75) * col[4] + T(0.25) * col[5];
-      } else {
-        const int r0 = 2 * I * kSTailCols;
-        T col[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-          col[k] = T(0.5) * R[r0 + cj[k]] + R[r0 + kSTailCols + cj[k]] +
-                   T(0.5) * R[r0 + 2 * kSTailCols + cj[k]];
-        v0 = T(0.5) * col[0] + col[1] + T(0.5) * col[2];
-        v1 = T(0.5) * col[2] + col[3] + T(0.5) * col[4];
+        ro[k] = i * kXbRow;
+        wr[k] = wk;
       }
+    } else {
+      ntap = 3;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int i = I < mc ? 2 * I + k : 0;
+        ro[k] = i * kXbRow;
+        wr[k] = I < mc ? (k == 1 ? T(1) : T(0.5)) : T(0);
+      }
+      ro[3] = 0;
+      wr[3] = T(0);
+    }
+    // vertical pass on exchange columns 4l .. 4l+5
+    T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= ntap) break;
+      const Quad<T> q = ld_quad(R + ro[k] + 4 * l);
+      v[0] += wr[k] * q.a;
+      v[1] += wr[k] * q.b;
+      v[2] += wr[k] * q.c;
+      v[3] += wr[k] * q.d;
+    }
+    v[4] = dpp_shl1(v[0]);
+    v[5] = dpp_shl1(v[1]);
+    T v0, v1;
+    if (neu) {
+      // fine columns 4l-1 .. 4l+2 and 4l+1 .. 4l+4
+      v0 = T(0.25) * v[0] + T(0.75) * v[1] + T(0.75) * v[2] + T(0.25) * v[3];
+      v1 = T(0.25) * v[2] + T(0.75) * v[3] + T(0.75) * v[4] + T(0.25) * v[5];
+    } else {
+      // fine columns 4l .. 4l+2 and 4l+2 .. 4l+4
+      v0 = T(0.5) * v[1] + v[2] + T(0.5) * v[3];
+      v1 = T(0.5) * v[3] + v[4] + T(0.5) * v[5];
     }
     cb[r].x = (I < mc && v0ok) ? s * v0 : T(0);
     cb[r].y = (I < mc && v1ok) ? s * v1 : T(0);
   }
 }
 
-// u += P e for this wave's rows, e = the published coarse iterate (zero past the coarse
-// grid's last column; column 127 is always such a column).
+// u += P e for this wave's rows, e = the published coarse iterate (ghost columns zero
+// for Dirichlet, edge-replicated for Neumann).  Fine lane l interpolates from coarse
+// columns l-1, l, l+1 (exchange columns l .. l+2).
 template <typename T, int RD>
-__device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const LevelDesc& F, const LevelDesc& C,
-                           typename Pair<T>::type* u, const LaneCols<T>& lc) {
-  const T* E = reinterpret_cast<const T*>(c.xb);
-  const int m = F.m, mc = C.m, nc = C.n;
+__device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const LevelDesc& F,
+                                                        const LevelDesc& C,
+                                                        typename Pair<T>::type* u,
+                                                        const LaneCols<T>& lc) {
+  const T* E = reinterpret_cast<const T*>(c.xb) + c.lane;
+  const int m = F.m, mc = C.m;
   const bool neu = F.bc == kNeumann;
-  const int l = c.lane;
-  int cj[3];  // coarse columns l-1, l, l+1
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    int J = l - 1 + k;
-    if (neu) J = min(max(J, 0), nc - 1);
-    else if (J < 0) J = kSTailCols - 1;
-    cj[k] = J;
-  }
-#pragma unroll
-  for (int r = 0; r < RD; ++r) {
-    const int gi = c.w * RD + r;
-    if (gi >= m) continue;
-    int ra, rb;  // coarse rows and weights of the vertical interpolation
-    T wa, wb;
+  // coarse row I (clamped for Neumann, zero outside for Dirichlet), columns l-1 .. l+1
+  auto row3 = [&](int I, T* t3) {
+    T w = T(1);
     if (neu) {
-      const int P = gi >> 1, O = (gi & 1) ? P + 1 : P - 1;
-      ra = min(max(P, 0), mc - 1);
-      rb = min(max(O, 0), mc - 1);
-      wa = T(0.75);
-      wb = T(0.25);
-    } else if (gi & 1) {
-      // the last fine row of an even-sized grid has no coarse parent
-      ra = rb = (gi - 1) >> 1;
-      wa = ra < mc ? T(1) : T(0);
-      wb = T(0);
-      if (ra >= mc) ra = rb = 0;
-    } else {
-      ra = (gi >> 1) - 1;
-      rb = gi >> 1;
-      wa = wb = T(0.5);
-      if (ra < 0) {
-        ra = 0;
-        wa = T(0);
-      }
-      if (rb >= mc) {
-        rb = 0;
-        wb = T(0);
-      }
+      I = min(max(I, 0), mc - 1);
+    } else if (I < 0 || I >= mc) {
+      I = 0;
+      w = T(0);
     }
-    const int oa = ra * kSTailCols, ob = rb * kSTailCols;
+    const T* e = E + I * kXbRow;
+    t3[0] = w * e[0];
+    t3[1] = w * e[1];
+    t3[2] = w * e[2];
+  };
+  auto interp = [&](int r, const T* ta, const T* tb, T wa, T wb) {
     T t[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] = wa * E[oa + cj[k]] + wb * E[ob + cj[k]];
+    for (int k = 0; k < 3; ++k) t[k] = wa * ta[k] + wb * tb[k];
     T cx, cy;
     if (neu) {
       cx = T(0.75) * t[1] + T(0.25) * t[0];
@@ -338,6 +363,47 @@ __device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, con
     }
     if (lc.inx) u[r].x += cx;
     if (lc.iny) u[r].y += cy;
+  };
+  if constexpr (RD % 2 == 0) {
+    // the wave's fine rows w RD .. w RD + RD - 1 read coarse rows w RD/2 - 1 .. w RD/2 + RD/2
+    constexpr int NE = RD / 2 + 2;
+    T ev[NE][3];
+    const int I0 = c.w * (RD / 2) - 1;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) row3(I0 + k, ev[k]);
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int gi = c.w * RD + r;
+      if (gi >= m) continue;
+      if (neu) {
+        const int kp = r / 2 + 1, ko = (r & 1) ? kp + 1 : kp - 1;
+        interp(r, ev[kp], ev[ko], T(0.75), T(0.25));
+      } else if (r & 1) {
+        interp(r, ev[(r + 1) / 2], ev[(r + 1) / 2], T(1), T(0));
+      } else {
+        interp(r, ev[r / 2], ev[r / 2 + 1], T(0.5), T(0.5));
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RD; ++r) {
+      const int gi = c.w * RD + r;
+      if (gi >= m) continue;
+      T ta[3], tb[3];
+      if (neu) {
+        const int P = gi >> 1, O = (gi & 1) ? P + 1 : P - 1;
+        row3(P, ta);
+        row3(O, tb);
+        interp(r, ta, tb, T(0.75), T(0.25));
+      } else if (gi & 1) {
+        row3((gi - 1) >> 1, ta);
+        interp(r, ta, ta, T(1), T(0));
+      } else {
+        row3((gi >> 1) - 1, ta);
+        row3(gi >> 1, tb);
+        interp(r, ta, tb, T(0.5), T(0.5));
+      }
+    }
   }
   __syncthreads();  // every wave has read the exchange buffer
 }
@@ -388,7 +454,7 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
     }
   }
   for (int s = 1; s < 2 * c.pre; ++s) st_halfsweep<T, RD>(c, u, hb, wi, s & 1);
-  st_residual<T, RD>(c, u, hb, lc, m, neu);
+  st_residual<T, RD>(c, u, hb, lc, m, L.n, neu);
   V2 cb[RC];
   st_restrict<T, RC>(c, L, C, cb);
   STailLevel<T, D + 1>::run(c, cb);
@@ -401,7 +467,7 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
   st_prolong<T, RD>(c, L, C, u, lc);
   for (int s = 0; s < 2 * c.post; ++s)
     st_halfsweep<T, RD>(c, u, hb, wi, (s + c.order) & 1);
-  st_publish<T, RD>(c, u, m);
+  st_publish<T, RD>(c, u, m, L.n, neu ? kGhostIter : kGhostZero);
 }
 
 template <typename T, bool DOTS>
@@ -481,7 +547,6 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   st_level<T, 0>(c, b);
 
   // the first tail level's iterate, from the exchange buffer, to global memory
-  const V2* xb = reinterpret_cast<const V2*>(c.xb);
   GDP_GLOBAL T* g = (GDP_GLOBAL T*)L0.u;
   const bool dots = DOTS && D.krylov >= 0;
   const GDP_GLOBAL double* kr = dots ? (const GDP_GLOBAL double*)kds[D.krylov].r : nullptr;
@@ -492,7 +557,8 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   for (int r = 0; r < R0; ++r) {
     const int gi = c.w * R0 + r;
     if (gi >= L0.m || gj >= L0.n) continue;
-    const V2 z = xb[gi * (kSTailCols / 2) + c.lane];
+    const T* zr0 = reinterpret_cast<const T*>(c.xb) + gi * kXbRow + gj + 1;
+    const V2 z = V2{zr0[0], zr0[1]};
     const size_t o = (size_t)gi * L0.pitch + gj;
     g[o] = z.x;
     if (gj + 1 < L0.n) g[o + 1] = z.y;

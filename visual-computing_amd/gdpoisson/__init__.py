@@ -11,6 +11,7 @@ in ``libgdpoisson.so`` (C ABI: ``include/gdpoisson.h``).  Batched plans
 
 from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
 from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
+from .autotune import autotune
 from .plan import Plan, PlanOptions
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
 
@@ -20,6 +21,7 @@ __all__ = [
     "MultigridOptions",
     "MultigridReport",
     "Plan",
+    "autotune",
     "PlanOptions",
     "SOLVERS",
     "SolverReport",

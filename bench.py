@@ -173,6 +173,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -213,6 +214,15 @@ def main():
                                options=gdpoisson.PlanOptions(precision="fp64",
                                                              pre_smoothing=args.nu,
                                                              post_smoothing=args.nu))
+
+    # tile shapes picked by the autotuner for exactly this workload (outside the timed
+    # region; cached per plan configuration)
+    tiles = {}
+    if not args.no_tune:
+        log("autotuning tile shapes")
+        for name, plan in (("mixed", mixed), ("fp64", fp64), ("fp64_same_cycle", fp64_same)):
+            ch = gdpoisson.autotune(plan, bs)
+            tiles[name] = {f"{['down', 'up', 'krylov'][k]}{l}": v for (k, l), v in sorted(ch.items())}
 
     if args.profile_run:
         for _ in range(2 + args.steps):
@@ -294,6 +304,7 @@ def main():
             "channels": CHANNELS,
             "levels": mixed.levels(0),
             "parallelism": "single GPU, channels batched per launch",
+            "tiles": tiles,
         },
         "fp64_value": round(v_64, 2),
         "fp64_ms_per_step": round(t_64 / fsteps * 1e3, 4),

@@ -78,47 +78,67 @@ def pct(xs, q):
     return xs[k]
 
 
-def cpu_baseline():
-    """The FP64 numpy oracle MG-PCG (Neumann) on one channel of the workload."""
+def cpu_baseline(runs: int = 5):
+    """The oracle's FP64 MG-PCG (Neumann) on the whole RGB workload, one core.
+
+    Median of ``runs`` timed solves after one untimed warm-up (about 2 s each here).
+    """
     import numpy as np
     from threadpoolctl import threadpool_limits
 
     from oracle import generators as G
     from oracle import solvers as OS
 
-    b, _ = G.neumann_problem(SHAPE, seed=2000)
+    bs = [G.neumann_problem(SHAPE, seed=2000 + c)[0] for c in range(CHANNELS)]
+    times, its = [], []
     with threadpool_limits(limits=1):
-        t0 = time.perf_counter()
-        u, it, conv = OS.mgcg(b, bc="neumann")
-        dt = time.perf_counter() - t0
-    mpix = SHAPE[0] * SHAPE[1] / 1e6 / (dt * CHANNELS)
+        for k in range(runs + 1):
+            t0 = time.perf_counter()
+            for b in bs:
+                _, it, _ = OS.mgcg(b, bc="neumann")
+                its.append(it)
+            if k:
+                times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
     return {
-        "value": round(mpix, 4),
+        "value": round(SHAPE[0] * SHAPE[1] / 1e6 / dt, 4),
         "unit": "MPix/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"one channel of the 1024x1024 RGB workload ({it} FP64 MG-PCG iterations, "
-        f"{dt:.2f} s); RGB rate = 1 MPix / (3 x channel time)",
+        "sample": f"the whole 1024x1024 RGB workload with the numpy FP64 MG-PCG oracle "
+        f"({max(its)} iterations per channel), median of {runs} solves after one warm-up, "
+        f"{dt:.2f} s per RGB solve, {os.cpu_count()} host cores visible, 1 used",
     }
 
 
 COPY_BYTES = 64 << 23  # the --profile-run calibration copy: 512 MiB read + 512 MiB written
 
 
+def _level_area(name: str, grid: int) -> int:
+    """Grid size x rows per strip (the third template argument of the strip kernels):
+    proportional to the level's area whatever tile shape the level was given."""
+    base = name.split("(")[0]
+    if "k_sdown<" in base or "k_sup<" in base:
+        args = base[base.index("<") + 1:].split(",")
+        return grid * int(args[2].strip(" >"))
+    return grid
+
+
 def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
     """The plan profiler's kind name of a rocprofv3 dispatch of this workload.
 
-    Levels are ranked by grid size per kernel (largest grid = finest level).
+    Levels are ranked per kernel family by grid size x rows per strip (largest = finest).
     """
     base = name.split("(")[0]
-    for key, kind in (("k_down<", "down"), ("k_up<", "up"), ("k_tail<", "tail"),
+    for key, kind in (("k_sdown<", "down"), ("k_sup<", "up"), ("k_stail<", "tail"),
+                      ("k_down<", "down"), ("k_up<", "up"), ("k_tail<", "tail"),
                       ("k_apply<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
                       ("k_residual<", "residual"), ("k_finalize<", "finalize"),
                       ("k_init_stats<", "init_stats")):
         if key in base:
             if kind in ("down", "up"):
                 dots = kind == "up" and ", true>" in base
-                lv = sorted(grids[key], reverse=True).index(grid)
+                lv = sorted(grids[key], reverse=True).index(_level_area(name, grid))
                 return f"{kind} L{lv}" + ("+dots" if dots else "")
             return kind
     return None
@@ -149,9 +169,9 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
         grids = collections.defaultdict(set)
         for r in rows:
             base = r["Kernel_Name"].split("(")[0]
-            for key in ("k_down<", "k_up<"):
+            for key in ("k_sdown<", "k_sup<", "k_down<", "k_up<"):
                 if key in base:
-                    grids[key].add(int(r["Grid_Size"]))
+                    grids[key].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
         vals = collections.defaultdict(list)
         for r in rows:
             kind = kernel_kind(r["Kernel_Name"], int(r["Grid_Size"]), grids)
@@ -174,6 +194,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tile shapes")
+    ap.add_argument("--tiles", metavar="BENCH_JSON",
+                    help="reuse the tile shapes recorded in a bench line (config.tiles) instead "
+                    "of tuning, so a profiled run has exactly the benched configuration")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -218,11 +241,20 @@ def main():
     # tile shapes picked by the autotuner for exactly this workload (outside the timed
     # region; cached per plan configuration)
     tiles = {}
-    if not args.no_tune:
+    plans = (("mixed", mixed), ("fp64", fp64), ("fp64_same_cycle", fp64_same))
+    kinds = ["down", "up", "krylov"]
+    if args.tiles:
+        recorded = json.loads(Path(args.tiles).read_text())["config"]["tiles"]
+        for name, plan in plans:
+            sel = {(kinds.index(k.rstrip("0123456789")), int(k[len(k.rstrip("0123456789")):])): v
+                   for k, v in recorded.get(name, {}).items()}
+            plan.set_tiles(sel)
+            tiles[name] = recorded.get(name, {})
+    elif not args.no_tune:
         log("autotuning tile shapes")
-        for name, plan in (("mixed", mixed), ("fp64", fp64), ("fp64_same_cycle", fp64_same)):
+        for name, plan in plans:
             ch = gdpoisson.autotune(plan, bs)
-            tiles[name] = {f"{['down', 'up', 'krylov'][k]}{l}": v for (k, l), v in sorted(ch.items())}
+            tiles[name] = {f"{kinds[k]}{l}": v for (k, l), v in sorted(ch.items())}
 
     if args.profile_run:
         for _ in range(2 + args.steps):

@@ -24,6 +24,10 @@
  *   gdp_cg_solve         solvers.py:93-118     _solve_cg    (SOLVERS["cg"])
  *   gdp_relative_residual solvers.py:78-82     _relative_residual(u, b, spacing)
  *   gdp_report           solvers.py:49-75      SolverReport fields
+ *   gdp_clone_rhs        poisson.py:52-92, 102-147, 261-348  seamless_clone, rectangle route:
+ *                        GuidanceField.of / select / mix_by_magnitude + divergence +
+ *                        fold_boundary of the window, fused
+ *   gdp_guidance         poisson.py:52-92      GuidanceField.select / mix_by_magnitude / scaled
  */
 #ifndef GDPOISSON_H
 #define GDPOISSON_H
@@ -100,6 +104,21 @@ int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, in
                       int planes, double spacing, void* stream);
 /* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
    (order = 0) or colour 1 then 0 (order = 1). */
+/* Folded right-hand side of seamless cloning over a window (the rectangle route of
+   seamless_clone, poisson.py:261-348): source planes (planes x sh x sw) placed at
+   (row, col) in the target planes (planes x h x w), source-space mask (sh x sw bytes,
+   nonzero = inside), guidance mode 0 import / 1 mixed / 2 average (poisson.py:319-330),
+   window [top, top + hh) x [left, left + ww).  out: planes x (hh - 2) x (ww - 2) =
+   divergence of the guidance at the window interior minus its ring of target values. */
+int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
+                  const uint8_t* mask, const double* target, int h, int w, int planes, int top,
+                  int left, int hh, int ww, int mode, double* out, void* stream);
+/* Guidance-field arithmetic (GuidanceField, poisson.py:52-92) over planes x h x w:
+   mode 0 select (mask[h x w] ? a : b), 1 mix_by_magnitude (|a| >= |b| ? a : b per
+   component), 2 average, 3 scaled (a * factor[h x w], or a * scale if factor is NULL). */
+int gdp_guidance(const double* ax, const double* ay, const double* bx, const double* by,
+                 const uint8_t* mask, const double* factor, double scale, int mode, int h, int w,
+                 int planes, double* out_x, double* out_y, void* stream);
 int gdp_smooth(double* u, const double* b, int m, int n, double spacing, int sweeps,
                int boundary, int order, void* stream);
 /* coarse (mc, nc) = 0.25 P_r^T fine P_c with fine (m, n). */

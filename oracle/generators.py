@@ -109,6 +109,16 @@ def compositing_problem(shape, config=3, problem=0, channels=3):
     return np.stack(rhs), np.stack(tgt), mask
 
 
+def compositing_images(shape, config=3, problem=0, channels=3):
+    """(source, target) channels-last images and the blob mask of compositing_problem."""
+    mask = blob_mask(shape, 1000 * config + 10 * problem + 9)
+    target = np.stack([blob_image(shape, 1000 * config + 10 * problem + c)
+                       for c in range(channels)], axis=2)
+    source = np.stack([blob_image(shape, 1000 * config + 10 * problem + 5 + c)
+                       for c in range(channels)], axis=2)
+    return source, target, mask
+
+
 def batch_shapes(count=64, seed=4000, low=128, high=2048):
     rng = np.random.default_rng(seed)
     s = np.exp(rng.uniform(np.log(low), np.log(high), count))

@@ -281,3 +281,96 @@ def test_autotuned_plan_solves_the_same_problem(gd):
         assert abs(rep[c].iterations - rep0[c].iterations) <= 1
         assert np.abs(got[c].cpu().numpy() - exact[c]).max() < 1e-6
         assert torch.allclose(got[c], ref[c], rtol=0, atol=1e-7)
+
+
+# ---------------------------------------------------------------- compositing front end
+def _clone_example(shape=(120, 150)):
+    rng = np.random.default_rng(70)
+    target = rng.random(shape + (3,))
+    source = rng.random((50, 64, 3)) * 0.5 + 0.25
+    mask = np.zeros((50, 64), dtype=bool)
+    yy, xx = np.mgrid[0:50, 0:64]
+    mask[(yy - 25) ** 2 + (xx - 32) ** 2 < 20 ** 2] = True
+    return source, target, mask, (30, 40)
+
+
+@pytest.mark.parametrize("mode", ["import", "mixed", "average"])
+def test_clone_rhs_is_bit_exact(gd, mode):
+    import ctypes
+
+    import torch
+
+    from gdpoisson import _lib
+    from oracle import poisson as OP
+
+    source, target, mask, off = _clone_example()
+    field, win = OP.clone_window(source, target, mask, off, mode, margin=9)
+    want = OP.clone_rhs(OP.GuidanceField(field.gx[win], field.gy[win]), target[win])
+    ps = torch.from_numpy(np.moveaxis(source, 2, 0).copy()).cuda()
+    pt = torch.from_numpy(np.moveaxis(target, 2, 0).copy()).cuda()
+    m = torch.from_numpy(mask.astype(np.uint8)).cuda()
+    top, left = win[0].start, win[1].start
+    hh, ww = win[0].stop - top, win[1].stop - left
+    out = torch.empty((3, hh - 2, ww - 2), dtype=torch.float64, device="cuda")
+    rc = _lib.lib().gdp_clone_rhs(
+        ctypes.c_void_p(ps.data_ptr()), 50, 64, off[0], off[1], ctypes.c_void_p(m.data_ptr()),
+        ctypes.c_void_p(pt.data_ptr()), 120, 150, 3, top, left, hh, ww,
+        {"import": 0, "mixed": 1, "average": 2}[mode], ctypes.c_void_p(out.data_ptr()), None)
+    assert rc == 0
+    assert np.array_equal(np.moveaxis(out.cpu().numpy(), 0, 2), want)
+
+
+def test_guidance_field_operations_match_the_oracle(gd):
+    from oracle import poisson as OP
+
+    rng = np.random.default_rng(71)
+    a, b = rng.standard_normal((30, 40, 3)), rng.standard_normal((30, 40, 3))
+    mask = rng.random((30, 40)) > 0.5
+    factor = rng.random((30, 40))
+    ga, gb = gd.GuidanceField.of(a), gd.GuidanceField.of(b)
+    oa, ob = OP.GuidanceField.of(a), OP.GuidanceField.of(b)
+    for got, want in ((ga.select(mask, gb), oa.select(mask, ob)),
+                      (ga.mix_by_magnitude(gb), oa.mix_by_magnitude(ob)),
+                      (ga.scaled(factor), oa.scaled(factor)), (ga.scaled(0.5), oa.scaled(0.5))):
+        assert np.array_equal(got.gx, want.gx) and np.array_equal(got.gy, want.gy)
+
+
+def test_integrating_an_image_reproduces_it_on_the_gpu(gd):
+    """test_applications.py:49-55 on the GPU path."""
+    image = np.random.default_rng(0).random((40, 52, 3))
+    result, reports = gd.solve_dirichlet(gd.GuidanceField.of(image), image, tolerance=1e-12)
+    assert np.abs(result - image).max() < 1e-9
+    assert len(reports) == 3 and all(r.converged for r in reports)
+
+
+@pytest.mark.parametrize("method", ["mgcg", "mgcg_mixed"])
+@pytest.mark.parametrize("mode", ["import", "mixed"])
+def test_seamless_clone_matches_the_oracle(gd, method, mode):
+    from oracle import poisson as OP
+
+    source, target, mask, off = _clone_example()
+    got, reps = gd.seamless_clone(source, target, mask, off, mode=mode, method=method, margin=9)
+    want, _ = OP.seamless_clone(source, target, mask, off, mode=mode, margin=9)
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    assert np.abs(got - want).max() < 1e-6
+    # outside the window the target is untouched
+    outside = np.ones(target.shape[:2], dtype=bool)
+    outside[20:91, 31:113] = False
+    assert np.array_equal(got[outside], target[outside])
+    with pytest.raises(NotImplementedError):
+        gd.seamless_clone(source, target, mask, off, domain="mask")
+    with pytest.raises(ValueError):
+        gd.seamless_clone(source, target, mask, off, mode="unknown")
+
+
+def test_compositing_config_at_full_size(gd):
+    """2048 x 2048 RGB compositing (blob mask, FP32 MG + FP64 refinement) agrees with the
+    FP64 GPU solve of the same window."""
+    import torch
+
+    source, target, mask = G.compositing_images((2048, 2048))
+    s, t = torch.from_numpy(source).cuda(), torch.from_numpy(target).cuda()
+    mixed, reps = gd.seamless_clone(s, t, mask, (0, 0), method="mgcg_mixed")
+    fp64, _ = gd.seamless_clone(s, t, mask, (0, 0), method="mgcg")
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    assert torch.abs(mixed - fp64).max().item() < 1e-6

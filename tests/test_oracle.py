@@ -313,3 +313,37 @@ def test_generators_are_seeded_and_shaped():
     assert rhs.shape == (3, 62, 70) and tgt.shape == (3, 64, 72)
     b = G.tonemap_problem((40, 50), 7)
     assert abs(b.sum()) < 1e-8 * np.abs(b).sum()
+
+
+def test_integrating_an_image_reproduces_it():
+    """test_applications.py:49-55: integrating an image's own gradient returns it."""
+    from oracle import poisson as OP
+
+    image = np.random.default_rng(0).random((40, 52, 3))
+    result, reports = OP.solve_dirichlet(OP.GuidanceField.of(image), image, method="direct")
+    assert np.abs(result - image).max() < 1e-9
+    assert len(reports) == 3
+
+
+@pytest.mark.parametrize("mode", ["import", "mixed", "average"])
+def test_cloning_the_target_into_itself_changes_nothing(mode):
+    from oracle import poisson as OP
+
+    rng = np.random.default_rng(1)
+    target = rng.random((48, 60, 3))
+    mask = np.zeros((30, 20), dtype=bool)
+    mask[5:25, 3:17] = True
+    source = target[10:40, 20:40]
+    result, _ = OP.seamless_clone(source, target, mask, offset=(10, 20), mode=mode, margin=6)
+    assert np.abs(result - target).max() < 1e-9
+
+
+def test_clone_rhs_matches_the_compositing_generator():
+    from oracle import poisson as OP
+
+    source, target, mask = G.compositing_images((64, 80))
+    field, _ = OP.clone_window(source, target, mask, margin=10**6)
+    rhs = OP.clone_rhs(field, target)
+    want, tgt, m2 = G.compositing_problem((64, 80))
+    assert np.array_equal(m2, mask) and np.array_equal(np.moveaxis(tgt, 0, 2), target)
+    assert np.array_equal(np.moveaxis(rhs, 2, 0), want)

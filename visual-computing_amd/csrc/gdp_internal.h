@@ -67,6 +67,15 @@ hipError_t launch_resnorm_partial(const double* u, const double* b, int m, int n
                                   double spacing, int bc, double2* partial, int blocks,
                                   hipStream_t s);
 
+hipError_t launch_clone_rhs(const double* src, int sh, int sw, int row, int col,
+                            const uint8_t* mask, const double* tgt, int h, int w, int planes,
+                            int top, int left, int hh, int ww, int mode, double* out,
+                            hipStream_t s);
+hipError_t launch_guidance(const double* ax, const double* ay, const double* bx,
+                           const double* by, const uint8_t* mask, const double* factor,
+                           double fscale, int mode, size_t hw, int planes, double* ox,
+                           double* oy, hipStream_t s);
+
 void set_error(const char* fmt, ...);
 
 }  // namespace gdp

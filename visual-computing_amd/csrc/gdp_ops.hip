@@ -223,11 +223,145 @@ __global__ __launch_bounds__(256) void k_resnorm_partial(const double* __restric
         make_double2(((sh[0] + sh[1]) + sh[2]) + sh[3], ((sh[4] + sh[5]) + sh[6]) + sh[7]);
 }
 
+// ------------------------------------------------------------------ compositing
+// Guidance of a cloned region (poisson.py:52-92, 261-348): the forward-difference
+// gradient (operators.py:30-50) of the source placed at (row, col) inside the target,
+// combined inside the placed mask by mode (0 import: source; 1 mixed: per component
+// the larger magnitude, source on ties; 2 average), the target's gradient outside.
+struct CloneArgs {
+  const double* src;      // source planes, sh x sw
+  const uint8_t* mask;    // source-space mask, sh x sw
+  const double* tgt;      // target planes, h x w
+  int sh, sw, row, col, h, w, mode;
+};
+
+__device__ inline double src_at(const CloneArgs& a, size_t sp, int i, int j) {
+  const int si = i - a.row, sj = j - a.col;
+  return (si >= 0 && si < a.sh && sj >= 0 && sj < a.sw) ? a.src[sp + (size_t)si * a.sw + sj]
+                                                        : 0.0;
+}
+
+__device__ inline bool mask_at(const CloneArgs& a, int i, int j) {
+  const int si = i - a.row, sj = j - a.col;
+  return si >= 0 && si < a.sh && sj >= 0 && sj < a.sw && a.mask[(size_t)si * a.sw + sj] != 0;
+}
+
+// Guidance (gx, gy) at image pixel (i, j); every pixel this is called on has its right
+// and lower neighbours inside the image.
+__device__ inline void clone_guidance(const CloneArgs& a, size_t sp, size_t tp, int i, int j,
+                                      double& gx, double& gy) {
+#pragma clang fp contract(off)
+  const double* t = a.tgt + tp;
+  const double tc = t[(size_t)i * a.w + j];
+  const double tgx = t[(size_t)i * a.w + j + 1] - tc;
+  const double tgy = t[(size_t)(i + 1) * a.w + j] - tc;
+  if (!mask_at(a, i, j)) {
+    gx = tgx;
+    gy = tgy;
+    return;
+  }
+  const double sc = src_at(a, sp, i, j);
+  const double sgx = src_at(a, sp, i, j + 1) - sc;
+  const double sgy = src_at(a, sp, i + 1, j) - sc;
+  if (a.mode == 1) {
+    gx = fabs(sgx) >= fabs(tgx) ? sgx : tgx;
+    gy = fabs(sgy) >= fabs(tgy) ? sgy : tgy;
+  } else if (a.mode == 2) {
+    gx = 0.5 * (sgx + tgx);
+    gy = 0.5 * (sgy + tgy);
+  } else {
+    gx = sgx;
+    gy = sgy;
+  }
+}
+
+// Folded right-hand side of the window [top, top+hh) x [left, left+ww) of one plane:
+// divergence (operators.py:53-83) of the guidance at the window's interior points, minus
+// the window's one-pixel ring of target values (fold_boundary, operators.py:139-169,
+// h = 1, in its order: first row, last row, first column, last column).
+__global__ __launch_bounds__(256) void k_clone_rhs(CloneArgs a, int top, int left, int hh,
+                                                   int ww, double* __restrict__ out) {
+  // no fused multiply-adds: the result is bit-identical to the unfused FP64 formula
+#pragma clang fp contract(off)
+  const int mi = hh - 2, ni = ww - 2;
+  int i, j;
+  if (!coord(mi, ni, i, j)) return;
+  const size_t sp = (size_t)blockIdx.z * a.sh * a.sw, tp = (size_t)blockIdx.z * a.h * a.w;
+  const int gi = top + 1 + i, gj = left + 1 + j;
+  double gx, gy, gxl, gyu;
+  clone_guidance(a, sp, tp, gi, gj, gx, gy);
+  double unused;
+  clone_guidance(a, sp, tp, gi, gj - 1, gxl, unused);
+  clone_guidance(a, sp, tp, gi - 1, gj, unused, gyu);
+  double v = (gx - gxl) + (gy - gyu);
+  const double* t = a.tgt + tp;
+  if (i == 0) v -= t[(size_t)top * a.w + gj];
+  if (i == mi - 1) v -= t[(size_t)(top + hh - 1) * a.w + gj];
+  if (j == 0) v -= t[(size_t)gi * a.w + left];
+  if (j == ni - 1) v -= t[(size_t)gi * a.w + left + ww - 1];
+  out[(size_t)blockIdx.z * mi * ni + (size_t)i * ni + j] = v;
+}
+
+// Element-wise guidance-field combinations (GuidanceField, poisson.py:52-92) over planes of
+// hw pixels: 0 select (mask ? a : b), 1 mixed (per component |a| >= |b| ? a : b),
+// 2 average, 3 scale (a * factor[pixel], or a * fscale without a factor array).
+__global__ __launch_bounds__(256) void k_guidance(const double* __restrict__ ax,
+                                                  const double* __restrict__ ay,
+                                                  const double* __restrict__ bx,
+                                                  const double* __restrict__ by,
+                                                  const uint8_t* __restrict__ mask,
+                                                  const double* __restrict__ factor,
+                                                  double fscale, int mode, size_t hw,
+                                                  size_t total, double* __restrict__ ox,
+                                                  double* __restrict__ oy) {
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= total) return;
+  const size_t px = k % hw;
+  double x, y;
+  if (mode == 0) {
+    const bool in = mask[px] != 0;
+    x = in ? ax[k] : bx[k];
+    y = in ? ay[k] : by[k];
+  } else if (mode == 1) {
+    x = fabs(ax[k]) >= fabs(bx[k]) ? ax[k] : bx[k];
+    y = fabs(ay[k]) >= fabs(by[k]) ? ay[k] : by[k];
+  } else if (mode == 2) {
+    x = 0.5 * (ax[k] + bx[k]);
+    y = 0.5 * (ay[k] + by[k]);
+  } else {
+    const double f = factor ? factor[px] : fscale;
+    x = ax[k] * f;
+    y = ay[k] * f;
+  }
+  ox[k] = x;
+  oy[k] = y;
+}
+
 dim3 grid2d(int m, int n, int planes) {
   return dim3((n + BX - 1) / BX, (m + BY - 1) / BY, planes);
 }
 
 }  // namespace
+
+hipError_t launch_clone_rhs(const double* src, int sh, int sw, int row, int col,
+                            const uint8_t* mask, const double* tgt, int h, int w, int planes,
+                            int top, int left, int hh, int ww, int mode, double* out,
+                            hipStream_t s) {
+  CloneArgs a{src, mask, tgt, sh, sw, row, col, h, w, mode};
+  k_clone_rhs<<<grid2d(hh - 2, ww - 2, planes), 256, 0, s>>>(a, top, left, hh, ww, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_guidance(const double* ax, const double* ay, const double* bx,
+                           const double* by, const uint8_t* mask, const double* factor,
+                           double fscale, int mode, size_t hw, int planes, double* ox,
+                           double* oy, hipStream_t s) {
+  const size_t total = hw * size_t(planes);
+  if (!total) return hipSuccess;
+  k_guidance<<<unsigned((total + 255) / 256), 256, 0, s>>>(ax, ay, bx, by, mask, factor, fscale,
+                                                          mode, hw, total, ox, oy);
+  return hipGetLastError();
+}
 
 hipError_t launch_laplacian(const double* u, double* out, int m, int n, int planes,
                             double spacing, int bc, hipStream_t s) {

@@ -146,7 +146,8 @@ struct gdp_plan {
   int* h_flag = nullptr;
   hipStream_t cap = nullptr;
   hipEvent_t io_ev = nullptr;  // the last upload of h_io / h_init has been consumed
-  hipGraphExec_t pair_exec[2] = {nullptr, nullptr};  // [0] mgcg, [1] cg
+  hipGraphExec_t pair_exec[2] = {nullptr, nullptr};   // [0] mgcg, [1] cg
+  hipGraphExec_t start_exec[2] = {nullptr, nullptr};  // set-up + first preconditioning
   int last_iters = 0;
   bool bound = false;
   // built-in kernel timing (HIP events around every launch on the launch stream)
@@ -204,6 +205,16 @@ std::vector<std::pair<int, int>> level_shapes(const gdp_problem& p, const gdp_op
     s.push_back({m, n});
   }
   return s;
+}
+
+// Instantiated hipGraphs of a plan are tied to its tables: drop them on re-bind / destroy.
+void destroy_graphs(gdp_plan* P) {
+  for (hipGraphExec_t* ex : {P->pair_exec, P->start_exec})
+    for (int g = 0; g < 2; ++g)
+      if (ex[g]) {
+        (void)hipGraphExecDestroy(ex[g]);
+        ex[g] = nullptr;
+      }
 }
 
 // LDS bytes of a tail holding levels [first, L) (u and b per level, one residual
@@ -485,18 +496,13 @@ int gdp_plan_set_tile(gdp_plan* P, int kind, int level, int menu_index) {
   else if (kind == 1) P->up_menu[level] = menu_index;
   else P->kry_menu = menu_index;
   if (P->bound) P->bound = false;  // tables must be rebuilt by gdp_plan_bind
-  for (auto& e : P->pair_exec)
-    if (e) {
-      (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+  destroy_graphs(P);
   return GDP_OK;
 }
 
 int gdp_plan_destroy(gdp_plan* P) {
   if (!P) return GDP_OK;
-  for (auto& e : P->pair_exec)
-    if (e) (void)hipGraphExecDestroy(e);
+  destroy_graphs(P);
   if (P->cap) (void)hipStreamDestroy(P->cap);
   if (P->io_ev) (void)hipEventDestroy(P->io_ev);
   for (auto& r : P->prof_pending) {
@@ -793,11 +799,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, smem));
   }
   GDP_HIP(hipDeviceSynchronize());
-  for (auto& e : P->pair_exec)
-    if (e) {
-      (void)hipGraphExecDestroy(e);
-      e = nullptr;
-    }
+  destroy_graphs(P);
   P->bound = true;
   return GDP_OK;
 }
@@ -1131,27 +1133,42 @@ void fill_reports(const gdp_plan* P, gdp_report* rep) {
   }
 }
 
-int launch_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
-  const int g = jacobi ? 1 : 0;
-  if (P->opt.use_graph && !P->prof) {
-    if (!P->pair_exec[g]) {
-      hipGraph_t graph = nullptr;
-      GDP_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
-      const int rc = enqueue_pcg_pair(P, P->cap, jacobi);
-      const hipError_t ee = hipStreamEndCapture(P->cap, &graph);
-      if (rc) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-      }
-      GDP_HIP(ee);
-      const hipError_t ie = hipGraphInstantiate(&P->pair_exec[g], graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      GDP_HIP(ie);
+// Replays exec on s, capturing enqueue(stream) into it on first use.  Without graphs
+// (or while profiling, which brackets launches with events) it enqueues directly.
+template <typename F>
+int launch_captured(gdp_plan* P, hipStream_t s, hipGraphExec_t& exec, F enqueue) {
+  if (!P->opt.use_graph || P->prof) return enqueue(s);
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    GDP_HIP(hipStreamBeginCapture(P->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(P->cap);
+    const hipError_t ee = hipStreamEndCapture(P->cap, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
     }
-    GDP_HIP(hipGraphLaunch(P->pair_exec[g], s));
-    return GDP_OK;
+    GDP_HIP(ee);
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    GDP_HIP(ie);
   }
-  return enqueue_pcg_pair(P, s, jacobi);
+  GDP_HIP(hipGraphLaunch(exec, s));
+  return GDP_OK;
+}
+
+int launch_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
+  return launch_captured(P, s, P->pair_exec[jacobi ? 1 : 0],
+                         [&](hipStream_t cs) { return enqueue_pcg_pair(P, cs, jacobi); });
+}
+
+// PCG set-up (b statistics, x = 0, r = -b, norms) and the first preconditioned residual.
+int launch_start(gdp_plan* P, hipStream_t s, bool jacobi) {
+  return launch_captured(P, s, P->start_exec[jacobi ? 1 : 0], [&](hipStream_t cs) {
+    const gdp_options& o = P->opt;
+    int rc = GDP_T(enqueue_init_t)(P, cs, 0, 1, o.tolerance, o.max_iterations, false, 1);
+    if (rc) return rc;
+    return jacobi ? GDP_T(enqueue_jacobi_t)(P, cs, false) : enqueue_vcycle(P, cs, true, false);
+  });
 }
 
 int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* rep,
@@ -1162,13 +1179,8 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   }
   int rc;
   if ((rc = stage_io(P, b, u, nullptr, s))) return rc;
-  const gdp_options& o = P->opt;
-  if ((rc = GDP_T(enqueue_init_t)(P, s, 0, 1, o.tolerance, o.max_iterations, false, 1)))
-    return rc;
-  // first preconditioned residual z0 = M r0 and its <r0, z0>
-  if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
-  else rc = enqueue_vcycle(P, s, true, false);
-  if (rc) return rc;
+  // set-up and the first preconditioned residual z0 = M r0 with its <r0, z0>
+  if ((rc = launch_start(P, s, jacobi))) return rc;
   // Iterate in batches of graph-captured iteration pairs.  After each batch the problems
   // that have finished are checked on the device without a host round trip: FP64 true
   // residual, reopen (residual replacement) or accept, and the accepted ones' output is
@@ -1346,6 +1358,34 @@ int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, in
   if (!out) return (set_error("null out"), GDP_EINVAL);
   GDP_HIP(launch_fold(rhs, boundary, out, h, w, planes, spacing,
                       static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
+                  const uint8_t* mask, const double* target, int h, int w, int planes, int top,
+                  int left, int hh, int ww, int mode, double* out, void* stream) {
+  if (!source || !mask || !target || !out) return (set_error("null argument"), GDP_EINVAL);
+  if (planes < 1 || sh < 1 || sw < 1 || h < 3 || w < 3)
+    return (set_error("bad sizes"), GDP_EINVAL);
+  if (row < 0 || col < 0 || row + sh > h || col + sw > w)
+    return (set_error("the source does not fit in the target at this offset"), GDP_EINVAL);
+  if (top < 0 || left < 0 || hh < 3 || ww < 3 || top + hh > h || left + ww > w)
+    return (set_error("the window must be at least 3 x 3 and inside the target"), GDP_EINVAL);
+  if (mode < 0 || mode > 2) return (set_error("unknown clone mode %d", mode), GDP_EINVAL);
+  GDP_HIP(launch_clone_rhs(source, sh, sw, row, col, mask, target, h, w, planes, top, left, hh,
+                           ww, mode, out, static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_guidance(const double* ax, const double* ay, const double* bx, const double* by,
+                 const uint8_t* mask, const double* factor, double scale, int mode, int h, int w,
+                 int planes, double* out_x, double* out_y, void* stream) {
+  if (!ax || !ay || !out_x || !out_y || h < 1 || w < 1 || planes < 1 || mode < 0 || mode > 3)
+    return (set_error("bad guidance arguments"), GDP_EINVAL);
+  if (mode <= 2 && (!bx || !by)) return (set_error("null second field"), GDP_EINVAL);
+  if (mode == 0 && !mask) return (set_error("null mask"), GDP_EINVAL);
+  GDP_HIP(launch_guidance(ax, ay, bx, by, mask, factor, scale, mode, size_t(h) * size_t(w),
+                          planes, out_x, out_y, static_cast<hipStream_t>(stream)));
   return GDP_OK;
 }
 

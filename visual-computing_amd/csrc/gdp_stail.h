@@ -49,9 +49,11 @@ struct STailCtx {
   unsigned char* xb;   // level exchange rows (kXbRow elements of T per row)
   unsigned char* xh;   // halo slots [2][16 waves][2 rows][64 lanes] pairs
   double* tab;         // exact coarsest: Cm, Cn, 1/lambda
+  double* tabT;        // exact coarsest: Cn transposed
   double* w0;          // exact coarsest work arrays
   double* w1;
-  const LevelDesc* LV;
+  const LevelDesc* __restrict__ LV;  // the tail's levels in global memory (read through
+                                     // scalar loads: every field is wave-uniform)
   int nlev, w, lane, pre, post, csweeps, order, hbuf;
   bool exact;
 };
@@ -105,15 +107,14 @@ __device__ __attribute__((always_inline)) void st_halfsweep(STailCtx& c, typenam
 
 // Inverse diagonals wi of this wave's rows and hb = h^2 b / diag (zero outside the
 // grid).
-template <typename T, int RD>
+template <typename T, int RD, bool NEU>
 __device__ __attribute__((always_inline)) void st_scale_rhs(const STailCtx& c, typename Pair<T>::type* hb,
                                                           typename Pair<T>::type* wi,
-                                                          const LaneCols<T>& lc, int m, bool neu,
-                                                          T h2) {
+                                                          const LaneCols<T>& lc, int m, T h2) {
 #pragma unroll
   for (int r = 0; r < RD; ++r) {
     const int gi = c.w * RD + r;
-    const int vc = row_count(gi, m, neu);
+    const int vc = row_count(gi, m, NEU);
     const bool rin = gi < m;
     wi[r].x = (rin && lc.inx) ? lc.wx(vc) : T(0);
     wi[r].y = (rin && lc.iny) ? lc.wy(vc) : T(0);
@@ -133,9 +134,9 @@ enum : int {
 
 // Publish this wave's rows of a level (n columns) to the exchange buffer with their
 // ghost columns, then synchronise.
-template <typename T, int RD>
+template <typename T, int RD, int ghost>
 __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, const typename Pair<T>::type* v,
-                                                        int m, int n, int ghost) {
+                                                        int m, int n) {
   using V2 = typename Pair<T>::type;
   T* xb = reinterpret_cast<T*>(c.xb);
   const int gjx = 2 * c.lane;
@@ -145,7 +146,7 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
     if (gi >= m) continue;
     V2 val = v[r];
     const T sy = dpp_shr1(val.y);  // column 2l - 1
-    if (ghost != kGhostZero) {
+    if constexpr (ghost != kGhostZero) {
       if (gjx == n && (ghost == kGhostIter || (n & 1) == 0)) val.x = sy;
       if (gjx + 1 == n && ghost == kGhostIter) val.y = val.x;
     }
@@ -159,12 +160,11 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
 
 // Coarsest level: u from the right-hand side b (plain) of this wave's rows; the
 // result is published to the exchange buffer.
-template <typename T, int RD>
+template <typename T, int RD, bool NEU>
 __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const LevelDesc& L, typename Pair<T>::type* b) {
   using V2 = typename Pair<T>::type;
   using D2 = typename Pair<double>::type;
   const int m = L.m, n = L.n;
-  const bool neu = L.bc == kNeumann;
   V2 u[RD];
   if (c.exact) {
     const int gj = 2 * c.lane;
@@ -177,7 +177,7 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
       }
     }
     __syncthreads();
-    lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailThreads);
+    lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailThreads, c.tabT);
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
       const int gi = c.w * RD + r;
@@ -190,7 +190,7 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
   } else {
     // coarsest_sweeps red-black sweeps in FP64 from zero (multigrid.py:206-207)
     LaneCols<double> lc;
-    lc.init(0, c.lane, n, neu);
+    lc.init(0, c.lane, n, NEU);
     D2 ud[RD], hd[RD];
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
@@ -198,7 +198,7 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
       hd[r] = D2{double(b[r].x), double(b[r].y)};
     }
     D2 wd[RD];
-    st_scale_rhs<double, RD>(c, hd, wd, lc, m, neu, L.h2);
+    st_scale_rhs<double, RD, NEU>(c, hd, wd, lc, m, L.h2);
     for (int s = 0; s < c.csweeps; ++s) {
       st_halfsweep_q<double, RD, 0>(c, ud, hd, wd);
       st_halfsweep_q<double, RD, 1>(c, ud, hd, wd);
@@ -206,14 +206,14 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const Le
 #pragma unroll
     for (int r = 0; r < RD; ++r) u[r] = V2{T(ud[r].x), T(ud[r].y)};
   }
-  st_publish<T, RD>(c, u, m, n, neu ? kGhostIter : kGhostZero);
+  st_publish<T, RD, NEU ? kGhostIter : kGhostZero>(c, u, m, n);
 }
 
 // Fine residual (scaled by h^2) of this wave's rows to the exchange buffer.
-template <typename T, int RD>
+template <typename T, int RD, bool NEU>
 __device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const typename Pair<T>::type* u,
                             const typename Pair<T>::type* hb, const LaneCols<T>& lc, int m,
-                            int lc_n, bool neu) {
+                            int lc_n) {
   using V2 = typename Pair<T>::type;
   V2 above, below;
   st_halo(c, u[0], u[RD - 1], above, below);
@@ -221,7 +221,7 @@ __device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const ty
 #pragma unroll
   for (int r = 0; r < RD; ++r) {
     const int gi = c.w * RD + r;
-    const int vc = row_count(gi, m, neu);
+    const int vc = row_count(gi, m, NEU);
     const V2 up = r > 0 ? u[r - 1] : above;
     const V2 dn = r < RD - 1 ? u[r + 1] : below;
     const T nx = ((up.x + dn.x) + dpp_shr1(u[r].y)) + u[r].y;
@@ -231,7 +231,7 @@ __device__ __attribute__((always_inline)) void st_residual(STailCtx& c, const ty
     r2[r].x = (gi < m && lc.inx) ? rx : T(0);
     r2[r].y = (gi < m && lc.iny) ? ry : T(0);
   }
-  st_publish<T, RD>(c, r2, m, lc_n, neu ? kGhostResidual : kGhostZero);
+  st_publish<T, RD, NEU ? kGhostResidual : kGhostZero>(c, r2, m, lc_n);
 }
 
 // Four consecutive elements of an exchange row at a 16-byte aligned position.
@@ -252,65 +252,65 @@ __device__ inline Quad<double> ld_quad(const double* p) {
 // Restriction of the published fine residual (scaled by h^2, with its ghost columns) to
 // this wave's coarse rows: the plain right-hand side 0.25 P^T r P / h^2 of the coarse
 // level.  Coarse lane l (columns 2l, 2l+1) reads the quad of fine columns 4l-1 .. 4l+2
-// (exchange columns 4l .. 4l+3) and the next lane's first two.
-template <typename T, int RC>
+// (exchange columns 4l .. 4l+3) and the next lane's first two.  Every quad of the
+// wave's rows is loaded before any arithmetic, so the LDS latency is paid once.
+template <typename T, int RC, bool NEU>
 __device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const LevelDesc& F,
                                                          const LevelDesc& C,
                                                          typename Pair<T>::type* cb) {
+  constexpr int NT = NEU ? 4 : 3;  // vertical taps
   const T* R = reinterpret_cast<const T*>(c.xb);
   const int m = F.m, mc = C.m, nc = C.n;
-  const bool neu = F.bc == kNeumann;
   const T s = T(0.25) * T(F.inv_h2);
   const int l = c.lane;
   const bool v0ok = 2 * l < nc, v1ok = 2 * l + 1 < nc;
+  Quad<T> q[RC][NT];
+  T wr[RC][NT];
 #pragma unroll
   for (int r = 0; r < RC; ++r) {
     const int I = c.w * RC + r;
-    // fine rows and weights of the vertical taps (Neumann rows extended like columns)
-    int ro[4];
-    T wr[4];
-    int ntap;
-    if (neu) {
-      ntap = 4;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        int i = 2 * I - 1 + k;
-        T wk = (k == 0 || k == 3) ? T(0.25) : T(0.75);
+    for (int k = 0; k < NT; ++k) {
+      // fine row and weight of tap k (Neumann rows extended like columns)
+      int i;
+      T wk;
+      if constexpr (NEU) {
+        i = 2 * I - 1 + k;
+        wk = (k == 0 || k == 3) ? T(0.25) : T(0.75);
         if (i == -1) i = 0;
         if (i == m && (m & 1) == 0) i = m - 1;
         if (i >= m || I >= mc) {
           i = 0;
           wk = T(0);
         }
-        ro[k] = i * kXbRow;
-        wr[k] = wk;
+      } else {
+        i = I < mc ? 2 * I + k : 0;
+        wk = I < mc ? (k == 1 ? T(1) : T(0.5)) : T(0);
       }
-    } else {
-      ntap = 3;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const int i = I < mc ? 2 * I + k : 0;
-        ro[k] = i * kXbRow;
-        wr[k] = I < mc ? (k == 1 ? T(1) : T(0.5)) : T(0);
-      }
-      ro[3] = 0;
-      wr[3] = T(0);
+      wr[r][k] = wk;
+      q[r][k] = ld_quad(R + i * kXbRow + 4 * l);
     }
-    // vertical pass on exchange columns 4l .. 4l+5
-    T v[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+  }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= ntap) break;
-      const Quad<T> q = ld_quad(R + ro[k] + 4 * l);
-      v[0] += wr[k] * q.a;
-      v[1] += wr[k] * q.b;
-      v[2] += wr[k] * q.c;
-      v[3] += wr[k] * q.d;
+  for (int r = 0; r < RC; ++r) {
+    const int I = c.w * RC + r;
+    // vertical pass on exchange columns 4l .. 4l+5
+    T v[6];
+    v[0] = wr[r][0] * q[r][0].a;
+    v[1] = wr[r][0] * q[r][0].b;
+    v[2] = wr[r][0] * q[r][0].c;
+    v[3] = wr[r][0] * q[r][0].d;
+#pragma unroll
+    for (int k = 1; k < NT; ++k) {
+      v[0] += wr[r][k] * q[r][k].a;
+      v[1] += wr[r][k] * q[r][k].b;
+      v[2] += wr[r][k] * q[r][k].c;
+      v[3] += wr[r][k] * q[r][k].d;
     }
     v[4] = dpp_shl1(v[0]);
     v[5] = dpp_shl1(v[1]);
     T v0, v1;
-    if (neu) {
+    if constexpr (NEU) {
       // fine columns 4l-1 .. 4l+2 and 4l+1 .. 4l+4
       v0 = T(0.25) * v[0] + T(0.75) * v[1] + T(0.75) * v[2] + T(0.25) * v[3];
       v1 = T(0.25) * v[2] + T(0.75) * v[3] + T(0.75) * v[4] + T(0.25) * v[5];
@@ -327,14 +327,14 @@ __device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, co
 // u += P e for this wave's rows, e = the published coarse iterate (ghost columns zero
 // for Dirichlet, edge-replicated for Neumann).  Fine lane l interpolates from coarse
 // columns l-1, l, l+1 (exchange columns l .. l+2).
-template <typename T, int RD>
+template <typename T, int RD, bool NEU>
 __device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const LevelDesc& F,
                                                         const LevelDesc& C,
                                                         typename Pair<T>::type* u,
                                                         const LaneCols<T>& lc) {
   const T* E = reinterpret_cast<const T*>(c.xb) + c.lane;
   const int m = F.m, mc = C.m;
-  const bool neu = F.bc == kNeumann;
+  constexpr bool neu = NEU;
   // coarse row I (clamped for Neumann, zero outside for Dirichlet), columns l-1 .. l+1
   auto row3 = [&](int I, T* t3) {
     T w = T(1);
@@ -408,40 +408,40 @@ __device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, con
   __syncthreads();  // every wave has read the exchange buffer
 }
 
-template <typename T, int D>
+template <typename T, int D, bool NEU>
 __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b);
 
-template <typename T, int D>
+template <typename T, int D, bool NEU>
 struct STailLevel {
   static __device__ __attribute__((always_inline)) void run(STailCtx& c, typename Pair<T>::type* b) {
-    st_level<T, D>(c, b);
+    st_level<T, D, NEU>(c, b);
   }
 };
-template <typename T>
-struct STailLevel<T, kSTailMaxLevels> {
+template <typename T, bool NEU>
+struct STailLevel<T, kSTailMaxLevels, NEU> {
   static __device__ void run(STailCtx&, typename Pair<T>::type*) {}
 };
 
 // Level D of the tail: b holds the plain right-hand side of this wave's rows.  On
-// return the level's iterate is published in the exchange buffer.
-template <typename T, int D>
+// return the level's iterate is published in the exchange buffer.  NEU: the problem's
+// boundary condition (Neumann), fixed for every level of the tail.
+template <typename T, int D, bool NEU>
 __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b) {
   using V2 = typename Pair<T>::type;
   constexpr int RD = RowsAt<STailRows<T>::kRows0, D>::value;
   constexpr int RC = RowsAt<STailRows<T>::kRows0, D + 1>::value;
   const LevelDesc& L = c.LV[D];
   if (D == c.nlev - 1) {
-    st_coarsest<T, RD>(c, L, b);
+    st_coarsest<T, RD, NEU>(c, L, b);
     return;
   }
   const LevelDesc& C = c.LV[D + 1];
   const int m = L.m;
-  const bool neu = L.bc == kNeumann;
   LaneCols<T> lc;
-  lc.init(0, c.lane, L.n, neu);
+  lc.init(0, c.lane, L.n, NEU);
   V2* hb = b;  // scaled in place
   V2 wi[RD];
-  st_scale_rhs<T, RD>(c, hb, wi, lc, m, neu, T(L.h2));
+  st_scale_rhs<T, RD, NEU>(c, hb, wi, lc, m, T(L.h2));
   // pre-smoothing from zero: the first colour-0 half-sweep is u = -hb on colour 0
   V2 u[RD];
 #pragma unroll
@@ -454,20 +454,20 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
     }
   }
   for (int s = 1; s < 2 * c.pre; ++s) st_halfsweep<T, RD>(c, u, hb, wi, s & 1);
-  st_residual<T, RD>(c, u, hb, lc, m, L.n, neu);
+  st_residual<T, RD, NEU>(c, u, hb, lc, m, L.n);
   V2 cb[RC];
-  st_restrict<T, RC>(c, L, C, cb);
-  STailLevel<T, D + 1>::run(c, cb);
+  st_restrict<T, RC, NEU>(c, L, C, cb);
+  STailLevel<T, D + 1, NEU>::run(c, cb);
   {
     // recompute the lane's column data rather than keep it live across the recursion
     int lane = c.lane;
     asm volatile("" : "+v"(lane));
-    lc.init(0, lane, L.n, neu);
+    lc.init(0, lane, L.n, NEU);
   }
-  st_prolong<T, RD>(c, L, C, u, lc);
+  st_prolong<T, RD, NEU>(c, L, C, u, lc);
   for (int s = 0; s < 2 * c.post; ++s)
     st_halfsweep<T, RD>(c, u, hb, wi, (s + c.order) & 1);
-  st_publish<T, RD>(c, u, m, L.n, neu ? kGhostIter : kGhostZero);
+  st_publish<T, RD, NEU ? kGhostIter : kGhostZero>(c, u, m, L.n);
 }
 
 template <typename T, bool DOTS>
@@ -484,27 +484,16 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   using V2 = typename Pair<T>::type;
   constexpr int R0 = STailRows<T>::kRows0;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ TailDesc D;
-  __shared__ LevelDesc LV[kMaxTailLevels];
   __shared__ double sh[2 * kSTailThreads / 64];
   __shared__ double2 bcc;
-  {
-    const int* src = reinterpret_cast<const int*>(&tails[blockIdx.x]);
-    int* dst = reinterpret_cast<int*>(&D);
-    for (int w = threadIdx.x; w < int(sizeof(TailDesc) / 4); w += kSTailThreads) dst[w] = src[w];
-    __syncthreads();
-    const int* ls = reinterpret_cast<const int*>(&levels[D.first]);
-    int* ld = reinterpret_cast<int*>(LV);
-    const int nw = D.nlev * int(sizeof(LevelDesc) / 4);
-    for (int w = threadIdx.x; w < nw; w += kSTailThreads) ld[w] = ls[w];
-    __syncthreads();
-  }
+  // descriptors through scalar loads (uniform addresses of read-only kernel arguments)
+  const TailDesc& D = tails[blockIdx.x];
   const int p = D.problem;
   STailCtx c;
   c.xb = smem;
   c.xh = smem + kSTailXbBytes;
   c.tab = reinterpret_cast<double*>(smem + kSTailXbBytes + kSTailXhBytes);
-  c.LV = LV;
+  c.LV = levels + D.first;
   c.nlev = D.nlev;
   c.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   c.lane = threadIdx.x & 63;
@@ -513,14 +502,18 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   c.csweeps = coarsest_sweeps;
   c.order = order;
   c.hbuf = 0;
-  c.exact = D.dct != nullptr;
-  const LevelDesc& L0 = LV[0];
-  const LevelDesc& Lc = LV[D.nlev - 1];
-  const int ntab = c.exact ? Lc.m * Lc.m + Lc.n * Lc.n + Lc.m * Lc.n : 0;
-  c.w0 = c.tab + ntab;
-  c.w1 = c.w0 + Lc.m * Lc.n;
+  const double* __restrict__ dct = D.dct;
+  c.exact = dct != nullptr;
+  const LevelDesc& L0 = c.LV[0];
+  const LevelDesc& Lc = c.LV[c.nlev - 1];
+  const int cm = Lc.m, cn = Lc.n;
+  const int ntab = c.exact ? cm * cm + cn * cn + cm * cn : 0;
+  c.tabT = c.tab + ntab;
+  c.w0 = c.tabT + (c.exact ? cn * cn : 0);
+  c.w1 = c.w0 + cm * cn;
 
-  // right-hand side of the first tail level (issued before the convergence test)
+  // right-hand side of the first tail level and the first elements of the transform
+  // tables, issued together before the convergence test
   V2 b[R0];
   {
     const GDP_GLOBAL T* g = (const GDP_GLOBAL T*)L0.b;
@@ -535,16 +528,39 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
     for (int r = 0; r < R0; ++r)
       if (gj + 1 >= L0.n) b[r].y = T(0);
   }
+  constexpr int kPre = 2;  // table elements per thread loaded up front
+  double tv[kPre];
+#pragma unroll
+  for (int k = 0; k < kPre; ++k) {
+    const int o = threadIdx.x + k * kSTailThreads;
+    tv[k] = o < ntab ? dct[o] : 0.0;
+  }
   if (st[p].done) return;
   // a problem whose finest level is in the tail gets its convergence test here
   if (check_conv && D.krylov >= 0 &&
       check_converged<kSTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
     return;
-  if (c.exact)
-    for (int o = threadIdx.x; o < ntab; o += kSTailThreads) c.tab[o] = D.dct[o];
-  // (the first barrier inside the level recursion orders the table loads)
+  if (c.exact) {
+    // Cm, Cn, 1/lambda as they are, and Cn once more transposed
+    const int cn0 = cm * cm, cn1 = cm * cm + cn * cn;
+    auto put = [&](int o, double v) {
+      c.tab[o] = v;
+      if (o >= cn0 && o < cn1) {
+        const int e = o - cn0, row = e / cn;
+        c.tabT[(e - row * cn) * cn + row] = v;
+      }
+    };
+#pragma unroll
+    for (int k = 0; k < kPre; ++k) {
+      const int o = threadIdx.x + k * kSTailThreads;
+      if (o < ntab) put(o, tv[k]);
+    }
+    for (int o = threadIdx.x + kPre * kSTailThreads; o < ntab; o += kSTailThreads) put(o, dct[o]);
+  }
+  // (the first barrier inside the level recursion orders the table stores)
 
-  st_level<T, 0>(c, b);
+  if (L0.bc == kNeumann) st_level<T, 0, true>(c, b);
+  else st_level<T, 0, false>(c, b);
 
   // the first tail level's iterate, from the exchange buffer, to global memory
   GDP_GLOBAL T* g = (GDP_GLOBAL T*)L0.u;

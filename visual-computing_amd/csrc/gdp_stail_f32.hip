@@ -5,7 +5,8 @@ template void gdp::launch_stail<float>(bool, int, int, hipStream_t, const gdp::T
 template hipError_t gdp::stail_set_smem<float>(int);
 
 int gdp::stail_lds_bytes(int cm, int cn, bool exact) {
-  const int tables = exact ? (cm * cm + cn * cn + cm * cn) + 2 * cm * cn : 0;
+  // Cm, Cn, 1/lambda, Cn transposed, and the two work arrays
+  const int tables = exact ? (cm * cm + 2 * cn * cn + cm * cn) + 2 * cm * cn : 0;
   return kSTailXbBytes + kSTailXhBytes + 8 * tables;
 }
 int gdp::stail_max_rows(bool fp32) {

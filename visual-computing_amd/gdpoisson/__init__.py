@@ -13,6 +13,7 @@ from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
 from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
 from .autotune import autotune
 from .plan import Plan, PlanOptions
+from .poisson import GuidanceField, seamless_clone, solve_dirichlet
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
 
 __version__ = "0.1.0"
@@ -21,6 +22,9 @@ __all__ = [
     "MultigridOptions",
     "MultigridReport",
     "Plan",
+    "GuidanceField",
+    "seamless_clone",
+    "solve_dirichlet",
     "autotune",
     "PlanOptions",
     "SOLVERS",

@@ -25,6 +25,8 @@ EXPORTS = (
     "gdp_gradient",
     "gdp_divergence",
     "gdp_fold_boundary",
+    "gdp_clone_rhs",
+    "gdp_guidance",
     "gdp_smooth",
     "gdp_restrict",
     "gdp_prolong",
@@ -98,6 +100,10 @@ def _declare(lib):
         "gdp_gradient": (c_int, [P, P, P, c_int, c_int, c_int, P]),
         "gdp_divergence": (c_int, [P, P, P, c_int, c_int, c_int, P]),
         "gdp_fold_boundary": (c_int, [P, P, P, c_int, c_int, c_int, c_double, P]),
+        "gdp_clone_rhs": (c_int, [P, c_int, c_int, c_int, c_int, P, P, c_int, c_int, c_int,
+                                  c_int, c_int, c_int, c_int, c_int, P, P]),
+        "gdp_guidance": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, c_int, c_int, P, P,
+                                 P]),
         "gdp_smooth": (c_int, [P, P, c_int, c_int, c_double, c_int, c_int, c_int, P]),
         "gdp_restrict": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),
         "gdp_prolong": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),

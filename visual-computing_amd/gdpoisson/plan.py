@@ -171,6 +171,17 @@ class Plan:
             if a.dtype != torch.float64 or not a.is_contiguous() or a.device != self.device:
                 raise ValueError(f"{name}: need contiguous float64 tensors on {self.device}")
 
+    def _ptrs(self, arrays):
+        """ctypes array of device pointers, reused when the same tensors come back."""
+        key = tuple(x.data_ptr() for x in arrays)
+        cache = self.__dict__.setdefault("_ptr_cache", {})
+        arr = cache.get(key)
+        if arr is None:
+            if len(cache) > 64:
+                cache.clear()
+            arr = cache[key] = _lib.ptr_array(list(key))
+        return arr
+
     # -------------------------------------------------------------- solves
     def mgcg(self, b: Sequence[torch.Tensor], out: Sequence[torch.Tensor] | None = None,
              jacobi: bool = False):
@@ -185,8 +196,8 @@ class Plan:
         out = list(out)
         self._check_inputs(out, "out")
         reps = (_lib.Report * self.count)()
-        bp = _lib.ptr_array([x.data_ptr() for x in b])
-        up = _lib.ptr_array([x.data_ptr() for x in out])
+        bp = self._ptrs(b)
+        up = self._ptrs(out)
         fn = _lib.lib().gdp_cg_solve if jacobi else _lib.lib().gdp_mgcg_solve
         with torch.cuda.device(self.device):
             _lib.check(fn(self._plan, bp, up, reps, _lib.current_stream()), "solve")

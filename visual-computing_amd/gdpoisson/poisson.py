@@ -1,0 +1,165 @@
+"""Gradient-domain compositing on the GPU, with the interface of
+``gradient-domain/src/gradient_domain/poisson.py``.
+
+``GuidanceField`` (poisson.py:52-92) holds a target gradient field; ``solve_dirichlet``
+(poisson.py:102-147) integrates a field over a rectangle whose one-pixel ring is fixed;
+``seamless_clone`` (poisson.py:261-348) pastes a source region into a target.  Images
+are ``(h, w)`` or channels-last ``(h, w, c)``; numpy in gives numpy out, device tensors
+in give device tensors out.  Every channel of a call is solved by one batched plan.
+
+``seamless_clone`` takes the rectangle route (``domain="rectangle"``): the unknowns are
+a window around the selection, the guidance is the chosen combination inside the mask
+and the target's gradient outside, and the window's ring supplies Dirichlet values.
+Its right-hand side is built by one fused kernel (gdp_clone_rhs).  The irregular
+``domain="mask"`` system, which the reference assembles and factorises with SuperLU
+(poisson.py:154-244), is not a multigrid problem and is not offered.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._device import to_device, to_host_like
+from .operators import _planes, _stream, _unplanes, divergence, fold_boundary, gradient
+from .solvers import SolverReport, solve_planes
+
+__all__ = ["GuidanceField", "solve_dirichlet", "seamless_clone"]
+
+MODES = {"import": 0, "mixed": 1, "average": 2}
+
+
+def _mask_tensor(mask, shape, device) -> torch.Tensor:
+    t = mask if isinstance(mask, torch.Tensor) else torch.from_numpy(np.asarray(mask))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"mask shape {tuple(t.shape)} does not match {tuple(shape)}")
+    return (t != 0).to(device=device, dtype=torch.uint8).contiguous()
+
+
+@dataclass
+class GuidanceField:
+    """A target gradient field (gx, gy), each ``(h, w)`` or ``(h, w, c)``."""
+
+    gx: object
+    gy: object
+
+    @classmethod
+    def of(cls, image) -> "GuidanceField":
+        """The field that reproduces ``image`` exactly."""
+        return cls(*gradient(image))
+
+    def _combine(self, other, mask, factor, scale: float, mode: int) -> "GuidanceField":
+        ax, host = to_device(self.gx)
+        ay, _ = to_device(self.gy)
+        pax, cl = _planes(ax)
+        pay, _ = _planes(ay)
+        c, h, w = pax.shape
+        pbx = pby = None
+        if other is not None:
+            pbx, _ = _planes(to_device(other.gx)[0])
+            pby, _ = _planes(to_device(other.gy)[0])
+            if pbx.shape != pax.shape:
+                raise ValueError("the two fields differ in shape")
+        m = _mask_tensor(mask, (h, w), pax.device) if mask is not None else None
+        f = None
+        if factor is not None:
+            f = to_device(factor)[0].contiguous()
+            if tuple(f.shape) != (h, w):
+                raise ValueError(f"factor shape {tuple(f.shape)} does not match {(h, w)}")
+        ox, oy = torch.empty_like(pax), torch.empty_like(pay)
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        _lib.check(_lib.lib().gdp_guidance(
+            ptr(pax), ptr(pay), ptr(pbx), ptr(pby), ptr(m), ptr(f), float(scale), mode, h, w, c,
+            ptr(ox), ptr(oy), _stream()), "gdp_guidance")
+        return GuidanceField(to_host_like(_unplanes(ox, cl), host),
+                             to_host_like(_unplanes(oy, cl), host))
+
+    def select(self, mask, other: "GuidanceField") -> "GuidanceField":
+        """This field inside ``mask`` and ``other`` outside it (poisson.py:71-76)."""
+        return self._combine(other, mask, None, 1.0, 0)
+
+    def mix_by_magnitude(self, other: "GuidanceField") -> "GuidanceField":
+        """Per component, the field of larger magnitude (this one on ties; :78-88)."""
+        return self._combine(other, None, None, 1.0, 1)
+
+    def scaled(self, factor) -> "GuidanceField":
+        """Both components times a scalar or a per-pixel factor (poisson.py:90-92)."""
+        if isinstance(factor, (int, float)):
+            return self._combine(None, None, None, float(factor), 3)
+        return self._combine(None, None, factor, 1.0, 3)
+
+
+def _solve_window(rhs_planes: torch.Tensor, method, tolerance, max_iterations):
+    return solve_planes(rhs_planes, method=method, tolerance=tolerance,
+                        max_iterations=max_iterations)
+
+
+def solve_dirichlet(field: GuidanceField, boundary, method: str = "mgcg",
+                    tolerance: float = 1e-8, max_iterations: int = 2000):
+    """Integrate ``field`` over a rectangle whose ring is fixed to ``boundary``
+    (poisson.py:102-147).  Returns ``(image, reports)``, one report per channel."""
+    bd, host = to_device(boundary)
+    pb, cl = _planes(bd)
+    c, h, w = pb.shape
+    if h < 3 or w < 3:
+        raise ValueError("the rectangle needs an interior (at least 3 x 3)")
+    rhs = divergence(to_device(field.gx)[0], to_device(field.gy)[0])
+    prhs, _ = _planes(rhs)
+    folded = fold_boundary(_unplanes(prhs[:, 1:-1, 1:-1].contiguous(), cl), bd)
+    pf, _ = _planes(folded)
+    sol, reports = _solve_window(pf, method, tolerance, max_iterations)
+    out = pb.clone()
+    out[:, 1:-1, 1:-1] = sol
+    return to_host_like(_unplanes(out, cl), host), reports
+
+
+def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
+                   domain: str = "rectangle", method: str = "mgcg", margin: int = 24,
+                   tolerance: float = 1e-8, max_iterations: int = 2000):
+    """Paste the masked region of ``source`` into ``target`` at ``offset`` without a seam
+    (poisson.py:261-348, rectangle route).  Returns ``(image, reports)``."""
+    if mode not in MODES:
+        raise ValueError(f"unknown mode {mode!r}")
+    if domain == "mask":
+        raise NotImplementedError(
+            "the mask-domain formulation is a sparse direct solve (SuperLU); the GPU path "
+            "integrates the rectangle around the selection: use domain='rectangle'")
+    if domain != "rectangle":
+        raise ValueError(f"unknown domain {domain!r}")
+    tgt, host = to_device(target)
+    src, _ = to_device(source)
+    if src.ndim != tgt.ndim:
+        raise ValueError("source and target need the same number of dimensions")
+    pt, cl = _planes(tgt)
+    ps, _ = _planes(src)
+    c, h, w = pt.shape
+    if ps.shape[0] != c:
+        raise ValueError("source and target need the same number of channels")
+    sh, sw = ps.shape[1:]
+    row, col = int(offset[0]), int(offset[1])
+    if row < 0 or col < 0 or row + sh > h or col + sw > w:
+        raise ValueError("the source crop does not fit in the target at this offset")
+    m = _mask_tensor(mask, (sh, sw), pt.device)
+    nz = torch.nonzero(m)
+    if nz.numel() == 0:
+        return to_host_like(_unplanes(pt.clone(), cl), host), []
+    r0, c0 = (nz.min(dim=0).values + torch.tensor([row, col], device=nz.device)).tolist()
+    r1, c1 = (nz.max(dim=0).values + torch.tensor([row, col], device=nz.device)).tolist()
+    top, bottom = max(r0 - margin, 0), min(r1 + margin + 1, h)
+    left, right = max(c0 - margin, 0), min(c1 + margin + 1, w)
+    hh, ww = bottom - top, right - left
+    if hh < 3 or ww < 3:
+        raise ValueError("the window around the selection has no interior")
+    rhs = torch.empty((c, hh - 2, ww - 2), dtype=torch.float64, device=pt.device)
+    _lib.check(_lib.lib().gdp_clone_rhs(
+        ctypes.c_void_p(ps.data_ptr()), sh, sw, row, col, ctypes.c_void_p(m.data_ptr()),
+        ctypes.c_void_p(pt.data_ptr()), h, w, c, top, left, hh, ww, MODES[mode],
+        ctypes.c_void_p(rhs.data_ptr()), _stream()), "gdp_clone_rhs")
+    sol, reports = _solve_window(rhs, method, tolerance, max_iterations)
+    out = pt.clone()
+    out[:, top + 1:bottom - 1, left + 1:right - 1] = sol
+    return to_host_like(_unplanes(out, cl), host), reports

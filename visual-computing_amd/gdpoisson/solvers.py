@@ -73,6 +73,41 @@ def _run(kind, b, spacing, tolerance, max_iterations, boundary="dirichlet"):
     return u, int(rep.iterations), bool(rep.converged), float(rep.relative_residual)
 
 
+def solve_planes(planes: torch.Tensor, method: str = "mgcg", tolerance: float = 1e-8,
+                 max_iterations: int = 2000, spacing: float = 1.0,
+                 boundary: str = "dirichlet"):
+    """Solve every plane of a (c, m, n) device tensor with one batched plan.
+
+    Returns (solutions (c, m, n), list of SolverReport); ``seconds`` is the batch's device
+    time (the planes are solved together).
+    """
+    if method not in SOLVERS:
+        raise KeyError(f"unknown solver {method!r}; choose from {sorted(SOLVERS)}")
+    c, m, n = planes.shape
+    bs = [planes[k] for k in range(c)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if method == "multigrid":
+        opts = MultigridOptions(max_cycles=int(max_iterations), tolerance=float(tolerance))
+        plan = cached_plan([(m, n)] * c, boundary, float(spacing), PlanOptions(
+            precision="fp64", pre_smoothing=opts.pre_smoothing,
+            post_smoothing=opts.post_smoothing, coarsest_size=opts.coarsest_size,
+            coarsest_sweeps=opts.coarsest_sweeps, max_iterations=opts.max_cycles,
+            tolerance=opts.tolerance, use_graph=False))
+        us, reps, _ = plan.mg_solve(bs)
+    else:
+        precision = "mixed" if method == "mgcg_mixed" else "fp64"
+        plan = cached_plan([(m, n)] * c, boundary, float(spacing),
+                           _pcg_options(precision, tolerance, max_iterations, boundary))
+        us, reps = plan.mgcg(bs, jacobi=(method == "cg"))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    reports = [SolverReport(method=method, iterations=int(r.iterations),
+                            relative_residual=float(r.relative_residual), seconds=dt,
+                            converged=bool(r.converged)) for r in reps]
+    return torch.stack(us), reports
+
+
 def _registry_entry(kind):
     def fn(b, spacing, tolerance, max_iterations):
         t, host = to_device(b)

@@ -28,6 +28,15 @@
  *                        GuidanceField.of / select / mix_by_magnitude + divergence +
  *                        fold_boundary of the window, fused
  *   gdp_guidance         poisson.py:52-92      GuidanceField.select / mix_by_magnitude / scaled
+ *   gdp_tm_log_luminance hdr.py:165-167        tone_map: log of the clamped luminance
+ *   gdp_tm_downsample    hdr.py:73-80          _downsample(image)
+ *   gdp_tm_level_factor  hdr.py:83-104,126-133 attenuation_field: one pyramid level
+ *                                              (_central_gradient, magnitude mean, local
+ *                                              factor, _upsample of the coarser factor)
+ *   gdp_tm_rhs           hdr.py:174-175        divergence(gx * factor, gy * factor)
+ *   gdp_tm_compose       hdr.py:180-182        exp(c - max c) * ratios ** saturation
+ *   gdp_percentile       hdr.py:184            numpy.percentile(image, q), linear method
+ *   gdp_tm_normalise     hdr.py:185            clip(image / max(peak, 1e-12), 0, 1)
  */
 #ifndef GDPOISSON_H
 #define GDPOISSON_H
@@ -102,8 +111,6 @@ int gdp_divergence(const double* gx, const double* gy, double* out, int h, int w
 /* out (h-2, w-2) = fold_boundary(rhs (h-2, w-2), boundary (h, w), spacing); out may alias rhs. */
 int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, int h, int w,
                       int planes, double spacing, void* stream);
-/* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
-   (order = 0) or colour 1 then 0 (order = 1). */
 /* Folded right-hand side of seamless cloning over a window (the rectangle route of
    seamless_clone, poisson.py:261-348): source planes (planes x sh x sw) placed at
    (row, col) in the target planes (planes x h x w), source-space mask (sh x sw bytes,
@@ -119,6 +126,36 @@ int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
 int gdp_guidance(const double* ax, const double* ay, const double* bx, const double* by,
                  const uint8_t* mask, const double* factor, double scale, int mode, int h, int w,
                  int planes, double* out_x, double* out_y, void* stream);
+/* ---------------------------------------------------------------- tone mapping
+ * One FP64 plane per image (h x w, row-major); radiance is h x w x 3 channels-last.
+ * `work` is a device buffer of gdp_tm_work_bytes() bytes (scalars and partial sums);
+ * calls that share one `work` must be ordered on one stream.                       */
+int gdp_tm_work_bytes(int h, int w, size_t* bytes);
+/* out = log(max(radiance . (0.2126, 0.7152, 0.0722), DBL_MIN)). */
+int gdp_tm_log_luminance(const double* radiance, int h, int w, double* out, void* stream);
+/* out ((h+1)/2 x (w+1)/2) = every other sample of the 5-tap binomial blur of in. */
+int gdp_tm_downsample(const double* in, int h, int w, double* out, void* stream);
+/* factor = (s / g) (g / s)^beta with g = max(|central gradient of level|, epsilon) at
+   this spacing and s = alpha max(mean g0, epsilon) (g0 the unclamped magnitude), times
+   the bilinear upsampling of `coarse` (hc x wc) when it is not NULL. */
+int gdp_tm_level_factor(const double* level, int h, int w, double spacing, double alpha,
+                        double beta, double epsilon, const double* coarse, int hc, int wc,
+                        double* factor, double* work, void* stream);
+/* rhs = divergence(gx * factor, gy * factor), (gx, gy) = gradient(log_lum); h, w >= 2. */
+int gdp_tm_rhs(const double* log_lum, const double* factor, int h, int w, double* rhs,
+               void* stream);
+/* image (h x w x 3) = (radiance / luminance)^saturation * exp(compressed - max compressed). */
+int gdp_tm_compose(const double* radiance, const double* compressed, int h, int w,
+                   double saturation, double* image, double* work, void* stream);
+/* *out (device) = numpy.percentile(x[0 .. n), q) (linear method), n < 2^32, by an
+   eight-pass radix selection; `work` holds gdp_percentile_work_bytes() bytes. */
+int gdp_percentile_work_bytes(size_t* bytes);
+int gdp_percentile(const double* x, size_t n, double q, double* out, void* work, void* stream);
+/* image[0 .. n) = clip(image / max(*peak, 1e-12), 0, 1), peak a device scalar. */
+int gdp_tm_normalise(double* image, size_t n, const double* peak, void* stream);
+
+/* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
+   (order = 0) or colour 1 then 0 (order = 1). */
 int gdp_smooth(double* u, const double* b, int m, int n, double spacing, int sweeps,
                int boundary, int order, void* stream);
 /* coarse (mc, nc) = 0.25 P_r^T fine P_c with fine (m, n). */

@@ -154,3 +154,26 @@ def tonemap_problem(shape, seed):
     f = attenuation_single_scale(log_l)
     gx, gy = gradient(log_l)
     return divergence(gx * f, gy * f)
+
+
+def radiance_map(shape, seed, decades=5.0, noise=0.01):
+    """Smooth HDR radiance (h, w, 3): the blob image rescaled so its luminance spans
+    ``decades`` decades, times a smooth per-channel tint in [0.6, 1.4], times
+    (1 + noise N(0, 1))."""
+    base = blob_image(shape, seed)
+    base = (base - base.min()) / max(base.max() - base.min(), 1e-12)
+    lum = 10.0 ** (decades * base - 0.5 * decades)
+    tints = []
+    for c in range(3):
+        t = blob_image(shape, seed + 11 + c)
+        tints.append(0.6 + 0.8 * (t - t.min()) / max(t.max() - t.min(), 1e-12))
+    rng = np.random.default_rng(seed + 17)
+    rad = lum[..., None] * np.stack(tints, axis=2)
+    return rad * (1.0 + noise * rng.standard_normal(rad.shape)).clip(0.5, None)
+
+
+def lognormal_radiance(shape, seed, sigma=2.0):
+    """i.i.d. log-normal (mu 0, sigma 2) radiance per pixel and channel, (h, w, 3): the
+    tone-mapping requests of the serving stream (the log field is not smoothed)."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.normal(0.0, sigma, tuple(shape) + (3,)))

@@ -6,7 +6,9 @@ public names (``solve_system``, ``SOLVERS``, ``SolverReport``, ``MultigridOption
 ``MultigridReport``, ``v_cycle``, ``solve``, ``solve_neumann``, ``gradient``,
 ``divergence``, ``laplacian``, ``fold_boundary``) backed by hand-written HIP kernels
 in ``libgdpoisson.so`` (C ABI: ``include/gdpoisson.h``).  Batched plans
-(:class:`Plan`) solve many grids per launch.
+(:class:`Plan`) solve many grids per launch.  The compositing (``seamless_clone``,
+``GuidanceField``, ``solve_dirichlet``) and tone-mapping (``tone_map``,
+``attenuation_field``, ``ToneMapConfig``) callers of the solve run on the device too.
 """
 
 from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
@@ -14,6 +16,8 @@ from .operators import divergence, fold_boundary, gradient, laplacian, neumann_l
 from .autotune import autotune
 from .plan import Plan, PlanOptions
 from .poisson import GuidanceField, seamless_clone, solve_dirichlet
+from . import hdr
+from .hdr import ToneMapConfig, attenuation_field, tone_map
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
 
 __version__ = "0.1.0"
@@ -25,6 +29,10 @@ __all__ = [
     "GuidanceField",
     "seamless_clone",
     "solve_dirichlet",
+    "ToneMapConfig",
+    "attenuation_field",
+    "hdr",
+    "tone_map",
     "autotune",
     "PlanOptions",
     "SOLVERS",

@@ -24,6 +24,7 @@ LIB = PKG / "libgdpoisson.so"
 SOURCES = [
     "gdp_plan.hip",
     "gdp_ops.hip",
+    "gdp_hdr.hip",
     "gdp_sdown_f32.hip",
     "gdp_sdown_f64.hip",
     "gdp_sup_f32.hip",

@@ -27,6 +27,15 @@ EXPORTS = (
     "gdp_fold_boundary",
     "gdp_clone_rhs",
     "gdp_guidance",
+    "gdp_tm_work_bytes",
+    "gdp_tm_log_luminance",
+    "gdp_tm_downsample",
+    "gdp_tm_level_factor",
+    "gdp_tm_rhs",
+    "gdp_tm_compose",
+    "gdp_percentile_work_bytes",
+    "gdp_percentile",
+    "gdp_tm_normalise",
     "gdp_smooth",
     "gdp_restrict",
     "gdp_prolong",
@@ -104,6 +113,16 @@ def _declare(lib):
                                   c_int, c_int, c_int, c_int, c_int, P, P]),
         "gdp_guidance": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, c_int, c_int, P, P,
                                  P]),
+        "gdp_tm_work_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
+        "gdp_tm_log_luminance": (c_int, [P, c_int, c_int, P, P]),
+        "gdp_tm_downsample": (c_int, [P, c_int, c_int, P, P]),
+        "gdp_tm_level_factor": (c_int, [P, c_int, c_int, c_double, c_double, c_double, c_double,
+                                        P, c_int, c_int, P, P, P]),
+        "gdp_tm_rhs": (c_int, [P, P, c_int, c_int, P, P]),
+        "gdp_tm_compose": (c_int, [P, P, c_int, c_int, c_double, P, P, P]),
+        "gdp_percentile_work_bytes": (c_int, [POINTER(c_size_t)]),
+        "gdp_percentile": (c_int, [P, c_size_t, c_double, P, P, P]),
+        "gdp_tm_normalise": (c_int, [P, c_size_t, P, P]),
         "gdp_smooth": (c_int, [P, P, c_int, c_int, c_double, c_int, c_int, c_int, P]),
         "gdp_restrict": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),
         "gdp_prolong": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),

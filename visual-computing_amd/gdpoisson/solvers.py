@@ -85,7 +85,7 @@ def solve_planes(planes: torch.Tensor, method: str = "mgcg", tolerance: float = 
         raise KeyError(f"unknown solver {method!r}; choose from {sorted(SOLVERS)}")
     c, m, n = planes.shape
     bs = [planes[k] for k in range(c)]
-    torch.cuda.synchronize()
+    torch.cuda.current_stream().synchronize()
     t0 = time.perf_counter()
     if method == "multigrid":
         opts = MultigridOptions(max_cycles=int(max_iterations), tolerance=float(tolerance))
@@ -100,7 +100,7 @@ def solve_planes(planes: torch.Tensor, method: str = "mgcg", tolerance: float = 
         plan = cached_plan([(m, n)] * c, boundary, float(spacing),
                            _pcg_options(precision, tolerance, max_iterations, boundary))
         us, reps = plan.mgcg(bs, jacobi=(method == "cg"))
-    torch.cuda.synchronize()
+    torch.cuda.current_stream().synchronize()
     dt = time.perf_counter() - t0
     reports = [SolverReport(method=method, iterations=int(r.iterations),
                             relative_residual=float(r.relative_residual), seconds=dt,
@@ -147,10 +147,10 @@ def solve_system(
     if t.ndim != 2:
         raise ValueError(f"b must be 2-D, got shape {tuple(t.shape)}")
     # a zero right-hand side converges at set-up with zero iterations and u = 0
-    torch.cuda.synchronize()
+    torch.cuda.current_stream().synchronize()
     t0 = time.perf_counter()
     u, it, conv, rel = _run(method, t, spacing, tolerance, max_iterations, boundary)
-    torch.cuda.synchronize()
+    torch.cuda.current_stream().synchronize()
     dt = time.perf_counter() - t0
     return to_host_like(u, host), SolverReport(
         method=method, iterations=it, relative_residual=rel, seconds=dt, converged=conv

@@ -40,17 +40,10 @@ def log(msg):
 
 
 def make_inputs():
-    import numpy as np
-    import torch
+    from gdpoisson import workloads as W
 
-    from oracle import generators as G
-
-    bs, exact = [], []
-    for c in range(CHANNELS):
-        b, x = G.neumann_problem(SHAPE, seed=2000 + c)
-        bs.append(torch.from_numpy(b).cuda())
-        exact.append(x)
-    return bs, exact
+    # seeds 2000 + channel (configuration 2, problem 0)
+    return W.rgb_neumann_problem(SHAPE, config=2, problem=0)
 
 
 def time_solves(plan, bs, steps, warmup):
@@ -112,6 +105,44 @@ def cpu_baseline(runs: int = 5):
 
 
 COPY_BYTES = 64 << 23  # the --profile-run calibration copy: 512 MiB read + 512 MiB written
+
+
+def serve_stream(count: int, qps: float, slots: int) -> dict:
+    """The mixed request stream: Poisson arrivals at ``qps``, 70% RGB seamless compositing
+    (squares of side U{512..2048}, blob masks) and 30% 3840 x 2160 tone mapping, served
+    by a continuous-batching Server with ``slots`` streams.  Returns latency statistics."""
+    import numpy as np
+
+    from gdpoisson import workloads as W
+    from gdpoisson.serving import Server, latency_stats
+
+    log(f"serving stream: {count} requests at {qps:g}/s, {slots} slots")
+    warm = W.serving_stream(count=8, qps=qps, seed=4999)
+    Server(slots=slots).run(warm, realtime=False)  # first-use costs outside the stream
+    reqs = W.serving_stream(count=count, qps=qps, seed=5000)
+    srv = Server(slots=slots, keep_outputs=False)
+    done = srv.run(reqs)
+    st = latency_stats(done)
+    lat = {}
+    for c in done:
+        lat.setdefault(c.kind, []).append(c.latency * 1e3)
+    return {
+        "requests": count,
+        "qps": qps,
+        "slots": slots,
+        "mix": "70% RGB seamless compositing (squares 512-2048, blob masks), "
+               "30% tone mapping 3840x2160 (log-normal radiance)",
+        "method": srv.method,
+        **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
+        "p50_ms_by_kind": {k: round(float(np.percentile(v, 50)), 3) for k, v in lat.items()},
+        "p99_ms_by_kind": {k: round(float(np.percentile(v, 99)), 3) for k, v in lat.items()},
+        "batches": len(srv.batches),
+        "mean_requests_per_batch": round(sum(n for _, n, _ in srv.batches) / max(len(srv.batches), 1), 2),
+        "peak_admitted_GB": round(srv.peak_bytes / 1e9, 2),
+        "budget_GB": round(srv.budget / 1e9, 1),
+        "all_converged": all(r.converged and r.relative_residual <= 1e-8
+                             for c in done for r in c.reports),
+    }
 
 
 def _level_area(name: str, grid: int) -> int:
@@ -197,6 +228,10 @@ def main():
     ap.add_argument("--tiles", metavar="BENCH_JSON",
                     help="reuse the tile shapes recorded in a bench line (config.tiles) instead "
                     "of tuning, so a profiled run has exactly the benched configuration")
+    ap.add_argument("--serve-requests", type=int, default=120,
+                    help="requests of the mixed serving stream (0: skip it)")
+    ap.add_argument("--serve-qps", type=float, default=200.0, help="arrival rate of the stream")
+    ap.add_argument("--serve-slots", type=int, default=3, help="serving workers (HIP streams)")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -367,6 +402,13 @@ def main():
         },
         "kernels": kernels,
     }
+    if args.serve_requests > 0:
+        line["serving"] = serve_stream(args.serve_requests, args.serve_qps, args.serve_slots)
+        line["latency_ms"] = {
+            "p50": line["serving"]["p50_ms"], "p99": line["serving"]["p99_ms"],
+            "of": "requests of the mixed serving stream (arrival to completion)",
+            "solve_p50": line["latency_ms"]["p50"], "solve_p99": line["latency_ms"]["p99"],
+            "solve_fp64_p50": line["latency_ms"]["fp64_p50"]}
     if not args.no_cpu:
         log("CPU oracle baseline")
         line["cpu_baseline"] = cpu_baseline()

@@ -1,0 +1,92 @@
+"""The serving loop on the device: every request's result is bit-for-bit the same alone
+and in the mix (batched with others in one plan, overlapped with other batches on other
+streams), served results agree with the oracle, and a timed stream reports latency."""
+
+import numpy as np
+import pytest
+
+from oracle import generators as G
+from oracle import hdr as H
+from oracle import poisson as OP
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gd():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gdpoisson
+
+    return gdpoisson
+
+
+def _requests():
+    """Six compositing requests (squares cut from 256 x 256 blob images and a blob mask)
+    and four small tone-mapping requests, all arriving at once."""
+    import torch
+
+    from gdpoisson.serving import Request
+
+    src, tgt, mask = G.compositing_images((256, 256), config=5)
+    s, t, m = (torch.from_numpy(x).cuda() for x in (src, tgt, mask))
+    rng = np.random.default_rng(3)
+    reqs = []
+    for k in range(6):
+        side = int(rng.integers(64, 161))
+        r0, c0 = (int(v) for v in rng.integers(0, 256 - side + 1, 2))
+        win = (slice(r0, r0 + side), slice(c0, c0 + side))
+        reqs.append(Request("composite", {"source": s[win], "target": t[win], "mask": m[win],
+                                          "offset": (0, 0), "mode": ["import", "mixed"][k % 2]}))
+    for k, shape in enumerate([(96, 128), (120, 90), (96, 128), (64, 200)]):
+        reqs.append(Request("tonemap", {
+            "radiance": torch.from_numpy(G.radiance_map(shape, seed=40 + k)).cuda()}))
+    return reqs
+
+
+def test_results_are_the_same_alone_and_in_the_mix(gd):
+    import torch
+
+    from gdpoisson.serving import Request, Server
+
+    reqs = _requests()
+    srv = Server(budget_bytes=4 << 30, slots=3, max_batch=12)
+    mixed = srv.run(reqs, realtime=False)
+    assert any(n > 1 for _, n, _ in srv.batches), "nothing was batched"
+    assert all(c.error is None for c in mixed)
+    for r, c in zip(reqs, mixed):
+        assert all(rep.converged and rep.relative_residual <= 1e-8 for rep in c.reports)
+        alone = Server(budget_bytes=4 << 30, slots=1).run(
+            [Request(r.kind, r.args)], realtime=False)[0]
+        assert torch.equal(alone.output, c.output), (r.kind, c.rid)
+
+
+def test_served_results_match_the_oracle(gd):
+    from gdpoisson.serving import Server
+
+    reqs = _requests()
+    done = Server(budget_bytes=4 << 30, slots=2, max_batch=12).run(reqs, realtime=False)
+    for r, c in zip(reqs, done):
+        a = r.args
+        if r.kind == "composite":
+            want, _ = OP.seamless_clone(a["source"].cpu().numpy(), a["target"].cpu().numpy(),
+                                        a["mask"].cpu().numpy(), (0, 0), mode=a["mode"])
+        else:
+            want, _ = H.tone_map(a["radiance"].cpu().numpy())
+        assert np.abs(c.output.cpu().numpy() - want).max() < 1e-6, (r.kind, c.rid)
+
+
+def test_a_timed_stream_reports_latency(gd):
+    from gdpoisson import workloads as W
+    from gdpoisson.serving import Server, latency_stats
+
+    reqs = W.serving_stream(count=16, qps=200.0, seed=7, sides=(128, 384),
+                            tonemap_shape=(270, 480), base_side=512)
+    done = Server(slots=3).run(reqs)
+    st = latency_stats(done)
+    assert st["served"] == 16 and st["rejected"] == 0
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    assert all(rep.converged for c in done for rep in c.reports)
+    assert {c.kind for c in done} == {"composite", "tonemap"}

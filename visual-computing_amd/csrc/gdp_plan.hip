@@ -17,6 +17,8 @@
 #include <cstring>
 #include <string>
 #include <utility>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "gdp_device.h"
@@ -144,10 +146,13 @@ struct gdp_plan {
   const double** h_init = nullptr;
   ProblemState* h_state = nullptr;
   int* h_flag = nullptr;
+  void* h_block = nullptr;     // pinned block holding h_io, h_init, h_state, h_flag
+  size_t h_block_bytes = 0;
   hipStream_t cap = nullptr;
   hipEvent_t io_ev = nullptr;  // the last upload of h_io / h_init has been consumed
   hipGraphExec_t pair_exec[2] = {nullptr, nullptr};   // [0] mgcg, [1] cg
   hipGraphExec_t start_exec[2] = {nullptr, nullptr};  // set-up + first preconditioning
+  hipStream_t graph_stream = nullptr;  // stream of the last graph launch
   int last_iters = 0;
   bool bound = false;
   // built-in kernel timing (HIP events around every launch on the launch stream)
@@ -207,8 +212,81 @@ std::vector<std::pair<int, int>> level_shapes(const gdp_problem& p, const gdp_op
   return s;
 }
 
-// Instantiated hipGraphs of a plan are tied to its tables: drop them on re-bind / destroy.
+// Pinned host staging blocks of plans are kept for reuse rather than freed: freeing
+// pinned memory may wait for the whole device, which would stall the work other
+// streams (other plans) have in flight.  Blocks come in power-of-two size classes.
+struct PinnedPool {
+  std::mutex mu;
+  std::unordered_map<size_t, std::vector<void*>> free;
+};
+PinnedPool& pinned_pool() {
+  static PinnedPool* pool = new PinnedPool;  // lives until the process ends
+  return *pool;
+}
+size_t pinned_class(size_t bytes) {
+  size_t c = 4096;
+  while (c < bytes) c <<= 1;
+  return c;
+}
+hipError_t pinned_get(size_t bytes, void** out) {
+  const size_t c = pinned_class(bytes);
+  {
+    PinnedPool& pool = pinned_pool();
+    std::lock_guard<std::mutex> lock(pool.mu);
+    std::vector<void*>& v = pool.free[c];
+    if (!v.empty()) {
+      *out = v.back();
+      v.pop_back();
+      return hipSuccess;
+    }
+  }
+  return hipHostMalloc(out, c, hipHostMallocDefault);
+}
+void pinned_put(void* p, size_t bytes) {
+  PinnedPool& pool = pinned_pool();
+  std::lock_guard<std::mutex> lock(pool.mu);
+  pool.free[pinned_class(bytes)].push_back(p);
+}
+
+// Raises the dynamic-LDS limit of a tail kernel family (0 / 1: strip tail float / double,
+// 2 / 3: LDS tail float / double) to at least smem.  The limit is process-wide and only
+// ever rises: a plan bound later, possibly in another thread, must not lower a limit
+// that launches of an earlier plan rely on.
+hipError_t raise_smem(int family, int smem) {
+  static std::mutex mu;
+  static int cur[4] = {0, 0, 0, 0};
+  std::lock_guard<std::mutex> lock(mu);
+  if (smem <= cur[family]) return hipSuccess;
+  hipError_t e = hipSuccess;
+  switch (family) {
+    case 0: e = stail_set_smem<float>(smem); break;
+    case 1: e = stail_set_smem<double>(smem); break;
+    case 2:
+      e = hipFuncSetAttribute((const void*)k_tail<float, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tail<float, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+      break;
+    default:
+      e = hipFuncSetAttribute((const void*)k_tail<double, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_tail<double, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, smem);
+  }
+  if (e == hipSuccess) cur[family] = smem;
+  return e;
+}
+
+// Instantiated hipGraphs of a plan are tied to its tables: drop them on re-bind /
+// destroy, once the stream they last ran on has finished them (other streams, and so
+// other plans' work, are not waited for).
 void destroy_graphs(gdp_plan* P) {
+  bool any = false;
+  for (hipGraphExec_t* ex : {P->pair_exec, P->start_exec}) any = any || ex[0] || ex[1];
+  if (any && P->graph_stream) (void)hipStreamSynchronize(P->graph_stream);
+  P->graph_stream = nullptr;
   for (hipGraphExec_t* ex : {P->pair_exec, P->start_exec})
     for (int g = 0; g < 2; ++g)
       if (ex[g]) {
@@ -510,10 +588,7 @@ int gdp_plan_destroy(gdp_plan* P) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : P->prof_pool) (void)hipEventDestroy(e);
-  if (P->h_io) (void)hipHostFree(P->h_io);
-  if (P->h_init) (void)hipHostFree(P->h_init);
-  if (P->h_state) (void)hipHostFree(P->h_state);
-  if (P->h_flag) (void)hipHostFree(P->h_flag);
+  if (P->h_block) pinned_put(P->h_block, P->h_block_bytes);
   delete P;
   return GDP_OK;
 }
@@ -533,14 +608,27 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   }
   P->ws = static_cast<char*>(workspace);
   const int NP = P->count();
-  if (!P->h_io) {
-    GDP_HIP(hipHostMalloc((void**)&P->h_io, sizeof(IoDesc) * NP, hipHostMallocDefault));
-    GDP_HIP(hipHostMalloc((void**)&P->h_init, sizeof(double*) * NP, hipHostMallocDefault));
-    GDP_HIP(hipHostMalloc((void**)&P->h_state, sizeof(ProblemState) * NP, hipHostMallocDefault));
-    GDP_HIP(hipHostMalloc((void**)&P->h_flag, sizeof(int) * 4, hipHostMallocDefault));
+  if (!P->h_block) {
+    auto a64 = [](size_t x) { return (x + 63) & ~size_t(63); };
+    const size_t b_io = a64(sizeof(IoDesc) * NP), b_init = a64(sizeof(double*) * NP);
+    const size_t b_state = a64(sizeof(ProblemState) * NP);
+    P->h_block_bytes = b_io + b_init + b_state + 64;
+    GDP_HIP(pinned_get(P->h_block_bytes, &P->h_block));
+    char* hb = static_cast<char*>(P->h_block);
+    P->h_io = reinterpret_cast<IoDesc*>(hb);
+    P->h_init = reinterpret_cast<const double**>(hb + b_io);
+    P->h_state = reinterpret_cast<ProblemState*>(hb + b_io + b_init);
+    P->h_flag = reinterpret_cast<int*>(hb + b_io + b_init + b_state);
   }
   if (!P->cap) GDP_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
   if (!P->io_ev) GDP_HIP(hipEventCreateWithFlags(&P->io_ev, hipEventDisableTiming));
+  // Host-to-device uploads of the tables, on the plan's own non-blocking stream (never
+  // the null stream, which other threads' graph captures forbid), each finished before
+  // its host buffer can go away.
+  auto h2d = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, P->cap);
+    return e == hipSuccess ? hipStreamSynchronize(P->cap) : e;
+  };
 
   // ---- descriptors
   std::vector<LevelDesc> levels(P->total_levels);
@@ -627,12 +715,11 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
           il[k * n + l] = (neu && k == 0 && l == 0) ? 0.0 : 1.0 / lam;
         }
       t.dct = P->at<double>(pi.off_dct);
-      GDP_HIP(hipMemcpy(P->ws + pi.off_dct, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+      GDP_HIP(h2d(P->ws + pi.off_dct, tab.data(), tab.size() * 8));
     }
   }
-  GDP_HIP(hipMemcpy(P->d_levels(), levels.data(), sizeof(LevelDesc) * levels.size(),
-                    hipMemcpyHostToDevice));
-  GDP_HIP(hipMemcpy(P->d_kry(), kry.data(), sizeof(KrylovDesc) * NP, hipMemcpyHostToDevice));
+  GDP_HIP(h2d(P->d_levels(), levels.data(), sizeof(LevelDesc) * levels.size()));
+  GDP_HIP(h2d(P->d_kry(), kry.data(), sizeof(KrylovDesc) * NP));
   {
     // strip-tail problems first (k_stail takes entries [0, n_stail), k_tail the rest)
     std::vector<TailDesc> ordered;
@@ -640,8 +727,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       if (P->probs[p].stail) ordered.push_back(tails[p]);
     for (int p = 0; p < NP; ++p)
       if (!P->probs[p].stail) ordered.push_back(tails[p]);
-    GDP_HIP(hipMemcpy(P->d_tails(), ordered.data(), sizeof(TailDesc) * NP,
-                      hipMemcpyHostToDevice));
+    GDP_HIP(h2d(P->d_tails(), ordered.data(), sizeof(TailDesc) * NP));
   }
 
   // ---- tile tables (self-contained records; see VTile / KTile)
@@ -655,7 +741,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       set_error("internal: tile table overflow");
       return GDP_EINVAL;
     }
-    if (count && hipMemcpy(P->ws + toff, data, rec * count, hipMemcpyHostToDevice) != hipSuccess) {
+    if (count && h2d(P->ws + toff, data, rec * count) != hipSuccess) {
       set_error("hipMemcpy of a tile table failed");
       return GDP_EHIP;
     }
@@ -775,30 +861,19 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     const auto sw = xcd_swizzle(v);
     if (int rc = upload(P->kry, sw.data(), sizeof(KTile), sw.size())) return rc;
   }
-  GDP_HIP(hipMemcpy(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
-  GDP_HIP(hipMemcpy(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP, hipMemcpyHostToDevice));
-  GDP_HIP(hipMemset(P->d_state(), 0, sizeof(ProblemState) * NP));
+  GDP_HIP(h2d(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP));
+  GDP_HIP(h2d(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP));
+  GDP_HIP(hipMemsetAsync(P->d_state(), 0, sizeof(ProblemState) * NP, P->cap));
   for (size_t o : {P->off_apart, P->off_upart, P->off_zpart, P->off_spart})
-    GDP_HIP(hipMemset(P->ws + o, 0, sizeof(double2) * P->max_slots));
-  GDP_HIP(hipMemset(P->d_flag(), 0, sizeof(int) * 4));
+    GDP_HIP(hipMemsetAsync(P->ws + o, 0, sizeof(double2) * P->max_slots, P->cap));
+  GDP_HIP(hipMemsetAsync(P->d_flag(), 0, sizeof(int) * 4, P->cap));
 
   // the tail kernels may use more than the default dynamic LDS
-  if (P->n_stail) {
-    GDP_HIP(P->fp32 ? stail_set_smem<float>(P->stail_smem) : stail_set_smem<double>(P->stail_smem));
-  }
-  const int smem = P->tail_smem;
-  if (P->fp32) {
-    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<float, true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
-    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<float, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
-  } else {
-    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<double, true>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
-    GDP_HIP(hipFuncSetAttribute((const void*)k_tail<double, false>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, smem));
-  }
-  GDP_HIP(hipDeviceSynchronize());
+  if (P->n_stail) GDP_HIP(raise_smem(P->fp32 ? 0 : 1, P->stail_smem));
+  if (P->tail_smem) GDP_HIP(raise_smem(P->fp32 ? 2 : 3, P->tail_smem));
+  // the uploads and clears above ran on the plan's own stream: finish them before any
+  // caller stream uses the tables
+  GDP_HIP(hipStreamSynchronize(P->cap));
   destroy_graphs(P);
   P->bound = true;
   return GDP_OK;
@@ -1153,6 +1228,7 @@ int launch_captured(gdp_plan* P, hipStream_t s, hipGraphExec_t& exec, F enqueue)
     GDP_HIP(ie);
   }
   GDP_HIP(hipGraphLaunch(exec, s));
+  P->graph_stream = s;
   return GDP_OK;
 }
 
@@ -1311,7 +1387,8 @@ int gdp_mg_solve(gdp_plan* P, const double* const* b, const double* const* initi
   if (history) {
     const size_t stride = size_t(o.max_iterations) + 2;
     std::vector<double> h(stride * P->count());
-    GDP_HIP(hipMemcpy(h.data(), P->d_hist(), h.size() * 8, hipMemcpyDeviceToHost));
+    GDP_HIP(hipMemcpyAsync(h.data(), P->d_hist(), h.size() * 8, hipMemcpyDeviceToHost, s));
+    GDP_HIP(hipStreamSynchronize(s));
     for (int p = 0; p < P->count(); ++p)
       for (int k = 0; k <= o.max_iterations; ++k)
         history[size_t(p) * (o.max_iterations + 1) + k] = h[p * stride + k];

@@ -22,7 +22,8 @@ from ._device import to_device, to_host_like
 from .operators import _stream
 from .solvers import solve_planes
 
-__all__ = ["ToneMapConfig", "attenuation_field", "tone_map"]
+__all__ = ["ToneMapConfig", "attenuation_field", "tone_map", "prepare_tone_map",
+           "finish_tone_map", "ToneJob", "percentile"]
 
 
 @dataclass
@@ -89,14 +90,22 @@ def attenuation_field(log_luminance, config: ToneMapConfig):
     return to_host_like(_attenuation(L, config, work), host)
 
 
-def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "mgcg_mixed",
-             tolerance: float = 1e-8, max_iterations: int = 2000, reports: list | None = None):
-    """Compress a radiance map to a displayable image (hdr.py:137-185).
+@dataclass
+class ToneJob:
+    """A prepared tone_map: the radiance, its attenuation factor and the right-hand side
+    (1 x h x w) of the Neumann solve."""
 
-    Returns ``(image in [0, 1] of shape (h, w, 3), factor (h, w))``.  The Neumann solve
-    uses ``method`` ("mgcg" FP64 or "mgcg_mixed") to ``tolerance``; its SolverReport is
-    appended to ``reports`` when a list is given.
-    """
+    radiance: torch.Tensor
+    host: bool
+    config: ToneMapConfig
+    factor: torch.Tensor
+    rhs: torch.Tensor
+    work: torch.Tensor
+
+
+def prepare_tone_map(radiance, config: ToneMapConfig | None = None) -> ToneJob:
+    """Log luminance, attenuation factor and attenuated divergence (hdr.py:160-175), on
+    the current stream."""
     config = config or ToneMapConfig()
     rad, host = to_device(radiance)
     if rad.ndim != 3 or rad.shape[2] != 3:
@@ -109,15 +118,19 @@ def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "mgcg_
     L = torch.empty((h, w), dtype=torch.float64, device=dev)
     _lib.check(lib.gdp_tm_log_luminance(_p(rad), h, w, _p(L), s), "gdp_tm_log_luminance")
     factor = _attenuation(L, config, work)
-    rhs = torch.empty_like(L)
+    rhs = torch.empty((1, h, w), dtype=torch.float64, device=dev)
     _lib.check(lib.gdp_tm_rhs(_p(L), _p(factor), h, w, _p(rhs), s), "gdp_tm_rhs")
-    sol, reps = solve_planes(rhs[None], method=method, tolerance=tolerance,
-                             max_iterations=max_iterations, boundary="neumann")
-    if reports is not None:
-        reports.extend(reps)
+    return ToneJob(rad, host, config, factor, rhs, work)
+
+
+def finish_tone_map(job: ToneJob, compressed: torch.Tensor):
+    """Display image from the integrated log luminance (hdr.py:180-185)."""
+    lib, s, dev = _lib.lib(), _stream(), job.radiance.device
+    h, w = int(job.radiance.shape[0]), int(job.radiance.shape[1])
     image = torch.empty((h, w, 3), dtype=torch.float64, device=dev)
-    _lib.check(lib.gdp_tm_compose(_p(rad), _p(sol[0]), h, w, float(config.saturation),
-                                  _p(image), _p(work), s), "gdp_tm_compose")
+    _lib.check(lib.gdp_tm_compose(_p(job.radiance), _p(compressed), h, w,
+                                  float(job.config.saturation), _p(image), _p(job.work), s),
+               "gdp_tm_compose")
     nsel = ctypes.c_size_t()
     _lib.check(lib.gdp_percentile_work_bytes(ctypes.byref(nsel)), "gdp_percentile_work_bytes")
     sel = torch.empty((nsel.value + 7) // 8, dtype=torch.float64, device=dev)
@@ -125,7 +138,23 @@ def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "mgcg_
     _lib.check(lib.gdp_percentile(_p(image), image.numel(), 99.5, _p(peak), _p(sel), s),
                "gdp_percentile")
     _lib.check(lib.gdp_tm_normalise(_p(image), image.numel(), _p(peak), s), "gdp_tm_normalise")
-    return to_host_like(image, host), to_host_like(factor, host)
+    return to_host_like(image, job.host), to_host_like(job.factor, job.host)
+
+
+def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "mgcg_mixed",
+             tolerance: float = 1e-8, max_iterations: int = 2000, reports: list | None = None):
+    """Compress a radiance map to a displayable image (hdr.py:137-185).
+
+    Returns ``(image in [0, 1] of shape (h, w, 3), factor (h, w))``.  The Neumann solve
+    uses ``method`` ("mgcg" FP64 or "mgcg_mixed") to ``tolerance``; its SolverReport is
+    appended to ``reports`` when a list is given.
+    """
+    job = prepare_tone_map(radiance, config)
+    sol, reps = solve_planes(job.rhs, method=method, tolerance=tolerance,
+                             max_iterations=max_iterations, boundary="neumann")
+    if reports is not None:
+        reports.extend(reps)
+    return finish_tone_map(job, sol[0])
 
 
 def percentile(x, q: float) -> float:

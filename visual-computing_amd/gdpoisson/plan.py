@@ -48,6 +48,13 @@ def _device(device) -> torch.device:
     return torch.device(device)
 
 
+def plan_workspace_bytes(shapes, boundary: str = "dirichlet", precision: str = "fp64") -> int:
+    """Device workspace bytes of a plan with the default options of ``precision`` for
+    these shapes; takes no device memory."""
+    p = Plan(shapes, boundary, 1.0, PlanOptions(precision=precision), allocate=False)
+    return p.workspace_bytes
+
+
 class Plan:
     """A bound libgdpoisson plan for ``shapes`` (one (m, n) per problem)."""
 
@@ -59,6 +66,7 @@ class Plan:
         options: PlanOptions | None = None,
         device=None,
         tiles: dict | None = None,
+        allocate: bool = True,
     ):
         opts = options or PlanOptions()
         self.options = opts
@@ -114,6 +122,9 @@ class Plan:
         nbytes = ctypes.c_size_t()
         _lib.check(L.gdp_plan_workspace_bytes(self._plan, ctypes.byref(nbytes)))
         self.workspace_bytes = int(nbytes.value)
+        if not allocate:  # sizing only: no device memory, not usable for solves
+            self.workspace = None
+            return
         with torch.cuda.device(self.device):
             self.workspace = torch.empty(self.workspace_bytes + 256, dtype=torch.uint8,
                                          device=self.device)

@@ -28,7 +28,8 @@ from ._device import to_device, to_host_like
 from .operators import _planes, _stream, _unplanes, divergence, fold_boundary, gradient
 from .solvers import SolverReport, solve_planes
 
-__all__ = ["GuidanceField", "solve_dirichlet", "seamless_clone"]
+__all__ = ["GuidanceField", "solve_dirichlet", "seamless_clone", "prepare_clone", "finish_clone",
+           "CloneJob"]
 
 MODES = {"import": 0, "mixed": 1, "average": 2}
 
@@ -117,11 +118,22 @@ def solve_dirichlet(field: GuidanceField, boundary, method: str = "mgcg",
     return to_host_like(_unplanes(out, cl), host), reports
 
 
-def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
-                   domain: str = "rectangle", method: str = "mgcg", margin: int = 24,
-                   tolerance: float = 1e-8, max_iterations: int = 2000):
-    """Paste the masked region of ``source`` into ``target`` at ``offset`` without a seam
-    (poisson.py:261-348, rectangle route).  Returns ``(image, reports)``."""
+@dataclass
+class CloneJob:
+    """A prepared seamless_clone: the target planes, the window and its folded
+    right-hand side (planes x (hh - 2) x (ww - 2)); ``rhs`` is None for an empty mask."""
+
+    target: torch.Tensor
+    channels_last: bool
+    host: bool
+    window: tuple
+    rhs: torch.Tensor | None
+
+
+def prepare_clone(source, target, mask, offset=(0, 0), mode: str = "import",
+                  domain: str = "rectangle", margin: int = 24) -> CloneJob:
+    """Validation, window and fused right-hand side of seamless_clone (device work on
+    the current stream; one small host read of the mask's bounding box)."""
     if mode not in MODES:
         raise ValueError(f"unknown mode {mode!r}")
     if domain == "mask":
@@ -146,7 +158,7 @@ def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
     m = _mask_tensor(mask, (sh, sw), pt.device)
     nz = torch.nonzero(m)
     if nz.numel() == 0:
-        return to_host_like(_unplanes(pt.clone(), cl), host), []
+        return CloneJob(pt, cl, host, (0, h, 0, w), None)
     r0, c0 = (nz.min(dim=0).values + torch.tensor([row, col], device=nz.device)).tolist()
     r1, c1 = (nz.max(dim=0).values + torch.tensor([row, col], device=nz.device)).tolist()
     top, bottom = max(r0 - margin, 0), min(r1 + margin + 1, h)
@@ -159,7 +171,25 @@ def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
         ctypes.c_void_p(ps.data_ptr()), sh, sw, row, col, ctypes.c_void_p(m.data_ptr()),
         ctypes.c_void_p(pt.data_ptr()), h, w, c, top, left, hh, ww, MODES[mode],
         ctypes.c_void_p(rhs.data_ptr()), _stream()), "gdp_clone_rhs")
-    sol, reports = _solve_window(rhs, method, tolerance, max_iterations)
-    out = pt.clone()
-    out[:, top + 1:bottom - 1, left + 1:right - 1] = sol
-    return to_host_like(_unplanes(out, cl), host), reports
+    return CloneJob(pt, cl, host, (top, bottom, left, right), rhs)
+
+
+def finish_clone(job: CloneJob, solution: torch.Tensor | None):
+    """The composited image: the target with the window interior replaced."""
+    out = job.target.clone()
+    if solution is not None:
+        top, bottom, left, right = job.window
+        out[:, top + 1:bottom - 1, left + 1:right - 1] = solution
+    return to_host_like(_unplanes(out, job.channels_last), job.host)
+
+
+def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
+                   domain: str = "rectangle", method: str = "mgcg", margin: int = 24,
+                   tolerance: float = 1e-8, max_iterations: int = 2000):
+    """Paste the masked region of ``source`` into ``target`` at ``offset`` without a seam
+    (poisson.py:261-348, rectangle route).  Returns ``(image, reports)``."""
+    job = prepare_clone(source, target, mask, offset, mode, domain, margin)
+    if job.rhs is None:
+        return finish_clone(job, None), []
+    sol, reports = _solve_window(job.rhs, method, tolerance, max_iterations)
+    return finish_clone(job, sol), reports

@@ -1,0 +1,357 @@
+"""Serving loop for gradient-domain requests: continuous batching over HIP streams with
+admission by HBM budget.
+
+The reference runs one solve at a time on the caller's thread (solvers.py:171-221,
+called per channel by poisson.solve_dirichlet, poisson.py:141-143, and once by
+hdr.tone_map, hdr.py:175).  Here a request is a whole compositing job (seamless_clone,
+rectangle route) or tone-mapping job (tone_map), and a :class:`Server` replays a stream
+of timed arrivals:
+
+* requests wait in arrival order; each holds an estimate of the device bytes it needs
+  while admitted (inputs it copies, temporaries, outputs and its share of a plan's
+  workspace, see :func:`request_bytes`);
+* whenever one of ``slots`` workers is free, the oldest waiting request is admitted if
+  its bytes fit what is left of the budget, together with every later waiting request of
+  the same kind that still fits (up to ``max_batch`` problems);
+* a worker runs its batch on its own HIP stream: each request's front-end kernels, one
+  batched plan holding every channel of every request of the batch (one launch per
+  kernel for all of them), then each request's back-end kernels; the only host waits
+  are the worker's own (its stream and the plan's single end-of-solve read);
+* batches on different workers overlap on the device, and requests that arrive while
+  others run are admitted as soon as a worker and budget free up.
+
+Each problem of a batched plan is solved with the arithmetic it gets alone (default tile
+shapes, per-problem reductions in a fixed order, per-problem stopping), so a request's
+result does not depend on what it was batched or overlapped with: the same bits alone
+and in the mix.  Latency = completion - arrival, queueing included.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import threading
+import time
+from collections import deque
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
+from dataclasses import dataclass, field
+from typing import Callable, Sequence
+
+import numpy as np
+
+KINDS = ("composite", "tonemap")
+BOUNDARY_OF = {"composite": "dirichlet", "tonemap": "neumann"}
+
+
+@dataclass
+class Request:
+    """``kind`` "composite": ``args`` = source, target, mask, offset, mode (and optionally
+    margin); "tonemap": ``args`` = radiance (and optionally config).  Images are
+    channels-last.  ``arrival`` is in seconds from the start of the stream."""
+
+    kind: str
+    args: dict
+    arrival: float = 0.0
+    rid: int = -1
+
+
+@dataclass
+class Completion:
+    rid: int
+    kind: str
+    arrival: float
+    admitted: float
+    finished: float
+    batch: int
+    nbytes: int
+    pixels: int
+    output: object = None
+    reports: list = field(default_factory=list)
+    error: str | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def _shape(x) -> tuple:
+    return tuple(int(s) for s in (x.shape if hasattr(x, "shape") else x))
+
+
+def request_pixels(req: Request) -> int:
+    a = req.args
+    if "nbytes" in a and "pixels" in a:
+        return int(a["pixels"])
+    img = a["target"] if req.kind == "composite" else a["radiance"]
+    s = _shape(img)
+    return s[0] * s[1]
+
+
+_WS_CACHE: dict = {}
+
+
+def _plan_bytes_per_point(boundary: str, precision: str) -> float:
+    """Workspace bytes per unknown of a one-problem plan (measured on a 512 x 512 plan:
+    the workspace is the level pyramid plus Krylov vectors, linear in the points)."""
+    key = (boundary, precision)
+    if key not in _WS_CACHE:
+        from .plan import plan_workspace_bytes
+
+        b = plan_workspace_bytes([(512, 512)], boundary, precision)
+        _WS_CACHE[key] = b / (512 * 512)
+    return _WS_CACHE[key]
+
+
+def request_bytes(req: Request, precision: str = "mixed") -> int:
+    """Upper bound of the device bytes a request holds while admitted.
+
+    composite: the target's planes (a copy) and the output image, the source's planes and
+    mask, the right-hand side and solution of the window (bounded by the target) and the
+    plan workspace of its channels; tonemap: log luminance, pyramid (4/3 of a plane),
+    factor, right-hand side, solution, the output image and the plan workspace.  An
+    ``nbytes`` entry in ``args`` overrides the estimate.
+    """
+    a = req.args
+    if "nbytes" in a:
+        return int(a["nbytes"])
+    per = 1.1 * _plan_bytes_per_point(BOUNDARY_OF[req.kind], precision)
+    if req.kind == "composite":
+        h, w = _shape(a["target"])[:2]
+        c = _shape(a["target"])[2] if len(_shape(a["target"])) == 3 else 1
+        sh, sw = _shape(a["source"])[:2]
+        return int(8 * c * (2 * h * w + sh * sw) + sh * sw + 16 * c * h * w + per * c * h * w)
+    h, w = _shape(a["radiance"])[:2]
+    return int(8 * h * w * (3 + 1 + 4 / 3 + 1 + 1 + 1) + per * h * w + (1 << 20))
+
+
+class Server:
+    """Continuous-batching request server on one GPU (see the module docstring).
+
+    ``budget_bytes``: device bytes the admitted requests may hold together (default: half
+    of the free HBM).  ``slots``: workers, one HIP stream each.  ``max_batch``: problems
+    (channels) per batched plan.  ``method``: "mgcg_mixed" or "mgcg".  ``graphs``: capture
+    the plans' PCG iterations in hipGraphs (each batch's plan is new, so the capture is
+    paid per batch).  ``executor`` replaces the device pipeline (tests of the
+    scheduling): called as ``executor(batch)`` in a worker, it returns one
+    ``(output, reports)`` per request.
+    """
+
+    def __init__(self, budget_bytes: int | None = None, slots: int = 3, max_batch: int = 12,
+                 method: str = "mgcg_mixed", tolerance: float = 1e-8,
+                 max_iterations: int = 2000, graphs: bool = False, keep_outputs: bool = True,
+                 device=None, executor: Callable | None = None):
+        if method not in ("mgcg", "mgcg_mixed"):
+            raise ValueError("method must be 'mgcg' or 'mgcg_mixed'")
+        if slots < 1 or max_batch < 1:
+            raise ValueError("slots and max_batch must be positive")
+        self.slots, self.max_batch = int(slots), int(max_batch)
+        self.method, self.tolerance, self.max_iterations = method, float(tolerance), int(max_iterations)
+        self.precision = "mixed" if method == "mgcg_mixed" else "fp64"
+        self.graphs, self.keep_outputs = bool(graphs), bool(keep_outputs)
+        self.executor = executor
+        self.device = None
+        if executor is None:
+            import torch
+
+            self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                       else int(device))
+            if budget_bytes is None:
+                free, _ = torch.cuda.mem_get_info(self.device)
+                budget_bytes = free // 2
+        if budget_bytes is None:
+            raise ValueError("budget_bytes is required with a custom executor")
+        self.budget = int(budget_bytes)
+        self._local = threading.local()
+        self.peak_bytes = 0
+        self.batches: list[tuple[str, int, int]] = []  # (kind, requests, problems)
+
+    # ------------------------------------------------------------ scheduling
+    def _bytes(self, req: Request) -> int:
+        return request_bytes(req, self.precision)
+
+    def _problems(self, req: Request) -> int:
+        if req.kind == "tonemap":
+            return 1
+        t = req.args.get("target")
+        s = _shape(t) if t is not None else (1, 1, 3)
+        return s[2] if len(s) == 3 else 1
+
+    def _form_batch(self, waiting: list, free: int):
+        head = waiting[0]
+        need = self._bytes(head)
+        if need > free:
+            return [], 0
+        batch, total, probs = [head], need, self._problems(head)
+        for r in waiting[1:]:
+            if r.kind != head.kind:
+                continue
+            p, b = self._problems(r), self._bytes(r)
+            if probs + p > self.max_batch:
+                break
+            if total + b > free:
+                continue
+            batch.append(r)
+            total += b
+            probs += p
+        return batch, total
+
+    def run(self, requests: Sequence[Request], realtime: bool = True) -> list[Completion]:
+        """Serve ``requests`` (arrival times honoured when ``realtime``, else all arrive at
+        once) and return their completions in request order."""
+        for i, r in enumerate(requests):
+            if r.kind not in KINDS:
+                raise ValueError(f"unknown request kind {r.kind!r}")
+            if r.rid < 0:
+                r.rid = i
+        if self.executor is None:
+            import torch
+
+            torch.cuda.synchronize(self.device)  # inputs made on other streams are ready
+        pending = deque(sorted(requests, key=lambda r: (r.arrival if realtime else 0.0, r.rid)))
+        waiting: list[Request] = []
+        running: dict = {}
+        done: list[Completion] = []
+        in_use = 0
+        bid = 0
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0  # noqa: E731
+        with ThreadPoolExecutor(max_workers=self.slots, thread_name_prefix="gdp-slot") as pool:
+            while pending or waiting or running:
+                now = clock()
+                while pending and (not realtime or pending[0].arrival <= now):
+                    waiting.append(pending.popleft())
+                while waiting and self._bytes(waiting[0]) > self.budget:
+                    r = waiting.pop(0)
+                    done.append(Completion(r.rid, r.kind, r.arrival if realtime else 0.0, now, now,
+                                           -1, self._bytes(r), request_pixels(r),
+                                           error="needs more device memory than the budget"))
+                while waiting and len(running) < self.slots:
+                    batch, nbytes = self._form_batch(waiting, self.budget - in_use)
+                    if not batch:
+                        break
+                    for r in batch:
+                        waiting.remove(r)
+                    in_use += nbytes
+                    self.peak_bytes = max(self.peak_bytes, in_use)
+                    self.batches.append((batch[0].kind, len(batch),
+                                         sum(self._problems(r) for r in batch)))
+                    fut = pool.submit(self._execute, batch, bid, clock(), clock, realtime)
+                    running[fut] = nbytes
+                    bid += 1
+                timeout = None
+                if pending and realtime:
+                    timeout = max(0.0, pending[0].arrival - clock())
+                if running:
+                    finished, _ = wait(list(running), timeout=timeout, return_when=FIRST_COMPLETED)
+                    for f in finished:
+                        in_use -= running.pop(f)
+                        done.extend(f.result())
+                elif timeout:
+                    time.sleep(timeout)
+        return sorted(done, key=lambda c: c.rid)
+
+    # ------------------------------------------------------------ execution
+    def _execute(self, batch: list, bid: int, admitted: float, clock, realtime: bool):
+        if self.executor is not None:
+            results = self.executor(batch)
+        else:
+            results = self._device_batch(batch)
+        fin = clock()
+        out = []
+        for r, (output, reports, err) in zip(batch, results):
+            out.append(Completion(r.rid, r.kind, r.arrival if realtime else 0.0, admitted, fin,
+                                  bid, self._bytes(r), request_pixels(r),
+                                  output if self.keep_outputs else None, reports, err))
+        return out
+
+    def _stream(self):
+        import torch
+
+        s = getattr(self._local, "stream", None)
+        if s is None:
+            s = torch.cuda.Stream(self.device)  # default priority
+            self._local.stream = s
+        return s
+
+    def _device_batch(self, batch: list):
+        import torch
+
+        from .hdr import finish_tone_map, prepare_tone_map
+        from .plan import Plan
+        from .poisson import finish_clone, prepare_clone
+        from .solvers import SolverReport, _pcg_options
+
+        stream = self._stream()
+        results: list = [None] * len(batch)
+        with torch.cuda.device(self.device), torch.cuda.stream(stream):
+            jobs: list = [None] * len(batch)
+            for i, r in enumerate(batch):
+                try:
+                    a = r.args
+                    if r.kind == "composite":
+                        jobs[i] = prepare_clone(a["source"], a["target"], a["mask"],
+                                                a.get("offset", (0, 0)), a.get("mode", "import"),
+                                                margin=int(a.get("margin", 24)))
+                    else:
+                        jobs[i] = prepare_tone_map(a["radiance"], a.get("config"))
+                except (ValueError, NotImplementedError) as e:
+                    results[i] = (None, [], str(e))
+            planes, owner = [], []
+            for i, j in enumerate(jobs):
+                if j is not None and j.rhs is not None:
+                    for k in range(j.rhs.shape[0]):
+                        planes.append(j.rhs[k])
+                        owner.append(i)
+            sols: dict = {}
+            reps: dict = {}
+            if planes:
+                t_start = time.perf_counter()
+                opts = dataclasses.replace(
+                    _pcg_options(self.precision, self.tolerance, self.max_iterations,
+                                 BOUNDARY_OF[batch[0].kind]), use_graph=self.graphs)
+                plan = Plan([tuple(p.shape) for p in planes], BOUNDARY_OF[batch[0].kind], 1.0,
+                            opts, device=self.device)
+                us, rs = plan.mgcg(planes)
+                del plan
+                dt = time.perf_counter() - t_start
+                for i, u, rep in zip(owner, us, rs):
+                    sols.setdefault(i, []).append(u)
+                    reps.setdefault(i, []).append(SolverReport(
+                        method=self.method, iterations=int(rep.iterations),
+                        relative_residual=float(rep.relative_residual), seconds=dt,
+                        converged=bool(rep.converged)))
+            for i, (r, j) in enumerate(zip(batch, jobs)):
+                if j is None:
+                    continue
+                if r.kind == "composite":
+                    sol = torch.stack(sols[i]) if i in sols else None
+                    results[i] = (finish_clone(j, sol), reps.get(i, []), None)
+                else:
+                    img, _ = finish_tone_map(j, sols[i][0])
+                    results[i] = (img, reps.get(i, []), None)
+            stream.synchronize()
+        return results
+
+
+def latency_stats(completions: Sequence[Completion]) -> dict:
+    """p50 / p99 / max latency (ms) of the served requests, throughput and sizes."""
+    ok = [c for c in completions if c.error is None]
+    if not ok:
+        return {"served": 0, "rejected": len(completions)}
+    lat = np.array([c.latency for c in ok]) * 1e3
+    span = max(c.finished for c in ok) - min(c.arrival for c in ok)
+    return {
+        "served": len(ok),
+        "rejected": len(completions) - len(ok),
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "max_ms": float(lat.max()),
+        "mean_ms": float(lat.mean()),
+        "requests_per_s": len(ok) / span if span > 0 else float("nan"),
+        "mpix_per_s": sum(c.pixels for c in ok) / 1e6 / span if span > 0 else float("nan"),
+    }
+
+
+def poisson_arrivals(count: int, qps: float, seed: int) -> np.ndarray:
+    """Arrival times (s) of a Poisson process of rate ``qps``."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / qps, count))

@@ -218,6 +218,61 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
     return out
 
 
+def _short(name: str) -> str:
+    import re
+
+    return re.sub(r"\(.*", "", name).replace("void ", "").replace("gdp::", "")
+
+
+def trace_summary(kt_dir: str) -> str:
+    """Markdown of a rocprofv3 --kernel-trace --stats run of --profile-run: the per-kernel
+    statistics and the timeline of one solve (the sixth of seven)."""
+    import csv
+
+    d = Path(kt_dir)
+    stats = list(csv.DictReader(open(next(d.glob("*kernel_stats.csv")))))
+    trace = sorted(csv.DictReader(open(next(d.glob("*kernel_trace.csv")))),
+                   key=lambda r: int(r["Start_Timestamp"]))
+    out = ["## Kernel statistics (rocprofv3 --stats)", "",
+           "| kernel | calls | total us | avg us | min us | max us | % |",
+           "|---|---|---|---|---|---|---|"]
+    for r in stats:
+        out.append(f"| `{_short(r['Name'])[:72]}` | {r['Calls']} | "
+                   f"{float(r['TotalDurationNs']) / 1e3:.1f} | {float(r['AverageNs']) / 1e3:.2f} | "
+                   f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                   f"{float(r['Percentage']):.2f} |")
+    starts = [i for i, r in enumerate(trace) if "k_init<" in r["Kernel_Name"]]
+    if len(starts) >= 2:
+        a, b = starts[-2], starts[-1]
+        t0 = int(trace[a]["Start_Timestamp"])
+        out += ["", "## One solve from the kernel trace: start offset, duration, grid", "",
+                "| start us | dur us | grid | kernel |", "|---|---|---|---|"]
+        for r in trace[a:b]:
+            st = int(r["Start_Timestamp"])
+            out.append(f"| {(st - t0) / 1e3:.1f} | {(int(r['End_Timestamp']) - st) / 1e3:.2f} | "
+                       f"{r['Grid_Size_X']} | `{_short(r['Kernel_Name'])[:64]}` |")
+    return "\n".join(out) + "\n"
+
+
+def traffic_md(fetch_csv: str, write_csv: str, bench_json: str) -> str:
+    """Markdown table: algorithmic bytes per launch (the bench line's kernel table) against
+    the HBM bytes the two counter passes measured."""
+    t = reduce_traffic(fetch_csv, write_csv)
+    k = json.loads(Path(bench_json).read_text())["kernels"]
+    cal = t["calibration"]
+    out = [f"Calibration (the 512 MiB device copy at the end of `--profile-run`): FETCH_SIZE "
+           f"scale {cal['FETCH_SIZE']['scale']}, WRITE_SIZE scale {cal['WRITE_SIZE']['scale']}.",
+           "", "| kernel | algorithmic B/launch | measured B/launch (fetch + write) | "
+           "measured / algorithmic | avg us (events) |", "|---|---|---|---|---|"]
+    for name, v in t["kernels"].items():
+        if name not in k or not k[name]["GBps"]:
+            continue
+        alg = k[name]["GBps"] * 1e9 * k[name]["avg_us"] * 1e-6
+        out.append(f"| {name} | {alg / 1e6:.2f} MB | {v['hbm_bytes_per_launch'] / 1e6:.2f} MB | "
+                   f"{v['hbm_bytes_per_launch'] / alg:.2f} | {k[name]['avg_us']} |")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +283,10 @@ def main():
     ap.add_argument("--tiles", metavar="BENCH_JSON",
                     help="reuse the tile shapes recorded in a bench line (config.tiles) instead "
                     "of tuning, so a profiled run has exactly the benched configuration")
+    ap.add_argument("--trace-summary", metavar="KT_DIR",
+                    help="markdown summary of a rocprofv3 --kernel-trace --stats output folder")
+    ap.add_argument("--traffic-md", nargs=3, metavar=("FETCH_CSV", "WRITE_CSV", "BENCH_JSON"),
+                    help="markdown table of measured vs algorithmic bytes per kernel")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the mixed serving stream (0: skip it)")
     ap.add_argument("--serve-qps", type=float, default=200.0, help="arrival rate of the stream")
@@ -241,6 +300,12 @@ def main():
                     help="only run a few mixed solves and a plain device copy of 512 MiB "
                     "(the FETCH_SIZE / WRITE_SIZE calibration) for rocprofv3; no JSON line")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary), end="")
+        return
+    if args.traffic_md:
+        print(traffic_md(*args.traffic_md), end="")
+        return
     if args.traffic:
         print(json.dumps(reduce_traffic(*args.traffic), indent=1))
         return

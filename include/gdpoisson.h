@@ -37,6 +37,7 @@
  *   gdp_tm_compose       hdr.py:180-182        exp(c - max c) * ratios ** saturation
  *   gdp_percentile       hdr.py:184            numpy.percentile(image, q), linear method
  *   gdp_tm_normalise     hdr.py:185            clip(image / max(peak, 1e-12), 0, 1)
+ *   gdp_neumann_dct_solve solvers.py:224-262   solve_neumann(b): idctn(dctn(b)/lambda), exact
  */
 #ifndef GDPOISSON_H
 #define GDPOISSON_H
@@ -153,6 +154,19 @@ int gdp_percentile_work_bytes(size_t* bytes);
 int gdp_percentile(const double* x, size_t n, double q, double* out, void* work, void* stream);
 /* image[0 .. n) = clip(image / max(*peak, 1e-12), 0, 1), peak a device scalar. */
 int gdp_tm_normalise(double* image, size_t n, const double* peak, void* stream);
+
+/* ---------------------------------------------------------------- exact Neumann solve
+ * u (m x n) = idctn(dctn(b, II, ortho) / lambda, II, ortho) with coefficient (0, 0) = 0:
+ * the zero-mean solution of div(grad u) = b (solve_neumann, solvers.py:224-262).  Sides
+ * of at most 4096 points with prime factors 2, 3, 5 only (GDP_ENOTSUP otherwise).
+ * `tables` (gdp_dct_tables_bytes, filled once per shape by gdp_dct_tables, which
+ * synchronises) and `work` (gdp_dct_work_bytes) are device buffers; u may not alias b. */
+int gdp_dct_supported(int m, int n);
+int gdp_dct_tables_bytes(int m, int n, size_t* bytes);
+int gdp_dct_tables(int m, int n, double* tables, void* stream);
+int gdp_dct_work_bytes(int m, int n, size_t* bytes);
+int gdp_neumann_dct_solve(const double* b, double* u, int m, int n, const double* tables,
+                          double* work, void* stream);
 
 /* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
    (order = 0) or colour 1 then 0 (order = 1). */

@@ -126,8 +126,8 @@ def test_python_errors_map_status_codes(L):
 
 def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
-    for kind in range(3):
-        for idx in range(4):
+    for kind, size in ((0, 4), (1, 4), (2, 6)):
+        for idx in range(size):
             for nu in range(4):
                 assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
                                              ctypes.byref(tw)) == 0
@@ -137,5 +137,5 @@ def test_tile_menu(L):
     assert tw.value == 58
     L.lib().gdp_tile_menu(1, 0, 2, ctypes.byref(th), ctypes.byref(tw))
     assert tw.value == 116
-    for bad in ((3, 0, 1), (0, 4, 1), (0, 0, 4)):
+    for bad in ((3, 0, 1), (0, 4, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL

@@ -115,17 +115,18 @@ def test_percentile_of_positive_wide_range(gd):
     assert gd.hdr.percentile(x, 99.5) == np.percentile(x, 99.5)
 
 
-@pytest.mark.parametrize("method", ["mgcg", "mgcg_mixed"])
+@pytest.mark.parametrize("method", ["dct", "mgcg", "mgcg_mixed"])
 def test_tone_map_matches_oracle(gd, method):
     rad = G.radiance_map((96, 128), seed=2)
     reports = []
     img, f = gd.tone_map(rad, method=method, reports=reports)
     want, wf = H.tone_map(rad)
     assert len(reports) == 1 and reports[0].converged
-    assert reports[0].relative_residual <= 1e-8
+    assert reports[0].relative_residual <= (1e-12 if method == "dct" else 1e-8)
     assert img.shape == rad.shape and img.min() >= 0.0 and img.max() <= 1.0
     assert rel(f, wf) < 1e-12
-    assert np.abs(img - want).max() < 1e-6
+    # the exact DCT integration agrees with the oracle's to round-off
+    assert np.abs(img - want).max() < (1e-10 if method == "dct" else 1e-6)
 
 
 def test_tone_map_rejects_a_grayscale_input(gd):
@@ -140,7 +141,7 @@ def test_tone_map_of_a_serving_request(gd):
 
     rad = G.lognormal_radiance((2160, 3840), seed=5003)
     reports = []
-    img, f = gd.tone_map(torch.from_numpy(rad).cuda(), reports=reports)
+    img, f = gd.tone_map(torch.from_numpy(rad).cuda(), method="mgcg_mixed", reports=reports)
     assert reports[0].converged and reports[0].relative_residual <= 1e-8
     want, _ = H.tone_map(rad)
     assert torch.abs(img.cpu() - torch.from_numpy(want)).max().item() < 1e-6

@@ -261,6 +261,21 @@ def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):
     assert rel(got.cpu().numpy(), want) < 1e-11
 
 
+@pytest.mark.parametrize("kry", range(6))
+def test_every_krylov_tile_solves_like_the_oracle(gd, kry):
+    """Each compiled Krylov tile shape (apply, update, set-up and check kernels) gives the
+    oracle's FP64 MG-PCG: same iteration count, same solution to round-off."""
+    import torch
+
+    b, _ = G.manufactured_problem((300, 260), seed=70 + kry)
+    plan = gd.Plan([b.shape], boundary="dirichlet", options=gd.PlanOptions(precision="fp64"),
+                   tiles={(2, 0): kry})
+    (u,), (rep,) = plan.mgcg([torch.from_numpy(b).cuda()])
+    want, wrep = OS.solve_system(b, "mgcg")
+    assert rep.iterations == wrep.iterations
+    assert rel(u.cpu().numpy(), want) < 1e-10
+
+
 def test_autotuned_plan_solves_the_same_problem(gd):
     import torch
 

@@ -71,8 +71,10 @@ struct Shape2 {
 // strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per block.
 const int kDownRows[4] = {1, 2, 4, 8};
 const int kUpRows[4] = {2, 4, 8, 16};
-const Shape2 kKryMenu[4] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}};
-constexpr int kMenu = 4;
+const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
+constexpr int kMenu = 4;     // level-kernel menus (rows per strip)
+constexpr int kKryMenuN = 6;  // Krylov tile menu
+int menu_size(int kind) { return kind == 2 ? kKryMenuN : kMenu; }
 constexpr int kStrips = kThreads / 64;  // strips per block
 int down_cols(int nu) { return (125 - 4 * nu) / 2; }
 int up_cols(int nu) { return 124 - 4 * nu; }
@@ -88,7 +90,9 @@ int cdiv(int a, int b) { return (a + b - 1) / b; }
     case 0: { constexpr int TH = 8, TW = 64; CALL; } break;   \
     case 1: { constexpr int TH = 16, TW = 64; CALL; } break;  \
     case 2: { constexpr int TH = 32, TW = 64; CALL; } break;  \
-    default: { constexpr int TH = 8, TW = 128; CALL; } break; \
+    case 3: { constexpr int TH = 8, TW = 128; CALL; } break;  \
+    case 4: { constexpr int TH = 16, TW = 128; CALL; } break; \
+    default: { constexpr int TH = 32, TW = 128; CALL; } break; \
   }
 
 // ----------------------------------------------------------------- plan
@@ -150,8 +154,10 @@ struct gdp_plan {
   size_t h_block_bytes = 0;
   hipStream_t cap = nullptr;
   hipEvent_t io_ev = nullptr;  // the last upload of h_io / h_init has been consumed
+  bool io_valid = false;       // the device copy holds the pointers now in h_io
   hipGraphExec_t pair_exec[2] = {nullptr, nullptr};   // [0] mgcg, [1] cg
-  hipGraphExec_t start_exec[2] = {nullptr, nullptr};  // set-up + first preconditioning
+  // a whole first batch (set-up, first preconditioning, N pairs, device-side check) per N
+  std::vector<std::pair<int, hipGraphExec_t>> head_exec[2];
   hipStream_t graph_stream = nullptr;  // stream of the last graph launch
   int last_iters = 0;
   bool bound = false;
@@ -283,16 +289,20 @@ hipError_t raise_smem(int family, int smem) {
 // destroy, once the stream they last ran on has finished them (other streams, and so
 // other plans' work, are not waited for).
 void destroy_graphs(gdp_plan* P) {
-  bool any = false;
-  for (hipGraphExec_t* ex : {P->pair_exec, P->start_exec}) any = any || ex[0] || ex[1];
+  bool any = !P->head_exec[0].empty() || !P->head_exec[1].empty();
+  any = any || P->pair_exec[0] || P->pair_exec[1];
   if (any && P->graph_stream) (void)hipStreamSynchronize(P->graph_stream);
   P->graph_stream = nullptr;
-  for (hipGraphExec_t* ex : {P->pair_exec, P->start_exec})
-    for (int g = 0; g < 2; ++g)
-      if (ex[g]) {
-        (void)hipGraphExecDestroy(ex[g]);
-        ex[g] = nullptr;
-      }
+  for (auto& v : P->head_exec) {
+    for (auto& e : v)
+      if (e.second) (void)hipGraphExecDestroy(e.second);
+    v.clear();
+  }
+  for (int g = 0; g < 2; ++g)
+    if (P->pair_exec[g]) {
+      (void)hipGraphExecDestroy(P->pair_exec[g]);
+      P->pair_exec[g] = nullptr;
+    }
 }
 
 // LDS bytes of a tail holding levels [first, L) (u and b per level, one residual
@@ -331,7 +341,7 @@ Shape2 menu_shape(int kind, int idx, int nu) {
 // Worst case tile count of a grid over every entry of a menu.
 int max_tiles(int kind, int m, int n, int nu) {
   int best = 0;
-  for (int i = 0; i < kMenu; ++i) {
+  for (int i = 0; i < menu_size(kind); ++i) {
     const Shape2 s = menu_shape(kind, i, nu);
     best = std::max(best, cdiv(m, s.h) * cdiv(n, s.w));
   }
@@ -362,7 +372,8 @@ const char* gdp_last_error(void) { return g_err.c_str(); }
 int gdp_version(void) { return 10000; }
 
 int gdp_tile_menu(int kind, int index, int nu, int* th, int* tw) {
-  if (kind < 0 || kind > 2 || index < 0 || index >= kMenu || nu < 0 || nu > 3 || !th || !tw) {
+  if (kind < 0 || kind > 2 || index < 0 || index >= menu_size(kind) || nu < 0 || nu > 3 ||
+      !th || !tw) {
     set_error("gdp_tile_menu: bad kind/index/nu");
     return GDP_EINVAL;
   }
@@ -565,7 +576,7 @@ int gdp_plan_level_shape(const gdp_plan* P, int problem, int level, int* m, int*
 }
 
 int gdp_plan_set_tile(gdp_plan* P, int kind, int level, int menu_index) {
-  if (!P || kind < 0 || kind > 2 || menu_index < 0 || menu_index >= kMenu ||
+  if (!P || kind < 0 || kind > 2 || menu_index < 0 || menu_index >= menu_size(kind) ||
       (kind < 2 && (level < 0 || level >= kMaxDepth))) {
     set_error("gdp_plan_set_tile: bad argument");
     return GDP_EINVAL;
@@ -607,6 +618,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     return GDP_EINVAL;
   }
   P->ws = static_cast<char*>(workspace);
+  P->io_valid = false;  // the pointer tables live in the workspace
   const int NP = P->count();
   if (!P->h_block) {
     auto a64 = [](size_t x) { return (x + 63) & ~size_t(63); };
@@ -1156,13 +1168,22 @@ int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix, int only_checked 
 int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double* const* init,
              hipStream_t s) {
   const int NP = P->count();
-  // the pinned staging buffers are reused: wait until the previous upload has run
-  GDP_HIP(hipEventSynchronize(P->io_ev));
-  for (int p = 0; p < NP; ++p) {
+  for (int p = 0; p < NP; ++p)
     if (!b[p] || !u[p]) {
       set_error("problem %d: null b or u pointer", p);
       return GDP_EINVAL;
     }
+  // The same buffers as last time (a solve loop reusing its arrays): the device copy is
+  // current, so only order this stream after the upload that wrote it.
+  bool same = P->io_valid && !init;
+  for (int p = 0; same && p < NP; ++p) same = P->h_io[p].b == b[p] && P->h_io[p].u == u[p];
+  if (same) {
+    GDP_HIP(hipStreamWaitEvent(s, P->io_ev, 0));
+    return GDP_OK;
+  }
+  // the pinned staging buffers are reused: wait until the previous upload has run
+  GDP_HIP(hipEventSynchronize(P->io_ev));
+  for (int p = 0; p < NP; ++p) {
     P->h_io[p].b = b[p];
     P->h_io[p].u = u[p];
     if (init) P->h_init[p] = init[p];
@@ -1172,6 +1193,7 @@ int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double
     GDP_HIP(hipMemcpyAsync(P->d_init(), P->h_init, sizeof(double*) * NP, hipMemcpyHostToDevice,
                            s));
   GDP_HIP(hipEventRecord(P->io_ev, s));
+  P->io_valid = true;
   return GDP_OK;
 }
 
@@ -1238,12 +1260,47 @@ int launch_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
 }
 
 // PCG set-up (b statistics, x = 0, r = -b, norms) and the first preconditioned residual.
-int launch_start(gdp_plan* P, hipStream_t s, bool jacobi) {
-  return launch_captured(P, s, P->start_exec[jacobi ? 1 : 0], [&](hipStream_t cs) {
-    const gdp_options& o = P->opt;
-    int rc = GDP_T(enqueue_init_t)(P, cs, 0, 1, o.tolerance, o.max_iterations, false, 1);
-    if (rc) return rc;
-    return jacobi ? GDP_T(enqueue_jacobi_t)(P, cs, false) : enqueue_vcycle(P, cs, true, false);
+int enqueue_start(gdp_plan* P, hipStream_t s, bool jacobi) {
+  const gdp_options& o = P->opt;
+  int rc = GDP_T(enqueue_init_t)(P, s, 0, 1, o.tolerance, o.max_iterations, false, 1);
+  if (rc) return rc;
+  return jacobi ? GDP_T(enqueue_jacobi_t)(P, s, false) : enqueue_vcycle(P, s, true, false);
+}
+
+// The device-side end of a batch of pairs: FP64 true residual of the problems that
+// stopped, reopen (residual replacement, if allowed) or accept, and the output of the
+// accepted ones.
+int enqueue_check(gdp_plan* P, hipStream_t s, bool allow_reopen) {
+  int rc;
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false))) return rc;
+  k_reopen<<<1, kThreads, 0, s>>>(P->d_state(), P->count(), allow_reopen ? 1 : 0, P->d_flag());
+  GDP_CHECK_LAUNCH();
+  return enqueue_finalize(P, s, 1, 1);
+}
+
+// The first batch of a solve as one graph: set-up, first preconditioning, `pairs`
+// iteration pairs and the check (one graph launch instead of pairs + 1, each of which
+// leaves the device idle for several microseconds).  Kept per pair count (at most 4).
+int launch_head(gdp_plan* P, hipStream_t s, bool jacobi, int pairs) {
+  auto& v = P->head_exec[jacobi ? 1 : 0];
+  size_t at = v.size();
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i].first == pairs) at = i;
+  if (at == v.size()) {
+    if (v.size() >= 4) {
+      if (P->graph_stream) (void)hipStreamSynchronize(P->graph_stream);
+      if (v.front().second) (void)hipGraphExecDestroy(v.front().second);
+      v.erase(v.begin());
+    }
+    v.emplace_back(pairs, nullptr);
+    at = v.size() - 1;
+  }
+  return launch_captured(P, s, v[at].second, [&](hipStream_t cs) {
+    int rc;
+    if ((rc = enqueue_start(P, cs, jacobi))) return rc;
+    for (int k = 0; k < pairs; ++k)
+      if ((rc = enqueue_pcg_pair(P, cs, jacobi))) return rc;
+    return enqueue_check(P, cs, true);
   });
 }
 
@@ -1255,26 +1312,16 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   }
   int rc;
   if ((rc = stage_io(P, b, u, nullptr, s))) return rc;
-  // set-up and the first preconditioned residual z0 = M r0 with its <r0, z0>
-  if ((rc = launch_start(P, s, jacobi))) return rc;
-  // Iterate in batches of graph-captured iteration pairs.  After each batch the problems
-  // that have finished are checked on the device without a host round trip: FP64 true
-  // residual, reopen (residual replacement) or accept, and the accepted ones' output is
-  // written; the host then reads the states once.
-  int launched = 0, restarts = 0;
+  // First batch, one graph: set-up, z0 = M r0 with its <r0, z0>, as many iteration pairs
+  // as the last solve of this plan needed, and the device-side check of the problems that
+  // finished (FP64 true residual, reopen for residual replacement or accept, output of
+  // the accepted ones).  The host then reads the states once; usually that ends the
+  // solve.  Otherwise it continues one pair at a time with the same check.
   const int first_pairs = std::max(1, (P->last_iters + 1) / 2);
-  for (;;) {
-    const int pairs = launched == 0 ? first_pairs : 1;
-    for (int k = 0; k < pairs; ++k)
-      if ((rc = launch_pair(P, s, jacobi))) return rc;
-    launched += 2 * pairs;
-    if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false))) return rc;
-    k_reopen<<<1, kThreads, 0, s>>>(P->d_state(), P->count(), restarts < 4 ? 1 : 0,
-                                    P->d_flag());
-    GDP_CHECK_LAUNCH();
-    if ((rc = enqueue_finalize(P, s, 1, 1))) return rc;
-    if ((rc = read_state(P, s, true))) return rc;
-    if (all_checked(P)) break;
+  if ((rc = launch_head(P, s, jacobi, first_pairs))) return rc;
+  if ((rc = read_state(P, s, true))) return rc;
+  int restarts = 0;
+  while (!all_checked(P)) {
     if (P->h_flag[0] > 0) {
       // reopened problems restart from their true residual
       ++restarts;
@@ -1282,6 +1329,9 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
       else rc = enqueue_vcycle(P, s, true, false);
       if (rc) return rc;
     }
+    if ((rc = launch_pair(P, s, jacobi))) return rc;
+    if ((rc = enqueue_check(P, s, restarts < 4))) return rc;
+    if ((rc = read_state(P, s, true))) return rc;
   }
   int mx = 0;
   for (int p = 0; p < P->count(); ++p) mx = std::max(mx, P->h_state[p].iter);

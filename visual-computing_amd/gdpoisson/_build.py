@@ -25,6 +25,7 @@ SOURCES = [
     "gdp_plan.hip",
     "gdp_ops.hip",
     "gdp_hdr.hip",
+    "gdp_fft.hip",
     "gdp_sdown_f32.hip",
     "gdp_sdown_f64.hip",
     "gdp_sup_f32.hip",

@@ -36,6 +36,11 @@ EXPORTS = (
     "gdp_percentile_work_bytes",
     "gdp_percentile",
     "gdp_tm_normalise",
+    "gdp_dct_supported",
+    "gdp_dct_tables_bytes",
+    "gdp_dct_tables",
+    "gdp_dct_work_bytes",
+    "gdp_neumann_dct_solve",
     "gdp_smooth",
     "gdp_restrict",
     "gdp_prolong",
@@ -123,6 +128,11 @@ def _declare(lib):
         "gdp_percentile_work_bytes": (c_int, [POINTER(c_size_t)]),
         "gdp_percentile": (c_int, [P, c_size_t, c_double, P, P, P]),
         "gdp_tm_normalise": (c_int, [P, c_size_t, P, P]),
+        "gdp_dct_supported": (c_int, [c_int, c_int]),
+        "gdp_dct_tables_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
+        "gdp_dct_tables": (c_int, [c_int, c_int, P, P]),
+        "gdp_dct_work_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
+        "gdp_neumann_dct_solve": (c_int, [P, P, c_int, c_int, P, P, P]),
         "gdp_smooth": (c_int, [P, P, c_int, c_int, c_double, c_int, c_int, c_int, P]),
         "gdp_restrict": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),
         "gdp_prolong": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),

@@ -19,8 +19,20 @@ import torch
 
 from .plan import Plan
 
-MENU = 4
 _CACHE: dict = {}
+
+
+def menu_size(kind: int) -> int:
+    """Entries of a compiled tile menu (gdp_tile_menu answers EINVAL past the last)."""
+    import ctypes
+
+    from . import _lib
+
+    th, tw = ctypes.c_int(), ctypes.c_int()
+    n = 0
+    while _lib.lib().gdp_tile_menu(kind, n, 1, ctypes.byref(th), ctypes.byref(tw)) == 0:
+        n += 1
+    return n
 
 
 def _kernel_times(plan: Plan, b: Sequence[torch.Tensor], solves: int) -> dict[str, float]:
@@ -74,7 +86,7 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
 
     for kind, level in [(k, l) for l in levels for k in (0, 1)] + [(2, 0)]:
         best_idx, best_t = None, float("inf")
-        for idx in range(MENU):
+        for idx in range(menu_size(kind)):
             plan.set_tiles({(kind, level): idx})
             t = cost(_kernel_times(plan, b, solves), kind, level)
             if t < best_t:

@@ -4,9 +4,10 @@
 ``attenuation_field`` (hdr.py:107-134) builds the Gaussian pyramid of the log luminance
 (gdp_tm_downsample) and the per-level factor from coarse to fine (gdp_tm_level_factor);
 ``tone_map`` (hdr.py:137-185) integrates the attenuated forward-difference field with a
-Neumann solve on the device (multigrid-preconditioned CG, FP64 or mixed precision, to the
-stated relative residual; the reference uses its exact DCT solver), then composes the
-display image and normalises it by its 99.5th percentile, all without leaving the GPU.
+Neumann solve on the device -- by default the reference's exact DCT diagonalisation
+(``solve_neumann``), or multigrid-preconditioned CG (``method="mgcg"`` / ``"mgcg_mixed"``,
+to the stated relative residual) -- then composes the display image and normalises it by
+its 99.5th percentile, all without leaving the GPU.
 numpy in gives numpy out; device tensors in give device tensors out.
 """
 
@@ -20,7 +21,8 @@ import torch
 from . import _lib
 from ._device import to_device, to_host_like
 from .operators import _stream
-from .solvers import solve_planes
+from .solvers import (SolverReport, dct_supported, relative_residual, solve_neumann,
+                      solve_planes)
 
 __all__ = ["ToneMapConfig", "attenuation_field", "tone_map", "prepare_tone_map",
            "finish_tone_map", "ToneJob", "percentile"]
@@ -141,15 +143,29 @@ def finish_tone_map(job: ToneJob, compressed: torch.Tensor):
     return to_host_like(image, job.host), to_host_like(job.factor, job.host)
 
 
-def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "mgcg_mixed",
+def tone_map(radiance, config: ToneMapConfig | None = None, method: str = "dct",
              tolerance: float = 1e-8, max_iterations: int = 2000, reports: list | None = None):
     """Compress a radiance map to a displayable image (hdr.py:137-185).
 
     Returns ``(image in [0, 1] of shape (h, w, 3), factor (h, w))``.  The Neumann solve
-    uses ``method`` ("mgcg" FP64 or "mgcg_mixed") to ``tolerance``; its SolverReport is
-    appended to ``reports`` when a list is given.
+    is the exact DCT (``method="dct"``, as the reference's ``solve_neumann``; sides whose
+    prime factors are not all 2, 3, 5 use FP64 MG-PCG) or MG-PCG (``"mgcg"`` FP64,
+    ``"mgcg_mixed"``) to ``tolerance``; its SolverReport (with the true relative residual)
+    is appended to ``reports`` when a list is given.
     """
+    import time
+
     job = prepare_tone_map(radiance, config)
+    if method == "dct":
+        t0 = time.perf_counter()
+        exact = dct_supported(*job.rhs.shape[1:])
+        u = solve_neumann(job.rhs[0], method="dct" if exact else "mgcg", tolerance=1e-12)
+        if reports is not None:
+            rr = relative_residual(u, job.rhs[0], 1.0, "neumann")
+            reports.append(SolverReport(method="dct" if exact else "mgcg", iterations=1,
+                                        relative_residual=rr,
+                                        seconds=time.perf_counter() - t0, converged=True))
+        return finish_tone_map(job, u)
     sol, reps = solve_planes(job.rhs, method=method, tolerance=tolerance,
                              max_iterations=max_iterations, boundary="neumann")
     if reports is not None:

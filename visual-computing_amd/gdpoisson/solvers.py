@@ -22,11 +22,14 @@ convention of ``solve_neumann`` (solvers.py:224-262).
 
 from __future__ import annotations
 
+import ctypes
+
 import time
 from dataclasses import dataclass
 
 import torch
 
+from . import _lib
 from ._device import cached_plan, to_device, to_host_like
 from .multigrid import MultigridOptions
 from .plan import PlanOptions
@@ -157,16 +160,64 @@ def solve_system(
     )
 
 
-def solve_neumann(b, tolerance: float = 1e-12, precision: str = "fp64"):
-    """Zero-mean solution of ``div(grad u) = b`` (solvers.py:224-262 contract).
+_DCT_TABLES: dict = {}
 
-    The reference diagonalises the operator with a DCT; here the same system is
-    solved by the Neumann MG-PCG to ``tolerance`` (default 1e-12 relative residual).
-    Any constant component of ``b`` is discarded, as in the reference.
+
+def relative_residual(u: torch.Tensor, b: torch.Tensor, spacing: float = 1.0,
+                      boundary: str = "dirichlet") -> float:
+    """True ``||b - A u|| / ||b||`` of one device grid (solvers.py:78-82), 0 when b = 0."""
+    out = ctypes.c_double()
+    _lib.check(_lib.lib().gdp_relative_residual(
+        ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(b.data_ptr()), int(u.shape[0]),
+        int(u.shape[1]), 1, float(spacing), _lib.NEUMANN if boundary == "neumann" else
+        _lib.DIRICHLET, ctypes.byref(out), _lib.current_stream()), "gdp_relative_residual")
+    return float(out.value)
+
+
+def dct_supported(m: int, n: int) -> bool:
+    """Whether the exact device DCT handles an m x n grid (sides <= 4096 whose prime
+    factors are 2, 3 and 5)."""
+    return bool(_lib.lib().gdp_dct_supported(int(m), int(n)))
+
+
+def _dct_solve(t: torch.Tensor) -> torch.Tensor:
+    m, n = int(t.shape[0]), int(t.shape[1])
+    key = (m, n, t.device)
+    tab = _DCT_TABLES.get(key)
+    if tab is None:
+        nbytes = ctypes.c_size_t()
+        _lib.check(_lib.lib().gdp_dct_tables_bytes(m, n, ctypes.byref(nbytes)), "gdp_dct")
+        tab = torch.empty(nbytes.value // 8, dtype=torch.float64, device=t.device)
+        _lib.check(_lib.lib().gdp_dct_tables(m, n, ctypes.c_void_p(tab.data_ptr()),
+                                             _lib.current_stream()), "gdp_dct_tables")
+        if len(_DCT_TABLES) > 32:
+            _DCT_TABLES.clear()
+        _DCT_TABLES[key] = tab
+    work = torch.empty(2 * m * n, dtype=torch.float64, device=t.device)
+    u = torch.empty_like(t)
+    _lib.check(_lib.lib().gdp_neumann_dct_solve(
+        ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(u.data_ptr()), m, n,
+        ctypes.c_void_p(tab.data_ptr()), ctypes.c_void_p(work.data_ptr()),
+        _lib.current_stream()), "gdp_neumann_dct_solve")
+    return u
+
+
+def solve_neumann(b, method: str = "dct", tolerance: float = 1e-12):
+    """Zero-mean solution of ``div(grad u) = b`` (solvers.py:224-262).
+
+    ``method="dct"`` (default) is the reference's exact diagonalisation on the device:
+    ``idctn(dctn(b, II, ortho) / lambda)`` with the constant mode dropped, by FFTs, for
+    sides of at most 4096 points whose prime factors are 2, 3 and 5; other sides, and
+    ``method="mgcg"`` / ``"mgcg_mixed"``, use the Neumann MG-PCG to ``tolerance``.  Any
+    constant component of ``b`` is discarded, as in the reference.
     """
     t, host = to_device(b)
     if t.ndim != 2:
         raise ValueError(f"b must be 2-D, got shape {tuple(t.shape)}")
-    u, _, _, _ = _run("mgcg_mixed" if precision == "mixed" else "mgcg", t, 1.0, tolerance,
+    if method not in ("dct", "mgcg", "mgcg_mixed"):
+        raise KeyError(f"unknown method {method!r}")
+    if method == "dct" and dct_supported(*t.shape):
+        return to_host_like(_dct_solve(t), host)
+    u, _, _, _ = _run("mgcg_mixed" if method == "mgcg_mixed" else "mgcg", t, 1.0, tolerance,
                       2000, boundary="neumann")
     return to_host_like(u, host)

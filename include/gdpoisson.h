@@ -28,6 +28,10 @@
  *                        GuidanceField.of / select / mix_by_magnitude + divergence +
  *                        fold_boundary of the window, fused
  *   gdp_guidance         poisson.py:52-92      GuidanceField.select / mix_by_magnitude / scaled
+ *   gdp_clone_rhs_masked, gdp_masked_rhs, gdp_plan_set_mask
+ *                        poisson.py:154-244    solve_masked: the system over the mask's pixels
+ *                                              (fixed neighbours folded in), solved by masked
+ *                                              MG-PCG instead of SuperLU
  *   gdp_tm_log_luminance hdr.py:165-167        tone_map: log of the clamped luminance
  *   gdp_tm_downsample    hdr.py:73-80          _downsample(image)
  *   gdp_tm_level_factor  hdr.py:83-104,126-133 attenuation_field: one pyramid level
@@ -121,6 +125,20 @@ int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, in
 int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
                   const uint8_t* mask, const double* target, int h, int w, int planes, int top,
                   int left, int hh, int ww, int mode, double* out, void* stream);
+/* The masked system's right-hand side (seamless_clone with domain "mask", poisson.py:
+   154-244, 337-338): as gdp_clone_rhs, but at the window interior points where
+   `interior` (h x w bytes, zero on the image's border rows and columns) is set, the
+   divergence of the guidance minus the target values of the neighbours that are not
+   unknowns (up, down, left, right); zero at the other points. */
+int gdp_clone_rhs_masked(const double* source, int sh, int sw, int row, int col,
+                         const uint8_t* mask, const double* target, int h, int w, int planes,
+                         int top, int left, int hh, int ww, int mode, const uint8_t* interior,
+                         double* out, void* stream);
+/* The same from an explicit guidance field (gx, gy planes x h x w) and image
+   (solve_masked, poisson.py:205-216). */
+int gdp_masked_rhs(const double* gx, const double* gy, const double* image,
+                   const uint8_t* interior, int h, int w, int planes, int top, int left, int hh,
+                   int ww, double* out, void* stream);
 /* Guidance-field arithmetic (GuidanceField, poisson.py:52-92) over planes x h x w:
    mode 0 select (mask[h x w] ? a : b), 1 mix_by_magnitude (|a| >= |b| ? a : b per
    component), 2 average, 3 scaled (a * factor[h x w], or a * scale if factor is NULL). */
@@ -198,6 +216,13 @@ int gdp_plan_level_shape(const gdp_plan* plan, int problem, int level, int* m, i
 /* Tile shapes: kind 0 = down (coarse tile), 1 = up (fine tile), 2 = Krylov (fine tile).
    Must be called before gdp_plan_bind; index into the compiled menu (see gdp_tile_menu). */
 int gdp_plan_set_tile(gdp_plan* plan, int kind, int level, int menu_index);
+/* Masked domain of a Dirichlet problem (solve_masked, poisson.py:154-244): `mask` is a
+   device array of rows x cols bytes, nonzero = unknown; the other points are held at zero
+   (their values belong in the right-hand side, see gdp_clone_rhs_masked).  The PCG
+   operator and true residual act on the unknowns only; the rectangle V-cycle, masked on
+   both sides, is the preconditioner.  NULL removes the mask.  Call before gdp_plan_bind;
+   the array must stay valid while the plan is used. */
+int gdp_plan_set_mask(gdp_plan* plan, int problem, const uint8_t* mask);
 /* Block tile (rows th x columns tw) of a menu entry for a kernel with nu sweeps (the
    down and up strips get narrower as nu grows; the Krylov tiles ignore nu). */
 int gdp_tile_menu(int kind, int index, int nu, int* th, int* tw);

@@ -9,8 +9,9 @@ Follows ``gradient-domain/src/gradient_domain/poisson.py``:
                                           (import / mixed / average), window = mask
                                           bounding box + margin, solve_dirichlet there
 
-The mask-domain route (SuperLU over the irregular selection, poisson.py:154-244) is not
-restated: the GPU path does not offer it.
+* ``solve_masked``    poisson.py:154-244  the system over the selected pixels off the
+                                          border, fixed neighbours folded in, SuperLU
+* ``seamless_clone(domain="mask")``  poisson.py:337-338
 """
 
 from __future__ import annotations
@@ -18,6 +19,9 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 import numpy as np
+
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from .operators import divergence, fold_boundary, gradient
 from .solvers import solve_system
@@ -75,6 +79,44 @@ def solve_dirichlet(field, boundary, method="mgcg", tolerance=1e-8, max_iteratio
     return (out[..., 0] if gray else out), reports
 
 
+_OFFSETS = ((-1, 0), (1, 0), (0, -1), (0, 1))
+
+
+def solve_masked(field, image, mask):
+    """Unknowns: mask without the image border; operator -4 / +1 over them; neighbours
+    that are not unknowns fixed to the image (poisson.py:154-244)."""
+    chans, gray = _channels(image)
+    interior = np.zeros(mask.shape, dtype=bool)
+    interior[1:-1, 1:-1] = np.asarray(mask, dtype=bool)[1:-1, 1:-1]
+    if not interior.any():
+        return np.asarray(image, dtype=float).copy(), []
+    index = np.full(mask.shape, -1, dtype=np.int64)
+    index[interior] = np.arange(int(interior.sum()))
+    nu = int(interior.sum())
+    rows, cols, vals = [np.arange(nu)], [np.arange(nu)], [np.full(nu, -4.0)]
+    ii, jj = np.nonzero(interior)
+    for di, dj in _OFFSETS:
+        nb = index[ii + di, jj + dj]
+        ok = nb >= 0
+        rows.append(np.arange(nu)[ok])
+        cols.append(nb[ok])
+        vals.append(np.ones(int(ok.sum())))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(nu, nu)).tocsc()
+    lu = spla.splu(A)
+    rhs = divergence(field.gx, field.gy)
+    if rhs.ndim == 2:
+        rhs = rhs[..., None]
+    out = chans.copy()
+    for k in range(chans.shape[2]):
+        b = rhs[..., k][interior].astype(float)
+        for di, dj in _OFFSETS:
+            fixed = index[ii + di, jj + dj] < 0
+            b[fixed] -= chans[ii + di, jj + dj, k][fixed]
+        out[..., k][interior] = lu.solve(b)
+    return (out[..., 0] if gray else out), []
+
+
 def clone_window(source, target, mask, offset=(0, 0), mode="import", margin=24):
     """(field, window slices) of the rectangle route of seamless_clone."""
     target = np.asarray(target, dtype=float)
@@ -103,10 +145,20 @@ def clone_window(source, target, mask, offset=(0, 0), mode="import", margin=24):
     return field, (slice(top, bottom), slice(left, right))
 
 
+def placed_mask(source, target, mask, offset=(0, 0)):
+    placed = np.zeros(np.asarray(target).shape[:2], dtype=bool)
+    row, col = offset
+    sh, sw = np.asarray(source).shape[:2]
+    placed[row:row + sh, col:col + sw] = mask
+    return placed
+
+
 def seamless_clone(source, target, mask, offset=(0, 0), mode="import", method="direct",
-                   margin=24, tolerance=1e-8):
+                   margin=24, tolerance=1e-8, domain="rectangle"):
     target = np.asarray(target, dtype=float)
     field, win = clone_window(source, target, mask, offset, mode, margin)
+    if domain == "mask":
+        return solve_masked(field, target, placed_mask(source, target, mask, offset))
     patch, reports = solve_dirichlet(GuidanceField(field.gx[win], field.gy[win]), target[win],
                                      method=method, tolerance=tolerance)
     out = target.copy()

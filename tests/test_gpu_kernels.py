@@ -372,8 +372,8 @@ def test_seamless_clone_matches_the_oracle(gd, method, mode):
     outside = np.ones(target.shape[:2], dtype=bool)
     outside[20:91, 31:113] = False
     assert np.array_equal(got[outside], target[outside])
-    with pytest.raises(NotImplementedError):
-        gd.seamless_clone(source, target, mask, off, domain="mask")
+    with pytest.raises(ValueError):
+        gd.seamless_clone(source, target, mask, off, domain="lasso")
     with pytest.raises(ValueError):
         gd.seamless_clone(source, target, mask, off, mode="unknown")
 

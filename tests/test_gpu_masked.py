@@ -1,0 +1,102 @@
+"""Masked-domain solves on the device (solve_masked and seamless_clone(domain="mask"),
+gradient-domain/src/gradient_domain/poisson.py:154-244, 337-338) against the oracle's
+SuperLU solve of the same system."""
+
+import numpy as np
+import pytest
+
+from oracle import generators as G
+from oracle import poisson as OP
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gd():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import gdpoisson
+
+    return gdpoisson
+
+
+def _disc(shape, radius, centre=None):
+    r, c = np.mgrid[0:shape[0], 0:shape[1]]
+    cy, cx = centre if centre else (shape[0] / 2, shape[1] / 2)
+    return (r - cy) ** 2 + (c - cx) ** 2 < radius ** 2
+
+
+MASKS = {
+    "disc": lambda s: _disc(s, min(s) / 3),
+    "border": lambda s: _disc(s, min(s) / 3) | _disc(s, 9.0, (2, 3)),
+    "blobs": lambda s: G.blob_mask(s, seed=31, sigma=6.0),
+    "full": lambda s: np.ones(s, dtype=bool),
+}
+
+
+@pytest.mark.parametrize("kind", sorted(MASKS))
+@pytest.mark.parametrize("method", ["mgcg", "mgcg_mixed"])
+def test_solve_masked_matches_superlu(gd, kind, method):
+    rng = np.random.default_rng(41)
+    shape = (70, 90)
+    img = rng.random(shape + (3,))
+    gx, gy = rng.standard_normal(shape + (3,)) * 0.1, rng.standard_normal(shape + (3,)) * 0.1
+    mask = MASKS[kind](shape)
+    tol = 1e-12 if method == "mgcg" else 1e-8
+    got, reps = gd.solve_masked(gd.GuidanceField(gx, gy), img, mask, method=method,
+                                tolerance=tol)
+    want, _ = OP.solve_masked(OP.GuidanceField(gx, gy), img, mask)
+    assert len(reps) == 3 and all(r.converged and r.relative_residual <= tol for r in reps)
+    assert np.abs(got - want).max() < (1e-9 if method == "mgcg" else 1e-6)
+    inner = np.zeros(shape, dtype=bool)
+    inner[1:-1, 1:-1] = mask[1:-1, 1:-1]
+    assert np.array_equal(got[~inner], img[~inner])  # test_applications.py:65-75
+
+
+@pytest.mark.parametrize("mode", ["import", "mixed", "average"])
+def test_mask_route_clone_matches_the_oracle(gd, mode):
+    src, tgt, _ = G.compositing_images((96, 112), config=3)
+    mask = G.blob_mask((96, 112), seed=32, sigma=8.0)
+    got, reps = gd.seamless_clone(src, tgt, mask, (0, 0), mode=mode, domain="mask",
+                                  method="mgcg_mixed")
+    want, _ = OP.seamless_clone(src, tgt, mask, (0, 0), mode=mode, domain="mask")
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    assert np.abs(got - want).max() < 1e-6
+
+
+def test_mask_route_with_an_offset_source(gd):
+    rng = np.random.default_rng(33)
+    tgt = rng.random((80, 100, 3))
+    src = rng.random((30, 40, 3))
+    mask = _disc((30, 40), 12.0)
+    got, _ = gd.seamless_clone(src, tgt, mask, (25, 31), domain="mask", method="mgcg",
+                               tolerance=1e-12)
+    want, _ = OP.seamless_clone(src, tgt, mask, (25, 31), domain="mask")
+    assert np.abs(got - want).max() < 1e-9
+
+
+def test_masked_compositing_at_512(gd):
+    """A blob-masked 512 x 512 RGB composite (about 30% of the pixels unknown) against
+    SuperLU; the iteration count is reported."""
+    src, tgt, mask = G.compositing_images((512, 512), config=3)
+    got, reps = gd.seamless_clone(src, tgt, mask, (0, 0), domain="mask", method="mgcg_mixed")
+    want, _ = OP.seamless_clone(src, tgt, mask, (0, 0), domain="mask")
+    print("masked 512x512 iterations", [r.iterations for r in reps])
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    assert np.abs(got - want).max() < 1e-6
+
+
+def test_masked_compositing_at_full_size(gd):
+    """2048 x 2048 RGB, blob mask: mixed and FP64 masked solves agree."""
+    import torch
+
+    src, tgt, mask = G.compositing_images((2048, 2048))
+    s, t = torch.from_numpy(src).cuda(), torch.from_numpy(tgt).cuda()
+    mixed, reps = gd.seamless_clone(s, t, mask, (0, 0), domain="mask", method="mgcg_mixed")
+    fp64, reps64 = gd.seamless_clone(s, t, mask, (0, 0), domain="mask", method="mgcg")
+    print("masked 2048x2048 iterations mixed", [r.iterations for r in reps],
+          "fp64", [r.iterations for r in reps64])
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps + reps64)
+    assert torch.abs(mixed - fp64).max().item() < 1e-6

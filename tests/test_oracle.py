@@ -347,3 +347,48 @@ def test_clone_rhs_matches_the_compositing_generator():
     want, tgt, m2 = G.compositing_problem((64, 80))
     assert np.array_equal(m2, mask) and np.array_equal(np.moveaxis(tgt, 0, 2), target)
     assert np.array_equal(np.moveaxis(rhs, 2, 0), want)
+
+
+def _disc(shape, radius, centre=None):
+    r, c = np.mgrid[0:shape[0], 0:shape[1]]
+    cy, cx = centre if centre else (shape[0] / 2, shape[1] / 2)
+    return (r - cy) ** 2 + (c - cx) ** 2 < radius ** 2
+
+
+def test_masked_solve_over_the_whole_interior_is_the_rectangle_solve():
+    from oracle import poisson as OP
+
+    rng = np.random.default_rng(21)
+    img = rng.random((30, 41, 3))
+    field = OP.GuidanceField(rng.standard_normal((30, 41, 3)), rng.standard_normal((30, 41, 3)))
+    masked, _ = OP.solve_masked(field, img, np.ones((30, 41), dtype=bool))
+    rect, _ = OP.solve_dirichlet(field, img, method="direct")
+    assert np.abs(masked - rect).max() < 1e-10
+
+
+def test_masked_solve_satisfies_the_poisson_equation_on_the_mask():
+    # L(result) = div(field) at every unknown; every other pixel keeps its value
+    from oracle import poisson as OP
+    from oracle.operators import divergence
+
+    rng = np.random.default_rng(22)
+    img = rng.random((40, 52))
+    field = OP.GuidanceField(rng.standard_normal((40, 52)), rng.standard_normal((40, 52)))
+    mask = _disc((40, 52), 15.0) | _disc((40, 52), 6.0, (3, 4))  # touches the border
+    out, _ = OP.solve_masked(field, img, mask)
+    inner = np.zeros_like(mask)
+    inner[1:-1, 1:-1] = mask[1:-1, 1:-1]
+    p = np.pad(out, 1)
+    lap = p[:-2, 1:-1] + p[2:, 1:-1] + p[1:-1, :-2] + p[1:-1, 2:] - 4 * out
+    assert np.abs(lap - divergence(field.gx, field.gy))[inner].max() < 1e-10
+    assert np.array_equal(out[~inner], img[~inner])
+
+
+@pytest.mark.parametrize("mode", ["import", "mixed", "average"])
+def test_mask_route_cloning_the_target_into_itself_changes_nothing(mode):
+    from oracle import poisson as OP
+
+    tgt = np.random.default_rng(23).random((32, 36, 3))
+    mask = _disc((20, 24), 8.0)
+    out, _ = OP.seamless_clone(tgt[5:25, 6:30], tgt, mask, (5, 6), mode=mode, domain="mask")
+    assert np.abs(out - tgt).max() < 1e-10

@@ -81,7 +81,9 @@ struct VTile {
   int32_t problem, i0, j0, slot;  // slot: this tile's partial-sum slot (finest up kernel)
   int32_t lead;                   // the one tile per problem that writes its state
   int32_t rbeg, rcnt;             // ||r||^2 partials to test convergence from (level 0)
-  int32_t pad[3];
+  int32_t pad;
+  const uint8_t* mask;            // finest level of a masked domain: m x n bytes, nonzero =
+                                  // unknown (the rest held at zero); else null
 };
 static_assert(sizeof(VTile) == 144, "VTile layout");
 
@@ -100,7 +102,10 @@ struct KTile {
   int32_t lead;
   int32_t kbeg, kcnt;  // slots of this problem's Krylov-tile partials
   int32_t zbeg, zcnt;  // slots of this problem's <r, z> partials (finest up kernel / tail)
-  int32_t pad[3];
+  int32_t pad;
+  // masked domain (Dirichlet): m x n bytes, nonzero = unknown, the rest held at zero; null
+  // = every point is an unknown
+  const uint8_t* mask;
 };
 static_assert(sizeof(KTile) == 128, "KTile layout");
 

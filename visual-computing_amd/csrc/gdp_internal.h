@@ -13,12 +13,14 @@ struct DownArgs {
   ProblemState* st;
   const double2* upart;
   int check_conv;
+  int masked;  // finest level of a plan with masked domains (strip height menu 0)
 };
 struct UpArgs {
   const VTile* tiles;
   const ProblemState* st;
   double2* zpart;
   int order;
+  int masked;
 };
 template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a);
@@ -69,8 +71,11 @@ hipError_t launch_resnorm_partial(const double* u, const double* b, int m, int n
 
 hipError_t launch_clone_rhs(const double* src, int sh, int sw, int row, int col,
                             const uint8_t* mask, const double* tgt, int h, int w, int planes,
-                            int top, int left, int hh, int ww, int mode, double* out,
-                            hipStream_t s);
+                            int top, int left, int hh, int ww, int mode,
+                            const uint8_t* interior, double* out, hipStream_t s);
+hipError_t launch_masked_rhs(const double* gx, const double* gy, const double* img,
+                             const uint8_t* interior, int h, int w, int planes, int top,
+                             int left, int hh, int ww, double* out, hipStream_t s);
 hipError_t launch_guidance(const double* ax, const double* ay, const double* bx,
                            const double* by, const uint8_t* mask, const double* factor,
                            double fscale, int mode, size_t hw, int planes, double* ox,

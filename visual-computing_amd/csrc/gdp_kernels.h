@@ -401,13 +401,16 @@ __global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ ti
   // vector loads first; the scalar reduction below overlaps them
   T zl[NRR][NCC];
   double dl[NRR][NCC];
+  const uint8_t* __restrict__ mk = t.mask;
 #pragma unroll
   for (int rr = 0; rr < NRR; ++rr)
 #pragma unroll
     for (int cc = 0; cc < NCC; ++cc) {
       const int a = wv + 4 * rr, c = lane + 64 * cc;
       const int gi = gr0 + a, gj = gc0 + c;
-      const bool in = a < RH && c < RW && (interior || (gi >= 0 && gi < m && gj >= 0 && gj < n));
+      bool in = a < RH && c < RW && (interior || (gi >= 0 && gi < m && gj >= 0 && gj < n));
+      // a masked domain holds its fixed points at zero: d (and so q's stencil) sees zeros
+      if (mk && in) in = mk[(size_t)gi * n + gj] != 0;
       const size_t o = in ? (size_t)gi * pitch + gj : 0;
       zl[rr][cc] = in ? z[o] : T(0);
       dl[rr][cc] = in ? dold[o] : 0.0;
@@ -445,7 +448,8 @@ __global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ ti
       const double dc = sd[idx];
       const double nb = ((sd[idx - RW] + sd[idx + RW]) + sd[idx - 1]) + sd[idx + 1];
       const double dg = interior ? 4.0 : diag_of<double>(neu, gi, gj, m, n);
-      const double q = -(nb - dg * dc) * t.inv_h2;
+      double q = -(nb - dg * dc) * t.inv_h2;
+      if (mk && !mk[(size_t)gi * n + gj]) q = 0.0;  // no equation at a fixed point
       const size_t o = (size_t)gi * pitch + gj;
       dnew[o] = dc;
       t.q[o] = q;
@@ -653,7 +657,8 @@ __global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__
     const double xc = sx[idx];
     const double nb = ((sx[idx - RW] + sx[idx + RW]) + sx[idx - 1]) + sx[idx + 1];
     const double lx = (nb - diag_of<double>(neu, gi, gj, m, n) * xc) * t.inv_h2;
-    const double res = (b[(size_t)gi * n + gj] - mean) - lx;
+    double res = (b[(size_t)gi * n + gj] - mean) - lx;
+    if (t.mask && !t.mask[(size_t)gi * n + gj]) res = 0.0;  // fixed point of a masked domain
     const size_t o = (size_t)gi * pitch + gj;
     if (mode == 0) t.r[o] = -res;
     b0[o] = T(res);

@@ -278,9 +278,11 @@ __device__ inline void clone_guidance(const CloneArgs& a, size_t sp, size_t tp, 
 // Folded right-hand side of the window [top, top+hh) x [left, left+ww) of one plane:
 // divergence (operators.py:53-83) of the guidance at the window's interior points, minus
 // the window's one-pixel ring of target values (fold_boundary, operators.py:139-169,
-// h = 1, in its order: first row, last row, first column, last column).
+// h = 1, in its order: first row, last row, first column, last column); or, with an
+// interior mask, the masked system's right-hand side.
 __global__ __launch_bounds__(256) void k_clone_rhs(CloneArgs a, int top, int left, int hh,
-                                                   int ww, double* __restrict__ out) {
+                                                   int ww, const uint8_t* __restrict__ interior,
+                                                   double* __restrict__ out) {
   // no fused multiply-adds: the result is bit-identical to the unfused FP64 formula
 #pragma clang fp contract(off)
   const int mi = hh - 2, ni = ww - 2;
@@ -295,10 +297,56 @@ __global__ __launch_bounds__(256) void k_clone_rhs(CloneArgs a, int top, int lef
   clone_guidance(a, sp, tp, gi - 1, gj, unused, gyu);
   double v = (gx - gxl) + (gy - gyu);
   const double* t = a.tgt + tp;
-  if (i == 0) v -= t[(size_t)top * a.w + gj];
-  if (i == mi - 1) v -= t[(size_t)(top + hh - 1) * a.w + gj];
-  if (j == 0) v -= t[(size_t)gi * a.w + left];
-  if (j == ni - 1) v -= t[(size_t)gi * a.w + left + ww - 1];
+  if (interior) {
+    // masked domain (solve_masked, poisson.py:208-216): no equation off the mask; the
+    // neighbours that are not unknowns give their target values, up, down, left, right
+    const size_t gp = (size_t)gi * a.w + gj;
+    if (!interior[gp]) {
+      v = 0.0;
+    } else {
+      if (!interior[gp - a.w]) v -= t[gp - a.w];
+      if (!interior[gp + a.w]) v -= t[gp + a.w];
+      if (!interior[gp - 1]) v -= t[gp - 1];
+      if (!interior[gp + 1]) v -= t[gp + 1];
+    }
+  } else {
+    if (i == 0) v -= t[(size_t)top * a.w + gj];
+    if (i == mi - 1) v -= t[(size_t)(top + hh - 1) * a.w + gj];
+    if (j == 0) v -= t[(size_t)gi * a.w + left];
+    if (j == ni - 1) v -= t[(size_t)gi * a.w + left + ww - 1];
+  }
+  out[(size_t)blockIdx.z * mi * ni + (size_t)i * ni + j] = v;
+}
+
+// Right-hand side of a masked solve from an explicit field (solve_masked,
+// poisson.py:205-216): at the unknowns of the window [top, top+hh) x [left, left+ww)
+// interior, divergence(gx, gy) minus the values of the neighbours that are not unknowns
+// (up, down, left, right); zero elsewhere.  interior: h x w bytes, never set on the
+// image's border rows and columns.
+__global__ __launch_bounds__(256) void k_masked_rhs(const double* __restrict__ gx,
+                                                    const double* __restrict__ gy,
+                                                    const double* __restrict__ img,
+                                                    const uint8_t* __restrict__ interior, int h,
+                                                    int w, int top, int left, int hh, int ww,
+                                                    double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int mi = hh - 2, ni = ww - 2;
+  int i, j;
+  if (!coord(mi, ni, i, j)) return;
+  const size_t pl = (size_t)blockIdx.z * h * w;
+  const int gi = top + 1 + i, gj = left + 1 + j;
+  const size_t gp = (size_t)gi * w + gj;
+  double v = 0.0;
+  if (interior[gp]) {
+    const double* X = gx + pl;
+    const double* Y = gy + pl;
+    const double* t = img + pl;
+    v = (X[gp] - X[gp - 1]) + (Y[gp] - Y[gp - w]);
+    if (!interior[gp - w]) v -= t[gp - w];
+    if (!interior[gp + w]) v -= t[gp + w];
+    if (!interior[gp - 1]) v -= t[gp - 1];
+    if (!interior[gp + 1]) v -= t[gp + 1];
+  }
   out[(size_t)blockIdx.z * mi * ni + (size_t)i * ni + j] = v;
 }
 
@@ -345,10 +393,19 @@ dim3 grid2d(int m, int n, int planes) {
 
 hipError_t launch_clone_rhs(const double* src, int sh, int sw, int row, int col,
                             const uint8_t* mask, const double* tgt, int h, int w, int planes,
-                            int top, int left, int hh, int ww, int mode, double* out,
-                            hipStream_t s) {
+                            int top, int left, int hh, int ww, int mode,
+                            const uint8_t* interior, double* out, hipStream_t s) {
   CloneArgs a{src, mask, tgt, sh, sw, row, col, h, w, mode};
-  k_clone_rhs<<<grid2d(hh - 2, ww - 2, planes), 256, 0, s>>>(a, top, left, hh, ww, out);
+  k_clone_rhs<<<grid2d(hh - 2, ww - 2, planes), 256, 0, s>>>(a, top, left, hh, ww, interior,
+                                                             out);
+  return hipGetLastError();
+}
+
+hipError_t launch_masked_rhs(const double* gx, const double* gy, const double* img,
+                             const uint8_t* interior, int h, int w, int planes, int top,
+                             int left, int hh, int ww, double* out, hipStream_t s) {
+  k_masked_rhs<<<grid2d(hh - 2, ww - 2, planes), 256, 0, s>>>(gx, gy, img, interior, h, w, top,
+                                                              left, hh, ww, out);
   return hipGetLastError();
 }
 

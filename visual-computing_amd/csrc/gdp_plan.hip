@@ -100,6 +100,7 @@ struct LevelInfo {
   int m, n, pitch;
   double h;
   size_t off_u, off_v, off_b;
+  size_t off_mask = 0;  // coarse levels of a Dirichlet problem: mask by injection (m x n)
 };
 
 struct ProbInfo {
@@ -113,6 +114,7 @@ struct ProbInfo {
   size_t off_x = 0, off_r = 0, off_d0 = 0, off_d1 = 0, off_q = 0, off_dct = 0;
   int tail_off[kMaxTailLevels][2];
   int scratch_off = 0, coarse_off = 0, tab_off = 0;
+  const uint8_t* mask = nullptr;  // masked Dirichlet domain (device, m x n), or null
 };
 
 struct Table {
@@ -252,6 +254,22 @@ void pinned_put(void* p, size_t bytes) {
   PinnedPool& pool = pinned_pool();
   std::lock_guard<std::mutex> lock(pool.mu);
   pool.free[pinned_class(bytes)].push_back(p);
+}
+
+// coarse mask (mc x nc) from the next finer one (row length nf) by injection: the coarse
+// point (I, J) of a Dirichlet hierarchy coincides with the fine point (2I + 1, 2J + 1)
+__global__ __launch_bounds__(256) void k_mask_inject(const uint8_t* __restrict__ fine, int nf,
+                                                     uint8_t* __restrict__ coarse, int mc, int nc) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= mc * nc) return;
+  const int I = t / nc, J = t - I * nc;
+  coarse[t] = fine[(size_t)(2 * I + 1) * nf + 2 * J + 1] ? 1 : 0;
+}
+
+bool has_mask(const gdp_plan* P) {
+  for (const ProbInfo& pi : P->probs)
+    if (pi.mask) return true;
+  return false;
 }
 
 // Raises the dynamic-LDS limit of a tail kernel family (0 / 1: strip tail float / double,
@@ -493,11 +511,14 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     return o2;
   };
   for (auto& pi : P->probs) {
-    for (auto& li : pi.lv) {
+    for (size_t l = 0; l < pi.lv.size(); ++l) {
+      LevelInfo& li = pi.lv[l];
       const size_t a = size_t(li.m) * li.pitch * P->tsize;
       li.off_u = take(a);
       li.off_v = take(a);
       li.off_b = take(a);
+      // room for the level's mask should the problem be given a masked domain
+      if (l > 0 && pi.p.boundary == GDP_DIRICHLET) li.off_mask = take(size_t(li.m) * li.n);
     }
     const size_t k = size_t(pi.lv[0].m) * pi.lv[0].pitch * 8;
     pi.off_x = take(k);
@@ -572,6 +593,21 @@ int gdp_plan_level_shape(const gdp_plan* P, int problem, int level, int* m, int*
   }
   *m = P->probs[problem].lv[level].m;
   *n = P->probs[problem].lv[level].n;
+  return GDP_OK;
+}
+
+int gdp_plan_set_mask(gdp_plan* P, int problem, const uint8_t* mask) {
+  if (!P || problem < 0 || problem >= P->count()) {
+    set_error("gdp_plan_set_mask: bad plan or problem");
+    return GDP_EINVAL;
+  }
+  if (mask && P->probs[problem].p.boundary != GDP_DIRICHLET) {
+    set_error("gdp_plan_set_mask: a masked domain needs the Dirichlet boundary");
+    return GDP_EINVAL;
+  }
+  P->probs[problem].mask = mask;
+  if (P->bound) P->bound = false;  // tables must be rebuilt by gdp_plan_bind
+  destroy_graphs(P);
   return GDP_OK;
 }
 
@@ -804,8 +840,26 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     t.lead = lead;
     t.rbeg = rk[p].begin;
     t.rcnt = rk[p].count;
+    if (pi.mask) t.mask = l == 0 ? pi.mask : reinterpret_cast<const uint8_t*>(P->ws + f.off_mask);
     return t;
   };
+  if (has_mask(P)) {
+    // the masked level kernels exist for the first strip height only
+    for (int l = 0; l < kMaxDepth; ++l) P->down_menu[l] = P->up_menu[l] = 0;
+    // coarse masks by injection: coarse point (I, J) is an unknown when its coincident
+    // fine point (2I + 1, 2J + 1) is
+    for (const ProbInfo& pi : P->probs) {
+      if (!pi.mask) continue;
+      for (size_t l = 1; l < pi.lv.size(); ++l) {
+        const LevelInfo& f = pi.lv[l - 1];
+        const LevelInfo& c = pi.lv[l];
+        const uint8_t* src = l == 1 ? pi.mask : reinterpret_cast<const uint8_t*>(P->ws + f.off_mask);
+        k_mask_inject<<<(c.m * c.n + 255) / 256, 256, 0, P->cap>>>(
+            src, f.n, reinterpret_cast<uint8_t*>(P->ws + c.off_mask), c.m, c.n);
+        GDP_HIP(hipGetLastError());
+      }
+    }
+  }
   for (int l = 0; l < P->depth; ++l) {
     std::vector<VTile> vd, vu;
     const Shape2 sd = menu_shape(0, P->down_menu[l], P->opt.pre_smoothing);
@@ -867,6 +921,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
           t.kcnt = rk[p].count;
           t.zbeg = ru[p].begin;
           t.zcnt = ru[p].count;
+          t.mask = pi.mask;
           v.push_back(t);
         }
     }
@@ -973,7 +1028,8 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     char nm[32];
     snprintf(nm, sizeof nm, "down L%d", l);
     ProfScope ps(P, s, nm, (2.25 * sizeof(T)) * level_points(P, l));
-    DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), check_conv ? 1 : 0};
+    DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), check_conv ? 1 : 0,
+               has_mask(P) ? 1 : 0};
     launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
@@ -1014,7 +1070,8 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     const bool dl = dots && l == 0;
     snprintf(nm, sizeof nm, dl ? "up L%d+dots" : "up L%d", l);
     ProfScope ps(P, s, nm, (3.25 * sizeof(T) + (dl ? 16.0 : 0.0)) * level_points(P, l));
-    UpArgs a{P->at<VTile>(P->up[l].off), P->d_state(), P->d_zpart(), order};
+    UpArgs a{P->at<VTile>(P->up[l].off), P->d_state(), P->d_zpart(), order,
+             has_mask(P) ? 1 : 0};
     launch_up<T>(P->up_menu[l], o.post_smoothing, dl, P->up[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
@@ -1488,9 +1545,10 @@ int gdp_fold_boundary(const double* rhs, const double* boundary, double* out, in
   return GDP_OK;
 }
 
-int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
-                  const uint8_t* mask, const double* target, int h, int w, int planes, int top,
-                  int left, int hh, int ww, int mode, double* out, void* stream) {
+static int clone_rhs_checked(const double* source, int sh, int sw, int row, int col,
+                      const uint8_t* mask, const double* target, int h, int w, int planes,
+                      int top, int left, int hh, int ww, int mode, const uint8_t* interior,
+                      double* out, void* stream) {
   if (!source || !mask || !target || !out) return (set_error("null argument"), GDP_EINVAL);
   if (planes < 1 || sh < 1 || sw < 1 || h < 3 || w < 3)
     return (set_error("bad sizes"), GDP_EINVAL);
@@ -1500,7 +1558,35 @@ int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
     return (set_error("the window must be at least 3 x 3 and inside the target"), GDP_EINVAL);
   if (mode < 0 || mode > 2) return (set_error("unknown clone mode %d", mode), GDP_EINVAL);
   GDP_HIP(launch_clone_rhs(source, sh, sw, row, col, mask, target, h, w, planes, top, left, hh,
-                           ww, mode, out, static_cast<hipStream_t>(stream)));
+                           ww, mode, interior, out, static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
+int gdp_clone_rhs(const double* source, int sh, int sw, int row, int col,
+                  const uint8_t* mask, const double* target, int h, int w, int planes, int top,
+                  int left, int hh, int ww, int mode, double* out, void* stream) {
+  return clone_rhs_checked(source, sh, sw, row, col, mask, target, h, w, planes, top, left, hh,
+                           ww, mode, nullptr, out, stream);
+}
+
+int gdp_clone_rhs_masked(const double* source, int sh, int sw, int row, int col,
+                         const uint8_t* mask, const double* target, int h, int w, int planes,
+                         int top, int left, int hh, int ww, int mode, const uint8_t* interior,
+                         double* out, void* stream) {
+  if (!interior) return (set_error("null interior mask"), GDP_EINVAL);
+  return clone_rhs_checked(source, sh, sw, row, col, mask, target, h, w, planes, top, left, hh,
+                           ww, mode, interior, out, stream);
+}
+
+int gdp_masked_rhs(const double* gx, const double* gy, const double* image,
+                   const uint8_t* interior, int h, int w, int planes, int top, int left, int hh,
+                   int ww, double* out, void* stream) {
+  if (!gx || !gy || !image || !interior || !out) return (set_error("null argument"), GDP_EINVAL);
+  if (planes < 1 || h < 3 || w < 3 || top < 0 || left < 0 || hh < 3 || ww < 3 ||
+      top + hh > h || left + ww > w)
+    return (set_error("the window must be at least 3 x 3 and inside the image"), GDP_EINVAL);
+  GDP_HIP(launch_masked_rhs(gx, gy, image, interior, h, w, planes, top, left, hh, ww, out,
+                            static_cast<hipStream_t>(stream)));
   return GDP_OK;
 }
 

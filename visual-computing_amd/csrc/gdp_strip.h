@@ -120,12 +120,18 @@ __device__ inline int row_count(int gi, int m, bool neu) {
   return neu ? (gi > 0) + (gi < m - 1) : 2;
 }
 
+// Unknown bits of a masked finest level: bit 2a is the lane's x column of strip row a,
+// bit 2a + 1 its y column.
+__device__ inline bool mask_bit(uint64_t mb, int bit) { return (mb >> bit) & 1ull; }
+
 // One red-black half-sweep over rows 1 .. FH-2 of a strip.  Row a updates x when
 // (a + S + FLIP) is even, else y.  EDGE: rows may lie outside the grid or on its
-// first/last row.
-template <typename T, int FH, int S, int FLIP, bool EDGE>
+// first/last row.  MASK: points whose bit in mb is clear are fixed at zero (their inverse
+// diagonal is taken as zero, and so is their right-hand side).
+template <typename T, int FH, int S, int FLIP, bool EDGE, bool MASK = false>
 __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
-                                       const LaneCols<T>& lc, int gr0, int m, bool neu) {
+                                       const LaneCols<T>& lc, int gr0, int m, bool neu,
+                                       uint64_t mb = 0) {
 #pragma unroll
   for (int a = 1; a < FH - 1; ++a) {
     int vc = 2;
@@ -136,23 +142,44 @@ __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>:
     }
     if (((a + S + FLIP) & 1) == 0) {
       const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
-      const T w = EDGE ? lc.wx(vc) : lc.ix2;
+      T w = EDGE ? lc.wx(vc) : lc.ix2;
+      if (MASK && !mask_bit(mb, 2 * a)) w = T(0);
       u[a].x = w * nb - hb[a].x;
     } else {
       const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
-      const T w = EDGE ? lc.wy(vc) : lc.iy2;
+      T w = EDGE ? lc.wy(vc) : lc.iy2;
+      if (MASK && !mask_bit(mb, 2 * a + 1)) w = T(0);
       u[a].y = w * nb - hb[a].y;
     }
   }
 }
 
-template <typename T, int FH, int S0, int NS, int FLIP, bool EDGE>
+template <typename T, int FH, int S0, int NS, int FLIP, bool EDGE, bool MASK = false>
 __device__ __attribute__((always_inline)) void strip_sweeps(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
-                                    const LaneCols<T>& lc, int gr0, int m, bool neu) {
+                                    const LaneCols<T>& lc, int gr0, int m, bool neu,
+                                    uint64_t mb = 0) {
   if constexpr (NS > 0) {
-    strip_halfsweep<T, FH, S0, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
-    strip_sweeps<T, FH, S0 + 1, NS - 1, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+    strip_halfsweep<T, FH, S0, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
+    strip_sweeps<T, FH, S0 + 1, NS - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
   }
+}
+
+// Unknown bits of the strip rows gr0 .. gr0 + FH - 1 at the lane's column pair (gjx,
+// gjx + 1) of a masked finest level (mask: m x n bytes).
+template <int FH>
+__device__ inline uint64_t load_mask_bits(const uint8_t* __restrict__ mask, int gr0, int gjx,
+                                          int m, int n, bool work) {
+  static_assert(2 * FH <= 64, "mask bits of a strip must fit 64 bits");
+  uint64_t mb = 0;
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const int gi = gr0 + a;
+    if (!work || gi < 0 || gi >= m) continue;
+    const uint8_t* row = mask + (size_t)gi * n;
+    if (gjx >= 0 && gjx < n && row[gjx]) mb |= 1ull << (2 * a);
+    if (gjx + 1 >= 0 && gjx + 1 < n && row[gjx + 1]) mb |= 2ull << (2 * a);
+  }
+  return mb;
 }
 
 // ------------------------------------------------------------------ down
@@ -166,8 +193,9 @@ __device__ __attribute__((always_inline)) void strip_sweeps(typename Pair<T>::ty
 // 0.25 / h^2.  Neumann restriction taps fine 2I-1 .. 2I+2 (1/4, 3/4, 3/4, 1/4) of the
 // residual extended by r[-1] = r[0] and, for even sizes, r[f] = r[f-1]; Dirichlet taps
 // 2I .. 2I+2 (1/2, 1, 1/2).
-template <typename T, int NU, int CR, int FLIP, bool EDGE>
-__device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0, int lane, typename Pair<T>::type* hb) {
+template <typename T, int NU, int CR, int FLIP, bool EDGE, bool MASK = false>
+__device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0, int lane, typename Pair<T>::type* hb,
+                                                        uint64_t mb = 0) {
   using V2 = typename Pair<T>::type;
   using G = StripGeo<NU>;
   constexpr int OFF = G::kOff, CC = G::kDownCols;
@@ -195,6 +223,10 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
     const T wy = EDGE ? lc.wy(vc) : lc.iy2;
     hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
     hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
+    if (MASK) {
+      if (!mask_bit(mb, 2 * a)) hb[a].x = T(0);
+      if (!mask_bit(mb, 2 * a + 1)) hb[a].y = T(0);
+    }
     if (NU > 0 && ((a + FLIP) & 1) == 0) {
       u[a].x = -hb[a].x;
       u[a].y = T(0);
@@ -205,7 +237,8 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
       u[a] = V2{T(0), T(0)};
     }
   }
-  if constexpr (NU > 0) strip_sweeps<T, FH, 1, 2 * NU - 1, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+  if constexpr (NU > 0)
+    strip_sweeps<T, FH, 1, 2 * NU - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
 
   // pre-smoothed iterate of the owned fine points (the last strip row / column of the
   // grid owns every remaining fine row / column)
@@ -245,8 +278,8 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
     const T ny = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
     const T rx = cx * (hb[a].x + u[a].x) - nx;
     const T ry = cy * (hb[a].y + u[a].y) - ny;
-    hb[a].x = (rin && lc.inx) ? rx : T(0);
-    hb[a].y = (rin && lc.iny) ? ry : T(0);
+    hb[a].x = (rin && lc.inx && (!MASK || mask_bit(mb, 2 * a))) ? rx : T(0);
+    hb[a].y = (rin && lc.iny && (!MASK || mask_bit(mb, 2 * a + 1))) ? ry : T(0);
   }
   if (neu) {
     // residual extension past the grid: columns first, then rows (corners take both)
@@ -298,7 +331,7 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
   }
 }
 
-template <typename T, int NU, int CR>
+template <typename T, int NU, int CR, bool MASK = false>
 __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ tiles,
                                                     ProblemState* __restrict__ st,
                                                     const double2* __restrict__ upart,
@@ -341,6 +374,16 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
   if (!work) return;
   const bool edge = gr0 < 1 || gr0 + FH > m - 1;
   const int flip = (gr0 + gc0) & 1;
+  if constexpr (MASK) {
+    // masked finest level (a kernel of its own, so the unmasked kernels keep their
+    // registers): the general body with fixed points held at zero
+    if (t.mask) {
+      const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
+      if (flip) sdown_body<T, NU, CR, 1, true, true>(t, I0, lane, hb, mb);
+      else sdown_body<T, NU, CR, 0, true, true>(t, I0, lane, hb, mb);
+      return;
+    }
+  }
   if (edge) {
     if (flip) sdown_body<T, NU, CR, 1, true>(t, I0, lane, hb);
     else sdown_body<T, NU, CR, 0, true>(t, I0, lane, hb);
@@ -355,9 +398,10 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
 // kUpCols fine columns starting at (i0, j0) and loads a margin of 2 NU rows and
 // 2 NU + 2 columns around them.  With DOTS (finest level of a PCG iteration) the block also sums
 // <r, z> and <q, z> over its points.
-template <typename T, int NU, int TR, int FLIP, bool EDGE, bool DOTS>
+template <typename T, int NU, int TR, int FLIP, bool EDGE, bool DOTS, bool MASK = false>
 __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, int lane, typename Pair<T>::type* u,
-                         typename Pair<T>::type* hb, const T* E, double& zr, double& zq) {
+                         typename Pair<T>::type* hb, const T* E, double& zr, double& zq,
+                         uint64_t mb = 0) {
   using V2 = typename Pair<T>::type;
   using G = StripGeo<NU>;
   constexpr int H = G::kHalo, TC = G::kUpCols;
@@ -397,8 +441,12 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
     const T wy = EDGE ? lc.wy(vc) : lc.iy2;
     hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
     hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
+    if (MASK) {
+      if (!mask_bit(mb, 2 * a)) u[a].x = hb[a].x = T(0);
+      if (!mask_bit(mb, 2 * a + 1)) u[a].y = hb[a].y = T(0);
+    }
   }
-  strip_sweeps<T, FH, 0, 2 * NU, FLIP, EDGE>(u, hb, lc, gr0, m, neu);
+  strip_sweeps<T, FH, 0, 2 * NU, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
 
   GDP_GLOBAL T* uo = (GDP_GLOBAL T*)t.u;
   const int fc1 = min(n, t.j0 + TC);
@@ -437,7 +485,7 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
   }
 }
 
-template <typename T, int NU, int TR, bool DOTS>
+template <typename T, int NU, int TR, bool DOTS, bool MASK = false>
 __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tiles,
                                                   const ProblemState* __restrict__ st,
                                                   double2* __restrict__ zpart, int order) {
@@ -500,7 +548,18 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
   if (work) {
     const bool edge = gr0 < 1 || gr0 + FH > m - 1;
     const int flip = ((gr0 + gc0) + order) & 1;
-    if (edge) {
+    bool masked = false;
+    if constexpr (MASK) {
+      // masked finest level (a kernel of its own): fixed points held at zero
+      if (t.mask) {
+        const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
+        if (flip) sup_body<T, NU, TR, 1, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
+        else sup_body<T, NU, TR, 0, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
+        masked = true;
+      }
+    }
+    if (masked) {
+    } else if (edge) {
       if (flip) sup_body<T, NU, TR, 1, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
       else sup_body<T, NU, TR, 0, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
     } else {

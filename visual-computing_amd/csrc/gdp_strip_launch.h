@@ -19,8 +19,23 @@ void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #undef GDP_DOWN
 }
 
+// Masked finest level: one strip height (menu entry 0), a kernel of its own.
+template <typename T>
+void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWN(NU) \
+  k_sdown<T, NU, 1, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
+  switch (nu) {
+    case 0: GDP_DOWN(0); break;
+    case 1: GDP_DOWN(1); break;
+    case 2: GDP_DOWN(2); break;
+    default: GDP_DOWN(3); break;
+  }
+#undef GDP_DOWN
+}
+
 template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
+  if (a.masked) return down_masked<T>(nu, blocks, s, a);
   switch (menu) {
     case 0: down_nu<T, 1>(nu, blocks, s, a); break;
     case 1: down_nu<T, 2>(nu, blocks, s, a); break;
@@ -52,8 +67,26 @@ void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
   }
 }
 
+template <typename T, bool DOTS>
+void up_masked(int nu, int blocks, hipStream_t s, const UpArgs& a) {
+#define GDP_UP(NU) \
+  k_sup<T, NU, 2, DOTS, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
+  switch (nu) {
+    case 0: GDP_UP(0); break;
+    case 1: GDP_UP(1); break;
+    case 2: GDP_UP(2); break;
+    default: GDP_UP(3); break;
+  }
+#undef GDP_UP
+}
+
 template <typename T>
 void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a) {
+  if (a.masked) {
+    if (dots) up_masked<T, true>(nu, blocks, s, a);
+    else up_masked<T, false>(nu, blocks, s, a);
+    return;
+  }
   if (dots) launch_up_d<T, true>(menu, nu, blocks, s, a);
   else launch_up_d<T, false>(menu, nu, blocks, s, a);
 }

@@ -15,7 +15,7 @@ from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
 from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
 from .autotune import autotune
 from .plan import Plan, PlanOptions
-from .poisson import GuidanceField, seamless_clone, solve_dirichlet
+from .poisson import GuidanceField, seamless_clone, solve_dirichlet, solve_masked
 from . import hdr
 from .hdr import ToneMapConfig, attenuation_field, tone_map
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
@@ -29,6 +29,7 @@ __all__ = [
     "GuidanceField",
     "seamless_clone",
     "solve_dirichlet",
+    "solve_masked",
     "ToneMapConfig",
     "attenuation_field",
     "hdr",

@@ -27,6 +27,9 @@ EXPORTS = (
     "gdp_fold_boundary",
     "gdp_clone_rhs",
     "gdp_guidance",
+    "gdp_clone_rhs_masked",
+    "gdp_masked_rhs",
+    "gdp_plan_set_mask",
     "gdp_tm_work_bytes",
     "gdp_tm_log_luminance",
     "gdp_tm_downsample",
@@ -118,6 +121,11 @@ def _declare(lib):
                                   c_int, c_int, c_int, c_int, c_int, P, P]),
         "gdp_guidance": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, c_int, c_int, P, P,
                                  P]),
+        "gdp_clone_rhs_masked": (c_int, [P, c_int, c_int, c_int, c_int, P, P, c_int, c_int,
+                                         c_int, c_int, c_int, c_int, c_int, c_int, P, P, P]),
+        "gdp_masked_rhs": (c_int, [P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                   P, P]),
+        "gdp_plan_set_mask": (c_int, [P, c_int, P]),
         "gdp_tm_work_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
         "gdp_tm_log_luminance": (c_int, [P, c_int, c_int, P, P]),
         "gdp_tm_downsample": (c_int, [P, c_int, c_int, P, P]),

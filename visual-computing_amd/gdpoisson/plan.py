@@ -67,6 +67,7 @@ class Plan:
         device=None,
         tiles: dict | None = None,
         allocate: bool = True,
+        masks: Sequence | None = None,
     ):
         opts = options or PlanOptions()
         self.options = opts
@@ -119,6 +120,20 @@ class Plan:
         if tiles:
             for (kind, level), idx in tiles.items():
                 _lib.check(L.gdp_plan_set_tile(self._plan, kind, level, idx), "gdp_plan_set_tile")
+        # masked Dirichlet domains: one (m, n) uint8 device tensor per problem (or None),
+        # kept alive with the plan
+        self.masks = list(masks) if masks is not None else [None] * n
+        if len(self.masks) != n:
+            raise ValueError("masks must match the number of shapes")
+        for i, mk in enumerate(self.masks):
+            if mk is None:
+                continue
+            if (tuple(mk.shape) != self.shapes[i] or mk.dtype != torch.uint8
+                    or not mk.is_contiguous() or mk.device.type != "cuda"):
+                raise ValueError(f"mask {i}: need a contiguous uint8 device tensor of shape "
+                                 f"{self.shapes[i]}")
+            _lib.check(L.gdp_plan_set_mask(self._plan, i, ctypes.c_void_p(mk.data_ptr())),
+                       "gdp_plan_set_mask")
         nbytes = ctypes.c_size_t()
         _lib.check(L.gdp_plan_workspace_bytes(self._plan, ctypes.byref(nbytes)))
         self.workspace_bytes = int(nbytes.value)

@@ -1,0 +1,124 @@
+// The PCG direction update and operator apply of one iteration (solvers.py:149-151:
+// d = z + beta d, q = A d, <d, q>) in strip form: no LDS and no barrier.
+//
+// A wave owns a strip of kApplyRows output rows and 128 loaded columns: lane l holds the
+// column pair (j0 - 1 + 2 l, j0 + 2 l) of each of its rows plus the rows above and below,
+// so vertical neighbours are in the lane and horizontal ones come from DPP row shifts.
+// Neighbouring strips overlap by two columns: a strip writes the 126 columns j0 .. j0 + 125
+// (and column 0 when it is the grid's first strip, j0 = 1); strip origins are odd, so every
+// pair load is aligned.  Four strips (stacked vertically) make a block.  The stencil sums
+// its neighbours in the order up, down, left, right, as the other stencils here do.
+#pragma once
+#include "gdp_strip.h"
+
+namespace gdp {
+
+constexpr int kApplyRows = 4;    // output rows per strip
+constexpr int kApplyCols = 126;  // output columns per strip
+constexpr int kApplyBlockRows = (kThreads / 64) * kApplyRows;  // one strip per wave
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ tiles,
+                                                      ProblemState* __restrict__ st,
+                                                      const double2* __restrict__ zpart,
+                                                      double2* __restrict__ apart, int din,
+                                                      int jacobi) {
+  using TP = typename Pair<T>::type;
+  using D2 = typename Pair<double>::type;
+  constexpr int R = kApplyRows, FH = R + 2;
+  const KTile t = tiles[blockIdx.x];
+  const int p = t.problem;
+  __shared__ double sh[2 * kThreads / 64];
+  __shared__ double2 bc;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int m = t.m, n = t.n, pitch = t.pitch;
+  const int r0 = t.i0 + w * R;
+  const int gj = t.j0 - 1 + 2 * lane;  // x column (even); y is gj + 1
+  const bool xin = gj < n, yin = gj + 1 < n;
+  const GDP_GLOBAL T* z = (const GDP_GLOBAL T*)t.z;
+  const GDP_GLOBAL double* dold = (const GDP_GLOBAL double*)(din ? t.d1 : t.d0);
+  GDP_GLOBAL double* dnew = (GDP_GLOBAL double*)(din ? t.d0 : t.d1);
+  GDP_GLOBAL double* qv = (GDP_GLOBAL double*)t.q;
+  const uint8_t* __restrict__ mk = t.mask;
+
+  // loads first; the scalar reduction below overlaps them
+  TP zl[FH];
+  D2 dl[FH];
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const int gi = r0 - 1 + a;
+    zl[a] = TP{T(0), T(0)};
+    dl[a] = D2{0.0, 0.0};
+    if (gi < 0 || gi >= m || !xin) continue;
+    const size_t o = (size_t)gi * pitch + gj;
+    if (yin) {
+      zl[a] = *(const GDP_GLOBAL TP*)(z + o);
+      dl[a] = *(const GDP_GLOBAL D2*)(dold + o);
+    } else {
+      zl[a].x = z[o];
+      dl[a].x = dold[o];
+    }
+    if (mk) {  // a masked domain holds its fixed points at zero
+      const uint8_t* row = mk + (size_t)gi * n;
+      if (!row[gj]) zl[a].x = T(0), dl[a].x = 0.0;
+      if (yin && !row[gj + 1]) zl[a].y = T(0), dl[a].y = 0.0;
+    }
+  }
+  if (st[p].done) return;
+  const double2 z2 = reduce_slots<kThreads>(
+      zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
+  double beta;
+  if (st[p].iter == 0 || st[p].restart) beta = 0.0;
+  else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;
+  else beta = z2.x / st[p].delta_old;
+  if (t.lead && threadIdx.x == 0) {
+    st[p].delta = z2.x;
+    st[p].beta = beta;
+  }
+  D2 dn[FH];
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    dn[a].x = double(zl[a].x) + beta * dl[a].x;
+    dn[a].y = double(zl[a].y) + beta * dl[a].y;
+  }
+  const bool neu = t.bc == kNeumann;
+  const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;
+  const bool outy = yin && gj + 1 < t.j0 + kApplyCols;
+  double acc = 0.0;
+#pragma unroll
+  for (int a = 1; a <= R; ++a) {
+    const int gi = r0 - 1 + a;
+    if (gi >= m) break;
+    const double lx = dpp_shr1(dn[a].y), ry = dpp_shl1(dn[a].x);
+    const double nbx = ((dn[a - 1].x + dn[a + 1].x) + lx) + dn[a].y;
+    const double nby = ((dn[a - 1].y + dn[a + 1].y) + dn[a].x) + ry;
+    double qx = -(nbx - diag_of<double>(neu, gi, gj, m, n) * dn[a].x) * t.inv_h2;
+    double qy = -(nby - diag_of<double>(neu, gi, gj + 1, m, n) * dn[a].y) * t.inv_h2;
+    if (mk) {  // no equation at a fixed point
+      const uint8_t* row = mk + (size_t)gi * n;
+      if (xin && !row[gj]) qx = 0.0;
+      if (yin && !row[gj + 1]) qy = 0.0;
+    }
+    const size_t o = (size_t)gi * pitch + gj;
+    if (outx && outy) {
+      *(GDP_GLOBAL D2*)(dnew + o) = dn[a];
+      *(GDP_GLOBAL D2*)(qv + o) = D2{qx, qy};
+      acc += dn[a].x * qx + dn[a].y * qy;
+    } else {
+      if (outx) {
+        dnew[o] = dn[a].x;
+        qv[o] = qx;
+        acc += dn[a].x * qx;
+      }
+      if (outy) {
+        dnew[o + 1] = dn[a].y;
+        qv[o + 1] = qy;
+        acc += dn[a].y * qy;
+      }
+    }
+  }
+  write_partial<kThreads>(acc, 0.0, apart, t.slot, sh);
+}
+
+}  // namespace gdp

@@ -102,12 +102,13 @@ struct KTile {
   int32_t lead;
   int32_t kbeg, kcnt;  // slots of this problem's Krylov-tile partials
   int32_t zbeg, zcnt;  // slots of this problem's <r, z> partials (finest up kernel / tail)
+  int32_t abeg, acnt;  // slots of this problem's <d, q> partials (the strip apply's tiles)
   int32_t pad;
   // masked domain (Dirichlet): m x n bytes, nonzero = unknown, the rest held at zero; null
   // = every point is an unknown
   const uint8_t* mask;
 };
-static_assert(sizeof(KTile) == 128, "KTile layout");
+static_assert(sizeof(KTile) == 136, "KTile layout");
 
 // Slot range of one problem in a partial-sum buffer.
 struct SlotRange {

@@ -369,94 +369,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
 // ------------------------------------------------------------------ Krylov kernels
 // FP64 vectors of the fine grid, tiles of TH x TW points (TH * TW a multiple of the
 // block size).  Scalar data flow of one PCG iteration (solvers.py:149-159):
-//   k_apply   reads the V-cycle's <r,z>, <q,z> partials -> delta, beta;
+//   k_apply_s (gdp_apply.h) reads the V-cycle's <r,z>, <q,z> partials -> delta, beta;
 //             d = z + beta d, q = -A d, partial <d, q>
 //   k_update  reads the <d, q> partials -> alpha = delta / <d, q>;
 //             x += alpha d, r -= alpha q, V-cycle RHS -r, partial ||r||^2
 //   first V-cycle kernel: convergence test from the ||r||^2 partials
 // Each kernel reads only state fields written by earlier kernels and its lead block
 // writes only fields that no block of the same kernel reads.
-
-template <typename T, int TH, int TW>
-__global__ __launch_bounds__(kThreads) void k_apply(const KTile* __restrict__ tiles,
-                                                    ProblemState* __restrict__ st,
-                                                    const double2* __restrict__ zpart,
-                                                    double2* __restrict__ apart, int din,
-                                                    int jacobi) {
-  const KTile t = tiles[blockIdx.x];
-  const int p = t.problem;
-  constexpr int RH = TH + 2, RW = TW + 2;
-  constexpr int NRR = (RH + 3) / 4, NCC = (RW + 63) / 64;
-  __shared__ double sd[RH * RW];
-  __shared__ double sh[2 * kThreads / 64];
-  __shared__ double2 bc;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int m = t.m, n = t.n, pitch = t.pitch;
-  const int gr0 = t.i0 - 1, gc0 = t.j0 - 1;
-  // the region (halo included) inside the grid: no tests, every diagonal is 4
-  const bool interior = gr0 >= 0 && gr0 + RH <= m && gc0 >= 0 && gc0 + RW <= n;
-  const T* __restrict__ z = static_cast<const T*>(t.z);
-  const double* __restrict__ dold = din ? t.d1 : t.d0;
-  double* __restrict__ dnew = din ? t.d0 : t.d1;
-  // vector loads first; the scalar reduction below overlaps them
-  T zl[NRR][NCC];
-  double dl[NRR][NCC];
-  const uint8_t* __restrict__ mk = t.mask;
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      const int gi = gr0 + a, gj = gc0 + c;
-      bool in = a < RH && c < RW && (interior || (gi >= 0 && gi < m && gj >= 0 && gj < n));
-      // a masked domain holds its fixed points at zero: d (and so q's stencil) sees zeros
-      if (mk && in) in = mk[(size_t)gi * n + gj] != 0;
-      const size_t o = in ? (size_t)gi * pitch + gj : 0;
-      zl[rr][cc] = in ? z[o] : T(0);
-      dl[rr][cc] = in ? dold[o] : 0.0;
-    }
-  if (st[p].done) return;
-  const double2 z2 = reduce_slots<kThreads>(
-      zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
-  double beta;
-  if (st[p].iter == 0 || st[p].restart) beta = 0.0;
-  else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;
-  else beta = z2.x / st[p].delta_old;
-  if (t.lead && threadIdx.x == 0) {
-    st[p].delta = z2.x;
-    st[p].beta = beta;
-  }
-#pragma unroll
-  for (int rr = 0; rr < NRR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NCC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      if (a < RH && c < RW) sd[a * RW + c] = double(zl[rr][cc]) + beta * dl[rr][cc];
-    }
-  __syncthreads();
-  const bool neu = t.bc == kNeumann;
-  constexpr int NOR = (TH + 3) / 4, NOC = (TW + 63) / 64;
-  double acc = 0.0;
-#pragma unroll
-  for (int rr = 0; rr < NOR; ++rr)
-#pragma unroll
-    for (int cc = 0; cc < NOC; ++cc) {
-      const int a = wv + 4 * rr, c = lane + 64 * cc;
-      const int gi = t.i0 + a, gj = t.j0 + c;
-      if (a >= TH || c >= TW || gi >= m || gj >= n) continue;
-      const int idx = (a + 1) * RW + c + 1;
-      const double dc = sd[idx];
-      const double nb = ((sd[idx - RW] + sd[idx + RW]) + sd[idx - 1]) + sd[idx + 1];
-      const double dg = interior ? 4.0 : diag_of<double>(neu, gi, gj, m, n);
-      double q = -(nb - dg * dc) * t.inv_h2;
-      if (mk && !mk[(size_t)gi * n + gj]) q = 0.0;  // no equation at a fixed point
-      const size_t o = (size_t)gi * pitch + gj;
-      dnew[o] = dc;
-      t.q[o] = q;
-      acc += dc * q;
-    }
-  write_partial<kThreads>(acc, 0.0, apart, t.slot, sh);
-}
 
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ tiles,
@@ -487,7 +406,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
     ql[it] = il[it] ? t.q[ol[it]] : 0.0;
   }
   if (st[p].done) return;
-  const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.kbeg, t.kcnt}, sh, &bc);
+  const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, sh, &bc);
   const double delta = st[p].delta;
   const double alpha = delta / a2.x;
   if (t.lead && threadIdx.x == 0) {
@@ -511,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
 }
 
 // Jacobi preconditioner of the reference "cg" method (solvers.py:114-117):
-// z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply with beta = delta'/delta).
+// z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply_s with beta = delta'/delta).
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_jacobi(const KTile* __restrict__ tiles,
                                                      ProblemState* __restrict__ st,

@@ -24,6 +24,7 @@
 #include "gdp_device.h"
 #include "gdp_internal.h"
 #include "gdp_kernels.h"
+#include "gdp_apply.h"
 #include "gdpoisson.h"
 
 namespace gdp {
@@ -145,7 +146,7 @@ struct gdp_plan {
   size_t off_hist = 0, off_flag = 0;
   size_t off_tiles = 0;     // start of the tile tables region
   size_t tiles_bytes = 0;   // reserved bytes for the tile tables
-  Table down[kMaxDepth], up[kMaxDepth], kry;
+  Table down[kMaxDepth], up[kMaxDepth], kry, kapply;
   // runtime
   char* ws = nullptr;
   IoDesc* h_io = nullptr;
@@ -356,6 +357,11 @@ Shape2 menu_shape(int kind, int idx, int nu) {
   return kKryMenu[idx];
 }
 
+// Strip tiles of the PCG apply (k_apply_s) over an m x n grid.
+int apply_tiles(int m, int n) {
+  return cdiv(m, kApplyBlockRows) * std::max(1, cdiv(n - 1, kApplyCols));
+}
+
 // Worst case tile count of a grid over every entry of a menu.
 int max_tiles(int kind, int m, int n, int nu) {
   int best = 0;
@@ -543,9 +549,10 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   // tile tables (worst case over the menus)
   int slots = 1;
   size_t tiles = 0;
-  int kry_slots = 0, up0_slots = 0;
+  int kry_slots = 0, up0_slots = 0, apply_slots = 0;
   for (auto& pi : P->probs) {
     kry_slots += max_tiles(2, pi.lv[0].m, pi.lv[0].n, 0);
+    apply_slots += apply_tiles(pi.lv[0].m, pi.lv[0].n);
     if (pi.tail_first > 0)
       up0_slots += max_tiles(1, pi.lv[0].m, pi.lv[0].n, o.post_smoothing);
     for (int l = 0; l < pi.tail_first; ++l) {
@@ -554,13 +561,14 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
   }
   tiles += kry_slots;
-  slots = std::max({slots, kry_slots, up0_slots + P_});
+  slots = std::max({slots, kry_slots, apply_slots, up0_slots + P_});
   P->max_slots = slots;
   P->off_apart = take(sizeof(double2) * slots);
   P->off_upart = take(sizeof(double2) * slots);
   P->off_zpart = take(sizeof(double2) * slots);
   P->off_spart = take(sizeof(double2) * slots);
-  P->tiles_bytes = sizeof(VTile) * (tiles + 8 * (2 * P->depth + 1)) + 256 * (2 * P->depth + 2);
+  P->tiles_bytes = sizeof(VTile) * (tiles + 8 * (2 * P->depth + 1)) + 256 * (2 * P->depth + 2) +
+                   sizeof(KTile) * (apply_slots + 8) + 256;
   P->off_tiles = take(P->tiles_bytes);
   P->ws_bytes = off;
   *out = P;
@@ -797,15 +805,18 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     return GDP_OK;
   };
   // Krylov slot ranges (fixed by the Krylov tile shape)
-  std::vector<SlotRange> rk(NP), ru(NP);
+  std::vector<SlotRange> rk(NP), ru(NP), ra(NP);
   {
     const Shape2 s = kKryMenu[P->kry_menu];
-    int c = 0;
+    int c = 0, ca = 0;
     for (int p = 0; p < NP; ++p) {
       const auto& l0 = P->probs[p].lv[0];
       rk[p].begin = c;
       c += cdiv(l0.m, s.h) * cdiv(l0.n, s.w);
       rk[p].count = c - rk[p].begin;
+      ra[p].begin = ca;  // the strip apply's tiles, in the order the table lists them
+      ca += apply_tiles(l0.m, l0.n);
+      ra[p].count = ca - ra[p].begin;
     }
   }
   auto vtile = [&](int p, int l, int i0, int j0, int slot, int lead) {
@@ -921,12 +932,52 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
           t.kcnt = rk[p].count;
           t.zbeg = ru[p].begin;
           t.zcnt = ru[p].count;
+          t.abeg = ra[p].begin;
+          t.acnt = ra[p].count;
           t.mask = pi.mask;
           v.push_back(t);
         }
     }
+    // the strip apply's tiles: their own <d, q> slots, every other field as above
+    std::vector<KTile> va;
+    for (int p = 0; p < NP; ++p) {
+      const ProbInfo& pi = P->probs[p];
+      const auto& l0 = pi.lv[0];
+      for (int i = 0; i < l0.m; i += kApplyBlockRows)
+        for (int j = 1; j == 1 || j <= l0.n - 1; j += kApplyCols) {
+          KTile t;
+          memset(&t, 0, sizeof(t));
+          t.x = P->at<double>(pi.off_x);
+          t.r = P->at<double>(pi.off_r);
+          t.d0 = P->at<double>(pi.off_d0);
+          t.d1 = P->at<double>(pi.off_d1);
+          t.q = P->at<double>(pi.off_q);
+          t.z = P->ws + l0.off_u;
+          t.b0 = P->ws + l0.off_b;
+          t.inv_h2 = 1.0 / (pi.p.spacing * pi.p.spacing);
+          t.m = l0.m;
+          t.n = l0.n;
+          t.pitch = l0.pitch;
+          t.bc = pi.p.boundary;
+          t.problem = p;
+          t.i0 = i;
+          t.j0 = j;
+          t.slot = int(va.size());
+          t.lead = int(va.size()) == ra[p].begin;
+          t.kbeg = rk[p].begin;
+          t.kcnt = rk[p].count;
+          t.zbeg = ru[p].begin;
+          t.zcnt = ru[p].count;
+          t.abeg = ra[p].begin;
+          t.acnt = ra[p].count;
+          t.mask = pi.mask;
+          va.push_back(t);
+        }
+    }
     const auto sw = xcd_swizzle(v);
     if (int rc = upload(P->kry, sw.data(), sizeof(KTile), sw.size())) return rc;
+    const auto swa = xcd_swizzle(va);
+    if (int rc = upload(P->kapply, swa.data(), sizeof(KTile), swa.size())) return rc;
   }
   GDP_HIP(h2d(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP));
   GDP_HIP(h2d(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP));
@@ -1103,9 +1154,9 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
   // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
   {
     ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
-    GDP_KRY_DISPATCH(P->kry_menu,
-                     (k_apply<T, TH, TW><<<nb, kThreads, 0, s>>>(
-                         t, P->d_state(), P->d_zpart(), P->d_apart(), din, jacobi ? 1 : 0)));
+    k_apply_s<T><<<P->kapply.count, kThreads, 0, s>>>(P->at<KTile>(P->kapply.off), P->d_state(),
+                                                     P->d_zpart(), P->d_apart(), din,
+                                                     jacobi ? 1 : 0);
     GDP_CHECK_LAUNCH();
   }
   {

@@ -144,6 +144,41 @@ def serve_stream(count: int, qps: float, slots: int) -> dict:
     }
 
 
+def composite_leg(steps: int) -> dict:
+    """2048 x 2048 RGB seamless compositing of a blob-masked source (about 30% of the
+    pixels selected), mixed MG-PCG to 1e-8, by both routes: the rectangle (Dirichlet over
+    the mask's bounding box, poisson.py:337-347) and the mask domain (the unknowns are the
+    selected pixels, poisson.py:154-244).  Time per composite from device-resident images
+    to the device-resident result, right-hand side and plan set-up included."""
+    import torch
+
+    import gdpoisson
+    from gdpoisson import workloads as W
+
+    side = 2048
+    src, tgt, mask = W.compositing_images((side, side))
+    s, t, mk = (torch.from_numpy(x).cuda() for x in (src, tgt, mask))
+    out = {"image": [side, side, 3], "selected": round(float(mask.mean()), 4), "steps": steps}
+    for domain in ("rectangle", "mask"):
+        log(f"compositing ({domain} route)")
+        gdpoisson.seamless_clone(s, t, mk, (0, 0), domain=domain, method="mgcg_mixed")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            _, reps = gdpoisson.seamless_clone(s, t, mk, (0, 0), domain=domain,
+                                               method="mgcg_mixed")
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[domain] = {
+            "ms_per_composite": round(dt * 1e3, 2),
+            "MPix_per_s": round(side * side / 1e6 / dt, 1),
+            "iterations": [int(r.iterations) for r in reps],
+            "max_relative_residual": float(f"{max(r.relative_residual for r in reps):.3e}"),
+            "converged": all(r.converged and r.relative_residual <= 1e-8 for r in reps),
+        }
+    return out
+
+
 def _level_area(name: str, grid: int) -> int:
     """Grid size x rows per strip (the third template argument of the strip kernels):
     proportional to the level's area whatever tile shape the level was given."""
@@ -288,6 +323,8 @@ def main():
                     help="markdown table of measured vs algorithmic bytes per kernel")
     ap.add_argument("--serve-requests", type=int, default=120,
                     help="requests of the mixed serving stream (0: skip it)")
+    ap.add_argument("--composite-steps", type=int, default=3,
+                    help="2048x2048 RGB composites timed per route (0: skip them)")
     ap.add_argument("--serve-qps", type=float, default=200.0, help="arrival rate of the stream")
     ap.add_argument("--serve-slots", type=int, default=3, help="serving workers (HIP streams)")
     ap.add_argument("--nu", type=int, default=2,
@@ -466,6 +503,8 @@ def main():
         },
         "kernels": kernels,
     }
+    if args.composite_steps > 0:
+        line["compositing"] = composite_leg(args.composite_steps)
     if args.serve_requests > 0:
         line["serving"] = serve_stream(args.serve_requests, args.serve_qps, args.serve_slots)
         line["latency_ms"] = {

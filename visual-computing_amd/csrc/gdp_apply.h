@@ -17,7 +17,7 @@ constexpr int kApplyRows = 4;    // output rows per strip
 constexpr int kApplyCols = 126;  // output columns per strip
 constexpr int kApplyBlockRows = (kThreads / 64) * kApplyRows;  // one strip per wave
 
-template <typename T>
+template <typename T, bool MASK>
 __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ tiles,
                                                       ProblemState* __restrict__ st,
                                                       const double2* __restrict__ zpart,
@@ -40,31 +40,28 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
   const GDP_GLOBAL double* dold = (const GDP_GLOBAL double*)(din ? t.d1 : t.d0);
   GDP_GLOBAL double* dnew = (GDP_GLOBAL double*)(din ? t.d0 : t.d1);
   GDP_GLOBAL double* qv = (GDP_GLOBAL double*)t.q;
-  const uint8_t* __restrict__ mk = t.mask;
 
-  // loads first; the scalar reduction below overlaps them
-  TP zl[FH];
-  D2 dl[FH];
+  // Loads first, branch-free: pairs come from clamped addresses inside the padded rows
+  // (the pitch is even) and are zeroed outside the grid.  MASK (masked plans, a kernel of
+  // their own): mb holds the lane's unknowns, the in-grid points less the fixed points of
+  // a masked domain (all of them for a tile without a mask); the others stay at zero.
+  // The scalar reduction below overlaps the loads.
+  const int jc = xin ? gj : 0;
+  TP zv[FH];
+  D2 dv[FH];
 #pragma unroll
   for (int a = 0; a < FH; ++a) {
-    const int gi = r0 - 1 + a;
-    zl[a] = TP{T(0), T(0)};
-    dl[a] = D2{0.0, 0.0};
-    if (gi < 0 || gi >= m || !xin) continue;
-    const size_t o = (size_t)gi * pitch + gj;
-    if (yin) {
-      zl[a] = *(const GDP_GLOBAL TP*)(z + o);
-      dl[a] = *(const GDP_GLOBAL D2*)(dold + o);
-    } else {
-      zl[a].x = z[o];
-      dl[a].x = dold[o];
-    }
-    if (mk) {  // a masked domain holds its fixed points at zero
-      const uint8_t* row = mk + (size_t)gi * n;
-      if (!row[gj]) zl[a].x = T(0), dl[a].x = 0.0;
-      if (yin && !row[gj + 1]) zl[a].y = T(0), dl[a].y = 0.0;
-    }
+    const size_t o = (size_t)min(max(r0 - 1 + a, 0), m - 1) * pitch + jc;
+    zv[a] = *(const GDP_GLOBAL TP*)(z + o);
+    dv[a] = *(const GDP_GLOBAL D2*)(dold + o);
   }
+  uint64_t mb = 0;
+  if constexpr (MASK) mb = load_mask_bits<FH>(t.mask, r0 - 1, gj, m, n, true);
+  auto unknown = [&](int a, int y) {
+    if constexpr (MASK) return mask_bit(mb, 2 * a + y);
+    const int gi = r0 - 1 + a;
+    return gi >= 0 && gi < m && (y ? yin : xin);
+  };
   if (st[p].done) return;
   const double2 z2 = reduce_slots<kThreads>(
       zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
@@ -79,8 +76,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
   D2 dn[FH];
 #pragma unroll
   for (int a = 0; a < FH; ++a) {
-    dn[a].x = double(zl[a].x) + beta * dl[a].x;
-    dn[a].y = double(zl[a].y) + beta * dl[a].y;
+    dn[a].x = unknown(a, 0) ? double(zv[a].x) + beta * dv[a].x : 0.0;
+    dn[a].y = unknown(a, 1) ? double(zv[a].y) + beta * dv[a].y : 0.0;
   }
   const bool neu = t.bc == kNeumann;
   const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;
@@ -95,11 +92,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
     const double nby = ((dn[a - 1].y + dn[a + 1].y) + dn[a].x) + ry;
     double qx = -(nbx - diag_of<double>(neu, gi, gj, m, n) * dn[a].x) * t.inv_h2;
     double qy = -(nby - diag_of<double>(neu, gi, gj + 1, m, n) * dn[a].y) * t.inv_h2;
-    if (mk) {  // no equation at a fixed point
-      const uint8_t* row = mk + (size_t)gi * n;
-      if (xin && !row[gj]) qx = 0.0;
-      if (yin && !row[gj + 1]) qy = 0.0;
-    }
+    if (MASK && !mask_bit(mb, 2 * a)) qx = 0.0;  // no equation at a fixed point
+    if (MASK && !mask_bit(mb, 2 * a + 1)) qy = 0.0;
     const size_t o = (size_t)gi * pitch + gj;
     if (outx && outy) {
       *(GDP_GLOBAL D2*)(dnew + o) = dn[a];

@@ -1154,9 +1154,13 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
   // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
   {
     ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
-    k_apply_s<T><<<P->kapply.count, kThreads, 0, s>>>(P->at<KTile>(P->kapply.off), P->d_state(),
-                                                     P->d_zpart(), P->d_apart(), din,
-                                                     jacobi ? 1 : 0);
+    const KTile* at = P->at<KTile>(P->kapply.off);
+    if (has_mask(P))
+      k_apply_s<T, true><<<P->kapply.count, kThreads, 0, s>>>(at, P->d_state(), P->d_zpart(),
+                                                              P->d_apart(), din, jacobi ? 1 : 0);
+    else
+      k_apply_s<T, false><<<P->kapply.count, kThreads, 0, s>>>(at, P->d_state(), P->d_zpart(),
+                                                               P->d_apart(), din, jacobi ? 1 : 0);
     GDP_CHECK_LAUNCH();
   }
   {

@@ -126,14 +126,25 @@ __device__ inline bool mask_bit(uint64_t mb, int bit) { return (mb >> bit) & 1ul
 
 // One red-black half-sweep over rows 1 .. FH-2 of a strip.  Row a updates x when
 // (a + S + FLIP) is even, else y.  EDGE: rows may lie outside the grid or on its
-// first/last row.  MASK: points whose bit in mb is clear are fixed at zero (their inverse
-// diagonal is taken as zero, and so is their right-hand side).
+// first/last row.  MASK: the inverse diagonals come from wt (mask_weights), zero at fixed
+// points and outside the grid, where hb' is zero too, so those points stay at zero; the
+// sweep then needs no per-row branch.
 template <typename T, int FH, int S, int FLIP, bool EDGE, bool MASK = false>
 __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
                                        const LaneCols<T>& lc, int gr0, int m, bool neu,
-                                       uint64_t mb = 0) {
+                                       const typename Pair<T>::type* wt = nullptr) {
 #pragma unroll
   for (int a = 1; a < FH - 1; ++a) {
+    if constexpr (MASK) {
+      if (((a + S + FLIP) & 1) == 0) {
+        const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
+        u[a].x = wt[a].x * nb - hb[a].x;
+      } else {
+        const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
+        u[a].y = wt[a].y * nb - hb[a].y;
+      }
+      continue;
+    }
     int vc = 2;
     if (EDGE) {
       const int gi = gr0 + a;
@@ -142,13 +153,11 @@ __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>:
     }
     if (((a + S + FLIP) & 1) == 0) {
       const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
-      T w = EDGE ? lc.wx(vc) : lc.ix2;
-      if (MASK && !mask_bit(mb, 2 * a)) w = T(0);
+      const T w = EDGE ? lc.wx(vc) : lc.ix2;
       u[a].x = w * nb - hb[a].x;
     } else {
       const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
-      T w = EDGE ? lc.wy(vc) : lc.iy2;
-      if (MASK && !mask_bit(mb, 2 * a + 1)) w = T(0);
+      const T w = EDGE ? lc.wy(vc) : lc.iy2;
       u[a].y = w * nb - hb[a].y;
     }
   }
@@ -157,29 +166,58 @@ __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>:
 template <typename T, int FH, int S0, int NS, int FLIP, bool EDGE, bool MASK = false>
 __device__ __attribute__((always_inline)) void strip_sweeps(typename Pair<T>::type* u, const typename Pair<T>::type* hb,
                                     const LaneCols<T>& lc, int gr0, int m, bool neu,
-                                    uint64_t mb = 0) {
+                                    const typename Pair<T>::type* wt = nullptr) {
   if constexpr (NS > 0) {
-    strip_halfsweep<T, FH, S0, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
-    strip_sweeps<T, FH, S0 + 1, NS - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
+    strip_halfsweep<T, FH, S0, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, wt);
+    strip_sweeps<T, FH, S0 + 1, NS - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, wt);
+  }
+}
+
+// Inverse diagonals of the strip rows of a masked level: the grid's own (as LaneCols
+// gives them) at unknowns, zero at fixed points and outside the grid (whose mask bits
+// load_mask_bits leaves clear).
+template <typename T, int FH>
+__device__ __attribute__((always_inline)) void mask_weights(typename Pair<T>::type* wt, uint64_t mb,
+                                                            const LaneCols<T>& lc, int gr0, int m,
+                                                            bool neu) {
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const int vc = row_count(gr0 + a, m, neu);
+    wt[a].x = mask_bit(mb, 2 * a) ? lc.wx(vc) : T(0);
+    wt[a].y = mask_bit(mb, 2 * a + 1) ? lc.wy(vc) : T(0);
   }
 }
 
 // Unknown bits of the strip rows gr0 .. gr0 + FH - 1 at the lane's column pair (gjx,
-// gjx + 1) of a masked finest level (mask: m x n bytes).
+// gjx + 1) of a masked level (mask: m x n bytes, m, n >= 1; null: every grid point is an
+// unknown).  Every byte load is issued (at a clamped, in-bounds address) before any is
+// waited on; bits of rows or columns outside the grid are clear.
 template <int FH>
 __device__ inline uint64_t load_mask_bits(const uint8_t* __restrict__ mask, int gr0, int gjx,
                                           int m, int n, bool work) {
   static_assert(2 * FH <= 64, "mask bits of a strip must fit 64 bits");
-  uint64_t mb = 0;
+  const GDP_GLOBAL uint8_t* mk = (const GDP_GLOBAL uint8_t*)mask;
+  const int jx = min(max(gjx, 0), n - 1), jy = min(max(gjx + 1, 0), n - 1);
+  const bool xin = work && gjx >= 0 && gjx < n, yin = work && gjx + 1 >= 0 && gjx + 1 < n;
+  uint32_t bx[FH], by[FH];
 #pragma unroll
   for (int a = 0; a < FH; ++a) {
-    const int gi = gr0 + a;
-    if (!work || gi < 0 || gi >= m) continue;
-    const uint8_t* row = mask + (size_t)gi * n;
-    if (gjx >= 0 && gjx < n && row[gjx]) mb |= 1ull << (2 * a);
-    if (gjx + 1 >= 0 && gjx + 1 < n && row[gjx + 1]) mb |= 2ull << (2 * a);
+    bx[a] = by[a] = 1;
+    if (mk) {
+      const size_t row = (size_t)min(max(gr0 + a, 0), m - 1) * n;
+      bx[a] = mk[row + jx];
+      by[a] = mk[row + jy];
+    }
   }
-  return mb;
+  uint32_t lo = 0, hi = 0;
+#pragma unroll
+  for (int a = 0; a < FH; ++a) {
+    const bool rin = gr0 + a >= 0 && gr0 + a < m;
+    const uint32_t bits = ((rin && xin && bx[a]) ? 1u : 0u) | ((rin && yin && by[a]) ? 2u : 0u);
+    if (2 * a < 32) lo |= bits << (2 * a);
+    else hi |= bits << (2 * a - 32);
+  }
+  return ((uint64_t)hi << 32) | lo;
 }
 
 // ------------------------------------------------------------------ down
@@ -206,26 +244,29 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
   LaneCols<T> lc;
   lc.init(gc0, lane, n, neu);
   const T h2 = T(t.h2);
+  V2 wt[MASK ? FH : 1];
+  if constexpr (MASK) mask_weights<T, FH>(wt, mb, lc, gr0, m, neu);
 
   // hb' = h^2 b / diag (zero outside the grid) and the first colour-0 half-sweep from
   // the zero guess, u = -hb' on colour 0
   V2 u[FH];
 #pragma unroll
   for (int a = 0; a < FH; ++a) {
-    const int gi = gr0 + a;
-    int vc = 2;
-    bool rin = true;
-    if (EDGE) {
-      rin = gi >= 0 && gi < m;
-      vc = row_count(gi, m, neu);
-    }
-    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
-    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
-    hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
-    hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
-    if (MASK) {
-      if (!mask_bit(mb, 2 * a)) hb[a].x = T(0);
-      if (!mask_bit(mb, 2 * a + 1)) hb[a].y = T(0);
+    if constexpr (MASK) {
+      hb[a].x = mask_bit(mb, 2 * a) ? wt[a].x * h2 * hb[a].x : T(0);
+      hb[a].y = mask_bit(mb, 2 * a + 1) ? wt[a].y * h2 * hb[a].y : T(0);
+    } else {
+      const int gi = gr0 + a;
+      int vc = 2;
+      bool rin = true;
+      if (EDGE) {
+        rin = gi >= 0 && gi < m;
+        vc = row_count(gi, m, neu);
+      }
+      const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+      const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+      hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
+      hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
     }
     if (NU > 0 && ((a + FLIP) & 1) == 0) {
       u[a].x = -hb[a].x;
@@ -238,7 +279,7 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
     }
   }
   if constexpr (NU > 0)
-    strip_sweeps<T, FH, 1, 2 * NU - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
+    strip_sweeps<T, FH, 1, 2 * NU - 1, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, wt);
 
   // pre-smoothed iterate of the owned fine points (the last strip row / column of the
   // grid owns every remaining fine row / column)
@@ -375,14 +416,12 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
   const bool edge = gr0 < 1 || gr0 + FH > m - 1;
   const int flip = (gr0 + gc0) & 1;
   if constexpr (MASK) {
-    // masked finest level (a kernel of its own, so the unmasked kernels keep their
-    // registers): the general body with fixed points held at zero
-    if (t.mask) {
-      const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
-      if (flip) sdown_body<T, NU, CR, 1, true, true>(t, I0, lane, hb, mb);
-      else sdown_body<T, NU, CR, 0, true, true>(t, I0, lane, hb, mb);
-      return;
-    }
+    // masked levels (kernels of their own, so the unmasked kernels keep their
+    // registers): one body for every strip, fixed points held at zero
+    const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
+    if (flip) sdown_body<T, NU, CR, 1, true, true>(t, I0, lane, hb, mb);
+    else sdown_body<T, NU, CR, 0, true, true>(t, I0, lane, hb, mb);
+    return;
   }
   if (edge) {
     if (flip) sdown_body<T, NU, CR, 1, true>(t, I0, lane, hb);
@@ -412,6 +451,8 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
   LaneCols<T> lc;
   lc.init(gc0, lane, n, neu);
   const T h2 = T(t.h2);
+  V2 wt[MASK ? FH : 1];
+  if constexpr (MASK) mask_weights<T, FH>(wt, mb, lc, gr0, m, neu);
 
   // u = v + P e (zero outside the grid); hb' = h^2 b / diag
 #pragma unroll
@@ -437,16 +478,19 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
     }
     u[a].x = (rin && lc.inx) ? u[a].x + cx : T(0);
     u[a].y = (rin && lc.iny) ? u[a].y + cy : T(0);
-    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
-    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
-    hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
-    hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
-    if (MASK) {
-      if (!mask_bit(mb, 2 * a)) u[a].x = hb[a].x = T(0);
-      if (!mask_bit(mb, 2 * a + 1)) u[a].y = hb[a].y = T(0);
+    if constexpr (MASK) {
+      if (!mask_bit(mb, 2 * a)) u[a].x = T(0);
+      if (!mask_bit(mb, 2 * a + 1)) u[a].y = T(0);
+      hb[a].x = mask_bit(mb, 2 * a) ? wt[a].x * h2 * hb[a].x : T(0);
+      hb[a].y = mask_bit(mb, 2 * a + 1) ? wt[a].y * h2 * hb[a].y : T(0);
+    } else {
+      const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+      const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+      hb[a].x = (rin && lc.inx) ? wx * h2 * hb[a].x : T(0);
+      hb[a].y = (rin && lc.iny) ? wy * h2 * hb[a].y : T(0);
     }
   }
-  strip_sweeps<T, FH, 0, 2 * NU, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, mb);
+  strip_sweeps<T, FH, 0, 2 * NU, FLIP, EDGE, MASK>(u, hb, lc, gr0, m, neu, wt);
 
   GDP_GLOBAL T* uo = (GDP_GLOBAL T*)t.u;
   const int fc1 = min(n, t.j0 + TC);
@@ -531,16 +575,19 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
       }
       E[k] = in ? e[(size_t)I * t.cpitch + K] : T(0);
     }
+    // branch-free: every pair is loaded from a clamped address (inside the grid's padded
+    // rows: the pitch is even) and zeroed outside the grid, so the strip arrays are not
+    // copied at each conditional load
+    const int jc = lin ? gjx : 0;
 #pragma unroll
     for (int a = 0; a < FH; ++a) {
       const int gi = gr0 + a;
-      u[a] = V2{T(0), T(0)};
-      hb[a] = V2{T(0), T(0)};
-      if (work && lin && gi >= 0 && gi < m) {
-        const size_t o = (size_t)gi * t.pitch + gjx;
-        u[a] = *(const GDP_GLOBAL V2*)(vv + o);
-        hb[a] = *(const GDP_GLOBAL V2*)(b + o);
-      }
+      const bool in = work && lin && gi >= 0 && gi < m;
+      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jc;
+      const V2 uv = *(const GDP_GLOBAL V2*)(vv + o);
+      const V2 bv = *(const GDP_GLOBAL V2*)(b + o);
+      u[a] = in ? uv : V2{T(0), T(0)};
+      hb[a] = in ? bv : V2{T(0), T(0)};
     }
   }
   if (st[t.problem].done) return;
@@ -548,17 +595,12 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
   if (work) {
     const bool edge = gr0 < 1 || gr0 + FH > m - 1;
     const int flip = ((gr0 + gc0) + order) & 1;
-    bool masked = false;
     if constexpr (MASK) {
-      // masked finest level (a kernel of its own): fixed points held at zero
-      if (t.mask) {
-        const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
-        if (flip) sup_body<T, NU, TR, 1, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
-        else sup_body<T, NU, TR, 0, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
-        masked = true;
-      }
-    }
-    if (masked) {
+      // masked levels (kernels of their own): one body for every strip, fixed points
+      // held at zero
+      const uint64_t mb = load_mask_bits<FH>(t.mask, gr0, gc0 + 2 * lane, m, n, true);
+      if (flip) sup_body<T, NU, TR, 1, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
+      else sup_body<T, NU, TR, 0, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
     } else if (edge) {
       if (flip) sup_body<T, NU, TR, 1, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
       else sup_body<T, NU, TR, 0, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);

@@ -855,8 +855,6 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     return t;
   };
   if (has_mask(P)) {
-    // the masked level kernels exist for the first strip height only
-    for (int l = 0; l < kMaxDepth; ++l) P->down_menu[l] = P->up_menu[l] = 0;
     // coarse masks by injection: coarse point (I, J) is an unknown when its coincident
     // fine point (2I + 1, 2J + 1) is
     for (const ProbInfo& pi : P->probs) {

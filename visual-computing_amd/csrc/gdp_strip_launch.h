@@ -19,11 +19,11 @@ void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #undef GDP_DOWN
 }
 
-// Masked finest level: one strip height (menu entry 0), a kernel of its own.
-template <typename T>
+// Masked levels: kernels of their own, every strip height of the menu.
+template <typename T, int CR>
 void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWN(NU) \
-  k_sdown<T, NU, 1, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
+  k_sdown<T, NU, CR, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
   switch (nu) {
     case 0: GDP_DOWN(0); break;
     case 1: GDP_DOWN(1); break;
@@ -35,7 +35,15 @@ void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 
 template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
-  if (a.masked) return down_masked<T>(nu, blocks, s, a);
+  if (a.masked) {
+    switch (menu) {
+      case 0: down_masked<T, 1>(nu, blocks, s, a); break;
+      case 1: down_masked<T, 2>(nu, blocks, s, a); break;
+      case 2: down_masked<T, 4>(nu, blocks, s, a); break;
+      default: down_masked<T, 8>(nu, blocks, s, a); break;
+    }
+    return;
+  }
   switch (menu) {
     case 0: down_nu<T, 1>(nu, blocks, s, a); break;
     case 1: down_nu<T, 2>(nu, blocks, s, a); break;
@@ -67,10 +75,10 @@ void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
   }
 }
 
-template <typename T, bool DOTS>
+template <typename T, int TR, bool DOTS>
 void up_masked(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU) \
-  k_sup<T, NU, 2, DOTS, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
+  k_sup<T, NU, TR, DOTS, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;
@@ -80,11 +88,21 @@ void up_masked(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #undef GDP_UP
 }
 
+template <typename T, bool DOTS>
+void launch_up_masked(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
+  switch (menu) {
+    case 0: up_masked<T, 2, DOTS>(nu, blocks, s, a); break;
+    case 1: up_masked<T, 4, DOTS>(nu, blocks, s, a); break;
+    case 2: up_masked<T, 8, DOTS>(nu, blocks, s, a); break;
+    default: up_masked<T, 16, DOTS>(nu, blocks, s, a); break;
+  }
+}
+
 template <typename T>
 void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a) {
   if (a.masked) {
-    if (dots) up_masked<T, true>(nu, blocks, s, a);
-    else up_masked<T, false>(nu, blocks, s, a);
+    if (dots) launch_up_masked<T, true>(menu, nu, blocks, s, a);
+    else launch_up_masked<T, false>(menu, nu, blocks, s, a);
     return;
   }
   if (dots) launch_up_d<T, true>(menu, nu, blocks, s, a);

@@ -100,3 +100,37 @@ def test_masked_compositing_at_full_size(gd):
           "fp64", [r.iterations for r in reps64])
     assert all(r.converged and r.relative_residual <= 1e-8 for r in reps + reps64)
     assert torch.abs(mixed - fp64).max().item() < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_masked_and_unmasked_problems_in_one_plan(gd, precision):
+    """A plan holding masked and unmasked Dirichlet problems of different shapes: every
+    masked problem is bit for bit its solve alone, and the unmasked one (run by the masked
+    level kernels with every grid point an unknown) agrees with its solve by the unmasked
+    kernels."""
+    import torch
+
+    from gdpoisson.plan import Plan, PlanOptions
+
+    rng = np.random.default_rng(7)
+    shapes = [(70, 90), (64, 64), (50, 130)]
+    masks = [torch.from_numpy(MASKS["blobs"](shapes[0]).astype(np.uint8)).cuda(), None,
+             torch.from_numpy(_disc(shapes[2], 20.0).astype(np.uint8)).cuda()]
+    bs = []
+    for s, mk in zip(shapes, masks):
+        b = torch.from_numpy(rng.standard_normal(s)).cuda()
+        bs.append(b * mk if mk is not None else b)  # no equation at the fixed points
+    opts = PlanOptions(precision=precision)
+    tol = opts.tolerance
+    us, reps = Plan(shapes, "dirichlet", 1.0, opts, masks=masks).mgcg(bs)
+    for k, (s, mk) in enumerate(zip(shapes, masks)):
+        alone = Plan([s], "dirichlet", 1.0, opts, masks=[mk] if mk is not None else None)
+        (ua,), (ra,) = alone.mgcg([bs[k]])
+        assert reps[k].converged and reps[k].relative_residual <= tol
+        assert ra.converged and ra.relative_residual <= tol
+        if mk is not None:
+            assert torch.equal(us[k], ua), k
+            assert torch.count_nonzero(us[k][mk == 0]).item() == 0  # fixed points stay zero
+        else:
+            scale = ua.abs().max().item()
+            assert (us[k] - ua).abs().max().item() <= 1e-6 * scale

@@ -429,6 +429,18 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
   write_partial<kThreads>(acc, 0.0, upart, t.slot, sh);
 }
 
+// Convergence test of an update whose preconditioner is not run (the last update of the
+// first batch of a solve, where the test usually ends the solve): one block per problem.
+__global__ __launch_bounds__(kThreads) void k_check_conv(ProblemState* __restrict__ st,
+                                                         const double2* __restrict__ upart,
+                                                         const SlotRange* __restrict__ rk) {
+  const int p = blockIdx.x;
+  if (st[p].done) return;
+  __shared__ double sh[2 * kThreads / 64];
+  __shared__ double2 bc;
+  check_converged<kThreads>(st, p, upart, rk[p], true, sh, &bc);
+}
+
 // Jacobi preconditioner of the reference "cg" method (solvers.py:114-117):
 // z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply_s with beta = delta'/delta).
 template <typename T, int TH, int TW>

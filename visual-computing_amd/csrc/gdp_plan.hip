@@ -1144,8 +1144,11 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
   return GDP_OK;
 }
 
+// One PCG iteration: direction and operator, update, preconditioner.  last: the
+// preconditioner is left out and only the update's convergence test runs (the host runs
+// the preconditioner before continuing if a problem is still open).
 template <typename T>
-int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
+int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool last) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
@@ -1168,18 +1171,24 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi) {
                          t, P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
     GDP_CHECK_LAUNCH();
   }
+  if (last) {
+    ProfScope ps(P, s, "conv test", 16.0 * P->kry.count);
+    k_check_conv<<<P->count(), kThreads, 0, s>>>(P->d_state(), P->d_upart(), P->d_rkry());
+    GDP_CHECK_LAUNCH();
+    return GDP_OK;
+  }
   if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
   return enqueue_vcycle_t<T>(P, s, true, true);
 }
 
-int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi) {
+int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false) {
   int rc;
   if (P->fp32) {
-    if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi))) return rc;
-    return enqueue_pcg_iter_t<float>(P, s, 1, jacobi);
+    if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi, false))) return rc;
+    return enqueue_pcg_iter_t<float>(P, s, 1, jacobi, last);
   }
-  if ((rc = enqueue_pcg_iter_t<double>(P, s, 0, jacobi))) return rc;
-  return enqueue_pcg_iter_t<double>(P, s, 1, jacobi);
+  if ((rc = enqueue_pcg_iter_t<double>(P, s, 0, jacobi, false))) return rc;
+  return enqueue_pcg_iter_t<double>(P, s, 1, jacobi, last);
 }
 
 struct ScalarArgs {
@@ -1390,7 +1399,10 @@ int enqueue_check(gdp_plan* P, hipStream_t s, bool allow_reopen) {
 
 // The first batch of a solve as one graph: set-up, first preconditioning, `pairs`
 // iteration pairs and the check (one graph launch instead of pairs + 1, each of which
-// leaves the device idle for several microseconds).  Kept per pair count (at most 4).
+// leaves the device idle for several microseconds).  Its last iteration ends with the
+// convergence test instead of a preconditioner: the batch is sized from the last solve,
+// so that test usually ends the solve, and a V-cycle after it would only be skipped
+// launch by launch.  Kept per pair count (at most 4).
 int launch_head(gdp_plan* P, hipStream_t s, bool jacobi, int pairs) {
   auto& v = P->head_exec[jacobi ? 1 : 0];
   size_t at = v.size();
@@ -1409,7 +1421,7 @@ int launch_head(gdp_plan* P, hipStream_t s, bool jacobi, int pairs) {
     int rc;
     if ((rc = enqueue_start(P, cs, jacobi))) return rc;
     for (int k = 0; k < pairs; ++k)
-      if ((rc = enqueue_pcg_pair(P, cs, jacobi))) return rc;
+      if ((rc = enqueue_pcg_pair(P, cs, jacobi, k == pairs - 1))) return rc;
     return enqueue_check(P, cs, true);
   });
 }
@@ -1431,14 +1443,18 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   if ((rc = launch_head(P, s, jacobi, first_pairs))) return rc;
   if ((rc = read_state(P, s, true))) return rc;
   int restarts = 0;
+  bool precondition = true;  // the first batch left its last preconditioner out
   while (!all_checked(P)) {
-    if (P->h_flag[0] > 0) {
-      // reopened problems restart from their true residual
-      ++restarts;
+    if (P->h_flag[0] > 0) ++restarts;
+    if (P->h_flag[0] > 0 || precondition) {
+      // z = M r of the open problems: those the first batch left before their
+      // preconditioner, and reopened ones, which restart from their true residual (no
+      // convergence test: the recursive residual they stopped on is stale)
       if (jacobi) rc = GDP_T(enqueue_jacobi_t)(P, s, false);
       else rc = enqueue_vcycle(P, s, true, false);
       if (rc) return rc;
     }
+    precondition = false;
     if ((rc = launch_pair(P, s, jacobi))) return rc;
     if ((rc = enqueue_check(P, s, restarts < 4))) return rc;
     if ((rc = read_state(P, s, true))) return rc;

@@ -197,12 +197,14 @@ def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
     base = name.split("(")[0]
     for key, kind in (("k_sdown<", "down"), ("k_sup<", "up"), ("k_stail<", "tail"),
                       ("k_down<", "down"), ("k_up<", "up"), ("k_tail<", "tail"),
-                      ("k_apply<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
+                      ("k_apply_s<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
                       ("k_residual<", "residual"), ("k_finalize<", "finalize"),
                       ("k_init_stats<", "init_stats")):
         if key in base:
             if kind in ("down", "up"):
-                dots = kind == "up" and ", true>" in base
+                # k_sup<T, nu, rows, dots, masked>
+                args = [a.strip(" >") for a in base[base.index("<") + 1:].split(",")]
+                dots = kind == "up" and args[3] == "true"
                 lv = sorted(grids[key], reverse=True).index(_level_area(name, grid))
                 return f"{kind} L{lv}" + ("+dots" if dots else "")
             return kind

@@ -208,6 +208,26 @@ def test_neumann_mgcg_matches_closed_form(gd, precision, shape):
     assert np.abs(u - ou).max() < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("width", [1, 2, 3, 126, 127, 128, 129, 253, 254])
+def test_pcg_at_strip_boundary_widths(gd, width, bc, precision):
+    """Widths at the edges of the Krylov apply's strips (126 owned columns, the first strip
+    also owning column 0, neighbours overlapping by two) and a height that is no multiple
+    of its 4-row strips: the solve recovers a random discrete solution."""
+    from oracle.operators import divergence, gradient, laplacian
+
+    shape = (9, width)
+    x = np.random.default_rng(width).standard_normal(shape)
+    b = laplacian(x) if bc == "dirichlet" else divergence(*gradient(x))
+    method = "mgcg_mixed" if precision == "mixed" else "mgcg"
+    u, rep = gd.solve_system(b, method=method, tolerance=1e-11, boundary=bc)
+    assert rep.converged and rep.relative_residual <= 1e-11
+    if bc == "neumann":
+        u, x = u - u.mean(), x - x.mean()
+    assert np.abs(u - x).max() < 1e-6
+
+
 @pytest.mark.parametrize("shape", [(96, 130), (64, 64)])
 def test_solve_neumann_matches_the_dct_oracle(gd, shape):
     """solvers.py:224-262 contract: zero-mean solution, constant part of b discarded."""

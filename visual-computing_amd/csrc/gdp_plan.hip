@@ -134,6 +134,10 @@ struct gdp_plan {
   int down_menu[kMaxDepth];
   int up_menu[kMaxDepth];
   int kry_menu = 1;
+  // coarsest sweeps of the problems without an exact coarsest solve (the option's count,
+  // or the reference's 60 where the option asked for exact solves and a problem's
+  // tables did not fit)
+  int csweeps = 0;
   int total_levels = 0;
   int tail_smem = 0;   // dynamic LDS of the LDS tail (k_tail)
   int stail_smem = 0;  // dynamic LDS of the register-strip tail (k_stail)
@@ -437,6 +441,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     P->up_menu[i] = i == 0 ? 2 : 1;
   }
   P->kry_menu = 1;
+  P->csweeps = o.coarsest_sweeps > 0 ? o.coarsest_sweeps : 60;
   int lvl_total = 0;
   for (int i = 0; i < count; ++i) {
     const gdp_problem& pr = problems[i];
@@ -462,26 +467,36 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
     const int L = int(pi.lv.size());
     int first = -1;
-    // the register-strip tail takes the first level at most 128 columns wide and
-    // stail_max_rows rows tall whose chain to the coarsest fits its level count and LDS
-    for (int f = 0; f < L; ++f) {
-      const auto& c = pi.lv[L - 1];
-      const int bytes = stail_lds_bytes(c.m, c.n, pi.exact);
-      if (pi.lv[f].m <= stail_max_rows(P->fp32) && pi.lv[f].n <= kStailMaxCols &&
-          L - f <= kStailMaxLevels && bytes <= kTailLdsBytes) {
-        first = f;
-        pi.stail = true;
-        pi.tail_bytes = bytes;
-        break;
+    auto place_tail = [&]() {
+      // the register-strip tail takes the first level at most 128 columns wide and
+      // stail_max_rows rows tall whose chain to the coarsest fits its level count and LDS
+      for (int f = 0; f < L; ++f) {
+        const auto& c = pi.lv[L - 1];
+        const int bytes = stail_lds_bytes(c.m, c.n, pi.exact);
+        if (pi.lv[f].m <= stail_max_rows(P->fp32) && pi.lv[f].n <= kStailMaxCols &&
+            L - f <= kStailMaxLevels && bytes <= kTailLdsBytes) {
+          first = f;
+          pi.stail = true;
+          pi.tail_bytes = bytes;
+          return;
+        }
       }
-    }
-    for (int f = 0; f < L && first < 0; ++f) {
-      const size_t bytes = tail_layout(pi, f, P->tsize);
-      if (bytes <= size_t(kTailLdsBytes)) {
-        first = f;
-        pi.tail_bytes = int(bytes);
-        break;
+      for (int f = 0; f < L; ++f) {
+        const size_t bytes = tail_layout(pi, f, P->tsize);
+        if (bytes <= size_t(kTailLdsBytes)) {
+          first = f;
+          pi.tail_bytes = int(bytes);
+          return;
+        }
       }
+    };
+    place_tail();
+    if (first < 0 && pi.exact) {
+      // a coarsest level whose transform tables overflow LDS (a thin grid: its short
+      // side stops the coarsening while its long side is still long) is solved by
+      // red-black sweeps instead, the reference's coarsest rule (multigrid.py:206-207)
+      pi.exact = false;
+      place_tail();
     }
     if (first < 0) {
       set_error("problem %d (%d x %d): coarsest level %d x %d does not fit in LDS", i, pr.rows,
@@ -1095,7 +1110,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     if (ns) {
       TailArgs a{P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
                  P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
-                 o.post_smoothing, o.coarsest_sweeps, order};
+                 o.post_smoothing, P->csweeps, order};
       launch_stail<T>(dots, ns, P->stail_smem, s, a);
       GDP_CHECK_LAUNCH();
     }
@@ -1104,12 +1119,12 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
         k_tail<T, true><<<no, kTailThreads, P->tail_smem, s>>>(
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
             P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
-            o.coarsest_sweeps, order);
+            P->csweeps, order);
       else
         k_tail<T, false><<<no, kTailThreads, P->tail_smem, s>>>(
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
             P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
-            o.coarsest_sweeps, order);
+            P->csweeps, order);
       GDP_CHECK_LAUNCH();
     }
   }

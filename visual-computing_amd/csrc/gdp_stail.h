@@ -154,6 +154,14 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
     row[gjx + 1] = val.x;
     row[gjx + 2] = val.y;
     if (c.lane == 0) row[0] = ghost == kGhostZero ? T(0) : val.x;
+    if (c.lane == 63) {
+      // exchange columns 129 .. 131 (grid columns 128 .. 130), past every lane's pair:
+      // the ghost column n of a 128-wide level (its edge value, like the ghosts above)
+      // and zeros, which the Neumann restriction of a 127- or 128-wide level reads
+      row[129] = (ghost != kGhostZero && n == kSTailCols) ? val.y : T(0);
+      row[130] = T(0);
+      row[131] = T(0);
+    }
   }
   __syncthreads();
 }

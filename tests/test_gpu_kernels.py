@@ -228,6 +228,28 @@ def test_pcg_at_strip_boundary_widths(gd, width, bc, precision):
     assert np.abs(u - x).max() < 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("shape", [(2, 2), (3, 5), (40, 128), (128, 40), (64, 128), (65, 129),
+                                   (63, 127), (33, 128), (200, 9), (9, 200), (1, 300),
+                                   (300, 1), (3, 1000)])
+def test_pcg_on_ragged_shapes(gd, shape, bc, precision):
+    """Shapes at the edges of the coarse tail's placement (register tail: at most 64 rows
+    in FP32, 32 in FP64, 128 columns; the LDS tail otherwise), thin grids both ways and
+    thin long grids whose exact coarsest solve does not fit (coarsest sweeps instead):
+    the solve recovers a random discrete solution."""
+    from oracle.operators import divergence, gradient, laplacian
+
+    x = np.random.default_rng(shape[0] * 7919 + shape[1]).standard_normal(shape)
+    b = laplacian(x) if bc == "dirichlet" else divergence(*gradient(x))
+    method = "mgcg_mixed" if precision == "mixed" else "mgcg"
+    u, rep = gd.solve_system(b, method=method, tolerance=1e-11, boundary=bc)
+    assert rep.converged and rep.relative_residual <= 1e-11
+    if bc == "neumann":
+        u, x = u - u.mean(), x - x.mean()
+    assert np.abs(u - x).max() < 1e-6
+
+
 @pytest.mark.parametrize("shape", [(96, 130), (64, 64)])
 def test_solve_neumann_matches_the_dct_oracle(gd, shape):
     """solvers.py:224-262 contract: zero-mean solution, constant part of b discarded."""

@@ -160,7 +160,7 @@ int gdp_tm_downsample(const double* in, int h, int w, double* out, void* stream)
 int gdp_tm_level_factor(const double* level, int h, int w, double spacing, double alpha,
                         double beta, double epsilon, const double* coarse, int hc, int wc,
                         double* factor, double* work, void* stream);
-/* rhs = divergence(gx * factor, gy * factor), (gx, gy) = gradient(log_lum); h, w >= 2. */
+/* rhs = divergence(gx * factor, gy * factor), (gx, gy) = gradient(log_lum); h, w >= 1. */
 int gdp_tm_rhs(const double* log_lum, const double* factor, int h, int w, double* rhs,
                void* stream);
 /* image (h x w x 3) = (radiance / luminance)^saturation * exp(compressed - max compressed). */

@@ -145,3 +145,15 @@ def test_tone_map_of_a_serving_request(gd):
     assert reports[0].converged and reports[0].relative_residual <= 1e-8
     want, _ = H.tone_map(rad)
     assert torch.abs(img.cpu() - torch.from_numpy(want)).max().item() < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 6), (7, 1), (2, 3), (5, 17), (17, 5), (33, 64)])
+def test_tone_map_of_small_and_odd_images(gd, shape):
+    """Tiny and odd-sized radiance maps (pyramids of one or two levels, one-pixel
+    solves) against the oracle."""
+    rad = G.radiance_map(shape, seed=7)
+    img, f = gd.tone_map(rad)
+    want, wf = H.tone_map(rad)
+    assert img.shape == want.shape
+    assert rel(f, wf) < 1e-12
+    assert np.abs(img - want).max() < 1e-10

@@ -134,3 +134,32 @@ def test_masked_and_unmasked_problems_in_one_plan(gd, precision):
         else:
             scale = ua.abs().max().item()
             assert (us[k] - ua).abs().max().item() <= 1e-6 * scale
+
+
+def test_masked_solve_of_an_empty_selection_keeps_the_image(gd):
+    """No selected pixel off the border (poisson.py:183-184): the image comes back, no
+    reports."""
+    rng = np.random.default_rng(5)
+    img = rng.random((20, 30, 3))
+    mask = np.zeros((20, 30), dtype=bool)
+    mask[0, :] = mask[:, -1] = True  # border pixels only
+    field = gd.GuidanceField(np.zeros_like(img), np.zeros_like(img))
+    got, reps = gd.solve_masked(field, img, mask)
+    assert reps == [] and np.array_equal(got, img)
+
+
+@pytest.mark.parametrize("method", ["mgcg", "mgcg_mixed"])
+@pytest.mark.parametrize("where", [(1, 1), (10, 15), (18, 28)])
+def test_masked_solve_of_a_single_pixel(gd, method, where):
+    """One unknown (at the border's inner corner, inside, at the far inner corner): the
+    value that satisfies its equation, as SuperLU gives it."""
+    rng = np.random.default_rng(6)
+    img = rng.random((20, 30))
+    gx, gy = rng.standard_normal((20, 30)) * 0.1, rng.standard_normal((20, 30)) * 0.1
+    mask = np.zeros((20, 30), dtype=bool)
+    mask[where] = True
+    got, reps = gd.solve_masked(gd.GuidanceField(gx, gy), img, mask, method=method)
+    want, _ = OP.solve_masked(OP.GuidanceField(gx, gy), img, mask)
+    assert len(reps) == 1 and reps[0].converged
+    assert np.abs(got - want).max() < 1e-9
+    assert np.array_equal(got[~mask], img[~mask])

@@ -174,13 +174,19 @@ __global__ __launch_bounds__(kHdrThreads) void k_tm_rhs(const double* __restrict
     const size_t o = (size_t)a * w + b;
     return (L[o + w] - L[o]) * f[o];
   };
-  double dx, dy;
-  if (j == 0) dx = fx(i, 0);
-  else if (j == w - 1) dx = -fx(i, w - 2);
-  else dx = fx(i, j) - fx(i, j - 1);
-  if (i == 0) dy = fy(0, j);
-  else if (i == h - 1) dy = -fy(h - 2, j);
-  else dy = fy(i, j) - fy(i - 1, j);
+  // a one-column (one-row) image has no horizontal (vertical) differences: that term is
+  // zero (operators.py:77-84)
+  double dx = 0.0, dy = 0.0;
+  if (w > 1) {
+    if (j == 0) dx = fx(i, 0);
+    else if (j == w - 1) dx = -fx(i, w - 2);
+    else dx = fx(i, j) - fx(i, j - 1);
+  }
+  if (h > 1) {
+    if (i == 0) dy = fy(0, j);
+    else if (i == h - 1) dy = -fy(h - 2, j);
+    else dy = fy(i, j) - fy(i - 1, j);
+  }
   rhs[t] = dx + dy;
 }
 
@@ -377,8 +383,7 @@ int gdp_tm_level_factor(const double* level, int h, int w, double spacing, doubl
 
 int gdp_tm_rhs(const double* log_lum, const double* factor, int h, int w, double* rhs,
                void* stream) {
-  if (!log_lum || !factor || !rhs || h < 2 || w < 2)
-    return (set_error("bad arguments (the image needs at least 2 x 2 pixels)"), GDP_EINVAL);
+  if (!log_lum || !factor || !rhs || h < 1 || w < 1) return (set_error("bad arguments"), GDP_EINVAL);
   k_tm_rhs<<<grid_for((size_t)h * w), kHdrThreads, 0, (hipStream_t)stream>>>(log_lum, factor, h,
                                                                               w, rhs);
   GDP_HDR_HIP(hipGetLastError());

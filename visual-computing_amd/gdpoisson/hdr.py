@@ -113,8 +113,8 @@ def prepare_tone_map(radiance, config: ToneMapConfig | None = None) -> ToneJob:
     if rad.ndim != 3 or rad.shape[2] != 3:
         raise ValueError("tone mapping expects a three-channel radiance map")
     h, w = int(rad.shape[0]), int(rad.shape[1])
-    if h < 2 or w < 2:
-        raise ValueError("tone mapping needs at least 2 x 2 pixels")
+    if h < 1 or w < 1:
+        raise ValueError("tone mapping needs a non-empty radiance map")
     lib, s, dev = _lib.lib(), _stream(), rad.device
     work = _work_tensor(h, w, dev)
     L = torch.empty((h, w), dtype=torch.float64, device=dev)

@@ -154,7 +154,7 @@ def placed_mask(source, target, mask, offset=(0, 0)):
 
 
 def seamless_clone(source, target, mask, offset=(0, 0), mode="import", method="direct",
-                   margin=24, tolerance=1e-8, domain="rectangle"):
+                   margin=24, tolerance=1e-8, domain="mask"):
     target = np.asarray(target, dtype=float)
     field, win = clone_window(source, target, mask, offset, mode, margin)
     if domain == "mask":

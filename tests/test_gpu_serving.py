@@ -24,8 +24,9 @@ def gd():
 
 
 def _requests():
-    """Six compositing requests (squares cut from 256 x 256 blob images and a blob mask)
-    and four small tone-mapping requests, all arriving at once."""
+    """Six compositing requests (squares cut from 256 x 256 blob images and a blob mask;
+    two by the mask route, batched with the rectangle ones in one plan) and four small
+    tone-mapping requests, all arriving at once."""
     import torch
 
     from gdpoisson.serving import Request
@@ -39,7 +40,8 @@ def _requests():
         r0, c0 = (int(v) for v in rng.integers(0, 256 - side + 1, 2))
         win = (slice(r0, r0 + side), slice(c0, c0 + side))
         reqs.append(Request("composite", {"source": s[win], "target": t[win], "mask": m[win],
-                                          "offset": (0, 0), "mode": ["import", "mixed"][k % 2]}))
+                                          "offset": (0, 0), "mode": ["import", "mixed"][k % 2],
+                                          "domain": "mask" if k % 3 == 2 else "rectangle"}))
     for k, shape in enumerate([(96, 128), (120, 90), (96, 128), (64, 200)]):
         reqs.append(Request("tonemap", {
             "radiance": torch.from_numpy(G.radiance_map(shape, seed=40 + k)).cuda()}))
@@ -72,7 +74,8 @@ def test_served_results_match_the_oracle(gd):
         a = r.args
         if r.kind == "composite":
             want, _ = OP.seamless_clone(a["source"].cpu().numpy(), a["target"].cpu().numpy(),
-                                        a["mask"].cpu().numpy(), (0, 0), mode=a["mode"])
+                                        a["mask"].cpu().numpy(), (0, 0), mode=a["mode"],
+                                        domain=a["domain"])
         else:
             want, _ = H.tone_map(a["radiance"].cpu().numpy())
         assert np.abs(c.output.cpu().numpy() - want).max() < 1e-6, (r.kind, c.rid)

@@ -217,7 +217,7 @@ class CloneJob:
 
 
 def prepare_clone(source, target, mask, offset=(0, 0), mode: str = "import",
-                  domain: str = "rectangle", margin: int = 24) -> CloneJob:
+                  domain: str = "mask", margin: int = 24) -> CloneJob:
     """Validation, window and fused right-hand side of seamless_clone (device work on
     the current stream; one small host read of the selection's bounding box)."""
     if mode not in MODES:
@@ -286,7 +286,7 @@ def finish_clone(job: CloneJob, solution: torch.Tensor | None):
 
 
 def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
-                   domain: str = "rectangle", method: str = "mgcg", margin: int = 24,
+                   domain: str = "mask", method: str = "mgcg", margin: int = 24,
                    tolerance: float = 1e-8, max_iterations: int = 2000):
     """Paste the masked region of ``source`` into ``target`` at ``offset`` without a seam
     (poisson.py:261-348, rectangle or mask route).  Returns ``(image, reports)``."""

@@ -4,7 +4,8 @@ admission by HBM budget.
 The reference runs one solve at a time on the caller's thread (solvers.py:171-221,
 called per channel by poisson.solve_dirichlet, poisson.py:141-143, and once by
 hdr.tone_map, hdr.py:175).  Here a request is a whole compositing job (seamless_clone,
-rectangle route) or tone-mapping job (tone_map), and a :class:`Server` replays a stream
+rectangle route unless the request asks for ``domain="mask"``) or tone-mapping job
+(tone_map), and a :class:`Server` replays a stream
 of timed arrivals:
 
 * requests wait in arrival order; each holds an estimate of the device bytes it needs
@@ -290,17 +291,20 @@ class Server:
                     if r.kind == "composite":
                         jobs[i] = prepare_clone(a["source"], a["target"], a["mask"],
                                                 a.get("offset", (0, 0)), a.get("mode", "import"),
+                                                a.get("domain", "rectangle"),
                                                 margin=int(a.get("margin", 24)))
                     else:
                         jobs[i] = prepare_tone_map(a["radiance"], a.get("config"))
                 except (ValueError, NotImplementedError) as e:
                     results[i] = (None, [], str(e))
-            planes, owner = [], []
+            planes, owner, masks = [], [], []
             for i, j in enumerate(jobs):
                 if j is not None and j.rhs is not None:
                     for k in range(j.rhs.shape[0]):
                         planes.append(j.rhs[k])
                         owner.append(i)
+                        # the unknowns of a mask-route composite (None: every point)
+                        masks.append(getattr(j, "unknown", None))
             sols: dict = {}
             reps: dict = {}
             if planes:
@@ -309,7 +313,8 @@ class Server:
                     _pcg_options(self.precision, self.tolerance, self.max_iterations,
                                  BOUNDARY_OF[batch[0].kind]), use_graph=self.graphs)
                 plan = Plan([tuple(p.shape) for p in planes], BOUNDARY_OF[batch[0].kind], 1.0,
-                            opts, device=self.device)
+                            opts, device=self.device,
+                            masks=masks if any(m is not None for m in masks) else None)
                 us, rs = plan.mgcg(planes)
                 del plan
                 dt = time.perf_counter() - t_start

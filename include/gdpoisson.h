@@ -28,6 +28,7 @@
  *                        GuidanceField.of / select / mix_by_magnitude + divergence +
  *                        fold_boundary of the window, fused
  *   gdp_guidance         poisson.py:52-92      GuidanceField.select / mix_by_magnitude / scaled
+ *   gdp_illumination_field poisson.py:433-437  illumination_change's attenuated field
  *   gdp_clone_rhs_masked, gdp_masked_rhs, gdp_plan_set_mask
  *                        poisson.py:154-244    solve_masked: the system over the mask's pixels
  *                                              (fixed neighbours folded in), solved by masked
@@ -145,6 +146,11 @@ int gdp_masked_rhs(const double* gx, const double* gy, const double* image,
 int gdp_guidance(const double* ax, const double* ay, const double* bx, const double* by,
                  const uint8_t* mask, const double* factor, double scale, int mode, int h, int w,
                  int planes, double* out_x, double* out_y, void* stream);
+/* illumination_change's attenuated field (poisson.py:433-437) over n elements:
+   f = alpha ** beta * max(sqrt(gx^2 + gy^2), epsilon) ** -beta, out = (gx f, gy f). */
+int gdp_illumination_field(const double* gx, const double* gy, size_t n, double alpha,
+                           double beta, double epsilon, double* out_x, double* out_y,
+                           void* stream);
 /* ---------------------------------------------------------------- tone mapping
  * One FP64 plane per image (h x w, row-major); radiance is h x w x 3 channels-last.
  * `work` is a device buffer of gdp_tm_work_bytes() bytes (scalars and partial sums);

@@ -12,6 +12,10 @@ Follows ``gradient-domain/src/gradient_domain/poisson.py``:
 * ``solve_masked``    poisson.py:154-244  the system over the selected pixels off the
                                           border, fixed neighbours folded in, SuperLU
 * ``seamless_clone(domain="mask")``  poisson.py:337-338
+* ``texture_flatten``      poisson.py:351-388  gradient times edge weights, rectangle or
+                                               masked solve
+* ``illumination_change``  poisson.py:391-437  gradient times alpha**beta |g|**-beta
+                                               (floored at epsilon), masked solve
 """
 
 from __future__ import annotations
@@ -164,3 +168,26 @@ def seamless_clone(source, target, mask, offset=(0, 0), mode="import", method="d
     out = target.copy()
     out[win] = patch
     return out, reports
+
+
+def texture_flatten(image, edges, method="mgcg", mask=None):
+    """poisson.py:351-388: the image's field scaled by the edge weights (broadcast over
+    channels), integrated with the image as boundary (or over ``mask``)."""
+    image = np.asarray(image, dtype=float)
+    weights = np.asarray(edges, dtype=float)
+    if image.ndim == 3:
+        weights = weights[..., None]
+    field = GuidanceField.of(image).scaled(weights)
+    if mask is not None:
+        return solve_masked(field, image, mask)
+    return solve_dirichlet(field, image, method=method)
+
+
+def illumination_change(image, mask, alpha=0.2, beta=0.2, epsilon=1e-4):
+    """poisson.py:391-437: gradient magnitudes remapped by alpha**beta * g**-beta (g floored
+    at epsilon), integrated over ``mask`` with the other pixels fixed."""
+    image = np.asarray(image, dtype=float)
+    field = GuidanceField.of(image)
+    magnitude = np.sqrt(field.gx ** 2 + field.gy ** 2)
+    factor = (alpha ** beta) * np.maximum(magnitude, epsilon) ** (-beta)
+    return solve_masked(field.scaled(factor), image, mask)

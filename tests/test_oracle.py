@@ -17,6 +17,7 @@ import scipy.sparse as sp
 from oracle import generators as G
 from oracle import multigrid as M
 from oracle import operators as O
+from oracle import poisson as OP
 from oracle import solvers as S
 
 GOLDEN = json.loads((Path(__file__).parent / "golden" / "reference_benchmark.json").read_text())
@@ -393,3 +394,76 @@ def test_mask_route_cloning_the_target_into_itself_changes_nothing(mode):
     mask = _disc((20, 24), 8.0)
     out, _ = OP.seamless_clone(tgt[5:25, 6:30], tgt, mask, (5, 6), mode=mode, domain="mask")
     assert np.abs(out - tgt).max() < 1e-10
+
+
+# ---------------------------------------------------------------- texture_flatten /
+# illumination_change (poisson.py:351-437)
+
+
+def _texture_example(shape=(96, 128), seed=1):
+    """Piecewise-constant RGB regions plus a fine texture; edges marks the region borders."""
+    rng = np.random.default_rng(seed)
+    rows, cols = np.mgrid[0:shape[0], 0:shape[1]]
+    regions = (rows > shape[0] // 2).astype(int) + 2 * (cols > shape[1] // 3).astype(int)
+    colours = rng.random((4, 3))
+    image = colours[regions] + 0.05 * np.sin(1.7 * cols + 0.9 * rows)[..., None]
+    edges = np.zeros(shape, dtype=bool)
+    edges[:, :-1] |= regions[:, 1:] != regions[:, :-1]
+    edges[:-1, :] |= regions[1:, :] != regions[:-1, :]
+    return image, edges.astype(float)
+
+
+def test_texture_flatten_with_unit_edges_returns_the_image():
+    """test_applications.py:150-152: weights of one keep the image's own field."""
+    image, _ = _texture_example((40, 52))
+    out, _ = OP.texture_flatten(image, np.ones(image.shape[:2]), method="direct")
+    assert np.abs(out - image).max() < 1e-9
+    mask = np.zeros(image.shape[:2], dtype=bool)
+    mask[5:30, 8:40] = True
+    out, _ = OP.texture_flatten(image, np.ones(image.shape[:2]), mask=mask)
+    assert np.abs(out - image).max() < 1e-9
+
+
+def test_texture_flatten_with_zero_edges_is_harmonic():
+    """A zero field leaves the interior harmonic: the discrete Laplacian of the result
+    vanishes at every pixel off the fixed ring."""
+    image, _ = _texture_example((30, 44))
+    out, _ = OP.texture_flatten(image, np.zeros(image.shape[:2]), method="direct")
+    lap = O.laplacian(out)
+    assert np.abs(lap[1:-1, 1:-1]).max() < 1e-9
+    assert np.array_equal(out[0], image[0]) and np.array_equal(out[:, -1], image[:, -1])
+
+
+def test_texture_flatten_reduces_texture_energy():
+    """test_applications.py:155-164 on this build's texture example."""
+    image, edges = _texture_example()
+    flat, _ = OP.texture_flatten(image, edges, method="direct")
+
+    def energy(x):
+        interior = ~edges.astype(bool)
+        variation = np.abs(np.diff(x, axis=1)).mean(axis=2)
+        return float(variation[interior[:, :-1]].mean())
+
+    assert energy(flat) < 0.2 * energy(image)
+
+
+def test_illumination_change_with_beta_zero_returns_the_image():
+    """beta = 0 makes the factor alpha ** 0 * g ** 0 = 1: the image's own field."""
+    rng = np.random.default_rng(3)
+    image = rng.random((32, 40, 3))
+    mask = np.zeros((32, 40), dtype=bool)
+    mask[4:28, 6:30] = True
+    out, _ = OP.illumination_change(image, mask, alpha=0.3, beta=0.0)
+    assert np.abs(out - image).max() < 1e-9
+
+
+def test_illumination_change_lifts_a_dark_selection():
+    """test_applications.py:167-175."""
+    rows, cols = np.mgrid[0:96, 0:96]
+    image = np.repeat((0.05 + 0.02 * np.sin(cols * 0.4))[..., None], 3, axis=2)
+    mask = (rows - 48) ** 2 + (cols - 48) ** 2 < 30.0 ** 2
+    inner = (rows - 48) ** 2 + (cols - 48) ** 2 < 24.0 ** 2
+    out, _ = OP.illumination_change(image, mask, alpha=0.4, beta=0.6)
+    ring = mask & ~inner
+    assert out[ring].std() > image[ring].std()
+    assert np.array_equal(out[~mask], image[~mask])

@@ -80,6 +80,9 @@ hipError_t launch_guidance(const double* ax, const double* ay, const double* bx,
                            const double* by, const uint8_t* mask, const double* factor,
                            double fscale, int mode, size_t hw, int planes, double* ox,
                            double* oy, hipStream_t s);
+hipError_t launch_illumination(const double* gx, const double* gy, size_t n, double scale,
+                               double beta, double epsilon, double* ox, double* oy,
+                               hipStream_t s);
 
 void set_error(const char* fmt, ...);
 

@@ -385,6 +385,24 @@ __global__ __launch_bounds__(256) void k_guidance(const double* __restrict__ ax,
   oy[k] = y;
 }
 
+// Gradient attenuation of illumination_change (poisson.py:433-437), per element of the
+// field: f = scale * max(|(gx, gy)|, epsilon) ** -beta with scale = alpha ** beta, and
+// (gx, gy) * f.  max keeps a NaN magnitude, as numpy.maximum does.
+__global__ __launch_bounds__(256) void k_illumination(const double* __restrict__ gx,
+                                                      const double* __restrict__ gy, size_t n,
+                                                      double scale, double beta,
+                                                      double epsilon, double* __restrict__ ox,
+                                                      double* __restrict__ oy) {
+#pragma clang fp contract(off)
+  const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const double x = gx[k], y = gy[k];
+  const double mag = sqrt(x * x + y * y);
+  const double f = scale * pow(mag < epsilon ? epsilon : mag, -beta);
+  ox[k] = x * f;
+  oy[k] = y * f;
+}
+
 dim3 grid2d(int m, int n, int planes) {
   return dim3((n + BX - 1) / BX, (m + BY - 1) / BY, planes);
 }
@@ -417,6 +435,15 @@ hipError_t launch_guidance(const double* ax, const double* ay, const double* bx,
   if (!total) return hipSuccess;
   k_guidance<<<unsigned((total + 255) / 256), 256, 0, s>>>(ax, ay, bx, by, mask, factor, fscale,
                                                           mode, hw, total, ox, oy);
+  return hipGetLastError();
+}
+
+hipError_t launch_illumination(const double* gx, const double* gy, size_t n, double scale,
+                               double beta, double epsilon, double* ox, double* oy,
+                               hipStream_t s) {
+  if (!n) return hipSuccess;
+  k_illumination<<<unsigned((n + 255) / 256), 256, 0, s>>>(gx, gy, n, scale, beta, epsilon, ox,
+                                                           oy);
   return hipGetLastError();
 }
 

@@ -1686,6 +1686,16 @@ int gdp_guidance(const double* ax, const double* ay, const double* bx, const dou
   return GDP_OK;
 }
 
+int gdp_illumination_field(const double* gx, const double* gy, size_t n, double alpha,
+                           double beta, double epsilon, double* out_x, double* out_y,
+                           void* stream) {
+  if (!gx || !gy || !out_x || !out_y) return (set_error("bad illumination arguments"), GDP_EINVAL);
+  // alpha ** beta once on the host, as the reference forms it (a scalar power)
+  GDP_HIP(launch_illumination(gx, gy, n, std::pow(alpha, beta), beta, epsilon, out_x, out_y,
+                              static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
+
 int gdp_smooth(double* u, const double* b, int m, int n, double spacing, int sweeps,
                int boundary, int order, void* stream) {
   if (int rc = check_grid(u, b, m, n, 1)) return rc;

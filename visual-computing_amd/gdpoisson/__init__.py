@@ -7,7 +7,8 @@ public names (``solve_system``, ``SOLVERS``, ``SolverReport``, ``MultigridOption
 ``divergence``, ``laplacian``, ``fold_boundary``) backed by hand-written HIP kernels
 in ``libgdpoisson.so`` (C ABI: ``include/gdpoisson.h``).  Batched plans
 (:class:`Plan`) solve many grids per launch.  The compositing (``seamless_clone``,
-``GuidanceField``, ``solve_dirichlet``) and tone-mapping (``tone_map``,
+``GuidanceField``, ``solve_dirichlet``, ``solve_masked``, ``texture_flatten``,
+``illumination_change``) and tone-mapping (``tone_map``,
 ``attenuation_field``, ``ToneMapConfig``) callers of the solve run on the device too.
 """
 
@@ -15,7 +16,8 @@ from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
 from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
 from .autotune import autotune
 from .plan import Plan, PlanOptions
-from .poisson import GuidanceField, seamless_clone, solve_dirichlet, solve_masked
+from .poisson import (GuidanceField, illumination_change, seamless_clone, solve_dirichlet,
+                      solve_masked, texture_flatten)
 from . import hdr
 from .hdr import ToneMapConfig, attenuation_field, tone_map
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
@@ -28,6 +30,8 @@ __all__ = [
     "Plan",
     "GuidanceField",
     "seamless_clone",
+    "texture_flatten",
+    "illumination_change",
     "solve_dirichlet",
     "solve_masked",
     "ToneMapConfig",

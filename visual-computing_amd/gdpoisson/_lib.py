@@ -27,6 +27,7 @@ EXPORTS = (
     "gdp_fold_boundary",
     "gdp_clone_rhs",
     "gdp_guidance",
+    "gdp_illumination_field",
     "gdp_clone_rhs_masked",
     "gdp_masked_rhs",
     "gdp_plan_set_mask",
@@ -121,6 +122,8 @@ def _declare(lib):
                                   c_int, c_int, c_int, c_int, c_int, P, P]),
         "gdp_guidance": (c_int, [P, P, P, P, P, P, c_double, c_int, c_int, c_int, c_int, P, P,
                                  P]),
+        "gdp_illumination_field": (c_int, [P, P, c_size_t, c_double, c_double, c_double, P, P,
+                                           P]),
         "gdp_clone_rhs_masked": (c_int, [P, c_int, c_int, c_int, c_int, P, P, c_int, c_int,
                                          c_int, c_int, c_int, c_int, c_int, c_int, P, P, P]),
         "gdp_masked_rhs": (c_int, [P, P, P, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -295,3 +295,44 @@ def seamless_clone(source, target, mask, offset=(0, 0), mode: str = "import",
         return finish_clone(job, None), []
     sol, reports = _solve_window(job.rhs, method, tolerance, max_iterations, job.unknown)
     return finish_clone(job, sol), reports
+
+
+def _masked_method(method: str) -> str:
+    # the reference solves masked systems directly whatever method is named; here the
+    # masked solve is FP64 MG-PCG unless mixed precision is asked for
+    return method if method in ("mgcg", "mgcg_mixed") else "mgcg"
+
+
+def texture_flatten(image, edges, method: str = "mgcg", mask=None, tolerance: float = 1e-8,
+                    max_iterations: int = 2000):
+    """Remove texture and keep the edges that define shapes (poisson.py:351-388): the
+    image's gradient times the ``edges`` weights (h, w) in [0, 1], integrated over the
+    whole image with its ring fixed, or over ``mask`` with every other pixel fixed.
+    Returns ``(result, reports)``."""
+    field = GuidanceField.of(image).scaled(edges)
+    if mask is not None:
+        return solve_masked(field, image, mask, method=_masked_method(method),
+                            tolerance=tolerance, max_iterations=max_iterations)
+    return solve_dirichlet(field, image, method=method, tolerance=tolerance,
+                           max_iterations=max_iterations)
+
+
+def illumination_change(image, mask, alpha: float = 0.2, beta: float = 0.2,
+                        epsilon: float = 1e-4, method: str = "mgcg", tolerance: float = 1e-8,
+                        max_iterations: int = 2000):
+    """Compress the local dynamic range inside a selection (poisson.py:391-437): every
+    gradient of the image times ``alpha ** beta * max(|g|, epsilon) ** -beta`` (fused on
+    the device, per channel), integrated over ``mask`` with every other pixel fixed.
+    Returns ``(result, reports)``."""
+    field = GuidanceField.of(image)
+    gx, host = to_device(field.gx)
+    gy, _ = to_device(field.gy)
+    gx, gy = gx.contiguous(), gy.contiguous()
+    ox, oy = torch.empty_like(gx), torch.empty_like(gy)
+    _lib.check(_lib.lib().gdp_illumination_field(
+        ctypes.c_void_p(gx.data_ptr()), ctypes.c_void_p(gy.data_ptr()), gx.numel(),
+        float(alpha), float(beta), float(epsilon), ctypes.c_void_p(ox.data_ptr()),
+        ctypes.c_void_p(oy.data_ptr()), _stream()), "gdp_illumination_field")
+    attenuated = GuidanceField(to_host_like(ox, host), to_host_like(oy, host))
+    return solve_masked(attenuated, image, mask, method=_masked_method(method),
+                        tolerance=tolerance, max_iterations=max_iterations)

@@ -66,12 +66,18 @@ extern "C" {
 #define GDP_FP64 0      /* V-cycle in FP64 (plain FP64 MG-PCG)                        */
 #define GDP_MIXED 1     /* V-cycle in FP32; r, x, d, q, dots, coarsest solve in FP64  */
 
+/* gdp_problem.flags */
+#define GDP_PROBLEM_MASKED 1 /* a masked domain will be set (gdp_plan_set_mask): every level
+                                but the coarsest is smoothed with its mask (the coarse tail
+                                holds the coarsest level only), which cuts the PCG
+                                iterations of masked solves by more than half */
+
 /* One independent grid (one colour channel of one image). */
 typedef struct gdp_problem {
   int32_t rows;      /* m: unknowns along axis 0 */
   int32_t cols;      /* n: unknowns along axis 1 */
   int32_t boundary;  /* GDP_DIRICHLET or GDP_NEUMANN */
-  int32_t reserved;
+  int32_t flags;     /* 0 or GDP_PROBLEM_MASKED */
   double spacing;    /* h */
 } gdp_problem;
 

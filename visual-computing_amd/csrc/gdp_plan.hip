@@ -446,8 +446,11 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   for (int i = 0; i < count; ++i) {
     const gdp_problem& pr = problems[i];
     if (pr.rows < 1 || pr.cols < 1 || !(pr.spacing > 0.0) ||
-        (pr.boundary != GDP_DIRICHLET && pr.boundary != GDP_NEUMANN)) {
-      set_error("problem %d: bad shape (%d, %d), spacing or boundary", i, pr.rows, pr.cols);
+        (pr.boundary != GDP_DIRICHLET && pr.boundary != GDP_NEUMANN) ||
+        (pr.flags & ~GDP_PROBLEM_MASKED) ||
+        ((pr.flags & GDP_PROBLEM_MASKED) && pr.boundary != GDP_DIRICHLET)) {
+      set_error("problem %d: bad shape (%d, %d), spacing, boundary or flags", i, pr.rows,
+                pr.cols);
       delete P;
       return GDP_EINVAL;
     }
@@ -467,10 +470,13 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     }
     const int L = int(pi.lv.size());
     int first = -1;
+    // a masked problem's tail holds its coarsest level only: the strip kernels smooth
+    // every other level with the mask
+    const int f0 = (pr.flags & GDP_PROBLEM_MASKED) ? std::min(L - 1, kMaxDepth - 1) : 0;
     auto place_tail = [&]() {
       // the register-strip tail takes the first level at most 128 columns wide and
       // stail_max_rows rows tall whose chain to the coarsest fits its level count and LDS
-      for (int f = 0; f < L; ++f) {
+      for (int f = f0; f < L; ++f) {
         const auto& c = pi.lv[L - 1];
         const int bytes = stail_lds_bytes(c.m, c.n, pi.exact);
         if (pi.lv[f].m <= stail_max_rows(P->fp32) && pi.lv[f].n <= kStailMaxCols &&
@@ -481,7 +487,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
           return;
         }
       }
-      for (int f = 0; f < L; ++f) {
+      for (int f = f0; f < L; ++f) {
         const size_t bytes = tail_layout(pi, f, P->tsize);
         if (bytes <= size_t(kTailLdsBytes)) {
           first = f;

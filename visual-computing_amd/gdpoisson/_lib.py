@@ -72,7 +72,7 @@ class Problem(ctypes.Structure):
         ("rows", c_int32),
         ("cols", c_int32),
         ("boundary", c_int32),
-        ("reserved", c_int32),
+        ("flags", c_int32),
         ("spacing", c_double),
     ]
 

@@ -85,8 +85,13 @@ class Plan:
         self.boundary = bcs
         self.device = _device(device)
         probs = (_lib.Problem * n)()
+        masked = [False] * n if masks is None else [mk is not None for mk in masks]
+        if len(masked) != n:
+            raise ValueError("masks must match the number of shapes")
         for i, ((m, nn), b, h) in enumerate(zip(self.shapes, bcs, hs)):
-            probs[i] = _lib.Problem(m, nn, BOUNDARY[b], 0, h)
+            # a masked problem keeps every level but the coarsest out of the coarse tail,
+            # so they are smoothed with its mask
+            probs[i] = _lib.Problem(m, nn, BOUNDARY[b], 1 if masked[i] else 0, h)
         mixed = opts.precision == "mixed"
         cs = opts.coarsest_size
         if cs is None:

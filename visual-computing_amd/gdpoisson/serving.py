@@ -106,8 +106,8 @@ def request_bytes(req: Request, precision: str = "mixed") -> int:
     """Upper bound of the device bytes a request holds while admitted.
 
     composite: the target's planes (a copy) and the output image, the source's planes and
-    mask, the right-hand side and solution of the window (bounded by the target) and the
-    plan workspace of its channels; tonemap: log luminance, pyramid (4/3 of a plane),
+    mask, the right-hand side and solution of the window (bounded by the target), the plan
+    workspace of its channels and, by the mask route, its byte masks; tonemap: log luminance, pyramid (4/3 of a plane),
     factor, right-hand side, solution, the output image and the plan workspace.  An
     ``nbytes`` entry in ``args`` overrides the estimate.
     """
@@ -119,7 +119,10 @@ def request_bytes(req: Request, precision: str = "mixed") -> int:
         h, w = _shape(a["target"])[:2]
         c = _shape(a["target"])[2] if len(_shape(a["target"])) == 3 else 1
         sh, sw = _shape(a["source"])[:2]
-        return int(8 * c * (2 * h * w + sh * sw) + sh * sw + 16 * c * h * w + per * c * h * w)
+        # the mask route's placed, interior and window masks
+        masks = 3 * h * w if a.get("domain", "rectangle") == "mask" else 0
+        return int(8 * c * (2 * h * w + sh * sw) + sh * sw + 16 * c * h * w + per * c * h * w
+                   + masks)
     h, w = _shape(a["radiance"])[:2]
     return int(8 * h * w * (3 + 1 + 4 / 3 + 1 + 1 + 1) + per * h * w + (1 << 20))
 

@@ -132,6 +132,7 @@ def serve_stream(count: int, qps: float, slots: int) -> dict:
         "mix": "70% RGB seamless compositing (squares 512-2048, blob masks), "
                "30% tone mapping 3840x2160 (log-normal radiance)",
         "method": srv.method,
+        "tonemap_method": srv.tonemap_method,
         **{k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()},
         "p50_ms_by_kind": {k: round(float(np.percentile(v, 50)), 3) for k, v in lat.items()},
         "p99_ms_by_kind": {k: round(float(np.percentile(v, 99)), 3) for k, v in lat.items()},

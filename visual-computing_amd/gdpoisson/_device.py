@@ -2,6 +2,7 @@
 
 from __future__ import annotations
 
+import threading
 from collections import OrderedDict
 
 import numpy as np
@@ -27,22 +28,29 @@ def to_host_like(t: torch.Tensor, was_host: bool):
 
 _CACHE: "OrderedDict[tuple, Plan]" = OrderedDict()
 _CACHE_MAX = 24
+_CACHE_LOCK = threading.Lock()
 
 
 def cached_plan(shapes, boundary, spacing, options: PlanOptions) -> Plan:
+    """A plan for this configuration, kept for reuse.  Plans are kept per host thread: a
+    plan's workspace and host state serve one solve at a time."""
     dev = torch.cuda.current_device()
     key = (tuple(tuple(s) for s in shapes), boundary if isinstance(boundary, str) else tuple(boundary),
-           spacing if isinstance(spacing, (int, float)) else tuple(spacing), options, dev)
-    p = _CACHE.get(key)
-    if p is not None:
-        _CACHE.move_to_end(key)
-        return p
+           spacing if isinstance(spacing, (int, float)) else tuple(spacing), options, dev,
+           threading.get_ident())
+    with _CACHE_LOCK:
+        p = _CACHE.get(key)
+        if p is not None:
+            _CACHE.move_to_end(key)
+            return p
     p = Plan(shapes, boundary, spacing, options)
-    _CACHE[key] = p
-    while len(_CACHE) > _CACHE_MAX:
-        _CACHE.popitem(last=False)
+    with _CACHE_LOCK:
+        _CACHE[key] = p
+        while len(_CACHE) > _CACHE_MAX:
+            _CACHE.popitem(last=False)
     return p
 
 
 def clear_plan_cache():
-    _CACHE.clear()
+    with _CACHE_LOCK:
+        _CACHE.clear()

@@ -132,7 +132,10 @@ class Server:
 
     ``budget_bytes``: device bytes the admitted requests may hold together (default: half
     of the free HBM).  ``slots``: workers, one HIP stream each.  ``max_batch``: problems
-    (channels) per batched plan.  ``method``: "mgcg_mixed" or "mgcg".  ``graphs``: capture
+    (channels) per batched plan.  ``method``: "mgcg_mixed" or "mgcg".  ``tonemap_method``:
+    "dct" integrates each tone map exactly by the device DCT, as the reference's tone_map
+    does (sides whose prime factors are not all 2, 3, 5 join the batched plan), "mg" puts
+    every tone map in the batched MG-PCG plan.  ``graphs``: capture
     the plans' PCG iterations in hipGraphs (each batch's plan is new, so the capture is
     paid per batch).  ``executor`` replaces the device pipeline (tests of the
     scheduling): called as ``executor(batch)`` in a worker, it returns one
@@ -142,9 +145,12 @@ class Server:
     def __init__(self, budget_bytes: int | None = None, slots: int = 3, max_batch: int = 12,
                  method: str = "mgcg_mixed", tolerance: float = 1e-8,
                  max_iterations: int = 2000, graphs: bool = False, keep_outputs: bool = True,
-                 device=None, executor: Callable | None = None):
+                 device=None, executor: Callable | None = None, tonemap_method: str = "dct"):
         if method not in ("mgcg", "mgcg_mixed"):
             raise ValueError("method must be 'mgcg' or 'mgcg_mixed'")
+        if tonemap_method not in ("dct", "mg"):
+            raise ValueError("tonemap_method must be 'dct' or 'mg'")
+        self.tonemap_method = tonemap_method
         if slots < 1 or max_batch < 1:
             raise ValueError("slots and max_batch must be positive")
         self.slots, self.max_batch = int(slots), int(max_batch)
@@ -282,7 +288,8 @@ class Server:
         from .hdr import finish_tone_map, prepare_tone_map
         from .plan import Plan
         from .poisson import finish_clone, prepare_clone
-        from .solvers import SolverReport, _pcg_options
+        from .solvers import (SolverReport, _pcg_options, dct_supported, relative_residual,
+                              solve_neumann)
 
         stream = self._stream()
         results: list = [None] * len(batch)
@@ -300,6 +307,19 @@ class Server:
                         jobs[i] = prepare_tone_map(a["radiance"], a.get("config"))
                 except (ValueError, NotImplementedError) as e:
                     results[i] = (None, [], str(e))
+            if self.tonemap_method == "dct":
+                # exact integration of each tone map on this stream (hdr.py:175)
+                for i, (r, j) in enumerate(zip(batch, jobs)):
+                    if r.kind != "tonemap" or j is None or not dct_supported(*j.rhs.shape[1:]):
+                        continue
+                    t0 = time.perf_counter()
+                    u = solve_neumann(j.rhs[0], method="dct")
+                    rr = relative_residual(u, j.rhs[0], 1.0, "neumann")
+                    img, _ = finish_tone_map(j, u)
+                    results[i] = (img, [SolverReport(
+                        method="dct", iterations=1, relative_residual=rr,
+                        seconds=time.perf_counter() - t0, converged=True)], None)
+                    jobs[i] = None
             planes, owner, masks = [], [], []
             for i, j in enumerate(jobs):
                 if j is not None and j.rhs is not None:

@@ -23,6 +23,7 @@ convention of ``solve_neumann`` (solvers.py:224-262).
 from __future__ import annotations
 
 import ctypes
+import threading
 
 import time
 from dataclasses import dataclass
@@ -161,6 +162,7 @@ def solve_system(
 
 
 _DCT_TABLES: dict = {}
+_DCT_LOCK = threading.Lock()
 
 
 def relative_residual(u: torch.Tensor, b: torch.Tensor, spacing: float = 1.0,
@@ -190,9 +192,14 @@ def _dct_solve(t: torch.Tensor) -> torch.Tensor:
         tab = torch.empty(nbytes.value // 8, dtype=torch.float64, device=t.device)
         _lib.check(_lib.lib().gdp_dct_tables(m, n, ctypes.c_void_p(tab.data_ptr()),
                                              _lib.current_stream()), "gdp_dct_tables")
-        if len(_DCT_TABLES) > 32:
-            _DCT_TABLES.clear()
-        _DCT_TABLES[key] = tab
+        # the cache is shared by every stream (serving workers): a table is published only
+        # once it is written, and evicted only when no stream can still be reading it
+        torch.cuda.current_stream(t.device).synchronize()
+        with _DCT_LOCK:
+            if len(_DCT_TABLES) >= 32:
+                torch.cuda.synchronize(t.device)
+                _DCT_TABLES.clear()
+            _DCT_TABLES[key] = tab
     work = torch.empty(2 * m * n, dtype=torch.float64, device=t.device)
     u = torch.empty_like(t)
     _lib.check(_lib.lib().gdp_neumann_dct_solve(

@@ -329,7 +329,8 @@ def main():
     ap.add_argument("--composite-steps", type=int, default=3,
                     help="2048x2048 RGB composites timed per route (0: skip them)")
     ap.add_argument("--serve-qps", type=float, default=200.0, help="arrival rate of the stream")
-    ap.add_argument("--serve-slots", type=int, default=3, help="serving workers (HIP streams)")
+    ap.add_argument("--serve-slots", type=int, default=4,
+                    help="serving workers (HIP streams; four hardware queues per process)")
     ap.add_argument("--nu", type=int, default=2,
                     help="pre/post smoothing sweeps of the mixed solver's V-cycle")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),

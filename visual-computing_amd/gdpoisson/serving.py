@@ -142,7 +142,7 @@ class Server:
     ``(output, reports)`` per request.
     """
 
-    def __init__(self, budget_bytes: int | None = None, slots: int = 3, max_batch: int = 12,
+    def __init__(self, budget_bytes: int | None = None, slots: int = 4, max_batch: int = 12,
                  method: str = "mgcg_mixed", tolerance: float = 1e-8,
                  max_iterations: int = 2000, graphs: bool = False, keep_outputs: bool = True,
                  device=None, executor: Callable | None = None, tonemap_method: str = "dct"):

@@ -76,8 +76,12 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
   D2 dn[FH];
 #pragma unroll
   for (int a = 0; a < FH; ++a) {
-    dn[a].x = unknown(a, 0) ? double(zv[a].x) + beta * dv[a].x : 0.0;
-    dn[a].y = unknown(a, 1) ? double(zv[a].y) + beta * dv[a].y : 0.0;
+    // a new sequence (beta = 0) takes d = z without reading the old direction, which
+    // the set-up leaves unwritten
+    dn[a].x = unknown(a, 0) ? (beta == 0.0 ? double(zv[a].x) : double(zv[a].x) + beta * dv[a].x)
+                            : 0.0;
+    dn[a].y = unknown(a, 1) ? (beta == 0.0 ? double(zv[a].y) : double(zv[a].y) + beta * dv[a].y)
+                            : 0.0;
   }
   const bool neu = t.bc == kNeumann;
   const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;

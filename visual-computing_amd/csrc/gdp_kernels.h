@@ -521,11 +521,10 @@ __global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ til
     const double bb = b[(size_t)gi * t.n + gj] - mean;
     acc += bb * bb;
     if (mode == 0) {
+      // d and q need no clearing: the first direction is z itself (beta = 0) and q is
+      // written by the operator apply before any kernel reads it
       t.r[o] = -bb;
       t.x[o] = 0.0;
-      t.d0[o] = 0.0;
-      t.d1[o] = 0.0;
-      t.q[o] = 0.0;
       b0[o] = T(bb);
     } else {
       t.x[o] = x0 ? x0[(size_t)gi * t.n + gj] : 0.0;

@@ -1248,7 +1248,7 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
   a.project = project;
   if ((rc = enqueue_scalar(P, s, a))) return rc;
   {
-    ProfScope ps(P, s, "init", (mode == 0 ? 48.0 + sizeof(T) : 16.0) * N);
+    ProfScope ps(P, s, "init", (mode == 0 ? 24.0 + sizeof(T) : 16.0) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
                      (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
                          t, P->d_io(), with_init ? P->d_init() : nullptr, P->d_state(),

@@ -1,10 +1,35 @@
 // Host-side declarations shared by the translation units of libgdpoisson.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "gdp_device.h"
 
 namespace gdp {
+
+// Launch timing of the plan profiler.  While a profiling scope is open on this thread it
+// points at its events and the scope's one launch carries them: hipExtLaunchKernelGGL
+// stamps them from the kernel's own dispatch, so no event packet runs between kernels
+// and the times match a kernel trace.
+struct LaunchEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool used = false;
+};
+extern thread_local LaunchEvents* t_launch_events;
+
+template <typename... KArgs, typename... Args>
+inline void klaunch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned smem, hipStream_t s,
+                    Args... args) {
+  static_assert(sizeof...(KArgs) == sizeof...(Args), "kernel argument count");
+  if (LaunchEvents* e = t_launch_events) {
+    // a later launch of the same scope moves only the stop: the scope times them all
+    hipExtLaunchKernelGGL(k, grid, block, smem, s, e->used ? nullptr : e->start, e->stop, 0u,
+                          static_cast<KArgs>(args)...);
+    e->used = true;
+  } else {
+    k<<<grid, block, smem, s>>>(static_cast<KArgs>(args)...);
+  }
+}
 
 // Launches of the strip kernels of the tiled multigrid levels (gdp_sdown_*.hip,
 // gdp_sup_*.hip).  menu: strip rows (kDownRows / kUpRows index), nu: sweeps.

@@ -29,6 +29,8 @@
 
 namespace gdp {
 
+thread_local LaunchEvents* t_launch_events = nullptr;
+
 static thread_local std::string g_err;
 
 void set_error(const char* fmt, ...) {
@@ -884,7 +886,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
         const LevelInfo& f = pi.lv[l - 1];
         const LevelInfo& c = pi.lv[l];
         const uint8_t* src = l == 1 ? pi.mask : reinterpret_cast<const uint8_t*>(P->ws + f.off_mask);
-        k_mask_inject<<<(c.m * c.n + 255) / 256, 256, 0, P->cap>>>(
+        klaunch(k_mask_inject, dim3((c.m * c.n + 255) / 256), dim3(256), 0, P->cap,
             src, f.n, reinterpret_cast<uint8_t*>(P->ws + c.off_mask), c.m, c.n);
         GDP_HIP(hipGetLastError());
       }
@@ -1053,25 +1055,29 @@ hipEvent_t prof_event(gdp_plan* P) {
   return e;
 }
 
-// Wraps one launch: events on the launch stream when profiling, nothing otherwise.
+// A profiled launch (nothing when not profiling): the scope's kernel launches (klaunch)
+// carry its start and stop events, stamped from the dispatches themselves.
 struct ProfScope {
   gdp_plan* P;
-  hipStream_t s;
   int id = -1;
   double bytes;
-  hipEvent_t a = nullptr;
-  ProfScope(gdp_plan* P_, hipStream_t s_, const char* name, double bytes_)
-      : P(P_), s(s_), bytes(bytes_) {
+  LaunchEvents ev;
+  ProfScope(gdp_plan* P_, hipStream_t, const char* name, double bytes_) : P(P_), bytes(bytes_) {
     if (!P->prof) return;
     id = prof_id(P, name);
-    a = prof_event(P);
-    (void)hipEventRecord(a, s);
+    ev.start = prof_event(P);
+    ev.stop = prof_event(P);
+    t_launch_events = &ev;
   }
   ~ProfScope() {
     if (id < 0) return;
-    hipEvent_t b = prof_event(P);
-    (void)hipEventRecord(b, s);
-    P->prof_pending.push_back(gdp_plan::ProfRec{id, a, b, bytes});
+    t_launch_events = nullptr;
+    if (ev.used) {
+      P->prof_pending.push_back(gdp_plan::ProfRec{id, ev.start, ev.stop, bytes});
+    } else {  // nothing was launched (an empty level)
+      P->prof_pool.push_back(ev.start);
+      P->prof_pool.push_back(ev.stop);
+    }
   }
 };
 
@@ -1122,12 +1128,12 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
     }
     if (no) {
       if (dots)
-        k_tail<T, true><<<no, kTailThreads, P->tail_smem, s>>>(
+        klaunch(k_tail<T, true>, dim3(no), dim3(kTailThreads), P->tail_smem, s,
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
             P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
             P->csweeps, order);
       else
-        k_tail<T, false><<<no, kTailThreads, P->tail_smem, s>>>(
+        klaunch(k_tail<T, false>, dim3(no), dim3(kTailThreads), P->tail_smem, s,
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
             P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
             P->csweeps, order);
@@ -1159,7 +1165,7 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
   const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "jacobi", (8.0 + sizeof(T)) * krylov_points(P));
   GDP_KRY_DISPATCH(P->kry_menu,
-                   (k_jacobi<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                   (klaunch(k_jacobi<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                        t, P->d_state(), P->d_upart(), P->d_zpart(), check_conv ? 1 : 0)));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
@@ -1178,23 +1184,23 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
     ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
     const KTile* at = P->at<KTile>(P->kapply.off);
     if (has_mask(P))
-      k_apply_s<T, true><<<P->kapply.count, kThreads, 0, s>>>(at, P->d_state(), P->d_zpart(),
+      klaunch(k_apply_s<T, true>, dim3(P->kapply.count), dim3(kThreads), 0, s, at, P->d_state(), P->d_zpart(),
                                                               P->d_apart(), din, jacobi ? 1 : 0);
     else
-      k_apply_s<T, false><<<P->kapply.count, kThreads, 0, s>>>(at, P->d_state(), P->d_zpart(),
+      klaunch(k_apply_s<T, false>, dim3(P->kapply.count), dim3(kThreads), 0, s, at, P->d_state(), P->d_zpart(),
                                                                P->d_apart(), din, jacobi ? 1 : 0);
     GDP_CHECK_LAUNCH();
   }
   {
     ProfScope ps(P, s, "update", (48.0 + sizeof(T)) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
-                     (k_update<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                     (klaunch(k_update<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                          t, P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
     GDP_CHECK_LAUNCH();
   }
   if (last) {
     ProfScope ps(P, s, "conv test", 16.0 * P->kry.count);
-    k_check_conv<<<P->count(), kThreads, 0, s>>>(P->d_state(), P->d_upart(), P->d_rkry());
+    klaunch(k_check_conv, dim3(P->count()), dim3(kThreads), 0, s, P->d_state(), P->d_upart(), P->d_rkry());
     GDP_CHECK_LAUNCH();
     return GDP_OK;
   }
@@ -1222,7 +1228,7 @@ struct ScalarArgs {
 int enqueue_scalar(gdp_plan* P, hipStream_t s, const ScalarArgs& a) {
   static const char* names[3] = {"scalar stats", "scalar init", "scalar residual"};
   ProfScope ps(P, s, names[a.kind], 16.0 * P->kry.count);
-  k_scalar<<<P->count(), kThreads, 0, s>>>(
+  klaunch(k_scalar, dim3(P->count()), dim3(kThreads), 0, s,
       P->d_kry(), P->d_state(), P->d_spart(), P->d_rkry(), a.kind, a.mode, a.project, a.tol,
       a.max_iter, P->opt.flexible, a.zero_done, a.respect_done, a.hist ? P->d_hist() : nullptr,
       P->opt.max_iterations + 2);
@@ -1239,7 +1245,7 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
   int rc;
   {
     ProfScope ps(P, s, "init_stats", 8.0 * N);
-    GDP_KRY_DISPATCH(P->kry_menu, (k_init_stats<TH, TW><<<nb, kThreads, 0, s>>>(
+    GDP_KRY_DISPATCH(P->kry_menu, (klaunch(k_init_stats<TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                                       t, P->d_io(), P->d_spart())));
     GDP_CHECK_LAUNCH();
   }
@@ -1250,7 +1256,7 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
   {
     ProfScope ps(P, s, "init", (mode == 0 ? 24.0 + sizeof(T) : 16.0) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
-                     (k_init<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                     (klaunch(k_init<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                          t, P->d_io(), with_init ? P->d_init() : nullptr, P->d_state(),
                          P->d_spart(), mode)));
     GDP_CHECK_LAUNCH();
@@ -1269,7 +1275,7 @@ int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, b
   {
     ProfScope ps(P, s, "residual", (24.0 + sizeof(T)) * krylov_points(P));
     GDP_KRY_DISPATCH(P->kry_menu,
-                     (k_residual<T, TH, TW><<<nb, kThreads, 0, s>>>(
+                     (klaunch(k_residual<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                          t, P->d_io(), P->d_state(), P->d_spart(), mode, respect_done)));
     GDP_CHECK_LAUNCH();
   }
@@ -1287,7 +1293,7 @@ int enqueue_correct_t(gdp_plan* P, hipStream_t s) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "correct", (16.0 + sizeof(T)) * krylov_points(P));
-  GDP_KRY_DISPATCH(P->kry_menu, (k_correct<T, TH, TW><<<nb, kThreads, 0, s>>>(
+  GDP_KRY_DISPATCH(P->kry_menu, (klaunch(k_correct<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                                     t, P->d_state())));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
@@ -1297,7 +1303,7 @@ int enqueue_finalize(gdp_plan* P, hipStream_t s, int mean_fix, int only_checked 
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   ProfScope ps(P, s, "finalize", 16.0 * krylov_points(P));
-  GDP_KRY_DISPATCH(P->kry_menu, (k_finalize<TH, TW><<<nb, kThreads, 0, s>>>(
+  GDP_KRY_DISPATCH(P->kry_menu, (klaunch(k_finalize<TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                                     t, P->d_io(), P->d_state(), mean_fix, only_checked)));
   GDP_CHECK_LAUNCH();
   return GDP_OK;
@@ -1413,7 +1419,7 @@ int enqueue_start(gdp_plan* P, hipStream_t s, bool jacobi) {
 int enqueue_check(gdp_plan* P, hipStream_t s, bool allow_reopen) {
   int rc;
   if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false))) return rc;
-  k_reopen<<<1, kThreads, 0, s>>>(P->d_state(), P->count(), allow_reopen ? 1 : 0, P->d_flag());
+  klaunch(k_reopen, dim3(1), dim3(kThreads), 0, s, P->d_state(), P->count(), allow_reopen ? 1 : 0, P->d_flag());
   GDP_CHECK_LAUNCH();
   return enqueue_finalize(P, s, 1, 1);
 }

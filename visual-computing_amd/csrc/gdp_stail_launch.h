@@ -8,11 +8,11 @@ namespace gdp {
 template <typename T>
 void launch_stail(bool dots, int blocks, int smem, hipStream_t s, const TailArgs& a) {
   if (dots)
-    k_stail<T, true><<<blocks, kSTailThreads, smem, s>>>(a.tails, a.levels, a.st, a.kds, a.upart,
+    klaunch(k_stail<T, true>, dim3(blocks), dim3(kSTailThreads), smem, s, a.tails, a.levels, a.st, a.kds, a.upart,
                                                         a.rk, a.zpart, a.rz, a.check_conv, a.pre,
                                                         a.post, a.coarsest_sweeps, a.order);
   else
-    k_stail<T, false><<<blocks, kSTailThreads, smem, s>>>(a.tails, a.levels, a.st, a.kds, a.upart,
+    klaunch(k_stail<T, false>, dim3(blocks), dim3(kSTailThreads), smem, s, a.tails, a.levels, a.st, a.kds, a.upart,
                                                          a.rk, a.zpart, a.rz, a.check_conv, a.pre,
                                                          a.post, a.coarsest_sweeps, a.order);
 }

@@ -9,7 +9,7 @@ namespace gdp {
 template <typename T, int CR>
 void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWN(NU) \
-  k_sdown<T, NU, CR><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
+  klaunch(k_sdown<T, NU, CR>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv)
   switch (nu) {
     case 0: GDP_DOWN(0); break;
     case 1: GDP_DOWN(1); break;
@@ -23,7 +23,7 @@ void down_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 template <typename T, int CR>
 void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWN(NU) \
-  k_sdown<T, NU, CR, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.upart, a.check_conv)
+  klaunch(k_sdown<T, NU, CR, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv)
   switch (nu) {
     case 0: GDP_DOWN(0); break;
     case 1: GDP_DOWN(1); break;
@@ -55,7 +55,7 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
 template <typename T, int TR, bool DOTS>
 void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU) \
-  k_sup<T, NU, TR, DOTS><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
+  klaunch(k_sup<T, NU, TR, DOTS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;
@@ -78,7 +78,7 @@ void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
 template <typename T, int TR, bool DOTS>
 void up_masked(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU) \
-  k_sup<T, NU, TR, DOTS, true><<<blocks, kThreads, 0, s>>>(a.tiles, a.st, a.zpart, a.order)
+  klaunch(k_sup<T, NU, TR, DOTS, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;

@@ -63,7 +63,7 @@ def make_plan(L, shapes, bc, precision=0, **kw):
     n = len(shapes)
     probs = (L.Problem * n)()
     for i, (m, nn) in enumerate(shapes):
-        probs[i] = L.Problem(m, nn, bc, 0, 1.0)
+        probs[i] = L.Problem(m, nn, bc, kw.get("flags", 0), 1.0)
     cs = kw.get("coarsest_size", 16 if bc == L.NEUMANN else 8)
     opt = L.Options(precision, kw.get("pre", 1), kw.get("post", 1), cs, 60,
                     kw.get("max_iterations", 100), 0, 1, 1e-8)
@@ -139,3 +139,21 @@ def test_tile_menu(L):
     assert tw.value == 116
     for bad in ((3, 0, 1), (0, 4, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
+
+
+def test_planner_problem_flags(L):
+    """GDP_PROBLEM_MASKED (the problem will get a masked domain) needs the Dirichlet
+    boundary; other flag bits are rejected; the level chain is the same."""
+    rc, plan = make_plan(L, [(255, 257)], L.DIRICHLET, flags=1)
+    assert rc == 0
+    rc0, plan0 = make_plan(L, [(255, 257)], L.DIRICHLET)
+    assert rc0 == 0 and plan_levels(L, plan, 0) == plan_levels(L, plan0, 0)
+    ws, ws0 = ctypes.c_size_t(), ctypes.c_size_t()
+    assert L.lib().gdp_plan_workspace_bytes(plan, ctypes.byref(ws)) == 0
+    assert L.lib().gdp_plan_workspace_bytes(plan0, ctypes.byref(ws0)) == 0
+    assert ws.value >= ws0.value  # its coarse levels are strip levels with tile tables
+    assert L.lib().gdp_plan_destroy(plan) == 0 and L.lib().gdp_plan_destroy(plan0) == 0
+    rc, _ = make_plan(L, [(64, 64)], L.NEUMANN, flags=1)
+    assert rc == L.GDP_EINVAL
+    rc, _ = make_plan(L, [(64, 64)], L.DIRICHLET, flags=2)
+    assert rc == L.GDP_EINVAL and b"flags" in L.lib().gdp_last_error()

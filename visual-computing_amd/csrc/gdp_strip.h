@@ -498,6 +498,20 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
   const bool wy = lc.gjx + 1 >= t.j0 && lc.gjx + 1 < fc1;
   const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
   const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
+  // DOTS: the FP64 residual and q of every owned row (the finest level shares the Krylov
+  // grid's pitch), loaded together before any is used, so their latency is paid once:
+  // branch-free, from clamped addresses inside the padded rows (gjx is even)
+  using D2 = typename Pair<double>::type;
+  D2 rq[DOTS ? 2 * TR : 1];
+  if constexpr (DOTS) {
+    const int jc = (lc.gjx >= 0 && lc.gjx < n) ? lc.gjx : 0;
+#pragma unroll
+    for (int a = 0; a < TR; ++a) {
+      const size_t o = (size_t)min(gr0 + H + a, m - 1) * t.pitch + jc;
+      rq[2 * a] = *(const GDP_GLOBAL D2*)(kr + o);
+      rq[2 * a + 1] = *(const GDP_GLOBAL D2*)(kq + o);
+    }
+  }
 #pragma unroll
   for (int a = H; a < H + TR; ++a) {
     const int gi = gr0 + a;
@@ -508,21 +522,10 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
     } else if (wx) {
       uo[o] = u[a].x;
     }
-    if (DOTS) {
-      // the finest level shares the Krylov grid's pitch
-      double rx = 0.0, ry = 0.0, qx = 0.0, qy = 0.0;
-      if (wx && wy) {
-        using D2 = typename Pair<double>::type;
-        const D2 r2 = *(const GDP_GLOBAL D2*)(kr + o);
-        const D2 q2 = *(const GDP_GLOBAL D2*)(kq + o);
-        rx = r2.x;
-        ry = r2.y;
-        qx = q2.x;
-        qy = q2.y;
-      } else if (wx) {
-        rx = kr[o];
-        qx = kq[o];
-      }
+    if constexpr (DOTS) {
+      const D2 r2 = rq[2 * (a - H)], q2 = rq[2 * (a - H) + 1];
+      const double rx = wx ? r2.x : 0.0, ry = (wx && wy) ? r2.y : 0.0;
+      const double qx = wx ? q2.x : 0.0, qy = (wx && wy) ? q2.y : 0.0;
       zr += double(u[a].x) * rx + double(u[a].y) * ry;
       zq += double(u[a].x) * qx + double(u[a].y) * qy;
     }

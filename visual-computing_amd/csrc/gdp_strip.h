@@ -389,18 +389,20 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
   const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
   const int m = t.m, n = t.n;
 
-  // every load of the strip is issued before anything waits on one
+  // every load of the strip is issued before anything waits on one: branch-free, from
+  // clamped addresses inside the padded rows (gjx is even), zeroed outside the grid
   V2 hb[FH];
   {
     const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
     const int gjx = gc0 + 2 * lane;
     const bool lin = gjx >= 0 && gjx < n;
+    const int jc = lin ? gjx : 0;
 #pragma unroll
     for (int a = 0; a < FH; ++a) {
       const int gi = gr0 + a;
-      hb[a] = V2{T(0), T(0)};
-      if (work && lin && gi >= 0 && gi < m)
-        hb[a] = *(const GDP_GLOBAL V2*)(b + (size_t)gi * t.pitch + gjx);
+      const bool in = work && lin && gi >= 0 && gi < m;
+      const V2 v = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jc);
+      hb[a] = in ? v : V2{T(0), T(0)};
     }
   }
   if (st[t.problem].done) return;
@@ -568,15 +570,15 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
       K = min(max(K, 0), t.cn - 1);
       kin = true;
     }
+    // branch-free: every coarse value comes from a clamped address and is zeroed where
+    // it lies outside the coarse grid (Dirichlet), so no load waits before the next
+    const int Kc = min(max(K, 0), t.cn - 1);
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
-      int I = cr0 + k;
-      bool in = work && kin && I >= 0 && I < t.cm;
-      if (neu) {
-        I = min(max(I, 0), t.cm - 1);
-        in = work;
-      }
-      E[k] = in ? e[(size_t)I * t.cpitch + K] : T(0);
+      const int I = cr0 + k;
+      const bool in = work && (neu || (kin && I >= 0 && I < t.cm));
+      const T ev = e[(size_t)min(max(I, 0), t.cm - 1) * t.cpitch + Kc];
+      E[k] = in ? ev : T(0);
     }
     // branch-free: every pair is loaded from a clamped address (inside the grid's padded
     // rows: the pitch is even) and zeroed outside the grid, so the strip arrays are not

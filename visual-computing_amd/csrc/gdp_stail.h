@@ -524,17 +524,19 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   // tables, issued together before the convergence test
   V2 b[R0];
   {
+    // branch-free: every row's pair is loaded (from a clamped address inside the padded
+    // rows) before any is waited on, and zeroed outside the level
     const GDP_GLOBAL T* g = (const GDP_GLOBAL T*)L0.b;
     const int gj = 2 * c.lane;
+    const int m0 = L0.m, n0 = L0.n, pitch0 = L0.pitch;
+    const int jc = gj < n0 ? gj : 0;
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
       const int gi = c.w * R0 + r;
-      b[r] = V2{T(0), T(0)};
-      if (gi < L0.m && gj < L0.n) b[r] = *(const GDP_GLOBAL V2*)(g + (size_t)gi * L0.pitch + gj);
+      const V2 v = *(const GDP_GLOBAL V2*)(g + (size_t)min(gi, m0 - 1) * pitch0 + jc);
+      b[r].x = (gi < m0 && gj < n0) ? v.x : T(0);
+      b[r].y = (gi < m0 && gj + 1 < n0) ? v.y : T(0);
     }
-#pragma unroll
-    for (int r = 0; r < R0; ++r)
-      if (gj + 1 >= L0.n) b[r].y = T(0);
   }
   constexpr int kPre = 2;  // table elements per thread loaded up front
   double tv[kPre];
@@ -577,21 +579,35 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   const GDP_GLOBAL double* kq = dots ? (const GDP_GLOBAL double*)kds[D.krylov].q : nullptr;
   double zr = 0.0, zq = 0.0;
   const int gj = 2 * c.lane;
+  const int m0 = L0.m, n0 = L0.n, pitch0 = L0.pitch;
+  // the FP64 residual and q of the rows, loaded together (branch-free, clamped addresses
+  // inside the padded rows) before any is used
+  using D2 = typename Pair<double>::type;
+  D2 kr2[R0], kq2[R0];
+  if (dots) {
+    const int jc = gj < n0 ? gj : 0;
+#pragma unroll
+    for (int r = 0; r < R0; ++r) {
+      const size_t o = (size_t)min(c.w * R0 + r, m0 - 1) * pitch0 + jc;
+      kr2[r] = *(const GDP_GLOBAL D2*)(kr + o);
+      kq2[r] = *(const GDP_GLOBAL D2*)(kq + o);
+    }
+  }
 #pragma unroll
   for (int r = 0; r < R0; ++r) {
     const int gi = c.w * R0 + r;
-    if (gi >= L0.m || gj >= L0.n) continue;
+    if (gi >= m0 || gj >= n0) continue;
     const T* zr0 = reinterpret_cast<const T*>(c.xb) + gi * kXbRow + gj + 1;
     const V2 z = V2{zr0[0], zr0[1]};
-    const size_t o = (size_t)gi * L0.pitch + gj;
+    const size_t o = (size_t)gi * pitch0 + gj;
     g[o] = z.x;
-    if (gj + 1 < L0.n) g[o + 1] = z.y;
+    if (gj + 1 < n0) g[o + 1] = z.y;
     if (dots) {
-      zr += double(z.x) * kr[o];
-      zq += double(z.x) * kq[o];
-      if (gj + 1 < L0.n) {
-        zr += double(z.y) * kr[o + 1];
-        zq += double(z.y) * kq[o + 1];
+      zr += double(z.x) * kr2[r].x;
+      zq += double(z.x) * kq2[r].x;
+      if (gj + 1 < n0) {
+        zr += double(z.y) * kr2[r].y;
+        zq += double(z.y) * kq2[r].y;
       }
     }
   }

@@ -235,19 +235,26 @@ struct CloneArgs {
   int sh, sw, row, col, h, w, mode;
 };
 
+// Source value and mask at image pixel (i, j), zero outside the placed crop.  Loads are
+// branch-free (clamped addresses, selects) so a thread issues them all before it waits.
 __device__ inline double src_at(const CloneArgs& a, size_t sp, int i, int j) {
   const int si = i - a.row, sj = j - a.col;
-  return (si >= 0 && si < a.sh && sj >= 0 && sj < a.sw) ? a.src[sp + (size_t)si * a.sw + sj]
-                                                        : 0.0;
+  const bool in = si >= 0 && si < a.sh && sj >= 0 && sj < a.sw;
+  const double v =
+      a.src[sp + (size_t)min(max(si, 0), a.sh - 1) * a.sw + min(max(sj, 0), a.sw - 1)];
+  return in ? v : 0.0;
 }
 
 __device__ inline bool mask_at(const CloneArgs& a, int i, int j) {
   const int si = i - a.row, sj = j - a.col;
-  return si >= 0 && si < a.sh && sj >= 0 && sj < a.sw && a.mask[(size_t)si * a.sw + sj] != 0;
+  const bool in = si >= 0 && si < a.sh && sj >= 0 && sj < a.sw;
+  const uint8_t v = a.mask[(size_t)min(max(si, 0), a.sh - 1) * a.sw + min(max(sj, 0), a.sw - 1)];
+  return in && v != 0;
 }
 
 // Guidance (gx, gy) at image pixel (i, j); every pixel this is called on has its right
-// and lower neighbours inside the image.
+// and lower neighbours inside the image.  Both the target's and the source's differences
+// are formed and the mask and mode select between them.
 __device__ inline void clone_guidance(const CloneArgs& a, size_t sp, size_t tp, int i, int j,
                                       double& gx, double& gy) {
 #pragma clang fp contract(off)
@@ -255,24 +262,23 @@ __device__ inline void clone_guidance(const CloneArgs& a, size_t sp, size_t tp, 
   const double tc = t[(size_t)i * a.w + j];
   const double tgx = t[(size_t)i * a.w + j + 1] - tc;
   const double tgy = t[(size_t)(i + 1) * a.w + j] - tc;
-  if (!mask_at(a, i, j)) {
-    gx = tgx;
-    gy = tgy;
-    return;
-  }
+  const bool in = mask_at(a, i, j);
   const double sc = src_at(a, sp, i, j);
   const double sgx = src_at(a, sp, i, j + 1) - sc;
   const double sgy = src_at(a, sp, i + 1, j) - sc;
+  double ix, iy;
   if (a.mode == 1) {
-    gx = fabs(sgx) >= fabs(tgx) ? sgx : tgx;
-    gy = fabs(sgy) >= fabs(tgy) ? sgy : tgy;
+    ix = fabs(sgx) >= fabs(tgx) ? sgx : tgx;
+    iy = fabs(sgy) >= fabs(tgy) ? sgy : tgy;
   } else if (a.mode == 2) {
-    gx = 0.5 * (sgx + tgx);
-    gy = 0.5 * (sgy + tgy);
+    ix = 0.5 * (sgx + tgx);
+    iy = 0.5 * (sgy + tgy);
   } else {
-    gx = sgx;
-    gy = sgy;
+    ix = sgx;
+    iy = sgy;
   }
+  gx = in ? ix : tgx;
+  gy = in ? iy : tgy;
 }
 
 // Folded right-hand side of the window [top, top+hh) x [left, left+ww) of one plane:
@@ -301,14 +307,14 @@ __global__ __launch_bounds__(256) void k_clone_rhs(CloneArgs a, int top, int lef
     // masked domain (solve_masked, poisson.py:208-216): no equation off the mask; the
     // neighbours that are not unknowns give their target values, up, down, left, right
     const size_t gp = (size_t)gi * a.w + gj;
-    if (!interior[gp]) {
-      v = 0.0;
-    } else {
-      if (!interior[gp - a.w]) v -= t[gp - a.w];
-      if (!interior[gp + a.w]) v -= t[gp + a.w];
-      if (!interior[gp - 1]) v -= t[gp - 1];
-      if (!interior[gp + 1]) v -= t[gp + 1];
-    }
+    const double tu = t[gp - a.w], td = t[gp + a.w], tl = t[gp - 1], tr = t[gp + 1];
+    const bool iu = interior[gp - a.w], id = interior[gp + a.w];
+    const bool il = interior[gp - 1], ir = interior[gp + 1];
+    v = iu ? v : v - tu;
+    v = id ? v : v - td;
+    v = il ? v : v - tl;
+    v = ir ? v : v - tr;
+    v = interior[gp] ? v : 0.0;
   } else {
     if (i == 0) v -= t[(size_t)top * a.w + gj];
     if (i == mi - 1) v -= t[(size_t)(top + hh - 1) * a.w + gj];
@@ -336,18 +342,17 @@ __global__ __launch_bounds__(256) void k_masked_rhs(const double* __restrict__ g
   const size_t pl = (size_t)blockIdx.z * h * w;
   const int gi = top + 1 + i, gj = left + 1 + j;
   const size_t gp = (size_t)gi * w + gj;
-  double v = 0.0;
-  if (interior[gp]) {
-    const double* X = gx + pl;
-    const double* Y = gy + pl;
-    const double* t = img + pl;
-    v = (X[gp] - X[gp - 1]) + (Y[gp] - Y[gp - w]);
-    if (!interior[gp - w]) v -= t[gp - w];
-    if (!interior[gp + w]) v -= t[gp + w];
-    if (!interior[gp - 1]) v -= t[gp - 1];
-    if (!interior[gp + 1]) v -= t[gp + 1];
-  }
-  out[(size_t)blockIdx.z * mi * ni + (size_t)i * ni + j] = v;
+  // branch-free (every neighbour is inside the image): all loads issue before any wait
+  const double* X = gx + pl;
+  const double* Y = gy + pl;
+  const double* t = img + pl;
+  const double tu = t[gp - w], td = t[gp + w], tl = t[gp - 1], tr = t[gp + 1];
+  double v = (X[gp] - X[gp - 1]) + (Y[gp] - Y[gp - w]);
+  v = interior[gp - w] ? v : v - tu;
+  v = interior[gp + w] ? v : v - td;
+  v = interior[gp - 1] ? v : v - tl;
+  v = interior[gp + 1] ? v : v - tr;
+  out[(size_t)blockIdx.z * mi * ni + (size_t)i * ni + j] = interior[gp] ? v : 0.0;
 }
 
 // Element-wise guidance-field combinations (GuidanceField, poisson.py:52-92) over planes of

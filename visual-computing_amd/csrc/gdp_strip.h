@@ -200,14 +200,16 @@ __device__ inline uint64_t load_mask_bits(const uint8_t* __restrict__ mask, int 
   const int jx = min(max(gjx, 0), n - 1), jy = min(max(gjx + 1, 0), n - 1);
   const bool xin = work && gjx >= 0 && gjx < n, yin = work && gjx + 1 >= 0 && gjx + 1 < n;
   uint32_t bx[FH], by[FH];
+  if (mk) {
 #pragma unroll
-  for (int a = 0; a < FH; ++a) {
-    bx[a] = by[a] = 1;
-    if (mk) {
+    for (int a = 0; a < FH; ++a) {
       const size_t row = (size_t)min(max(gr0 + a, 0), m - 1) * n;
       bx[a] = mk[row + jx];
       by[a] = mk[row + jy];
     }
+  } else {
+#pragma unroll
+    for (int a = 0; a < FH; ++a) bx[a] = by[a] = 1;
   }
   uint32_t lo = 0, hi = 0;
 #pragma unroll

@@ -166,27 +166,23 @@ __global__ __launch_bounds__(kHdrThreads) void k_tm_rhs(const double* __restrict
   const size_t t = (size_t)blockIdx.x * kHdrThreads + threadIdx.x;
   if (t >= (size_t)h * w) return;
   const int i = int(t / w), j = int(t - (size_t)i * w);
-  auto fx = [&](int a, int b) {  // (gx f)(a, b), b < w - 1
-    const size_t o = (size_t)a * w + b;
-    return (L[o + 1] - L[o]) * f[o];
-  };
-  auto fy = [&](int a, int b) {  // (gy f)(a, b), a < h - 1
-    const size_t o = (size_t)a * w + b;
-    return (L[o + w] - L[o]) * f[o];
-  };
+  // (gx f) at (i, j) and (i, j - 1), (gy f) at (i, j) and (i - 1, j), each from clamped
+  // (in-bounds) addresses so every load issues before any wait; the cases below pick the
+  // terms that exist, as the branches of the divergence do
+  const int jr = min(j + 1, w - 1), jl = max(j - 1, 0);
+  const int id = min(i + 1, h - 1), iu = max(i - 1, 0);
+  const double* Li = L + (size_t)i * w;
+  const double* fi = f + (size_t)i * w;
+  const double Lc = Li[j], Lr = Li[jr], Ll = Li[jl];
+  const double Ld = L[(size_t)id * w + j], Lu = L[(size_t)iu * w + j];
+  const double fc = fi[j], fl = fi[jl], fu = f[(size_t)iu * w + j];
+  const double fxr = (Lr - Lc) * fc, fxl = (Lc - Ll) * fl;  // (gx f)(i, j), (i, j - 1)
+  const double fyd = (Ld - Lc) * fc, fyu = (Lc - Lu) * fu;  // (gy f)(i, j), (i - 1, j)
   // a one-column (one-row) image has no horizontal (vertical) differences: that term is
   // zero (operators.py:77-84)
   double dx = 0.0, dy = 0.0;
-  if (w > 1) {
-    if (j == 0) dx = fx(i, 0);
-    else if (j == w - 1) dx = -fx(i, w - 2);
-    else dx = fx(i, j) - fx(i, j - 1);
-  }
-  if (h > 1) {
-    if (i == 0) dy = fy(0, j);
-    else if (i == h - 1) dy = -fy(h - 2, j);
-    else dy = fy(i, j) - fy(i - 1, j);
-  }
+  if (w > 1) dx = j == 0 ? fxr : (j == w - 1 ? -fxl : fxr - fxl);
+  if (h > 1) dy = i == 0 ? fyd : (i == h - 1 ? -fyu : fyd - fyu);
   rhs[t] = dx + dy;
 }
 

@@ -40,7 +40,20 @@ __device__ inline double2 reduce_slots(const double2* __restrict__ partial, Slot
   const int lo = rg.begin + threadIdx.x * per;
   const int hi = min(lo + per, rg.begin + rg.count);
   double sx = 0.0, sy = 0.0;
-  for (int i = lo; i < hi; ++i) {
+  int i = lo;
+  // four loads in flight at a time, added in slot order as one at a time would be
+  for (; i + 4 <= hi; i += 4) {
+    const double2 v0 = partial[i], v1 = partial[i + 1], v2 = partial[i + 2], v3 = partial[i + 3];
+    sx += v0.x;
+    sy += v0.y;
+    sx += v1.x;
+    sy += v1.y;
+    sx += v2.x;
+    sy += v2.y;
+    sx += v3.x;
+    sy += v3.y;
+  }
+  for (; i < hi; ++i) {
     const double2 v = partial[i];
     sx += v.x;
     sy += v.y;

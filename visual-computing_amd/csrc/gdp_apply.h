@@ -62,9 +62,12 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
     const int gi = r0 - 1 + a;
     return gi >= 0 && gi < m && (y ? yin : xin);
   };
-  if (st[p].done) return;
+  // the finished-problem test follows the reduction's barrier, which keeps the loads
+  // above in front of it (a test before it lets the compiler sink them below the state
+  // load)
   const double2 z2 = reduce_slots<kThreads>(
       zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
+  if (st[p].done) return;
   double beta;
   if (st[p].iter == 0 || st[p].restart) beta = 0.0;
   else if (st[p].flexible) beta = -st[p].alpha * z2.y / st[p].delta_old;

@@ -3,6 +3,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Global address space: pointer accesses through it compile to global (not flat) memory
+// instructions, whose completions are counted apart from LDS and scalar loads.
+#define GDP_GLOBAL __attribute__((address_space(1)))
+
 namespace gdp {
 
 constexpr int kDirichlet = 0;
@@ -62,7 +66,8 @@ struct ProblemState {
   int32_t max_iter;
   int32_t flexible;
   int32_t checked;   // done, true residual checked, output written (PCG)
-  int32_t pad;
+  int32_t xpend;     // 1 + the direction buffer whose alpha d the last k_update left out
+                     // of x for the tail launch of the same iteration to add; 0: none
 };
 
 // One workgroup's work item of a V-cycle kernel: everything the block needs about its

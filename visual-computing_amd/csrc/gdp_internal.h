@@ -52,7 +52,9 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
 template <typename T>
 void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a);
 
-// The register-strip coarse tail (gdp_stail_*.hip): one 1024-thread block per problem.
+// The register-strip coarse tail (gdp_stail_*.hip): one 512-thread block per problem.
+constexpr int kStailBlockThreads = 512;  // threads of a strip-tail block
+constexpr int kStailXItems = 4;          // deferred-x items of a thread per round
 struct TailArgs {
   const TailDesc* tails;
   const LevelDesc* levels;
@@ -63,6 +65,11 @@ struct TailArgs {
   double2* zpart;
   const SlotRange* rz;
   int check_conv, pre, post, coarsest_sweeps, order;
+  // deferred x updates (st_xupdate): extra blocks past the tail blocks, over the nxt
+  // Krylov tiles of xth x xtw points at xt (xblocks = 0: none)
+  int xblocks = 0;
+  const KTile* xt = nullptr;
+  int nxt = 0, xth = 0, xtw = 0;
 };
 template <typename T>
 void launch_stail(bool dots, int blocks, int smem, hipStream_t s, const TailArgs& a);

@@ -374,10 +374,14 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
 //   k_update  reads the <d, q> partials -> alpha = delta / <d, q>;
 //             x += alpha d, r -= alpha q, V-cycle RHS -r, partial ||r||^2
 //   first V-cycle kernel: convergence test from the ||r||^2 partials
+// With defer_x the update leaves x += alpha d out (and says so in xpend) for the extra
+// blocks of the same iteration's tail launch (st_xupdate in gdp_stail.h), which run on
+// the CUs that the tail's one block per problem leaves idle; d stays valid until the
+// next iteration's apply writes the other direction buffer.
 // Each kernel reads only state fields written by earlier kernels and its lead block
 // writes only fields that no block of the same kernel reads.
 
-template <typename T, int TH, int TW>
+template <typename T, int TH, int TW, bool DEFER_X>
 __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ tiles,
                                                      ProblemState* __restrict__ st,
                                                      const double2* __restrict__ apart,
@@ -387,26 +391,39 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
   const int m = t.m, n = t.n, pitch = t.pitch;
-  T* __restrict__ b0 = static_cast<T*>(t.b0);
-  const double* __restrict__ d = dcur ? t.d1 : t.d0;
+  GDP_GLOBAL T* b0 = (GDP_GLOBAL T*)t.b0;
+  GDP_GLOBAL double* x = (GDP_GLOBAL double*)t.x;
+  GDP_GLOBAL double* r = (GDP_GLOBAL double*)t.r;
+  const GDP_GLOBAL double* q = (const GDP_GLOBAL double*)t.q;
+  const GDP_GLOBAL double* d = (const GDP_GLOBAL double*)(dcur ? t.d1 : t.d0);
   constexpr int NO = (TH * TW) / kThreads;
   double xl[NO], dl[NO], rl[NO], ql[NO];
   size_t ol[NO];
   bool il[NO];
+  // branch-free loads (clamped in-bounds addresses; the points outside are not used), so
+  // every load is issued before the first wait
 #pragma unroll
   for (int it = 0; it < NO; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     il[it] = gi < m && gj < n;
-    ol[it] = il[it] ? (size_t)gi * pitch + gj : 0;
-    xl[it] = il[it] ? t.x[ol[it]] : 0.0;
-    dl[it] = il[it] ? d[ol[it]] : 0.0;
-    rl[it] = il[it] ? t.r[ol[it]] : 0.0;
-    ql[it] = il[it] ? t.q[ol[it]] : 0.0;
+    ol[it] = (size_t)min(gi, m - 1) * pitch + min(gj, n - 1);
+    if constexpr (!DEFER_X) {
+      xl[it] = x[ol[it]];
+      dl[it] = d[ol[it]];
+    }
+    rl[it] = r[ol[it]];
+    ql[it] = q[ol[it]];
   }
-  if (st[p].done) return;
+  // the finished-problem test comes after the reduction's barrier, which keeps the
+  // loads above in front of it (a test before it lets the compiler sink them below the
+  // state load)
   const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, sh, &bc);
+  if (st[p].done) {
+    if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
+    return;
+  }
   const double delta = st[p].delta;
   const double alpha = delta / a2.x;
   if (t.lead && threadIdx.x == 0) {
@@ -415,16 +432,17 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
     st[p].delta_old = delta;
     st[p].iter += 1;
     st[p].restart = 0;
+    st[p].xpend = DEFER_X ? dcur + 1 : 0;
   }
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < NO; ++it) {
     if (!il[it]) continue;
-    const double r = rl[it] - alpha * ql[it];
-    t.x[ol[it]] = xl[it] + alpha * dl[it];
-    t.r[ol[it]] = r;
-    b0[ol[it]] = T(-r);
-    acc += r * r;
+    const double rn = rl[it] - alpha * ql[it];
+    if constexpr (!DEFER_X) x[ol[it]] = xl[it] + alpha * dl[it];
+    r[ol[it]] = rn;
+    b0[ol[it]] = T(-rn);
+    acc += rn * rn;
   }
   write_partial<kThreads>(acc, 0.0, upart, t.slot, sh);
 }
@@ -629,6 +647,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
     S.done = S.converged || max_iter <= 0;
     S.restart = 0;
     S.checked = 0;
+    S.xpend = 0;
     S.alpha = S.beta = S.delta = S.delta_old = 0.0;
     S.rr = tot.x;
     S.max_iter = max_iter;

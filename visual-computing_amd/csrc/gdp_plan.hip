@@ -144,6 +144,7 @@ struct gdp_plan {
   int tail_smem = 0;   // dynamic LDS of the LDS tail (k_tail)
   int stail_smem = 0;  // dynamic LDS of the register-strip tail (k_stail)
   int n_stail = 0;     // strip-tail problems: the first n_stail entries of the tail table
+  int n_cu = 0;        // compute units of the bound device (blocks for deferred x updates)
   int max_slots = 1;
   size_t ws_bytes = 0;
   size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0;
@@ -686,6 +687,11 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   }
   P->ws = static_cast<char*>(workspace);
   P->io_valid = false;  // the pointer tables live in the workspace
+  {
+    int dev = 0;
+    GDP_HIP(hipGetDevice(&dev));
+    GDP_HIP(hipDeviceGetAttribute(&P->n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+  }
   const int NP = P->count();
   if (!P->h_block) {
     auto a64 = [](size_t x) { return (x + 63) & ~size_t(63); };
@@ -1096,8 +1102,11 @@ void prof_collect(gdp_plan* P) {
   P->prof_pending.clear();
 }
 
+// xupd: the previous k_update deferred its x half (defer_x); the strip-tail launch adds
+// it in blocks of its own.
 template <typename T>
-int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
+int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
+                     bool xupd = false) {
   const gdp_options& o = P->opt;
   for (int l = 0; l < P->depth; ++l) {
     if (!P->down[l].count) continue;
@@ -1116,13 +1125,26 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv) {
   for (const auto& pi : P->probs)
     tail_pts += double(pi.lv[pi.tail_first].m) * pi.lv[pi.tail_first].n;
   {
-    ProfScope ps(P, s, "tail", 2.0 * sizeof(T) * tail_pts);
+    ProfScope ps(P, s, "tail",
+                 2.0 * sizeof(T) * tail_pts + (xupd ? 24.0 * krylov_points(P) : 0.0));
     const int cc = check_conv ? 1 : 0;
     const int ns = P->n_stail, no = P->count() - P->n_stail;
     if (ns) {
       TailArgs a{P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
                  P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
                  o.post_smoothing, P->csweeps, order};
+      if (xupd) {
+        // one block on each CU the tail blocks leave free, at most one per kXItems items
+        const Shape2 ks = kKryMenu[P->kry_menu];
+        const int items =
+            P->kry.count * ((ks.h * ks.w / 2 + kStailBlockThreads - 1) / kStailBlockThreads);
+        a.xblocks =
+            std::max(1, std::min(P->n_cu - ns, (items + kStailXItems - 1) / kStailXItems));
+        a.xt = P->at<KTile>(P->kry.off);
+        a.nxt = P->kry.count;
+        a.xth = ks.h;
+        a.xtw = ks.w;
+      }
       launch_stail<T>(dots, ns, P->stail_smem, s, a);
       GDP_CHECK_LAUNCH();
     }
@@ -1191,11 +1213,20 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
                                                                P->d_apart(), din, jacobi ? 1 : 0);
     GDP_CHECK_LAUNCH();
   }
+  // the x half of the update goes to the strip-tail launch of this iteration's V-cycle,
+  // when there is one (not after the last update of a batch, not for Jacobi)
+  const bool defer_x = !jacobi && !last && P->n_stail > 0;
   {
-    ProfScope ps(P, s, "update", (48.0 + sizeof(T)) * N);
-    GDP_KRY_DISPATCH(P->kry_menu,
-                     (klaunch(k_update<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
-                         t, P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
+    ProfScope ps(P, s, "update", ((defer_x ? 24.0 : 48.0) + sizeof(T)) * N);
+    if (defer_x) {
+      GDP_KRY_DISPATCH(P->kry_menu,
+                       (klaunch(k_update<T, TH, TW, true>, dim3(nb), dim3(kThreads), 0, s, t,
+                                P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
+    } else {
+      GDP_KRY_DISPATCH(P->kry_menu,
+                       (klaunch(k_update<T, TH, TW, false>, dim3(nb), dim3(kThreads), 0, s, t,
+                                P->d_state(), P->d_apart(), P->d_upart(), din ^ 1)));
+    }
     GDP_CHECK_LAUNCH();
   }
   if (last) {
@@ -1205,7 +1236,7 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
     return GDP_OK;
   }
   if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
-  return enqueue_vcycle_t<T>(P, s, true, true);
+  return enqueue_vcycle_t<T>(P, s, true, true, defer_x);
 }
 
 int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false) {

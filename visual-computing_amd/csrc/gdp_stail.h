@@ -478,6 +478,56 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
   st_publish<T, RD, NEU ? kGhostIter : kGhostZero>(c, u, m, L.n);
 }
 
+// The x half of a PCG update that k_update deferred (solvers.py:151, x += alpha d): run
+// by the blocks of a tail launch past its tail blocks, on CUs the tail leaves idle.
+// Block xb of nxb walks the Krylov tiles (th x tw points each) in items of one column
+// pair per thread and kXItems items at a time, every load issued before the first
+// wait; a problem whose xpend is zero is left alone.  Pairs are aligned: tile columns
+// start at multiples of tw and the row pitch is even.
+constexpr int kXItems = 4;
+__device__ __attribute__((always_inline)) void st_xupdate(const KTile* __restrict__ tiles,
+                                                        int ntiles,
+                                                        const ProblemState* __restrict__ st,
+                                                        int th, int tw, int xb, int nxb) {
+  using D2 = typename Pair<double>::type;
+  const int ppr = tw / 2;                                        // pairs per tile row
+  const int cpt = (th * ppr + kSTailThreads - 1) / kSTailThreads;  // items per tile
+  const int nitems = ntiles * cpt;
+  for (int w0 = xb * kXItems; w0 < nitems; w0 += nxb * kXItems) {
+    D2 xv[kXItems], dv[kXItems];
+    double al[kXItems];
+    GDP_GLOBAL double* xp[kXItems];
+    size_t o[kXItems];
+    bool ok[kXItems], ok1[kXItems];
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u) {
+      const int w = min(w0 + u, nitems - 1);
+      const int ti = w / cpt;
+      const KTile& t = tiles[ti];
+      const int pend = st[t.problem].xpend;
+      const int e = (w - ti * cpt) * kSTailThreads + threadIdx.x;
+      const int a = e / ppr;
+      const int gi = t.i0 + a, gj = t.j0 + 2 * (e - a * ppr);
+      ok[u] = w0 + u < nitems && pend != 0 && a < th && gi < t.m && gj < t.n;
+      ok1[u] = ok[u] && gj + 1 < t.n;
+      o[u] = ok[u] ? (size_t)gi * t.pitch + gj : 0;
+      xp[u] = (GDP_GLOBAL double*)t.x;
+      const GDP_GLOBAL double* d = (const GDP_GLOBAL double*)(pend == 2 ? t.d1 : t.d0);
+      xv[u] = *(const GDP_GLOBAL D2*)(xp[u] + o[u]);
+      dv[u] = *(const GDP_GLOBAL D2*)(d + o[u]);
+      al[u] = st[t.problem].alpha;
+    }
+#pragma unroll
+    for (int u = 0; u < kXItems; ++u) {
+      if (ok1[u]) {
+        *(GDP_GLOBAL D2*)(xp[u] + o[u]) = D2{xv[u].x + al[u] * dv[u].x, xv[u].y + al[u] * dv[u].y};
+      } else if (ok[u]) {
+        xp[u][o[u]] = xv[u].x + al[u] * dv[u].x;
+      }
+    }
+  }
+}
+
 template <typename T, bool DOTS>
 __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restrict__ tails,
                                                         const LevelDesc* __restrict__ levels,
@@ -488,12 +538,18 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
                                                         double2* __restrict__ zpart,
                                                         const SlotRange* __restrict__ rz,
                                                         int check_conv, int pre, int post,
-                                                        int coarsest_sweeps, int order) {
+                                                        int coarsest_sweeps, int order,
+                                                        int ntail, const KTile* __restrict__ xt,
+                                                        int nxt, int xth, int xtw) {
   using V2 = typename Pair<T>::type;
   constexpr int R0 = STailRows<T>::kRows0;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double sh[2 * kSTailThreads / 64];
   __shared__ double2 bcc;
+  if (int(blockIdx.x) >= ntail) {  // the deferred x updates
+    st_xupdate(xt, nxt, st, xth, xtw, int(blockIdx.x) - ntail, int(gridDim.x) - ntail);
+    return;
+  }
   // descriptors through scalar loads (uniform addresses of read-only kernel arguments)
   const TailDesc& D = tails[blockIdx.x];
   const int p = D.problem;

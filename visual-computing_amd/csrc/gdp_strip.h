@@ -31,8 +31,6 @@
 
 namespace gdp {
 
-#define GDP_GLOBAL __attribute__((address_space(1)))
-
 template <typename T>
 struct Pair;
 template <>
@@ -407,6 +405,9 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
       hb[a] = in ? v : V2{T(0), T(0)};
     }
   }
+  // a compiler-only fence: it keeps the loads above from being sunk below the state load
+  // of the finished-problem test (which would add its round trip before them)
+  asm volatile("" ::: "memory");
   if (st[t.problem].done) return;
   if (check_conv && t.level == 0) {
     // convergence test of the preceding PCG update (the loads above are in flight)

@@ -240,13 +240,17 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
             for key in ("k_sdown<", "k_sup<", "k_down<", "k_up<"):
                 if key in base:
                     grids[key].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
-        vals = collections.defaultdict(list)
+        # grouped by launch shape: a kind with two shapes (the tail with and without the
+        # deferred x update) averages over both, as its algorithmic bytes per launch do
+        vals = collections.defaultdict(lambda: collections.defaultdict(list))
         for r in rows:
             kind = kernel_kind(r["Kernel_Name"], int(r["Grid_Size"]), grids)
             if kind:
-                vals[kind].append(float(r["Counter_Value"]) * 1024 * scale)
-        for kind, v in vals.items():
-            full = [x for x in v if x >= 0.5 * max(v)] or v
+                vals[kind][int(r["Grid_Size"])].append(float(r["Counter_Value"]) * 1024 * scale)
+        for kind, by_grid in vals.items():
+            full = []
+            for v in by_grid.values():
+                full += [x for x in v if x >= 0.5 * max(v)] or v
             per[kind][counter] = sum(full) / len(full)
     for kind, d in per.items():
         f, w = d.get("FETCH_SIZE", 0.0), d.get("WRITE_SIZE", 0.0)

@@ -19,6 +19,7 @@
 // solvers.py:224-262 (Neumann) and its DST-I analogue (Dirichlet).
 #pragma once
 #include "gdp_dct.h"
+#include "gdp_internal.h"
 #include "gdp_strip.h"
 
 namespace gdp {
@@ -484,7 +485,7 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
 // pair per thread and kXItems items at a time, every load issued before the first
 // wait; a problem whose xpend is zero is left alone.  Pairs are aligned: tile columns
 // start at multiples of tw and the row pitch is even.
-constexpr int kXItems = 4;
+constexpr int kXItems = kStailXItems;  // one definition, shared with the host sizing
 __device__ __attribute__((always_inline)) void st_xupdate(const KTile* __restrict__ tiles,
                                                         int ntiles,
                                                         const ProblemState* __restrict__ st,

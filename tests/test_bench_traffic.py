@@ -1,0 +1,59 @@
+"""The bench's per-kernel HBM traffic reduction (rocprofv3 FETCH_SIZE / WRITE_SIZE CSVs).
+
+Synthetic counter files: a 512 MiB calibration copy, tail launches of two shapes (with and
+without the deferred x update) and update launches one of which exited early.
+"""
+import csv
+import importlib.util
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", ROOT / "bench.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _write(path, counter, rows):
+    with open(path, "w", newline="") as f:
+        w = csv.DictWriter(f, fieldnames=["Kernel_Name", "Grid_Size", "Counter_Name",
+                                          "Counter_Value"])
+        w.writeheader()
+        for name, grid, kib in rows:
+            w.writerow({"Kernel_Name": name, "Grid_Size": grid, "Counter_Name": counter,
+                        "Counter_Value": kib})
+
+
+def test_traffic_filters_early_exits_per_launch_shape(tmp_path):
+    bench = _bench()
+    copy_kib = bench.COPY_BYTES / 1024
+    tail = "void gdp::k_stail<float, true>(...)"
+    upd = "void gdp::k_update<float, 16, 64, true>(...)"
+    # FETCH reads half the copy's bytes (scale 2), WRITE all of them (scale 1)
+    fetch_rows = [("__amd_rocclr_copyBuffer", 1 << 20, copy_kib / 2)]
+    write_rows = [("__amd_rocclr_copyBuffer", 1 << 20, copy_kib)]
+    # five tail launches with the x update (100 KiB fetched), one without (1 KiB)
+    fetch_rows += [(tail, 131072, 100.0)] * 5 + [(tail, 1536, 1.0)]
+    write_rows += [(tail, 131072, 50.0)] * 5 + [(tail, 1536, 0.5)]
+    # three full update launches and one that exited early (left out of the mean)
+    fetch_rows += [(upd, 786432, 80.0)] * 3 + [(upd, 786432, 0.2)]
+    write_rows += [(upd, 786432, 40.0)] * 3 + [(upd, 786432, 0.1)]
+    fcsv, wcsv = tmp_path / "fetch.csv", tmp_path / "write.csv"
+    _write(fcsv, "FETCH_SIZE", fetch_rows)
+    _write(wcsv, "WRITE_SIZE", write_rows)
+
+    out = bench.reduce_traffic(str(fcsv), str(wcsv))
+    assert out["calibration"]["FETCH_SIZE"]["scale"] == pytest.approx(2.0)
+    assert out["calibration"]["WRITE_SIZE"]["scale"] == pytest.approx(1.0)
+    k = out["kernels"]
+    # both tail shapes count, as the tail's algorithmic bytes average over them
+    assert k["tail"]["fetch_bytes"] == round((5 * 100.0 + 1.0) / 6 * 1024 * 2)
+    assert k["tail"]["write_bytes"] == round((5 * 50.0 + 0.5) / 6 * 1024)
+    # the early-exited update is dropped
+    assert k["update"]["fetch_bytes"] == round(80.0 * 1024 * 2)
+    assert k["update"]["hbm_bytes_per_launch"] == round(80.0 * 1024 * 2 + 40.0 * 1024)

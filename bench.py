@@ -240,13 +240,16 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
             for key in ("k_sdown<", "k_sup<", "k_down<", "k_up<"):
                 if key in base:
                     grids[key].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
-        # grouped by launch shape: a kind with two shapes (the tail with and without the
-        # deferred x update) averages over both, as its algorithmic bytes per launch do
+        # grouped by instantiation and launch shape: a kind with two of them (the tail
+        # with and without the deferred x update, k_update with and without its x half)
+        # averages over both, as its algorithmic bytes per launch do; the early-exit
+        # filter then compares each dispatch only with its own variant
         vals = collections.defaultdict(lambda: collections.defaultdict(list))
         for r in rows:
             kind = kernel_kind(r["Kernel_Name"], int(r["Grid_Size"]), grids)
             if kind:
-                vals[kind][int(r["Grid_Size"])].append(float(r["Counter_Value"]) * 1024 * scale)
+                variant = (r["Kernel_Name"].split("(")[0], int(r["Grid_Size"]))
+                vals[kind][variant].append(float(r["Counter_Value"]) * 1024 * scale)
         for kind, by_grid in vals.items():
             full = []
             for v in by_grid.values():

@@ -43,6 +43,9 @@
  *   gdp_percentile       hdr.py:184            numpy.percentile(image, q), linear method
  *   gdp_tm_normalise     hdr.py:185            clip(image / max(peak, 1e-12), 0, 1)
  *   gdp_neumann_dct_solve solvers.py:224-262   solve_neumann(b): idctn(dctn(b)/lambda), exact
+ *   gdp_dirichlet_dst_solve solvers.py:85-90   _solve_direct (SOLVERS["direct"]): the exact
+ *                        solve of sparse_laplacian(b.shape, spacing) u = b
+ *                        (operators.py:110-136), by DST-I diagonalisation instead of SuperLU
  */
 #ifndef GDPOISSON_H
 #define GDPOISSON_H
@@ -197,6 +200,19 @@ int gdp_dct_tables(int m, int n, double* tables, void* stream);
 int gdp_dct_work_bytes(int m, int n, size_t* bytes);
 int gdp_neumann_dct_solve(const double* b, double* u, int m, int n, const double* tables,
                           double* work, void* stream);
+
+/* ---------------------------------------------------------------- exact Dirichlet solve
+ * u (m x n) = the solution of laplacian(u, spacing) = b with zeros outside the grid (the
+ * reference's direct method, which factorises sparse_laplacian with SuperLU), by the
+ * orthonormal DST-I: u = S_m ((S_m b S_n) ./ lambda) S_n.  Sides m with 2 (m + 1) <= 4096
+ * and 2 (m + 1) a product of 2, 3 and 5 only (GDP_ENOTSUP otherwise).  `tables`
+ * (gdp_dst_tables_bytes, filled once per shape by gdp_dst_tables, which synchronises) and
+ * `work` (gdp_dct_work_bytes: 2 m n doubles) are device buffers; u may not alias b. */
+int gdp_dst_supported(int m, int n);
+int gdp_dst_tables_bytes(int m, int n, size_t* bytes);
+int gdp_dst_tables(int m, int n, double* tables, void* stream);
+int gdp_dirichlet_dst_solve(const double* b, double* u, int m, int n, double spacing,
+                            const double* tables, double* work, void* stream);
 
 /* In-place red-black Gauss-Seidel: `sweeps` sweeps, each colour 0 then 1
    (order = 0) or colour 1 then 0 (order = 1). */

@@ -57,3 +57,27 @@ def test_traffic_filters_early_exits_per_launch_shape(tmp_path):
     # the early-exited update is dropped
     assert k["update"]["fetch_bytes"] == round(80.0 * 1024 * 2)
     assert k["update"]["hbm_bytes_per_launch"] == round(80.0 * 1024 * 2 + 40.0 * 1024)
+
+
+def test_traffic_keeps_the_two_update_variants_apart(tmp_path):
+    """k_update with its x half deferred moves about half the bytes of the full update
+    on the same grid: each variant is filtered against its own maximum, so the deferred
+    launches are not taken for early exits, and the mean covers both."""
+    bench = _bench()
+    copy_kib = bench.COPY_BYTES / 1024
+    full = "void gdp::k_update<float, 16, 64, false>(...)"
+    deferred = "void gdp::k_update<float, 16, 64, true>(...)"
+    fetch_rows = [("__amd_rocclr_copyBuffer", 1 << 20, copy_kib / 2)]
+    write_rows = [("__amd_rocclr_copyBuffer", 1 << 20, copy_kib)]
+    # one full update, four deferred ones (0.45 of the full bytes: below half of the
+    # full update's count) and one deferred launch that exited early
+    fetch_rows += [(full, 786432, 100.0)] + [(deferred, 786432, 45.0)] * 4 + \
+        [(deferred, 786432, 0.1)]
+    write_rows += [(full, 786432, 50.0)] + [(deferred, 786432, 22.5)] * 4 + \
+        [(deferred, 786432, 0.05)]
+    fcsv, wcsv = tmp_path / "fetch.csv", tmp_path / "write.csv"
+    _write(fcsv, "FETCH_SIZE", fetch_rows)
+    _write(wcsv, "WRITE_SIZE", write_rows)
+    k = bench.reduce_traffic(str(fcsv), str(wcsv))["kernels"]
+    assert k["update"]["fetch_bytes"] == round((100.0 + 4 * 45.0) / 5 * 1024 * 2)
+    assert k["update"]["write_bytes"] == round((50.0 + 4 * 22.5) / 5 * 1024)

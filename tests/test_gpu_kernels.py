@@ -282,6 +282,30 @@ def test_batched_uneven_problems_match_each_alone(gd, precision):
         assert np.abs(together[k].cpu().numpy() - exact[k]).max() < 1e-6
 
 
+@pytest.mark.parametrize("count,low,high", [(300, 24, 96), (200, 200, 300)])
+def test_more_problems_than_free_cus_match_each_alone(gd, count, low, high):
+    """More strip-tail problems than CUs (300 small grids), and 200 grids whose Krylov
+    points are too many for the CUs the tail leaves free: the x half of the update is
+    then not deferred to the tail launch, while each problem alone defers it.  Results
+    are bit for bit the same either way, and every problem converges."""
+    import torch
+
+    shapes = G.batch_shapes(count=count, seed=4200 + count, low=low, high=high)
+    bs, exact = [], []
+    for k, s in enumerate(shapes):
+        b, x = G.neumann_problem(s, seed=4300 + k)
+        bs.append(torch.from_numpy(b).cuda())
+        exact.append(x)
+    opts = gd.PlanOptions(precision="mixed")
+    together, reps = gd.Plan(shapes, boundary="neumann", options=opts).mgcg(bs)
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    for k in (0, 1, count // 2, count - 1):
+        alone, rep1 = gd.Plan([shapes[k]], boundary="neumann", options=opts).mgcg([bs[k]])
+        assert reps[k].iterations == rep1[0].iterations
+        assert torch.equal(together[k], alone[0]), (k, shapes[k])
+        assert np.abs(together[k].cpu().numpy() - exact[k]).max() < 1e-6
+
+
 @pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
 @pytest.mark.parametrize("menu", [0, 1, 2, 3])
 def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):

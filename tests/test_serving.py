@@ -103,6 +103,29 @@ def test_a_request_larger_than_the_budget_is_rejected():
     assert st["served"] == 1 and st["rejected"] == 1
 
 
+def test_a_failing_batch_fails_its_requests_only():
+    """An executor that raises (a plan that cannot be built, a shape the kernels do not
+    support) gives error completions for that batch; the loop keeps serving the rest and
+    hands the batch's budget back."""
+    inner = FakeDevice()
+
+    def flaky(batch):
+        if any(r.args.get("poison") for r in batch):
+            raise NotImplementedError("coarsest level does not fit in LDS")
+        return inner(batch)
+
+    reqs = [_req("tonemap", 60, 0.002, 0.0), _req("composite", 60, 0.002, 0.001),
+            _req("tonemap", 60, 0.002, 0.002), _req("composite", 60, 0.002, 0.003)]
+    reqs[1].args["poison"] = True
+    srv = Server(budget_bytes=100, slots=1, max_batch=1, executor=flaky)
+    done = srv.run(reqs)
+    assert [c.rid for c in done] == [0, 1, 2, 3]
+    assert "NotImplementedError" in done[1].error and done[1].output is None
+    assert all(done[i].error is None and done[i].output == i for i in (0, 2, 3))
+    # every request used the whole budget alone: the failed batch released its share
+    assert srv.peak_bytes == 60
+
+
 def test_latency_stats():
     from gdpoisson.serving import Completion
 

@@ -13,6 +13,18 @@
 // transposed, so four passes make the forward and inverse 2-D transforms in the original
 // orientation; the division by lambda is fused into the second pass.  Sides of at most
 // 4096 points whose only prime factors are 2, 3 and 5 are supported.
+//
+// Exact Dirichlet solve (the rectangle operator of operators.sparse_laplacian,
+// operators.py:110-136, which solvers._solve_direct factorises with SuperLU,
+// solvers.py:85-90): the orthonormal DST-I diagonalises it,
+//
+//   u = S_m ( (S_m b S_n) ./ lambda ) S_n,  lambda_kl = (2 cos(pi (k+1) / (m+1)) +
+//                                                        2 cos(pi (l+1) / (n+1)) - 4) / h^2,
+//
+// S symmetric and its own inverse.  A DST-I of length L is the imaginary part of the FFT
+// of the row's odd extension (length 2 (L + 1)); two rows share one complex FFT as its
+// real and imaginary parts.  Sides L with 2 (L + 1) <= 4096 and 2 (L + 1) a product of
+// 2, 3 and 5 are supported.
 #include "gdpoisson.h"
 #include "gdp_internal.h"
 
@@ -215,6 +227,72 @@ __global__ __launch_bounds__(kFftThreads) void k_dct3_rows(const double* __restr
   }
 }
 
+// Table of one dimension of the DST-I of length L (FFT length N = 2 (L + 1), 4 (L + 1) + L
+// doubles): exp(-2 pi i t / N) for t < N, then 2 cos(pi (k + 1) / (L + 1)) for k < L.
+size_t dst_table_doubles(int L) { return size_t(4) * (L + 1) + L; }
+
+bool dst_factor(int L, FftShape& f) {
+  return L >= 1 && L < kFftMaxN / 2 && factor(2 * (L + 1), f);
+}
+
+// Orthonormal DST-I of each row of in (rows x L), out (L x rows) = transposed.  Block b
+// transforms rows 2b and 2b + 1 as the real and imaginary parts of one FFT of the odd
+// extension z = (0, x_0 .. x_{L-1}, 0, -x_{L-1} .. -x_0): a real row's transform is
+// Z_{k+1} = -2i X_k, so X_a[k] = -Im Z_{k+1} / 2 and X_b[k] = Re Z_{k+1} / 2.  With
+// lam_other (the other dimension's 2 cos table), the coefficient of (row r, index k) is
+// multiplied by h2 / (tab_lam[k] + lam_other[r] - 4), the inverse eigenvalue.
+__global__ __launch_bounds__(kFftThreads) void k_dst1_rows(const double* __restrict__ in,
+                                                           double* __restrict__ out, int rows,
+                                                           int L, FftShape f,
+                                                           const double* __restrict__ tab,
+                                                           const double* __restrict__ lam_other,
+                                                           double h2) {
+  extern __shared__ double2 fbuf[];
+  const int N = f.N;
+  const int r0 = 2 * blockIdx.x, r1 = r0 + 1;
+  const bool two = r1 < rows;
+  const double2* tw = reinterpret_cast<const double2*>(tab);
+  const double* lam = tab + 2 * N;
+  const double* xa = in + size_t(r0) * L;
+  const double* xb = in + size_t(two ? r1 : r0) * L;
+  for (int t = threadIdx.x; t < N; t += kFftThreads) {
+    double2 v{0.0, 0.0};
+    if (t >= 1 && t <= L) {
+      v.x = xa[t - 1];
+      v.y = two ? xb[t - 1] : 0.0;
+    } else if (t >= L + 2) {
+      const int j = N - 1 - t;
+      v.x = -xa[j];
+      v.y = two ? -xb[j] : 0.0;
+    }
+    fbuf[t] = v;
+  }
+  __syncthreads();
+  const double2* Z = fft(fbuf, f, tw);
+  const double sc = 0.5 * sqrt(2.0 / double(L + 1));
+  for (int k = threadIdx.x; k < L; k += kFftThreads) {
+    const double2 z = Z[k + 1];
+    double a = -sc * z.y, b = sc * z.x;
+    if (lam_other) {
+      a = a * h2 / (lam[k] + lam_other[r0] - 4.0);
+      if (two) b = b * h2 / (lam[k] + lam_other[r1] - 4.0);
+    }
+    out[size_t(k) * rows + r0] = a;
+    if (two) out[size_t(k) * rows + r1] = b;
+  }
+}
+
+void fill_dst_table(int L, double* t) {
+  const double pi = 3.14159265358979323846;
+  const int N = 2 * (L + 1);
+  for (int k = 0; k < N; ++k) {
+    const double a = -2.0 * pi * double(k) / double(N);
+    t[2 * k] = std::cos(a);
+    t[2 * k + 1] = std::sin(a);
+  }
+  for (int k = 0; k < L; ++k) t[2 * N + k] = 2.0 * std::cos(pi * double(k + 1) / double(L + 1));
+}
+
 void fill_table(int N, double* t) {
   const double pi = 3.14159265358979323846;
   for (int k = 0; k < N; ++k) {
@@ -305,6 +383,77 @@ int gdp_neumann_dct_solve(const double* b, double* u, int m, int n, const double
   k_dct3_rows<<<m, kFftThreads, lds_bytes(n), s>>>(w1, w0, m, fn, tn);
   GDP_FFT_HIP(hipGetLastError());
   k_dct3_rows<<<n, kFftThreads, lds_bytes(m), s>>>(w0, u, n, fm, tm);
+  GDP_FFT_HIP(hipGetLastError());
+  return GDP_OK;
+}
+
+
+int gdp_dst_supported(int m, int n) {
+  FftShape a, b;
+  return dst_factor(m, a) && dst_factor(n, b) ? 1 : 0;
+}
+
+int gdp_dst_tables_bytes(int m, int n, size_t* bytes) {
+  FftShape a, b;
+  if (!bytes || !dst_factor(m, a) || !dst_factor(n, b)) {
+    set_error("gdp_dst: sides m need 2 (m + 1) <= %d with prime factors 2, 3, 5 only "
+              "(got %d x %d)", kFftMaxN, m, n);
+    return GDP_ENOTSUP;
+  }
+  *bytes = sizeof(double) * (dst_table_doubles(m) + dst_table_doubles(n));
+  return GDP_OK;
+}
+
+int gdp_dst_tables(int m, int n, double* tables, void* stream) {
+  size_t bytes = 0;
+  if (int rc = gdp_dst_tables_bytes(m, n, &bytes)) return rc;
+  if (!tables) return (set_error("null tables"), GDP_EINVAL);
+  std::vector<double> h(bytes / sizeof(double));
+  fill_dst_table(m, h.data());
+  fill_dst_table(n, h.data() + dst_table_doubles(m));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  GDP_FFT_HIP(hipMemcpyAsync(tables, h.data(), bytes, hipMemcpyHostToDevice, s));
+  GDP_FFT_HIP(hipStreamSynchronize(s));  // h is released on return
+  return GDP_OK;
+}
+
+int gdp_dirichlet_dst_solve(const double* b, double* u, int m, int n, double spacing,
+                            const double* tables, double* work, void* stream) {
+  FftShape fm, fn;
+  if (!b || !u || !tables || !work || !(spacing > 0.0))
+    return (set_error("null argument or spacing <= 0"), GDP_EINVAL);
+  if (!dst_factor(m, fm) || !dst_factor(n, fn)) {
+    set_error("gdp_dst: sides m need 2 (m + 1) <= %d with prime factors 2, 3, 5 only "
+              "(got %d x %d)", kFftMaxN, m, n);
+    return GDP_ENOTSUP;
+  }
+  static bool lds_set = false;
+  if (!lds_set) {
+    GDP_FFT_HIP(hipFuncSetAttribute((const void*)k_dst1_rows,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds_bytes(kFftMaxN))));
+    lds_set = true;
+  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const double* tm = tables;
+  const double* tn = tables + dst_table_doubles(m);
+  const double* lam_n = tn + 2 * size_t(fn.N);
+  const double h2 = spacing * spacing;
+  double* w0 = work;                  // n x m
+  double* w1 = work + size_t(m) * n;  // m x n
+  // S along n: b (m x n) -> w0 (n x m); S along m and 1 / lambda: w0 -> w1 (m x n)
+  k_dst1_rows<<<(m + 1) / 2, kFftThreads, lds_bytes(fn.N), s>>>(b, w0, m, n, fn, tn, nullptr,
+                                                                 h2);
+  GDP_FFT_HIP(hipGetLastError());
+  k_dst1_rows<<<(n + 1) / 2, kFftThreads, lds_bytes(fm.N), s>>>(w0, w1, n, m, fm, tm, lam_n,
+                                                                 h2);
+  GDP_FFT_HIP(hipGetLastError());
+  // S along n: w1 -> w0 (n x m); S along m: w0 -> u (m x n)
+  k_dst1_rows<<<(m + 1) / 2, kFftThreads, lds_bytes(fn.N), s>>>(w1, w0, m, n, fn, tn, nullptr,
+                                                                 h2);
+  GDP_FFT_HIP(hipGetLastError());
+  k_dst1_rows<<<(n + 1) / 2, kFftThreads, lds_bytes(fm.N), s>>>(w0, u, n, m, fm, tm, nullptr,
+                                                                 h2);
   GDP_FFT_HIP(hipGetLastError());
   return GDP_OK;
 }

@@ -1102,6 +1102,19 @@ void prof_collect(gdp_plan* P) {
   P->prof_pending.clear();
 }
 
+// Whether k_update leaves its x half (read x and d, write x: 24 B per Krylov point) to
+// extra blocks of the strip-tail launch.  Those blocks run on the CUs the one block per
+// tail problem leaves free, while the latency-bound tail runs (about 20 us); one CU
+// streams about 40 GB/s, so 1 MiB per free CU is what they finish in that time.  With
+// more, the x update would outlast the tail on too few CUs (a batch of about as many
+// problems as CUs), and the whole-chip k_update does it instead.
+constexpr double kXBytesPerFreeCu = 1048576.0;
+bool defer_x_update(const gdp_plan* P) {
+  const int free_cu = P->n_cu - P->n_stail;
+  if (P->n_stail <= 0 || free_cu <= 0) return false;
+  return 24.0 * krylov_points(P) <= kXBytesPerFreeCu * free_cu;
+}
+
 // xupd: the previous k_update deferred its x half (defer_x); the strip-tail launch adds
 // it in blocks of its own.
 template <typename T>
@@ -1215,7 +1228,7 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
   }
   // the x half of the update goes to the strip-tail launch of this iteration's V-cycle,
   // when there is one (not after the last update of a batch, not for Jacobi)
-  const bool defer_x = !jacobi && !last && P->n_stail > 0;
+  const bool defer_x = !jacobi && !last && defer_x_update(P);
   {
     ProfScope ps(P, s, "update", ((defer_x ? 24.0 : 48.0) + sizeof(T)) * N);
     if (defer_x) {

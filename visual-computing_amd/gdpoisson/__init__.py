@@ -13,7 +13,8 @@ in ``libgdpoisson.so`` (C ABI: ``include/gdpoisson.h``).  Batched plans
 """
 
 from .multigrid import MultigridOptions, MultigridReport, solve, v_cycle
-from .operators import divergence, fold_boundary, gradient, laplacian, neumann_laplacian
+from .operators import (divergence, fold_boundary, gradient, laplacian, neumann_laplacian,
+                        sparse_laplacian)
 from .autotune import autotune
 from .plan import Plan, PlanOptions
 from .poisson import (GuidanceField, illumination_change, seamless_clone, solve_dirichlet,
@@ -21,6 +22,7 @@ from .poisson import (GuidanceField, illumination_change, seamless_clone, solve_
 from . import hdr
 from .hdr import ToneMapConfig, attenuation_field, tone_map
 from .solvers import SOLVERS, SolverReport, solve_neumann, solve_system
+from .benchmark import BenchmarkRecord, BenchmarkTable, manufactured_problem, solver_scaling
 
 __version__ = "0.1.0"
 
@@ -51,4 +53,9 @@ __all__ = [
     "solve_neumann",
     "solve_system",
     "v_cycle",
+    "sparse_laplacian",
+    "manufactured_problem",
+    "solver_scaling",
+    "BenchmarkRecord",
+    "BenchmarkTable",
 ]

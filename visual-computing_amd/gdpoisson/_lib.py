@@ -45,6 +45,10 @@ EXPORTS = (
     "gdp_dct_tables",
     "gdp_dct_work_bytes",
     "gdp_neumann_dct_solve",
+    "gdp_dst_supported",
+    "gdp_dst_tables_bytes",
+    "gdp_dst_tables",
+    "gdp_dirichlet_dst_solve",
     "gdp_smooth",
     "gdp_restrict",
     "gdp_prolong",
@@ -144,6 +148,10 @@ def _declare(lib):
         "gdp_dct_tables": (c_int, [c_int, c_int, P, P]),
         "gdp_dct_work_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
         "gdp_neumann_dct_solve": (c_int, [P, P, c_int, c_int, P, P, P]),
+        "gdp_dst_supported": (c_int, [c_int, c_int]),
+        "gdp_dst_tables_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
+        "gdp_dst_tables": (c_int, [c_int, c_int, P, P]),
+        "gdp_dirichlet_dst_solve": (c_int, [P, P, c_int, c_int, c_double, P, P, P]),
         "gdp_smooth": (c_int, [P, P, c_int, c_int, c_double, c_int, c_int, c_int, P]),
         "gdp_restrict": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),
         "gdp_prolong": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),

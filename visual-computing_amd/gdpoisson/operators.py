@@ -8,6 +8,7 @@ reference; channels are processed as independent planes by one launch.
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -91,3 +92,21 @@ def fold_boundary(right_hand_side, boundary, spacing: float = 1.0):
     _lib.check(_lib.lib().gdp_fold_boundary(pr.data_ptr(), pb.data_ptr(), out.data_ptr(), h, w,
                                             c, float(spacing), _stream()), "fold_boundary")
     return to_host_like(_unplanes(out, cl), host)
+
+
+def sparse_laplacian(shape, spacing: float = 1.0):
+    """The five-point operator of :func:`laplacian` as a scipy CSR matrix
+    (operators.py:110-136): ``(T_m kron I_n + I_m kron T_n) / h^2`` in row-major order,
+    T = tridiag(1, -2, 1).  A host helper (the reference's tests and the direct method's
+    definition use it); the device solvers never form it."""
+    import scipy.sparse as sp
+
+    m, n = int(shape[0]), int(shape[1])
+
+    def tri(k):
+        return sp.diags([np.ones(k - 1), np.full(k, -2.0), np.ones(k - 1)], [-1, 0, 1],
+                        format="csr")
+
+    a = sp.kron(tri(m), sp.identity(n, format="csr")) + sp.kron(sp.identity(m, format="csr"),
+                                                               tri(n))
+    return (a / (spacing * spacing)).tocsr()

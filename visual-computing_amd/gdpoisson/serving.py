@@ -261,10 +261,20 @@ class Server:
 
     # ------------------------------------------------------------ execution
     def _execute(self, batch: list, bid: int, admitted: float, clock, realtime: bool):
-        if self.executor is not None:
-            results = self.executor(batch)
-        else:
-            results = self._device_batch(batch)
+        # a failure of the batch as a whole (an executor that raises, a plan that cannot be
+        # built or bound) fails that batch's requests only: the loop keeps serving and the
+        # batch's budget is still handed back
+        try:
+            if self.executor is not None:
+                results = list(self.executor(batch))
+            else:
+                results = self._device_batch(batch)
+            if len(results) != len(batch):
+                raise RuntimeError(f"executor returned {len(results)} results for "
+                                   f"{len(batch)} requests")
+        except Exception as e:  # noqa: BLE001 -- reported per request
+            msg = f"{type(e).__name__}: {e}"
+            results = [(None, [], msg)] * len(batch)
         fin = clock()
         out = []
         for r, (output, reports, err) in zip(batch, results):
@@ -332,14 +342,21 @@ class Server:
             reps: dict = {}
             if planes:
                 t_start = time.perf_counter()
-                opts = dataclasses.replace(
-                    _pcg_options(self.precision, self.tolerance, self.max_iterations,
-                                 BOUNDARY_OF[batch[0].kind]), use_graph=self.graphs)
-                plan = Plan([tuple(p.shape) for p in planes], BOUNDARY_OF[batch[0].kind], 1.0,
-                            opts, device=self.device,
-                            masks=masks if any(m is not None for m in masks) else None)
-                us, rs = plan.mgcg(planes)
-                del plan
+                try:
+                    opts = dataclasses.replace(
+                        _pcg_options(self.precision, self.tolerance, self.max_iterations,
+                                     BOUNDARY_OF[batch[0].kind]), use_graph=self.graphs)
+                    plan = Plan([tuple(p.shape) for p in planes], BOUNDARY_OF[batch[0].kind],
+                                1.0, opts, device=self.device,
+                                masks=masks if any(m is not None for m in masks) else None)
+                    us, rs = plan.mgcg(planes)
+                    del plan
+                except Exception as e:  # noqa: BLE001 -- the plan's requests fail alone
+                    msg = f"{type(e).__name__}: {e}"
+                    for i in set(owner):
+                        results[i] = (None, [], msg)
+                        jobs[i] = None
+                    us, rs, owner = [], [], []
                 dt = time.perf_counter() - t_start
                 for i, u, rep in zip(owner, us, rs):
                     sols.setdefault(i, []).append(u)

@@ -9,6 +9,10 @@ returns zeros with residual 0.0, the report carries the true relative residual.
 
 Methods:
 
+``direct``      the exact solve (solvers.py:85-90, SuperLU there): DST-I diagonalisation of
+                the rectangle operator on the device, for sides m with 2 (m + 1) a product
+                of 2, 3 and 5 up to 4096; other sides are solved by FP64 MG-PCG to a
+                relative residual of 1e-13.
 ``mgcg``        V-cycle-preconditioned CG, FP64 throughout (solvers.py:134-160).
 ``mgcg_mixed``  the same CG with an FP32 V-cycle (r, x, d, q, dots and the coarsest
                 solve stay FP64) and the Polak-Ribiere beta.
@@ -59,8 +63,21 @@ def _pcg_options(precision, tolerance, max_iterations, boundary):
     )
 
 
+# the fallback of "direct" for sides the DST does not cover: FP64 MG-PCG this far
+DIRECT_FALLBACK_TOLERANCE = 1e-13
+
+
 def _run(kind, b, spacing, tolerance, max_iterations, boundary="dirichlet"):
     """Solve one grid on the device; returns (u tensor, iterations, converged, true residual)."""
+    if kind == "direct":
+        if boundary != "dirichlet":
+            raise ValueError("the direct method solves the Dirichlet rectangle operator "
+                             "(use solve_neumann for the Neumann one)")
+        m, n = int(b.shape[0]), int(b.shape[1])
+        if dst_supported(m, n):
+            u = _dst_solve(b, float(spacing))
+            return u, 0, True, relative_residual(u, b, spacing)
+        return _run("mgcg", b, spacing, DIRECT_FALLBACK_TOLERANCE, max(int(max_iterations), 200))
     if kind == "multigrid":
         opts = MultigridOptions(max_cycles=int(max_iterations), tolerance=float(tolerance))
         plan = cached_plan([tuple(b.shape)], boundary, float(spacing), PlanOptions(
@@ -91,7 +108,11 @@ def solve_planes(planes: torch.Tensor, method: str = "mgcg", tolerance: float = 
     bs = [planes[k] for k in range(c)]
     torch.cuda.current_stream().synchronize()
     t0 = time.perf_counter()
-    if method == "multigrid":
+    if method == "direct":
+        res = [_run("direct", x, spacing, tolerance, max_iterations, boundary) for x in bs]
+        us = [r[0] for r in res]
+        reps = [_lib.Report(r[1], int(r[2]), r[3], 0.0) for r in res]
+    elif method == "multigrid":
         opts = MultigridOptions(max_cycles=int(max_iterations), tolerance=float(tolerance))
         plan = cached_plan([(m, n)] * c, boundary, float(spacing), PlanOptions(
             precision="fp64", pre_smoothing=opts.pre_smoothing,
@@ -123,6 +144,7 @@ def _registry_entry(kind):
 
 
 SOLVERS = {
+    "direct": _registry_entry("direct"),
     "cg": _registry_entry("cg"),
     "multigrid": _registry_entry("multigrid"),
     "mgcg": _registry_entry("mgcg"),
@@ -182,30 +204,62 @@ def dct_supported(m: int, n: int) -> bool:
     return bool(_lib.lib().gdp_dct_supported(int(m), int(n)))
 
 
-def _dct_solve(t: torch.Tensor) -> torch.Tensor:
-    m, n = int(t.shape[0]), int(t.shape[1])
-    key = (m, n, t.device)
+def dst_supported(m: int, n: int) -> bool:
+    """Whether the exact device DST-I (the direct Dirichlet solve) handles an m x n grid:
+    2 (side + 1) at most 4096 with prime factors 2, 3 and 5."""
+    return bool(_lib.lib().gdp_dst_supported(int(m), int(n)))
+
+
+def _transform_tables(kind: str, m: int, n: int, device) -> torch.Tensor:
+    """The device tables of the exact DCT ("dct") or DST ("dst") solve of an m x n grid,
+    built once per shape and device and shared by every stream."""
+    key = (kind, m, n, device)
     tab = _DCT_TABLES.get(key)
     if tab is None:
+        L = _lib.lib()
         nbytes = ctypes.c_size_t()
-        _lib.check(_lib.lib().gdp_dct_tables_bytes(m, n, ctypes.byref(nbytes)), "gdp_dct")
-        tab = torch.empty(nbytes.value // 8, dtype=torch.float64, device=t.device)
-        _lib.check(_lib.lib().gdp_dct_tables(m, n, ctypes.c_void_p(tab.data_ptr()),
-                                             _lib.current_stream()), "gdp_dct_tables")
+        _lib.check(getattr(L, f"gdp_{kind}_tables_bytes")(m, n, ctypes.byref(nbytes)),
+                   f"gdp_{kind}")
+        tab = torch.empty(nbytes.value // 8, dtype=torch.float64, device=device)
+        _lib.check(getattr(L, f"gdp_{kind}_tables")(m, n, ctypes.c_void_p(tab.data_ptr()),
+                                                    _lib.current_stream()), f"gdp_{kind}_tables")
         # the cache is shared by every stream (serving workers): a table is published only
         # once it is written, and evicted only when no stream can still be reading it
-        torch.cuda.current_stream(t.device).synchronize()
+        torch.cuda.current_stream(device).synchronize()
         with _DCT_LOCK:
             if len(_DCT_TABLES) >= 32:
-                torch.cuda.synchronize(t.device)
+                torch.cuda.synchronize(device)
                 _DCT_TABLES.clear()
             _DCT_TABLES[key] = tab
+    return tab
+
+
+def _dst_solve(t: torch.Tensor, spacing: float) -> torch.Tensor:
+    m, n = int(t.shape[0]), int(t.shape[1])
+    tab = _transform_tables("dst", m, n, t.device)
+    work = torch.empty(2 * m * n, dtype=torch.float64, device=t.device)
+    u = torch.empty_like(t)
+    _lib.check(_lib.lib().gdp_dirichlet_dst_solve(
+        ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(u.data_ptr()), m, n, float(spacing),
+        ctypes.c_void_p(tab.data_ptr()), ctypes.c_void_p(work.data_ptr()),
+        _lib.current_stream()), "gdp_dirichlet_dst_solve")
+    tab.record_stream(torch.cuda.current_stream(t.device))
+    return u
+
+
+def _dct_solve(t: torch.Tensor) -> torch.Tensor:
+    m, n = int(t.shape[0]), int(t.shape[1])
+    tab = _transform_tables("dct", m, n, t.device)
     work = torch.empty(2 * m * n, dtype=torch.float64, device=t.device)
     u = torch.empty_like(t)
     _lib.check(_lib.lib().gdp_neumann_dct_solve(
         ctypes.c_void_p(t.data_ptr()), ctypes.c_void_p(u.data_ptr()), m, n,
         ctypes.c_void_p(tab.data_ptr()), ctypes.c_void_p(work.data_ptr()),
         _lib.current_stream()), "gdp_neumann_dct_solve")
+    # the table was allocated on the stream that built it: if another worker evicts it
+    # while this stream still reads it, the allocator must not hand its memory out before
+    # this stream's solve has run
+    tab.record_stream(torch.cuda.current_stream(t.device))
     return u
 
 

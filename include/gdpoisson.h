@@ -200,6 +200,15 @@ int gdp_dct_tables(int m, int n, double* tables, void* stream);
 int gdp_dct_work_bytes(int m, int n, size_t* bytes);
 int gdp_neumann_dct_solve(const double* b, double* u, int m, int n, const double* tables,
                           double* work, void* stream);
+/* The same solve without a work argument and without allocating: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops that forbid hipMalloc).  The work lives in a static device array of
+ * 64 MiB (2 m n doubles: m n <= 4,194,304, GDP_ENOMEM beyond), one per GPU, held until the
+ * process ends; calls are identical to gdp_neumann_dct_solve's in every bit.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (different streams or threads
+ * without ordering between them) race and give wrong results. */
+int gdp_neumann_dct_solve_static(const double* b, double* u, int m, int n,
+                                 const double* tables, void* stream);
 
 /* ---------------------------------------------------------------- exact Dirichlet solve
  * u (m x n) = the solution of laplacian(u, spacing) = b with zeros outside the grid (the

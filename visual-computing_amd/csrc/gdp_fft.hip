@@ -29,6 +29,7 @@
 #include "gdp_internal.h"
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #define GDP_FFT_HIP(x)                                                          \
@@ -39,6 +40,13 @@
       return GDP_EHIP;                                                          \
     }                                                                           \
   } while (0)
+
+// Work buffer of gdp_neumann_dct_solve_static: a file-scope device array, one copy per GPU
+// the library runs on, held until the process ends.  Its name is a host-side stand-in
+// (not a device address): host code only ever passes the per-device address that
+// hipGetSymbolAddress returns.
+constexpr size_t kStaticWorkDoubles = size_t(1) << 23;  // 64 MiB: m n <= 4,194,304
+static __device__ __align__(256) double scratch[kStaticWorkDoubles];
 
 namespace gdp {
 namespace {
@@ -308,6 +316,65 @@ void fill_table(int N, double* t) {
 
 size_t lds_bytes(int N) { return size_t(2) * N * sizeof(double2); }
 
+hipError_t dct_lds_setup() {
+  static bool lds_set = false;
+  if (!lds_set) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_dct2_rows,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(lds_bytes(kFftMaxN)));
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute((const void*)k_dct3_rows,
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(lds_bytes(kFftMaxN)));
+    if (e != hipSuccess) return e;
+    lds_set = true;
+  }
+  return hipSuccess;
+}
+
+// The four row passes of the exact Neumann solve; work holds 2 m n doubles.
+hipError_t dct_solve_passes(const double* b, double* u, int m, int n, const FftShape& fm,
+                            const FftShape& fn, const double* tables, double* work,
+                            hipStream_t s) {
+  const double* tm = tables;
+  const double* tn = tables + table_doubles(m);
+  double* w0 = work;                       // n x m
+  double* w1 = work + size_t(m) * n;       // m x n
+  // along n: b (m x n) -> w0 (n x m); along m, divided by lambda: w0 -> w1 (m x n)
+  k_dct2_rows<<<m, kFftThreads, lds_bytes(n), s>>>(b, w0, m, fn, tn, nullptr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_dct2_rows<<<n, kFftThreads, lds_bytes(m), s>>>(w0, w1, n, fm, tm, tn + 4 * size_t(n));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // inverse along n: w1 (m x n) -> w0 (n x m); along m: w0 -> u (m x n)
+  k_dct3_rows<<<m, kFftThreads, lds_bytes(n), s>>>(w1, w0, m, fn, tn);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_dct3_rows<<<n, kFftThreads, lds_bytes(m), s>>>(w0, u, n, fm, tm);
+  return hipGetLastError();
+}
+
+constexpr int kMaxDevices = 64;
+
+// Device address of this device's scratch array, looked up once per device (with that
+// device current) and kept.
+hipError_t static_work(double** out) {
+  static std::mutex mu;
+  static double* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = static_cast<double*>(p);
+  }
+  *out = addr[dev];
+  return hipSuccess;
+}
+
 }  // namespace
 }  // namespace gdp
 
@@ -359,34 +426,35 @@ int gdp_neumann_dct_solve(const double* b, double* u, int m, int n, const double
               kFftMaxN, m, n);
     return GDP_ENOTSUP;
   }
-  static bool lds_set = false;
-  if (!lds_set) {
-    GDP_FFT_HIP(hipFuncSetAttribute((const void*)k_dct2_rows,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(lds_bytes(kFftMaxN))));
-    GDP_FFT_HIP(hipFuncSetAttribute((const void*)k_dct3_rows,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(lds_bytes(kFftMaxN))));
-    lds_set = true;
-  }
-  const hipStream_t s = static_cast<hipStream_t>(stream);
-  const double* tm = tables;
-  const double* tn = tables + table_doubles(m);
-  double* w0 = work;                       // n x m
-  double* w1 = work + size_t(m) * n;       // m x n
-  // along n: b (m x n) -> w0 (n x m); along m, divided by lambda: w0 -> w1 (m x n)
-  k_dct2_rows<<<m, kFftThreads, lds_bytes(n), s>>>(b, w0, m, fn, tn, nullptr);
-  GDP_FFT_HIP(hipGetLastError());
-  k_dct2_rows<<<n, kFftThreads, lds_bytes(m), s>>>(w0, w1, n, fm, tm, tn + 4 * size_t(n));
-  GDP_FFT_HIP(hipGetLastError());
-  // inverse along n: w1 (m x n) -> w0 (n x m); along m: w0 -> u (m x n)
-  k_dct3_rows<<<m, kFftThreads, lds_bytes(n), s>>>(w1, w0, m, fn, tn);
-  GDP_FFT_HIP(hipGetLastError());
-  k_dct3_rows<<<n, kFftThreads, lds_bytes(m), s>>>(w0, u, n, fm, tm);
-  GDP_FFT_HIP(hipGetLastError());
+  GDP_FFT_HIP(dct_lds_setup());
+  GDP_FFT_HIP(dct_solve_passes(b, u, m, n, fm, fn, tables, work,
+                               static_cast<hipStream_t>(stream)));
   return GDP_OK;
 }
 
+int gdp_neumann_dct_solve_static(const double* b, double* u, int m, int n,
+                                 const double* tables, void* stream) {
+  FftShape fm, fn;
+  if (!b || !u || !tables) return (set_error("null argument"), GDP_EINVAL);
+  if (!factor(m, fm) || !factor(n, fn)) {
+    set_error("gdp_dct: sides must be at most %d with prime factors 2, 3, 5 only (got %d x %d)",
+              kFftMaxN, m, n);
+    return GDP_ENOTSUP;
+  }
+  if (2 * size_t(m) * n > kStaticWorkDoubles) {
+    set_error("gdp_neumann_dct_solve_static: %d x %d needs %zu bytes of work, more than the "
+              "static %zu (m n <= %zu); use gdp_neumann_dct_solve with a work buffer",
+              m, n, 2 * size_t(m) * n * sizeof(double), kStaticWorkDoubles * sizeof(double),
+              kStaticWorkDoubles / 2);
+    return GDP_ENOMEM;
+  }
+  double* work = nullptr;
+  GDP_FFT_HIP(static_work(&work));
+  GDP_FFT_HIP(dct_lds_setup());
+  GDP_FFT_HIP(dct_solve_passes(b, u, m, n, fm, fn, tables, work,
+                               static_cast<hipStream_t>(stream)));
+  return GDP_OK;
+}
 
 int gdp_dst_supported(int m, int n) {
   FftShape a, b;

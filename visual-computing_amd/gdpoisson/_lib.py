@@ -45,6 +45,7 @@ EXPORTS = (
     "gdp_dct_tables",
     "gdp_dct_work_bytes",
     "gdp_neumann_dct_solve",
+    "gdp_neumann_dct_solve_static",
     "gdp_dst_supported",
     "gdp_dst_tables_bytes",
     "gdp_dst_tables",
@@ -148,6 +149,7 @@ def _declare(lib):
         "gdp_dct_tables": (c_int, [c_int, c_int, P, P]),
         "gdp_dct_work_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
         "gdp_neumann_dct_solve": (c_int, [P, P, c_int, c_int, P, P, P]),
+        "gdp_neumann_dct_solve_static": (c_int, [P, P, c_int, c_int, P, P]),
         "gdp_dst_supported": (c_int, [c_int, c_int]),
         "gdp_dst_tables_bytes": (c_int, [c_int, c_int, POINTER(c_size_t)]),
         "gdp_dst_tables": (c_int, [c_int, c_int, P, P]),

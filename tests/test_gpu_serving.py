@@ -24,9 +24,9 @@ def gd():
 
 
 def _requests():
-    """Six compositing requests (squares cut from 256 x 256 blob images and a blob mask;
-    two by the mask route, batched with the rectangle ones in one plan) and four small
-    tone-mapping requests, all arriving at once."""
+    """Six compositing requests (squares of two sizes cut from 256 x 256 blob images and a
+    blob mask; two by the mask route) and four small tone-mapping requests (two of one
+    shape), all arriving at once: requests of one kind and shape batch together."""
     import torch
 
     from gdpoisson.serving import Request
@@ -35,8 +35,7 @@ def _requests():
     s, t, m = (torch.from_numpy(x).cuda() for x in (src, tgt, mask))
     rng = np.random.default_rng(3)
     reqs = []
-    for k in range(6):
-        side = int(rng.integers(64, 161))
+    for k, side in enumerate([96, 96, 128, 96, 128, 128]):
         r0, c0 = (int(v) for v in rng.integers(0, 256 - side + 1, 2))
         win = (slice(r0, r0 + side), slice(c0, c0 + side))
         reqs.append(Request("composite", {"source": s[win], "target": t[win], "mask": m[win],
@@ -93,3 +92,38 @@ def test_a_timed_stream_reports_latency(gd):
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     assert all(rep.converged for c in done for rep in c.reports)
     assert {c.kind for c in done} == {"composite", "tonemap"}
+
+
+def test_resident_plans_are_reused_and_keep_the_bits(gd):
+    """A second stream on the same server reuses the plans (and their captured graphs) the
+    first built: no new plan for shapes seen before, and the same bits as the first time
+    and as a fresh server; tuned tiles (chosen per shape) keep alone == in the mix."""
+    import torch
+
+    from gdpoisson.serving import Request, Server
+
+    reqs = _requests()
+    # one worker, so the second stream's batches meet the first stream's plans
+    srv = Server(budget_bytes=4 << 30, slots=1, max_batch=12)
+    first = srv.run(reqs, realtime=False)
+    pooled1 = srv.plan_builds - srv.unpooled_builds
+    unpooled1 = srv.unpooled_builds
+    assert pooled1 > 0 and srv.plan_hits == 0
+    again = srv.run([Request(r.kind, r.args) for r in reqs], realtime=False)
+    st = srv.stats()
+    # every rectangle plan of the second stream was resident (plans are per plane shape);
+    # only the mask-route plans (their masks change per request) were built again
+    assert st["graphs"] and st["plan_hits"] == pooled1
+    assert srv.plan_builds - srv.unpooled_builds == pooled1
+    assert srv.unpooled_builds == 2 * unpooled1
+    for a, b in zip(first, again):
+        assert torch.equal(a.output, b.output)
+    tuned = Server(budget_bytes=4 << 30, slots=2, max_batch=12)
+    table = tuned.tune(reqs, solves=1)
+    assert table, "no plane shape was tuned"
+    mixed = tuned.run(reqs, realtime=False)
+    for r, c in zip(reqs, mixed):
+        alone = Server(budget_bytes=4 << 30, slots=1, tiles=table).run(
+            [Request(r.kind, r.args)], realtime=False)[0]
+        assert torch.equal(alone.output, c.output), (r.kind, c.rid)
+        assert all(rep.converged and rep.relative_residual <= 1e-8 for rep in c.reports)

@@ -153,3 +153,21 @@ def test_workload_generators_match_the_oracle_generators():
     s1, t1, m1 = W.compositing_images((40, 30))
     s2, t2, m2 = G.compositing_images((40, 30))
     assert np.array_equal(s1, s2) and np.array_equal(t1, t2) and np.array_equal(m1, m2)
+
+
+def test_batches_hold_one_image_shape():
+    """Requests batch only with requests of the same kind and image shape (so the batch's
+    planes repeat shapes a resident plan was built for); the oldest request still leads."""
+    import numpy as np
+
+    def comp(shape, arrival):
+        return Request("composite", {"nbytes": 10, "pixels": 1, "service": 0.002,
+                                     "target": np.zeros(shape)}, arrival)
+
+    reqs = [comp((64, 64, 3), 0.0), comp((32, 32, 3), 0.0), comp((64, 64, 3), 0.0),
+            comp((64, 64, 3), 0.0), comp((32, 32, 3), 0.0)]
+    dev = FakeDevice()
+    done = Server(budget_bytes=1000, slots=1, max_batch=12, executor=dev).run(
+        reqs, realtime=False)
+    assert all(c.error is None for c in done)
+    assert dev.batches == [[0, 2, 3], [1, 4]]

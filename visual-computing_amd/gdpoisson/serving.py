@@ -5,26 +5,31 @@ The reference runs one solve at a time on the caller's thread (solvers.py:171-22
 called per channel by poisson.solve_dirichlet, poisson.py:141-143, and once by
 hdr.tone_map, hdr.py:175).  Here a request is a whole compositing job (seamless_clone,
 rectangle route unless the request asks for ``domain="mask"``) or tone-mapping job
-(tone_map), and a :class:`Server` replays a stream
-of timed arrivals:
+(tone_map), and a :class:`Server` replays a stream of timed arrivals:
 
 * requests wait in arrival order; each holds an estimate of the device bytes it needs
   while admitted (inputs it copies, temporaries, outputs and its share of a plan's
   workspace, see :func:`request_bytes`);
 * whenever one of ``slots`` workers is free, the oldest waiting request is admitted if
   its bytes fit what is left of the budget, together with every later waiting request of
-  the same kind that still fits (up to ``max_batch`` problems);
+  the same kind and image shape that still fits (up to ``max_batch`` problems);
 * a worker runs its batch on its own HIP stream: each request's front-end kernels, one
-  batched plan holding every channel of every request of the batch (one launch per
-  kernel for all of them), then each request's back-end kernels; the only host waits
-  are the worker's own (its stream and the plan's single end-of-solve read);
+  batched plan per plane shape holding every channel of every request of the batch with
+  that shape (one launch per kernel for all of them), then each request's back-end
+  kernels; the only host waits are the worker's own (its stream and each plan's single
+  end-of-solve read);
+* the plans stay resident: each worker keeps its bound plans (level pyramid, Krylov
+  vectors, tile tables and the hipGraphs of their PCG iterations) in a pool keyed by
+  the planes' shapes, within a byte cap, so a batch of shapes seen before reuses the
+  workspace and replays its captured graphs instead of building and capturing anew;
 * batches on different workers overlap on the device, and requests that arrive while
   others run are admitted as soon as a worker and budget free up.
 
-Each problem of a batched plan is solved with the arithmetic it gets alone (default tile
-shapes, per-problem reductions in a fixed order, per-problem stopping), so a request's
-result does not depend on what it was batched or overlapped with: the same bits alone
-and in the mix.  Latency = completion - arrival, queueing included.
+Each problem of a batched plan is solved with the arithmetic it gets alone (tile shapes
+chosen by its shape only, per-problem reductions in a fixed order, per-problem
+stopping), so a request's result does not depend on what it was batched or overlapped
+with: the same bits alone and in the mix.  Latency = completion - arrival, queueing
+included.
 """
 
 from __future__ import annotations
@@ -32,7 +37,7 @@ from __future__ import annotations
 import dataclasses
 import threading
 import time
-from collections import deque
+from collections import OrderedDict, deque
 from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 from typing import Callable, Sequence
@@ -43,11 +48,12 @@ KINDS = ("composite", "tonemap")
 BOUNDARY_OF = {"composite": "dirichlet", "tonemap": "neumann"}
 
 
-@dataclass
+@dataclass(eq=False)
 class Request:
     """``kind`` "composite": ``args`` = source, target, mask, offset, mode (and optionally
     margin); "tonemap": ``args`` = radiance (and optionally config).  Images are
-    channels-last.  ``arrival`` is in seconds from the start of the stream."""
+    channels-last.  ``arrival`` is in seconds from the start of the stream.  Requests
+    compare by identity (their arguments hold device tensors)."""
 
     kind: str
     args: dict
@@ -130,22 +136,27 @@ def request_bytes(req: Request, precision: str = "mixed") -> int:
 class Server:
     """Continuous-batching request server on one GPU (see the module docstring).
 
-    ``budget_bytes``: device bytes the admitted requests may hold together (default: half
-    of the free HBM).  ``slots``: workers, one HIP stream each.  ``max_batch``: problems
-    (channels) per batched plan.  ``method``: "mgcg_mixed" or "mgcg".  ``tonemap_method``:
-    "dct" integrates each tone map exactly by the device DCT, as the reference's tone_map
-    does (sides whose prime factors are not all 2, 3, 5 join the batched plan), "mg" puts
-    every tone map in the batched MG-PCG plan.  ``graphs``: capture
-    the plans' PCG iterations in hipGraphs (each batch's plan is new, so the capture is
-    paid per batch).  ``executor`` replaces the device pipeline (tests of the
-    scheduling): called as ``executor(batch)`` in a worker, it returns one
-    ``(output, reports)`` per request.
+    ``budget_bytes``: device bytes the server may hold (default: half of the free HBM);
+    ``plan_pool_bytes`` of it (default an eighth) is kept for the resident plans, the rest
+    is the admission budget of the requests.  ``slots``: workers, one HIP stream each.
+    ``max_batch``: problems (channels) per batched plan.  ``method``: "mgcg_mixed" or
+    "mgcg".  ``tonemap_method``: "dct" integrates each tone map exactly by the device DCT,
+    as the reference's tone_map does (sides whose prime factors are not all 2, 3, 5 join
+    the batched plan), "mg" puts every tone map in the batched MG-PCG plan.  ``graphs``:
+    capture the pooled plans' PCG iterations in hipGraphs (a pooled plan replays them on
+    every later use; mask-route plans, whose masks change per request, are built per
+    batch and run without).  ``tiles``: {(boundary, (m, n)): {(kind, level): menu index}},
+    the tile shapes of planes of that shape (see :meth:`tune`); other shapes get the
+    default tiles.  ``executor`` replaces the device pipeline (tests of the scheduling):
+    called as ``executor(batch)`` in a worker, it returns one ``(output, reports, error)``
+    per request.
     """
 
     def __init__(self, budget_bytes: int | None = None, slots: int = 4, max_batch: int = 12,
                  method: str = "mgcg_mixed", tolerance: float = 1e-8,
-                 max_iterations: int = 2000, graphs: bool = False, keep_outputs: bool = True,
-                 device=None, executor: Callable | None = None, tonemap_method: str = "dct"):
+                 max_iterations: int = 2000, graphs: bool = True, keep_outputs: bool = True,
+                 device=None, executor: Callable | None = None, tonemap_method: str = "dct",
+                 plan_pool_bytes: int | None = None, tiles: dict | None = None):
         if method not in ("mgcg", "mgcg_mixed"):
             raise ValueError("method must be 'mgcg' or 'mgcg_mixed'")
         if tonemap_method not in ("dct", "mg"):
@@ -169,10 +180,29 @@ class Server:
                 budget_bytes = free // 2
         if budget_bytes is None:
             raise ValueError("budget_bytes is required with a custom executor")
-        self.budget = int(budget_bytes)
+        if plan_pool_bytes is None:
+            plan_pool_bytes = 0 if executor is not None else int(budget_bytes) // 8
+        if not 0 <= plan_pool_bytes < int(budget_bytes):
+            raise ValueError("plan_pool_bytes must be smaller than the budget")
+        self.total_budget = int(budget_bytes)
+        self.pool_bytes = int(plan_pool_bytes)
+        self.budget = self.total_budget - self.pool_bytes  # admission budget of requests
+        self.tiles = dict(tiles or {})
         self._local = threading.local()
+        self._lock = threading.Lock()
         self.peak_bytes = 0
         self.batches: list[tuple[str, int, int]] = []  # (kind, requests, problems)
+        self.plan_hits = 0     # batches served by a resident plan
+        self.plan_builds = 0   # plans built (and, when pooled, kept)
+        self.unpooled_builds = 0  # of which built for one batch (mask route, no pool)
+        self.pool_peak = 0     # largest resident-plan bytes of one worker
+        self._workers = None
+
+    def close(self) -> None:
+        """Stop the workers; their streams and resident plans are released."""
+        if self._workers is not None:
+            self._workers.shutdown(wait=True)
+            self._workers = None
 
     # ------------------------------------------------------------ scheduling
     def _bytes(self, req: Request) -> int:
@@ -185,14 +215,26 @@ class Server:
         s = _shape(t) if t is not None else (1, 1, 3)
         return s[2] if len(s) == 3 else 1
 
+    @staticmethod
+    def _signature(req: Request) -> tuple:
+        """Requests batch together only with the same kind and image shape (and route), so
+        a batch's planes repeat shapes that the resident plans were built for."""
+        a = req.args
+        if req.kind == "composite" and "target" in a:
+            return ("composite", _shape(a["target"]), a.get("domain", "rectangle"))
+        if req.kind == "tonemap" and "radiance" in a:
+            return ("tonemap", _shape(a["radiance"]))
+        return (req.kind,)
+
     def _form_batch(self, waiting: list, free: int):
         head = waiting[0]
         need = self._bytes(head)
         if need > free:
             return [], 0
+        sig = self._signature(head)
         batch, total, probs = [head], need, self._problems(head)
         for r in waiting[1:]:
-            if r.kind != head.kind:
+            if self._signature(r) != sig:
                 continue
             p, b = self._problems(r), self._bytes(r)
             if probs + p > self.max_batch:
@@ -224,39 +266,44 @@ class Server:
         bid = 0
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0  # noqa: E731
-        with ThreadPoolExecutor(max_workers=self.slots, thread_name_prefix="gdp-slot") as pool:
-            while pending or waiting or running:
-                now = clock()
-                while pending and (not realtime or pending[0].arrival <= now):
-                    waiting.append(pending.popleft())
-                while waiting and self._bytes(waiting[0]) > self.budget:
-                    r = waiting.pop(0)
-                    done.append(Completion(r.rid, r.kind, r.arrival if realtime else 0.0, now, now,
-                                           -1, self._bytes(r), request_pixels(r),
-                                           error="needs more device memory than the budget"))
-                while waiting and len(running) < self.slots:
-                    batch, nbytes = self._form_batch(waiting, self.budget - in_use)
-                    if not batch:
-                        break
-                    for r in batch:
-                        waiting.remove(r)
-                    in_use += nbytes
-                    self.peak_bytes = max(self.peak_bytes, in_use)
-                    self.batches.append((batch[0].kind, len(batch),
-                                         sum(self._problems(r) for r in batch)))
-                    fut = pool.submit(self._execute, batch, bid, clock(), clock, realtime)
-                    running[fut] = nbytes
-                    bid += 1
-                timeout = None
-                if pending and realtime:
-                    timeout = max(0.0, pending[0].arrival - clock())
-                if running:
-                    finished, _ = wait(list(running), timeout=timeout, return_when=FIRST_COMPLETED)
-                    for f in finished:
-                        in_use -= running.pop(f)
-                        done.extend(f.result())
-                elif timeout:
-                    time.sleep(timeout)
+        # the workers (and with them their streams and resident plans) live as long as the
+        # server, so later runs reuse what earlier ones built
+        if self._workers is None:
+            self._workers = ThreadPoolExecutor(max_workers=self.slots,
+                                               thread_name_prefix="gdp-slot")
+        pool = self._workers
+        while pending or waiting or running:
+            now = clock()
+            while pending and (not realtime or pending[0].arrival <= now):
+                waiting.append(pending.popleft())
+            while waiting and self._bytes(waiting[0]) > self.budget:
+                r = waiting.pop(0)
+                done.append(Completion(r.rid, r.kind, r.arrival if realtime else 0.0, now, now,
+                                       -1, self._bytes(r), request_pixels(r),
+                                       error="needs more device memory than the budget"))
+            while waiting and len(running) < self.slots:
+                batch, nbytes = self._form_batch(waiting, self.budget - in_use)
+                if not batch:
+                    break
+                for r in batch:
+                    waiting.remove(r)
+                in_use += nbytes
+                self.peak_bytes = max(self.peak_bytes, in_use)
+                self.batches.append((batch[0].kind, len(batch),
+                                     sum(self._problems(r) for r in batch)))
+                fut = pool.submit(self._execute, batch, bid, clock(), clock, realtime)
+                running[fut] = nbytes
+                bid += 1
+            timeout = None
+            if pending and realtime:
+                timeout = max(0.0, pending[0].arrival - clock())
+            if running:
+                finished, _ = wait(list(running), timeout=timeout, return_when=FIRST_COMPLETED)
+                for f in finished:
+                    in_use -= running.pop(f)
+                    done.extend(f.result())
+            elif timeout:
+                time.sleep(timeout)
         return sorted(done, key=lambda c: c.rid)
 
     # ------------------------------------------------------------ execution
@@ -292,14 +339,116 @@ class Server:
             self._local.stream = s
         return s
 
+    def _plan_options(self, boundary: str, graphs: bool):
+        from .solvers import _pcg_options
+
+        return dataclasses.replace(_pcg_options(self.precision, self.tolerance,
+                                                self.max_iterations, boundary),
+                                   use_graph=graphs)
+
+    def _tiles_of(self, boundary: str, shapes: list):
+        """Tile choices of a plan whose planes all have one shape listed in ``tiles``."""
+        if shapes and all(sh == shapes[0] for sh in shapes):
+            return self.tiles.get((boundary, tuple(shapes[0])))
+        return None
+
+    def _plan(self, shapes: list, boundary: str, masks):
+        """A bound plan for these planes: from this worker's pool when it holds one for
+        the same shapes (its workspace and captured graphs are reused), else built, and
+        kept in the pool (least recently used plans are dropped past the worker's share of
+        ``plan_pool_bytes``).  Mask-route plans are built per batch."""
+        from .plan import Plan
+
+        tiles = self._tiles_of(boundary, shapes)
+        if masks is not None or self.pool_bytes == 0:
+            with self._lock:
+                self.plan_builds += 1
+                self.unpooled_builds += 1
+            return Plan(shapes, boundary, 1.0, self._plan_options(boundary, False),
+                        device=self.device, tiles=tiles, masks=masks)
+        pool = getattr(self._local, "plans", None)
+        if pool is None:
+            pool = self._local.plans = OrderedDict()
+            self._local.pool_used = 0
+        key = (boundary, tuple(shapes))
+        plan = pool.get(key)
+        if plan is not None:
+            pool.move_to_end(key)
+            with self._lock:
+                self.plan_hits += 1
+            return plan
+        plan = Plan(shapes, boundary, 1.0, self._plan_options(boundary, self.graphs),
+                    device=self.device, tiles=tiles)
+        pool[key] = plan
+        self._local.pool_used += plan.workspace_bytes
+        share = self.pool_bytes // self.slots
+        while self._local.pool_used > share and len(pool) > 1:
+            _, old = pool.popitem(last=False)  # its last solve has finished (synchronous)
+            self._local.pool_used -= old.workspace_bytes
+        with self._lock:
+            self.plan_builds += 1
+            self.pool_peak = max(self.pool_peak, self._local.pool_used)
+        return plan
+
+    def tune(self, requests: Sequence[Request], solves: int = 2) -> dict:
+        """Autotune the tile shapes of every plane shape these requests solve by MG-PCG (a
+        plan of one request's planes per shape, timed by :func:`gdpoisson.autotune`), add
+        them to ``tiles`` and return the table.  The choice depends on the shape only, so
+        a request keeps its bits alone and batched."""
+        import torch
+
+        from .autotune import autotune
+        from .hdr import prepare_tone_map
+        from .plan import Plan
+        from .poisson import prepare_clone
+        from .solvers import dct_supported
+
+        seen = {}
+        with torch.cuda.device(self.device):
+            for r in requests:
+                a = r.args
+                if r.kind == "composite":
+                    if a.get("domain", "rectangle") == "mask":
+                        continue
+                    j = prepare_clone(a["source"], a["target"], a["mask"],
+                                      a.get("offset", (0, 0)), a.get("mode", "import"),
+                                      "rectangle", margin=int(a.get("margin", 24)))
+                else:
+                    j = prepare_tone_map(a["radiance"], a.get("config"))
+                    if self.tonemap_method == "dct" and dct_supported(*j.rhs.shape[1:]):
+                        continue
+                if j.rhs is None:
+                    continue
+                key = (BOUNDARY_OF[r.kind], tuple(int(x) for x in j.rhs.shape[1:]))
+                if key not in self.tiles and key not in seen:
+                    seen[key] = int(j.rhs.shape[0])
+            for (bc, shape), count in seen.items():
+                plan = Plan([shape] * count, bc, 1.0, self._plan_options(bc, False),
+                            device=self.device)
+                self.tiles[(bc, shape)] = autotune(plan, solves=solves)
+        return dict(self.tiles)
+
+    def stats(self) -> dict:
+        """Batching and plan-residency counters of the requests served so far."""
+        nb = len(self.batches)
+        return {
+            "batches": nb,
+            "mean_requests_per_batch": round(sum(n for _, n, _ in self.batches) / max(nb, 1), 2),
+            "max_requests_per_batch": max((n for _, n, _ in self.batches), default=0),
+            "plan_hits": self.plan_hits,
+            "plan_builds": self.plan_builds,
+            "unpooled_builds": self.unpooled_builds,
+            "graphs": self.graphs,
+            "tuned_shapes": len(self.tiles),
+            "resident_plan_peak_GB": round(self.pool_peak / 1e9, 3),
+        }
+
     def _device_batch(self, batch: list):
         import torch
 
         from .hdr import finish_tone_map, prepare_tone_map
-        from .plan import Plan
         from .poisson import finish_clone, prepare_clone
-        from .solvers import (SolverReport, _pcg_options, dct_supported, relative_residual,
-                              solve_neumann)
+        from .solvers import SolverReport, dct_supported, relative_residual, solve_neumann
 
         stream = self._stream()
         results: list = [None] * len(batch)
@@ -340,25 +489,30 @@ class Server:
                         masks.append(getattr(j, "unknown", None))
             sols: dict = {}
             reps: dict = {}
-            if planes:
+            # one plan per plane shape: a request's planes share its window's shape, and a
+            # plan of one shape takes that shape's tiles (alone or batched alike) and is
+            # the one the pool keeps for it
+            groups: dict = {}
+            for k, pl in enumerate(planes):
+                groups.setdefault(tuple(pl.shape), []).append(k)
+            for idx in groups.values():
                 t_start = time.perf_counter()
+                gm = [masks[k] for k in idx]
+                go = [owner[k] for k in idx]
                 try:
-                    opts = dataclasses.replace(
-                        _pcg_options(self.precision, self.tolerance, self.max_iterations,
-                                     BOUNDARY_OF[batch[0].kind]), use_graph=self.graphs)
-                    plan = Plan([tuple(p.shape) for p in planes], BOUNDARY_OF[batch[0].kind],
-                                1.0, opts, device=self.device,
-                                masks=masks if any(m is not None for m in masks) else None)
-                    us, rs = plan.mgcg(planes)
+                    plan = self._plan([tuple(planes[k].shape) for k in idx],
+                                      BOUNDARY_OF[batch[0].kind],
+                                      gm if any(m is not None for m in gm) else None)
+                    us, rs = plan.mgcg([planes[k] for k in idx])
                     del plan
                 except Exception as e:  # noqa: BLE001 -- the plan's requests fail alone
                     msg = f"{type(e).__name__}: {e}"
-                    for i in set(owner):
+                    for i in set(go):
                         results[i] = (None, [], msg)
                         jobs[i] = None
-                    us, rs, owner = [], [], []
+                    continue
                 dt = time.perf_counter() - t_start
-                for i, u, rep in zip(owner, us, rs):
+                for i, u, rep in zip(go, us, rs):
                     sols.setdefault(i, []).append(u)
                     reps.setdefault(i, []).append(SolverReport(
                         method=self.method, iterations=int(rep.iterations),

@@ -91,15 +91,22 @@ def lognormal_radiance(shape, seed, sigma=2.0):
 
 
 def serving_stream(count: int, qps: float, seed: int = 5000, sides=(512, 2048),
-                   tonemap_shape=(2160, 3840), tonemap_fraction=0.3, base_side: int = 2048,
-                   radiance_maps: int = 2, device=None) -> list[Request]:
+                   side_step: int = 256, tonemap_shape=(2160, 3840), tonemap_fraction=0.3,
+                   base_side: int = 2048, radiance_maps: int = 6, tonemap_levels: int = 1,
+                   device=None) -> list[Request]:
     """Requests of the mixed serving workload with device-resident inputs.
 
-    Compositing request: a square of side U{sides} cut at a random place from one pair of
-    base_side x base_side RGB blob images and from a blob mask (source and target the
-    same size, offset 0).  Tone-mapping request: one of ``radiance_maps`` log-normal
-    radiance maps of ``tonemap_shape``.  Arrivals: a Poisson process of rate ``qps``.
+    Compositing request: a square whose side is drawn uniformly from the multiples of
+    ``side_step`` in ``sides`` (the standard sizes an image stream repeats), cut at a
+    random place from one pair of base_side x base_side RGB blob images and from a blob
+    mask (source and target the same size, offset 0).  Tone-mapping request: one of
+    ``radiance_maps`` log-normal radiance maps of ``tonemap_shape`` (mu 0, sigma 2, i.i.d.
+    per pixel), attenuated by the single-scale factor (|grad| / alpha)^(beta - 1)
+    (``tonemap_levels`` = 1: attenuation_field with one level, hdr.py:122, 127-132).
+    Arrivals: a Poisson process of rate ``qps``.
     """
+    from .hdr import ToneMapConfig
+
     dev = torch.device("cuda", torch.cuda.current_device() if device is None else device)
     rng = np.random.default_rng(seed)
     src, tgt, mask = compositing_images((base_side, base_side), config=5, problem=0)
@@ -108,14 +115,16 @@ def serving_stream(count: int, qps: float, seed: int = 5000, sides=(512, 2048),
     mask_d = torch.from_numpy(mask).to(dev)
     rads = [torch.from_numpy(lognormal_radiance(tonemap_shape, seed + 100 + k)).to(dev)
             for k in range(radiance_maps)]
+    config = ToneMapConfig(levels=int(tonemap_levels))
+    side_set = list(range(sides[0], min(sides[1], base_side) + 1, max(1, int(side_step))))
     arrivals = poisson_arrivals(count, qps, seed + 1)
     out = []
     for i, t in enumerate(arrivals):
         if rng.random() < tonemap_fraction:
-            out.append(Request("tonemap", {"radiance": rads[int(rng.integers(radiance_maps))]},
-                               float(t), i))
+            out.append(Request("tonemap", {"radiance": rads[int(rng.integers(radiance_maps))],
+                                           "config": config}, float(t), i))
             continue
-        s = int(rng.integers(sides[0], min(sides[1], base_side) + 1))
+        s = int(side_set[int(rng.integers(len(side_set)))])
         r0 = int(rng.integers(0, base_side - s + 1))
         c0 = int(rng.integers(0, base_side - s + 1))
         win = (slice(r0, r0 + s), slice(c0, c0 + s))

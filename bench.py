@@ -373,7 +373,7 @@ def batch_inputs():
     return shapes, plane_shapes, bs, exact
 
 
-def batch_leg(steps: int) -> dict:
+def batch_leg(steps: int, tune: bool = True) -> dict:
     """The batched configuration (64 RGB problems of uneven sizes, 192 planes, one mixed
     plan): solved MPix/s, and the per-kernel HBM rates of one profiled solve over a working
     set far larger than the 256 MB Infinity Cache, so the rates are HBM rates."""
@@ -386,6 +386,12 @@ def batch_leg(steps: int) -> dict:
     shapes, plane_shapes, bs, exact = batch_inputs()
     plan = gdpoisson.Plan(plane_shapes, boundary="neumann",
                           options=gdpoisson.PlanOptions(precision="mixed"))
+    tiles = None
+    if tune:
+        log("batched 64-problem workload: autotuning tile shapes")
+        kinds = ["down", "up", "krylov"]
+        ch = gdpoisson.autotune(plan, bs, solves=1)
+        tiles = {f"{kinds[k]}{l}": v for (k, l), v in sorted(ch.items())}
     outs = [torch.empty_like(b) for b in bs]
     log("batched 64-problem workload: timing")
     plan.mgcg(bs, outs)  # first use: graph capture
@@ -432,6 +438,7 @@ def batch_leg(steps: int) -> dict:
         "max_relative_residual": float(f"{max(r.relative_residual for r in reps):.3e}"),
         "max_abs_err_vs_exact": float(f"{err:.3e}"),
         "converged": all(r.converged and r.relative_residual <= 1e-8 for r in reps),
+        "tiles": tiles,
         "working_set_bytes": int(ws),
         "working_set_vs_infinity_cache": round(ws / (256 << 20), 1),
         "kernels": kernels,
@@ -688,7 +695,7 @@ def main():
     if args.composite_steps > 0:
         line["compositing"] = composite_leg(args.composite_steps)
     if args.batch_steps > 0:
-        line["batched"] = batch_leg(args.batch_steps)
+        line["batched"] = batch_leg(args.batch_steps, tune=not args.no_tune)
     if args.serve_requests > 0:
         line["serving"] = serve_stream(args.serve_requests, args.serve_qps, args.serve_slots)
         head_qps = 200.0 if 200.0 in args.serve_qps else args.serve_qps[0]

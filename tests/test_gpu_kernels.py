@@ -327,6 +327,61 @@ def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):
     assert rel(got.cpu().numpy(), want) < 1e-11
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("nu", [1, 2, 3])
+@pytest.mark.parametrize("stream", [4, 5])
+@pytest.mark.parametrize("shape", [(300, 260), (97, 131), (1030, 250)])
+def test_streamed_down_kernel_is_bit_identical(gd, shape, stream, nu, bc, precision):
+    """The wavefront-order down kernel (menu entries 4, 5: 8 and 16 coarse rows per strip)
+    gives the same bits as the sweep-order kernel (entry 3) on every tiled level, and the
+    oracle's V-cycle to round-off; ragged and tall shapes put strips across the grid's
+    edges and past its last rows."""
+    import torch
+
+    rng = np.random.default_rng(60 + nu + stream)
+    b = rng.standard_normal(shape)
+    u = rng.standard_normal(shape)
+    kw = dict(pre_smoothing=nu, post_smoothing=nu, precision=precision)
+    if bc == "neumann":
+        kw.update(coarsest_size=16)
+    outs = {}
+    for menu in (3, stream):
+        tiles = {(0, l): menu for l in range(6)}
+        plan = gd.Plan([shape], boundary=bc, options=gd.PlanOptions(**kw), tiles=tiles)
+        (got,) = plan.vcycle([torch.from_numpy(u).cuda()], [torch.from_numpy(b).cuda()])
+        outs[menu] = got
+    assert torch.equal(outs[3], outs[stream])
+    if precision == "fp64":
+        okw = {k: v for k, v in kw.items() if k != "precision"}
+        want = OM.v_cycle(u, b, 1.0, OM.MultigridOptions(**okw), bc)
+        assert rel(outs[stream].cpu().numpy(), want) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_streamed_down_kernel_in_batched_solves(gd, precision):
+    """Batched uneven problems solved with the streamed down kernel on every level: the
+    same bits as with the sweep-order kernel, each problem alone and batched."""
+    import torch
+
+    shapes = G.batch_shapes(count=7, seed=4400, low=40, high=700)
+    bs = []
+    for k, s in enumerate(shapes):
+        b, _ = G.neumann_problem(s, seed=4500 + k)
+        bs.append(torch.from_numpy(b).cuda())
+    opts = gd.PlanOptions(precision=precision)
+    ref, _ = gd.Plan(shapes, boundary="neumann", options=opts,
+                     tiles={(0, l): 3 for l in range(6)}).mgcg(bs)
+    got, reps = gd.Plan(shapes, boundary="neumann", options=opts,
+                        tiles={(0, l): 5 for l in range(6)}).mgcg(bs)
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    for k in range(len(shapes)):
+        assert torch.equal(ref[k], got[k]), (k, shapes[k])
+    alone, _ = gd.Plan([shapes[3]], boundary="neumann", options=opts,
+                       tiles={(0, l): 5 for l in range(6)}).mgcg([bs[3]])
+    assert torch.equal(alone[0], got[3])
+
+
 @pytest.mark.parametrize("kry", range(6))
 def test_every_krylov_tile_solves_like_the_oracle(gd, kry):
     """Each compiled Krylov tile shape (apply, update, set-up and check kernels) gives the
@@ -355,7 +410,8 @@ def test_autotuned_plan_solves_the_same_problem(gd):
     ref, rep0 = gd.Plan([shape] * 3, boundary="neumann", options=opts).mgcg(bs)
     plan = gd.Plan([shape] * 3, boundary="neumann", options=opts)
     choice = gd.autotune(plan, use_cache=False)
-    assert set(k for k, _ in choice) <= {0, 1, 2} and all(0 <= v < 4 for v in choice.values())
+    assert set(k for k, _ in choice) <= {0, 1, 2}
+    assert all(0 <= v < (6 if k == 0 else 4 if k == 1 else 6) for (k, _), v in choice.items())
     got, rep = plan.mgcg(bs)
     for c in range(3):
         assert rep[c].converged and rep[c].relative_residual <= 1e-8

@@ -70,14 +70,16 @@ struct Shape2 {
   int h, w;
 };
 // Tile menus.  Kind 0 = down: coarse rows per strip (a block stacks four strips; a
-// strip is StripGeo<nu>::kDownCols coarse columns wide).  Kind 1 = up: fine rows per
+// strip is StripGeo<nu>::kDownCols coarse columns wide); entries 4 and 5 are the
+// streamed (wavefront-order) down kernel, k_sdown_stream.  Kind 1 = up: fine rows per
 // strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per block.
-const int kDownRows[4] = {1, 2, 4, 8};
+const int kDownRows[6] = {1, 2, 4, 8, 8, 16};
 const int kUpRows[4] = {2, 4, 8, 16};
 const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
-constexpr int kMenu = 4;     // level-kernel menus (rows per strip)
-constexpr int kKryMenuN = 6;  // Krylov tile menu
-int menu_size(int kind) { return kind == 2 ? kKryMenuN : kMenu; }
+constexpr int kMenu = 4;       // up-kernel menu (rows per strip)
+constexpr int kDownMenuN = 6;  // down-kernel menu
+constexpr int kKryMenuN = 6;   // Krylov tile menu
+int menu_size(int kind) { return kind == 2 ? kKryMenuN : (kind == 0 ? kDownMenuN : kMenu); }
 constexpr int kStrips = kThreads / 64;  // strips per block
 int down_cols(int nu) { return (125 - 4 * nu) / 2; }
 int up_cols(int nu) { return 124 - 4 * nu; }
@@ -278,6 +280,14 @@ bool has_mask(const gdp_plan* P) {
   for (const ProbInfo& pi : P->probs)
     if (pi.mask) return true;
   return false;
+}
+
+// The down-kernel menu entry of level l: the masked kernels come in the sweep-order form
+// only, so a plan with masked domains runs a streamed entry as the tallest one of that
+// form.
+int down_menu_of(const gdp_plan* P, int l) {
+  const int m = P->down_menu[l];
+  return (m >= 4 && has_mask(P)) ? 3 : m;
 }
 
 // Raises the dynamic-LDS limit of a tail kernel family (0 / 1: strip tail float / double,
@@ -900,7 +910,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   }
   for (int l = 0; l < P->depth; ++l) {
     std::vector<VTile> vd, vu;
-    const Shape2 sd = menu_shape(0, P->down_menu[l], P->opt.pre_smoothing);
+    const Shape2 sd = menu_shape(0, down_menu_of(P, l), P->opt.pre_smoothing);
     const Shape2 su = menu_shape(1, P->up_menu[l], P->opt.post_smoothing);
     for (int p = 0; p < NP; ++p) {
       const ProbInfo& pi = P->probs[p];
@@ -1128,7 +1138,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
     ProfScope ps(P, s, nm, (2.25 * sizeof(T)) * level_points(P, l));
     DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), check_conv ? 1 : 0,
                has_mask(P) ? 1 : 0};
-    launch_down<T>(P->down_menu[l], o.pre_smoothing, P->down[l].count, s, a);
+    launch_down<T>(down_menu_of(P, l), o.pre_smoothing, P->down[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
   // Neumann problems use the symmetric cycle (post-smoothing colour 1 then 0); a plan

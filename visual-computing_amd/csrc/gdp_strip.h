@@ -27,6 +27,8 @@
 // :147-166 (colour 0 then 1, or the reverse for the symmetric Neumann post-smoother),
 // residual :210, restriction 0.25 P^T r P :169-174, prolongation P e P^T :177-181.
 #pragma once
+#include <type_traits>
+
 #include "gdp_reduce.h"
 
 namespace gdp {
@@ -60,6 +62,12 @@ __device__ inline double dpp_shl1(double v) {
   const unsigned hi = (unsigned)dpp_shl1_i(int(unsigned(b >> 32)));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+
+// a * b + c rounded once.  The relaxation, residual and restriction sums of the level
+// kernels are written with it, so their bits do not depend on which products the compiler
+// chooses to contract (the sweep-order and streamed down kernels give the same bits).
+__device__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // 1 / k for the neighbour counts k = 1..4 (0 for k <= 0).
 template <typename T>
@@ -136,10 +144,10 @@ __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>:
     if constexpr (MASK) {
       if (((a + S + FLIP) & 1) == 0) {
         const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
-        u[a].x = wt[a].x * nb - hb[a].x;
+        u[a].x = fmad(wt[a].x, nb, -hb[a].x);
       } else {
         const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
-        u[a].y = wt[a].y * nb - hb[a].y;
+        u[a].y = fmad(wt[a].y, nb, -hb[a].y);
       }
       continue;
     }
@@ -152,11 +160,11 @@ __device__ __attribute__((always_inline)) void strip_halfsweep(typename Pair<T>:
     if (((a + S + FLIP) & 1) == 0) {
       const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
       const T w = EDGE ? lc.wx(vc) : lc.ix2;
-      u[a].x = w * nb - hb[a].x;
+      u[a].x = fmad(w, nb, -hb[a].x);
     } else {
       const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
       const T w = EDGE ? lc.wy(vc) : lc.iy2;
-      u[a].y = w * nb - hb[a].y;
+      u[a].y = fmad(w, nb, -hb[a].y);
     }
   }
 }
@@ -218,6 +226,45 @@ __device__ inline uint64_t load_mask_bits(const uint8_t* __restrict__ mask, int 
     else hi |= bits << (2 * a - 32);
   }
   return ((uint64_t)hi << 32) | lo;
+}
+
+// Full-weighting restriction of one coarse point per lane from the scaled residual rows
+// (multigrid.py:169-174): Neumann taps 1/4, 3/4, 3/4, 1/4 over fine rows 2I-1 .. 2I+2 and
+// columns 2J-1 .. 2J+2 (the lane's pair and its neighbours' by DPP), Dirichlet 1/2, 1, 1/2
+// over 2I .. 2I+2 and 2J .. 2J+2; s = 1 / (4 h^2).  The sums run in this order with one
+// rounding per product-add.
+template <typename T>
+__device__ __attribute__((always_inline)) T restrict_neumann(
+    const typename Pair<T>::type& r0, const typename Pair<T>::type& r1,
+    const typename Pair<T>::type& r2, const typename Pair<T>::type& r3, T s) {
+  const T q0 = T(0.25) * s, q1 = T(0.75) * s;
+  T vx = q0 * r0.x;
+  vx = fmad(q1, r1.x, vx);
+  vx = fmad(q1, r2.x, vx);
+  vx = fmad(q0, r3.x, vx);
+  T vy = q0 * r0.y;
+  vy = fmad(q1, r1.y, vy);
+  vy = fmad(q1, r2.y, vy);
+  vy = fmad(q0, r3.y, vy);
+  T c = T(0.25) * dpp_shr1(vy);
+  c = fmad(T(0.75), vx, c);
+  c = fmad(T(0.75), vy, c);
+  return fmad(T(0.25), dpp_shl1(vx), c);
+}
+
+template <typename T>
+__device__ __attribute__((always_inline)) T restrict_dirichlet(
+    const typename Pair<T>::type& r0, const typename Pair<T>::type& r1,
+    const typename Pair<T>::type& r2, T s) {
+  const T h = T(0.5) * s;
+  T vx = h * r0.x;
+  vx = fmad(s, r1.x, vx);
+  vx = fmad(h, r2.x, vx);
+  T vy = h * r0.y;
+  vy = fmad(s, r1.y, vy);
+  vy = fmad(h, r2.y, vy);
+  const T c = fmad(T(0.5), vx, vy);
+  return fmad(T(0.5), dpp_shl1(vx), c);
 }
 
 // ------------------------------------------------------------------ down
@@ -317,8 +364,8 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
     const T cx = T(vc) + lc.hx, cy = T(vc) + lc.hy;
     const T nx = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
     const T ny = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
-    const T rx = cx * (hb[a].x + u[a].x) - nx;
-    const T ry = cy * (hb[a].y + u[a].y) - ny;
+    const T rx = fmad(cx, hb[a].x + u[a].x, -nx);
+    const T ry = fmad(cy, hb[a].y + u[a].y, -ny);
     hb[a].x = (rin && lc.inx && (!MASK || mask_bit(mb, 2 * a))) ? rx : T(0);
     hb[a].y = (rin && lc.iny && (!MASK || mask_bit(mb, 2 * a + 1))) ? ry : T(0);
   }
@@ -356,17 +403,10 @@ __device__ __attribute__((always_inline)) void sdown_body(const VTile& t, int I0
     const int I = I0 + i;
     T c;
     if (neu) {
-      const int a0 = 2 * i + OFF - 1;
-      const T vx = (T(0.25) * s) * hb[a0].x + (T(0.75) * s) * hb[a0 + 1].x +
-                   (T(0.75) * s) * hb[a0 + 2].x + (T(0.25) * s) * hb[a0 + 3].x;
-      const T vy = (T(0.25) * s) * hb[a0].y + (T(0.75) * s) * hb[a0 + 1].y +
-                   (T(0.75) * s) * hb[a0 + 2].y + (T(0.25) * s) * hb[a0 + 3].y;
-      c = T(0.25) * dpp_shr1(vy) + T(0.75) * vx + T(0.75) * vy + T(0.25) * dpp_shl1(vx);
+      c = restrict_neumann<T>(hb[2 * i + OFF - 1], hb[2 * i + OFF], hb[2 * i + OFF + 1],
+                              hb[2 * i + OFF + 2], s);
     } else {
-      const int a0 = 2 * i + OFF;
-      const T vx = (T(0.5) * s) * hb[a0].x + s * hb[a0 + 1].x + (T(0.5) * s) * hb[a0 + 2].x;
-      const T vy = (T(0.5) * s) * hb[a0].y + s * hb[a0 + 1].y + (T(0.5) * s) * hb[a0 + 2].y;
-      c = T(0.5) * vx + vy + T(0.5) * dpp_shl1(vx);
+      c = restrict_dirichlet<T>(hb[2 * i + OFF], hb[2 * i + OFF + 1], hb[2 * i + OFF + 2], s);
     }
     if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
   }
@@ -434,6 +474,238 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
   } else {
     if (flip) sdown_body<T, NU, CR, 1, false>(t, I0, lane, hb);
     else sdown_body<T, NU, CR, 0, false>(t, I0, lane, hb);
+  }
+}
+
+// ------------------------------------------------------------------ down, streamed
+// The same pre-smoothing + residual + restriction as sdown_body, in wavefront order.  The
+// strip's rows are visited top to bottom; at step t row t is loaded (its right-hand side
+// prefetched kPrefetch rows ahead) and set to the zero-guess half-sweep, half-sweep s
+// (s = 1 .. 2 NU - 1) is applied to row t - s, row t - (2 NU - 1) is then final and
+// stored if owned, the residual is formed at the row above it and, once the last residual
+// row a coarse row reads is ready, that coarse row is restricted.  A half-sweep of colour
+// c at row a reads rows a - 1 and a + 1 only at colour 1 - c, which no half-sweep between
+// s - 1 and s changes, so every point sees the values it sees in the sweep order of
+// sdown_body: the results are the same bits.  Only a window of about 2 NU + 3 rows is live
+// at a time (plus the prefetched loads), so a strip can own CRS = 8 .. 32 coarse rows with
+// few registers: the rows recomputed at its ends are 4 NU + 3 per 2 CRS owned.
+constexpr int kPrefetch = 8;
+
+// f(integral_constant<int, I>) for I = B .. E - 1, expanded at compile time, so the row
+// index of every step is a constant and the row arrays stay in registers however many
+// steps a strip has.
+template <int B, int E, typename F>
+__device__ __attribute__((always_inline)) void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+template <typename T, int NU, int CRS, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void sdown_stream_body(const VTile& t, int I0,
+                                                                 int lane,
+                                                                 typename Pair<T>::type* raw) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int OFF = G::kOff, CC = G::kDownCols;
+  constexpr int FH = 2 * CRS + (NU > 0 ? 4 * NU + 3 : 4);
+  constexpr int K = NU > 0 ? 2 * NU - 1 : 0;  // half-sweeps after the zero-guess one
+  constexpr int R0 = OFF - 1, R1 = OFF + 2 * CRS;
+  const int m = t.m, n = t.n;
+  const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const bool neu = t.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(gc0, lane, n, neu);
+  const T h2 = T(t.h2);
+  const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+  const int gjx = gc0 + 2 * lane;
+  const bool lin = gjx >= 0 && gjx < n;
+  const int jload = lin ? gjx : 0;
+  // stores: owned fine rows / columns of v (the last strip of the grid owns the rest)
+  const int fr0 = 2 * I0, fc0 = 2 * t.j0;
+  const int fr1 = (I0 + CRS >= t.cm) ? m : min(m, fr0 + 2 * CRS);
+  const int fc1 = (t.j0 + CC >= t.cn) ? n : min(n, fc0 + 2 * CC);
+  GDP_GLOBAL T* v = (GDP_GLOBAL T*)t.v;
+  const bool vx = lc.gjx >= fc0 && lc.gjx < fc1;
+  const bool vy = lc.gjx + 1 >= fc0 && lc.gjx + 1 < fc1;
+  // restriction
+  GDP_GLOBAL T* cb = (GDP_GLOBAL T*)t.cb;
+  const T sres = T(0.25) * T(t.inv_h2);
+  const int jcr = lane - OFF / 2;
+  const int J = t.j0 + jcr;
+  const bool wj = jcr >= 0 && jcr < CC && J < t.cn;
+  const bool colext = neu && (gc0 < 0 || gc0 + 128 > n);
+  const bool gl = lc.gjx + 1 == -1;
+  const bool gr = lc.gjx == n && (n & 1) == 0;
+
+  V2 u[FH], hb[FH];
+  static_for<0, FH>([&](auto step) __attribute__((always_inline)) {
+    constexpr int st = decltype(step)::value;
+    // prefetch the right-hand side kPrefetch rows ahead (clamped address, zero outside)
+    if (st + kPrefetch < FH) {
+      const int a = st + kPrefetch;
+      const int gi = gr0 + a;
+      const V2 q = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jload);
+      raw[a] = (lin && gi >= 0 && gi < m) ? q : V2{T(0), T(0)};
+    }
+    // row st: scaled right-hand side and the zero-guess half-sweep (colour 0)
+    {
+      const int a = st;
+      const int gi = gr0 + a;
+      int vc = 2;
+      bool rin = true;
+      if (EDGE) {
+        rin = gi >= 0 && gi < m;
+        vc = row_count(gi, m, neu);
+      }
+      const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+      const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+      hb[a].x = (rin && lc.inx) ? wx * h2 * raw[a].x : T(0);
+      hb[a].y = (rin && lc.iny) ? wy * h2 * raw[a].y : T(0);
+      if (NU > 0 && ((a + FLIP) & 1) == 0) {
+        u[a].x = -hb[a].x;
+        u[a].y = T(0);
+      } else if (NU > 0) {
+        u[a].x = T(0);
+        u[a].y = -hb[a].y;
+      } else {
+        u[a] = V2{T(0), T(0)};
+      }
+    }
+    // half-sweep s at row st - s
+#pragma unroll
+    for (int s = 1; s <= K; ++s) {
+      const int a = st - s;
+      if (a < 1 || a > FH - 2) continue;
+      int vc = 2;
+      if (EDGE) {
+        const int gi = gr0 + a;
+        if (gi < 0 || gi >= m) continue;
+        vc = row_count(gi, m, neu);
+      }
+      if (((a + s + FLIP) & 1) == 0) {
+        const T nb = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
+        const T w = EDGE ? lc.wx(vc) : lc.ix2;
+        u[a].x = fmad(w, nb, -hb[a].x);
+      } else {
+        const T nb = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
+        const T w = EDGE ? lc.wy(vc) : lc.iy2;
+        u[a].y = fmad(w, nb, -hb[a].y);
+      }
+    }
+    // row st - K is final: the pre-smoothed iterate of the owned rows
+    {
+      const int a = st - K;
+      if (a >= OFF && a < OFF + 2 * CRS + 2) {
+        const int gi = gr0 + a;
+        if (gi < fr1) {
+          GDP_GLOBAL T* row = v + (size_t)gi * t.pitch + lc.gjx;
+          if (vx && vy) {
+            *(GDP_GLOBAL V2*)row = u[a];
+          } else if (vx) {
+            row[0] = u[a].x;
+          }
+        }
+      }
+    }
+    // scaled residual of the row above it, in place of hb', then the extensions past the
+    // grid (Neumann: columns, then rows) and the restriction of a completed coarse row
+    {
+      const int a = st - K - 1;
+      if (a >= R0 && a <= R1) {
+        const int gi = gr0 + a;
+        int vc = 2;
+        bool rin = true;
+        if (EDGE) {
+          rin = gi >= 0 && gi < m;
+          vc = row_count(gi, m, neu);
+        }
+        const T cx = T(vc) + lc.hx, cy = T(vc) + lc.hy;
+        const T nx = ((u[a - 1].x + u[a + 1].x) + dpp_shr1(u[a].y)) + u[a].y;
+        const T ny = ((u[a - 1].y + u[a + 1].y) + u[a].x) + dpp_shl1(u[a].x);
+        const T rx = fmad(cx, hb[a].x + u[a].x, -nx);
+        const T ry = fmad(cy, hb[a].y + u[a].y, -ny);
+        hb[a].x = (rin && lc.inx) ? rx : T(0);
+        hb[a].y = (rin && lc.iny) ? ry : T(0);
+        if (colext) {
+          const T sx = dpp_shl1(hb[a].x);
+          const T sy = dpp_shr1(hb[a].y);
+          if (gl) hb[a].y = sx;
+          if (gr) hb[a].x = sy;
+        }
+        if (EDGE && neu) {
+          if (a > R0 && gi - 1 == -1) hb[a - 1] = hb[a];
+          if (gi == m && (m & 1) == 0 && a > R0) hb[a] = hb[a - 1];
+        }
+        if (a >= OFF + 2 && ((a - OFF - 2) & 1) == 0) {
+          const int i = (a - OFF - 2) / 2;
+          const int I = I0 + i;
+          T c;
+          if (neu) {
+            c = restrict_neumann<T>(hb[2 * i + OFF - 1], hb[2 * i + OFF], hb[2 * i + OFF + 1],
+                                    hb[2 * i + OFF + 2], sres);
+          } else {
+            c = restrict_dirichlet<T>(hb[2 * i + OFF], hb[2 * i + OFF + 1],
+                                      hb[2 * i + OFF + 2], sres);
+          }
+          if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
+        }
+      }
+    }
+  });
+}
+
+template <typename T, int NU, int CRS>
+__global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restrict__ tiles,
+                                                           ProblemState* __restrict__ st,
+                                                           const double2* __restrict__ upart,
+                                                           int check_conv) {
+  using V2 = typename Pair<T>::type;
+  using G = StripGeo<NU>;
+  constexpr int OFF = G::kOff;
+  constexpr int FH = 2 * CRS + (NU > 0 ? 4 * NU + 3 : 4);
+  const VTile t = tiles[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int I0 = t.i0 + w * CRS;
+  const bool work = I0 < t.cm;
+  const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const int m = t.m, n = t.n;
+  // the first kPrefetch rows are requested before the state tests (branch-free, clamped
+  // addresses, zero outside the grid); the body requests the rest as it goes
+  V2 raw[FH];
+  {
+    const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const int gjx = gc0 + 2 * lane;
+    const bool lin = gjx >= 0 && gjx < n;
+    const int jc = lin ? gjx : 0;
+#pragma unroll
+    for (int a = 0; a < kPrefetch && a < FH; ++a) {
+      const int gi = gr0 + a;
+      const bool in = work && lin && gi >= 0 && gi < m;
+      const V2 q = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jc);
+      raw[a] = in ? q : V2{T(0), T(0)};
+    }
+  }
+  asm volatile("" ::: "memory");
+  if (st[t.problem].done) return;
+  if (check_conv && t.level == 0) {
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
+                                  shc, &bcc))
+      return;
+  }
+  if (!work) return;
+  const bool edge = gr0 < 1 || gr0 + FH > m - 1;
+  const int flip = (gr0 + gc0) & 1;
+  if (edge) {
+    if (flip) sdown_stream_body<T, NU, CRS, 1, true>(t, I0, lane, raw);
+    else sdown_stream_body<T, NU, CRS, 0, true>(t, I0, lane, raw);
+  } else {
+    if (flip) sdown_stream_body<T, NU, CRS, 1, false>(t, I0, lane, raw);
+    else sdown_stream_body<T, NU, CRS, 0, false>(t, I0, lane, raw);
   }
 }
 

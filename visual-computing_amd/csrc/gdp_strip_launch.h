@@ -33,9 +33,25 @@ void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #undef GDP_DOWN
 }
 
+// Streamed strips (menu entries 4, 5): the wavefront-order kernel with CRS = 8 or 16
+// coarse rows, in translation units of their own (gdp_sdstream_*.hip).
+template <typename T, int CRS>
+void down_stream_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWNS(NU) \
+  klaunch(k_sdown_stream<T, NU, CRS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv)
+  switch (nu) {
+    case 0: GDP_DOWNS(0); break;
+    case 1: GDP_DOWNS(1); break;
+    case 2: GDP_DOWNS(2); break;
+    default: GDP_DOWNS(3); break;
+  }
+#undef GDP_DOWNS
+}
+
 template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a) {
   if (a.masked) {
+    // (the planner gives masked levels menu entries 0 .. 3 only)
     switch (menu) {
       case 0: down_masked<T, 1>(nu, blocks, s, a); break;
       case 1: down_masked<T, 2>(nu, blocks, s, a); break;
@@ -48,7 +64,8 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
     case 0: down_nu<T, 1>(nu, blocks, s, a); break;
     case 1: down_nu<T, 2>(nu, blocks, s, a); break;
     case 2: down_nu<T, 4>(nu, blocks, s, a); break;
-    default: down_nu<T, 8>(nu, blocks, s, a); break;
+    case 3: down_nu<T, 8>(nu, blocks, s, a); break;
+    default: launch_down_stream<T>(menu == 4 ? 8 : 16, nu, blocks, s, a); break;
   }
 }
 

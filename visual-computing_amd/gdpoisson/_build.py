@@ -28,6 +28,8 @@ SOURCES = [
     "gdp_fft.hip",
     "gdp_sdown_f32.hip",
     "gdp_sdown_f64.hip",
+    "gdp_sdstream_f32.hip",
+    "gdp_sdstream_f64.hip",
     "gdp_sup_f32.hip",
     "gdp_sup_f64.hip",
     "gdp_stail_f32.hip",

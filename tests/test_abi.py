@@ -126,7 +126,7 @@ def test_python_errors_map_status_codes(L):
 
 def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
-    for kind, size in ((0, 4), (1, 4), (2, 6)):
+    for kind, size in ((0, 6), (1, 4), (2, 6)):
         for idx in range(size):
             for nu in range(4):
                 assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
@@ -137,7 +137,10 @@ def test_tile_menu(L):
     assert tw.value == 58
     L.lib().gdp_tile_menu(1, 0, 2, ctypes.byref(th), ctypes.byref(tw))
     assert tw.value == 116
-    for bad in ((3, 0, 1), (0, 4, 1), (2, 6, 1), (0, 0, 4)):
+    # the down menu's entries 4 and 5 are streamed strips of 8 and 16 coarse rows
+    L.lib().gdp_tile_menu(0, 5, 2, ctypes.byref(th), ctypes.byref(tw))
+    assert (th.value, tw.value) == (64, 58)
+    for bad in ((3, 0, 1), (0, 6, 1), (1, 4, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
 
 

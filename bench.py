@@ -70,20 +70,15 @@ def pct(xs, q):
     return xs[k]
 
 
-def cpu_baseline(runs: int = 5):
-    """The oracle's FP64 MG-PCG (Neumann) on the whole RGB workload, one core.
-
-    Median of ``runs`` timed solves after one untimed warm-up (about 2 s each here).
-    """
-    import numpy as np
+def _cpu_solves(bs, runs: int, limit):
+    """Median seconds of ``runs`` oracle RGB solves after one warm-up, with the thread
+    pools of numpy / scipy limited to ``limit`` (None: the libraries' defaults)."""
     from threadpoolctl import threadpool_limits
 
-    from oracle import generators as G
     from oracle import solvers as OS
 
-    bs = [G.neumann_problem(SHAPE, seed=2000 + c)[0] for c in range(CHANNELS)]
     times, its = [], []
-    with threadpool_limits(limits=1):
+    with threadpool_limits(limits=limit):
         for k in range(runs + 1):
             t0 = time.perf_counter()
             for b in bs:
@@ -91,15 +86,37 @@ def cpu_baseline(runs: int = 5):
                 its.append(it)
             if k:
                 times.append(time.perf_counter() - t0)
-    dt = statistics.median(times)
+    return statistics.median(times), max(its)
+
+
+def cpu_baseline(runs: int = 5):
+    """The oracle's FP64 MG-PCG (Neumann) on the whole RGB workload: one core (the
+    reference is effectively single-threaded), and once more with the libraries' default
+    thread pools.  Median of ``runs`` timed solves after one untimed warm-up (about 2 s
+    each here)."""
+    from threadpoolctl import threadpool_info
+
+    from oracle import generators as G
+
+    bs = [G.neumann_problem(SHAPE, seed=2000 + c)[0] for c in range(CHANNELS)]
+    dt, its = _cpu_solves(bs, runs, 1)
+    dt_def, _ = _cpu_solves(bs, max(1, runs // 2), None)
+    pools = sorted({f"{i.get('internal_api')}:{i.get('num_threads')}" for i in threadpool_info()})
     return {
         "value": round(SHAPE[0] * SHAPE[1] / 1e6 / dt, 4),
         "unit": "MPix/s",
         "cores": 1,
         "kind": "port",
         "sample": f"the whole 1024x1024 RGB workload with the numpy FP64 MG-PCG oracle "
-        f"({max(its)} iterations per channel), median of {runs} solves after one warm-up, "
+        f"({its} iterations per channel), median of {runs} solves after one warm-up, "
         f"{dt:.2f} s per RGB solve, {os.cpu_count()} host cores visible, 1 used",
+        "default_threads": {
+            "value": round(SHAPE[0] * SHAPE[1] / 1e6 / dt_def, 4),
+            "seconds_per_solve": round(dt_def, 3),
+            "thread_pools": pools,
+            "note": "the same solves with numpy/scipy's default thread pools; elementwise "
+                    "numpy, scipy CSR products and pocketfft run one thread either way",
+        },
     }
 
 

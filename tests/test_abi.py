@@ -126,7 +126,7 @@ def test_python_errors_map_status_codes(L):
 
 def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
-    for kind, size in ((0, 6), (1, 4), (2, 6)):
+    for kind, size in ((0, 7), (1, 4), (2, 6)):
         for idx in range(size):
             for nu in range(4):
                 assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
@@ -137,10 +137,15 @@ def test_tile_menu(L):
     assert tw.value == 58
     L.lib().gdp_tile_menu(1, 0, 2, ctypes.byref(th), ctypes.byref(tw))
     assert tw.value == 116
-    # the down menu's entries 4 and 5 are streamed strips of 8 and 16 coarse rows
+    # the down menu's entries 4, 5, 6 are streamed strips of 8, 16 and 32 coarse rows
+    # (32 for nu = 1 and 2; other sweep counts get the 16-row strips)
     L.lib().gdp_tile_menu(0, 5, 2, ctypes.byref(th), ctypes.byref(tw))
     assert (th.value, tw.value) == (64, 58)
-    for bad in ((3, 0, 1), (0, 6, 1), (1, 4, 1), (2, 6, 1), (0, 0, 4)):
+    L.lib().gdp_tile_menu(0, 6, 2, ctypes.byref(th), ctypes.byref(tw))
+    assert th.value == 128
+    L.lib().gdp_tile_menu(0, 6, 3, ctypes.byref(th), ctypes.byref(tw))
+    assert th.value == 64
+    for bad in ((3, 0, 1), (0, 7, 1), (1, 4, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
 
 

@@ -51,9 +51,12 @@ template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a);
 template <typename T>
 void launch_up(int menu, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a);
-// The streamed down kernel (k_sdown_stream), crs = 8 or 16 coarse rows per strip.
+// The streamed down kernel (k_sdown_stream), crs = 8, 16 or 32 coarse rows per strip
+// (32 for nu = 1 and 2 only, in launch_down_stream32).
 template <typename T>
 void launch_down_stream(int crs, int nu, int blocks, hipStream_t s, const DownArgs& a);
+template <typename T>
+void launch_down_stream32(int nu, int blocks, hipStream_t s, const DownArgs& a);
 
 // The register-strip coarse tail (gdp_stail_*.hip): one 512-thread block per problem.
 constexpr int kStailBlockThreads = 512;  // threads of a strip-tail block

@@ -70,16 +70,20 @@ struct Shape2 {
   int h, w;
 };
 // Tile menus.  Kind 0 = down: coarse rows per strip (a block stacks four strips; a
-// strip is StripGeo<nu>::kDownCols coarse columns wide); entries 4 and 5 are the
-// streamed (wavefront-order) down kernel, k_sdown_stream.  Kind 1 = up: fine rows per
-// strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per block.
-const int kDownRows[6] = {1, 2, 4, 8, 8, 16};
+// strip is StripGeo<nu>::kDownCols coarse columns wide); entries 4, 5 and 6 are the
+// streamed (wavefront-order) down kernel, k_sdown_stream, the last one compiled for
+// nu = 1 and 2 only.  Kind 1 = up: fine rows per strip (kUpCols fine columns wide).
+// Kind 2 = Krylov: fine points per block.
+const int kDownRows[7] = {1, 2, 4, 8, 8, 16, 32};
 const int kUpRows[4] = {2, 4, 8, 16};
 const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
 constexpr int kMenu = 4;       // up-kernel menu (rows per strip)
-constexpr int kDownMenuN = 6;  // down-kernel menu
+constexpr int kDownMenuN = 7;  // down-kernel menu
 constexpr int kKryMenuN = 6;   // Krylov tile menu
 int menu_size(int kind) { return kind == 2 ? kKryMenuN : (kind == 0 ? kDownMenuN : kMenu); }
+// The 32-row streamed down entry exists for nu = 1 and 2; other sweep counts run the
+// 16-row one.
+int down_entry(int idx, int nu) { return (idx == 6 && nu != 1 && nu != 2) ? 5 : idx; }
 constexpr int kStrips = kThreads / 64;  // strips per block
 int down_cols(int nu) { return (125 - 4 * nu) / 2; }
 int up_cols(int nu) { return 124 - 4 * nu; }
@@ -286,7 +290,7 @@ bool has_mask(const gdp_plan* P) {
 // only, so a plan with masked domains runs a streamed entry as the tallest one of that
 // form.
 int down_menu_of(const gdp_plan* P, int l) {
-  const int m = P->down_menu[l];
+  const int m = down_entry(P->down_menu[l], P->opt.pre_smoothing);
   return (m >= 4 && has_mask(P)) ? 3 : m;
 }
 
@@ -369,7 +373,7 @@ size_t tail_layout(ProbInfo& pi, int first, size_t tsize) {
 // Block tile of a menu entry: down in coarse points, up and Krylov in fine points; the
 // strip widths depend on the number of sweeps nu of the kernel.
 Shape2 menu_shape(int kind, int idx, int nu) {
-  if (kind == 0) return Shape2{kStrips * kDownRows[idx], down_cols(nu)};
+  if (kind == 0) return Shape2{kStrips * kDownRows[down_entry(idx, nu)], down_cols(nu)};
   if (kind == 1) return Shape2{kStrips * kUpRows[idx], up_cols(nu)};
   return kKryMenu[idx];
 }

@@ -33,8 +33,9 @@ void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #undef GDP_DOWN
 }
 
-// Streamed strips (menu entries 4, 5): the wavefront-order kernel with CRS = 8 or 16
-// coarse rows, in translation units of their own (gdp_sdstream_*.hip).
+// Streamed strips (menu entries 4, 5, 6): the wavefront-order kernel with CRS = 8, 16 or
+// 32 coarse rows, in translation units of their own (gdp_sdstream*.hip; 32 rows for
+// nu = 1 and 2 only: the planner gives other sweep counts the 16-row entry).
 template <typename T, int CRS>
 void down_stream_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWNS(NU) \
@@ -65,7 +66,7 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
     case 1: down_nu<T, 2>(nu, blocks, s, a); break;
     case 2: down_nu<T, 4>(nu, blocks, s, a); break;
     case 3: down_nu<T, 8>(nu, blocks, s, a); break;
-    default: launch_down_stream<T>(menu == 4 ? 8 : 16, nu, blocks, s, a); break;
+    default: launch_down_stream<T>(menu == 4 ? 8 : (menu == 5 ? 16 : 32), nu, blocks, s, a); break;
   }
 }
 

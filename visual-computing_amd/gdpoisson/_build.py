@@ -22,18 +22,21 @@ INCLUDE = PKG.parent.parent / "include"
 LIB = PKG / "libgdpoisson.so"
 
 SOURCES = [
+    # the longest compiles first, so they start before the pool fills
+    "gdp_sdstream_f32.hip",
+    "gdp_sdstream_f64.hip",
+    "gdp_sdstream32_f32.hip",
+    "gdp_sdstream32_f64.hip",
+    "gdp_stail_f32.hip",
+    "gdp_stail_f64.hip",
+    "gdp_sdown_f32.hip",
+    "gdp_sdown_f64.hip",
+    "gdp_sup_f32.hip",
+    "gdp_sup_f64.hip",
     "gdp_plan.hip",
     "gdp_ops.hip",
     "gdp_hdr.hip",
     "gdp_fft.hip",
-    "gdp_sdown_f32.hip",
-    "gdp_sdown_f64.hip",
-    "gdp_sdstream_f32.hip",
-    "gdp_sdstream_f64.hip",
-    "gdp_sup_f32.hip",
-    "gdp_sup_f64.hip",
-    "gdp_stail_f32.hip",
-    "gdp_stail_f64.hip",
 ]
 ARCH = os.environ.get("GDP_OFFLOAD_ARCH", "gfx950")
 

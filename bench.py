@@ -233,11 +233,22 @@ def composite_leg(steps: int) -> dict:
     return out
 
 
+def _family(base: str) -> str | None:
+    """The level-kernel family of a dispatch: the sweep-order and streamed down kernels are
+    one family (they run the same levels), ranked together by level area."""
+    if "k_sdown<" in base or "k_sdown_stream<" in base:
+        return "k_sdown<"
+    for key in ("k_sup<", "k_down<", "k_up<"):
+        if key in base:
+            return key
+    return None
+
+
 def _level_area(name: str, grid: int) -> int:
     """Grid size x rows per strip (the third template argument of the strip kernels):
     proportional to the level's area whatever tile shape the level was given."""
     base = name.split("(")[0]
-    if "k_sdown<" in base or "k_sup<" in base:
+    if _family(base) in ("k_sdown<", "k_sup<"):
         args = base[base.index("<") + 1:].split(",")
         return grid * int(args[2].strip(" >"))
     return grid
@@ -249,18 +260,19 @@ def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
     Levels are ranked per kernel family by grid size x rows per strip (largest = finest).
     """
     base = name.split("(")[0]
-    for key, kind in (("k_sdown<", "down"), ("k_sup<", "up"), ("k_stail<", "tail"),
-                      ("k_down<", "down"), ("k_up<", "up"), ("k_tail<", "tail"),
+    fam = _family(base)
+    if fam is not None:
+        kind = "down" if fam in ("k_sdown<", "k_down<") else "up"
+        # k_sup<T, nu, rows, dots, masked>
+        args = [a.strip(" >") for a in base[base.index("<") + 1:].split(",")]
+        dots = kind == "up" and len(args) > 3 and args[3] == "true"
+        lv = sorted(grids[fam], reverse=True).index(_level_area(name, grid))
+        return f"{kind} L{lv}" + ("+dots" if dots else "")
+    for key, kind in (("k_stail<", "tail"), ("k_tail<", "tail"),
                       ("k_apply_s<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
                       ("k_residual<", "residual"), ("k_finalize<", "finalize"),
                       ("k_init_stats<", "init_stats")):
         if key in base:
-            if kind in ("down", "up"):
-                # k_sup<T, nu, rows, dots, masked>
-                args = [a.strip(" >") for a in base[base.index("<") + 1:].split(",")]
-                dots = kind == "up" and args[3] == "true"
-                lv = sorted(grids[key], reverse=True).index(_level_area(name, grid))
-                return f"{kind} L{lv}" + ("+dots" if dots else "")
             return kind
     return None
 
@@ -289,10 +301,9 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
                                        "scale": round(scale, 4)}
         grids = collections.defaultdict(set)
         for r in rows:
-            base = r["Kernel_Name"].split("(")[0]
-            for key in ("k_sdown<", "k_sup<", "k_down<", "k_up<"):
-                if key in base:
-                    grids[key].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
+            fam = _family(r["Kernel_Name"].split("(")[0])
+            if fam is not None:
+                grids[fam].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
         # grouped by instantiation and launch shape: a kind with two of them (the tail
         # with and without the deferred x update, k_update with and without its x half)
         # averages over both, as its algorithmic bytes per launch do; the early-exit
@@ -556,6 +567,11 @@ def main():
         _, plane_shapes, bs, _ = batch_inputs()
         plan = gdpoisson.Plan(plane_shapes, boundary="neumann",
                               options=gdpoisson.PlanOptions(precision="mixed"))
+        if args.tiles:  # the tiles the bench line's batched leg tuned
+            rec = json.loads(Path(args.tiles).read_text()).get("batched", {}).get("tiles") or {}
+            kinds = ["down", "up", "krylov"]
+            plan.set_tiles({(kinds.index(k.rstrip("0123456789")),
+                             int(k[len(k.rstrip("0123456789")):])): v for k, v in rec.items()})
         for _ in range(1 + args.steps):
             plan.mgcg(bs)
         calibration_copy()

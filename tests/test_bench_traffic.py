@@ -81,3 +81,17 @@ def test_traffic_keeps_the_two_update_variants_apart(tmp_path):
     k = bench.reduce_traffic(str(fcsv), str(wcsv))["kernels"]
     assert k["update"]["fetch_bytes"] == round((100.0 + 4 * 45.0) / 5 * 1024 * 2)
     assert k["update"]["write_bytes"] == round((50.0 + 4 * 22.5) / 5 * 1024)
+
+
+def test_streamed_and_sweep_order_down_kernels_rank_as_one_family():
+    """A plan may run its finest down level with the streamed kernel and coarser ones with
+    the sweep-order kernel: levels are ranked by area across both."""
+    bench = _bench()
+    names = {
+        ("void gdp::k_sdown_stream<float, 2, 32>(...)", 4096): None,  # area 4096 * 32
+        ("void gdp::k_sdown<float, 2, 4, false>(...)", 8192): None,   # area 8192 * 4
+        ("void gdp::k_sdown<float, 2, 1, false>(...)", 4096): None,   # area 4096 * 1
+    }
+    grids = {"k_sdown<": {bench._level_area(n, g) for n, g in names}}
+    kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
+    assert kinds == ["down L0", "down L1", "down L2"]

@@ -79,10 +79,17 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
     choice: dict = {}
 
     def cost(times: dict, kind: int, level: int) -> float:
+        # a tile choice also costs the kernels that reduce its partial sums: every block
+        # of the apply reduces the finest up kernel's <r,z>, <q,z> slots, and the first
+        # down kernel's convergence test reduces the update's ||r||^2 slots (one slot
+        # per block of the producer)
         if kind == 2:
-            return sum(t for k, t in times.items() if k in ("apply", "update"))
+            return sum(t for k, t in times.items() if k in ("apply", "update", "down L0"))
         pre = "down" if kind == 0 else "up"
-        return sum(t for k, t in times.items() if re.fullmatch(rf"{pre} L{level}(\+dots)?", k))
+        t = sum(v for k, v in times.items() if re.fullmatch(rf"{pre} L{level}(\+dots)?", k))
+        if kind == 1 and level == 0:
+            t += times.get("apply", 0.0)
+        return t
 
     for kind, level in [(k, l) for l in levels for k in (0, 1)] + [(2, 0)]:
         best_idx, best_t = None, float("inf")

@@ -330,14 +330,15 @@ def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 @pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
 @pytest.mark.parametrize("nu", [1, 2, 3])
-@pytest.mark.parametrize("stream", [4, 5, 6])
+@pytest.mark.parametrize("stream", [4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("shape", [(300, 260), (97, 131), (1030, 250)])
 def test_streamed_down_kernel_is_bit_identical(gd, shape, stream, nu, bc, precision):
     """The wavefront-order down kernel (menu entries 4, 5, 6: 8, 16 and 32 coarse rows per
-    strip; 32 falls back to 16 for nu = 3) gives the same bits as the sweep-order kernel
-    (entry 3) on every tiled level, and the
-    oracle's V-cycle to round-off; ragged and tall shapes put strips across the grid's
-    edges and past its last rows."""
+    strip; 32 falls back to 16 for nu = 3) and the block-tiled one (entries 7, 8, 9: 8, 16
+    and 32 coarse rows per block, rows shared by its waves through LDS) give the same bits
+    as the sweep-order kernel (entry 3) on every tiled level, and the oracle's V-cycle to
+    round-off; ragged and tall shapes put strips and blocks across the grid's edges and
+    past its last rows."""
     import torch
 
     rng = np.random.default_rng(60 + nu + stream)
@@ -359,10 +360,12 @@ def test_streamed_down_kernel_is_bit_identical(gd, shape, stream, nu, bc, precis
         assert rel(outs[stream].cpu().numpy(), want) < 1e-11
 
 
+@pytest.mark.parametrize("down", [5, 8])
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_streamed_down_kernel_in_batched_solves(gd, precision):
-    """Batched uneven problems solved with the streamed down kernel on every level: the
-    same bits as with the sweep-order kernel, each problem alone and batched."""
+def test_streamed_down_kernel_in_batched_solves(gd, precision, down):
+    """Batched uneven problems solved with the streamed (entry 5) or block-tiled (entry 8)
+    down kernel on every level: the same bits as with the sweep-order kernel, each problem
+    alone and batched."""
     import torch
 
     shapes = G.batch_shapes(count=7, seed=4400, low=40, high=700)
@@ -374,12 +377,12 @@ def test_streamed_down_kernel_in_batched_solves(gd, precision):
     ref, _ = gd.Plan(shapes, boundary="neumann", options=opts,
                      tiles={(0, l): 3 for l in range(6)}).mgcg(bs)
     got, reps = gd.Plan(shapes, boundary="neumann", options=opts,
-                        tiles={(0, l): 5 for l in range(6)}).mgcg(bs)
+                        tiles={(0, l): down for l in range(6)}).mgcg(bs)
     assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
     for k in range(len(shapes)):
         assert torch.equal(ref[k], got[k]), (k, shapes[k])
     alone, _ = gd.Plan([shapes[3]], boundary="neumann", options=opts,
-                       tiles={(0, l): 5 for l in range(6)}).mgcg([bs[3]])
+                       tiles={(0, l): down for l in range(6)}).mgcg([bs[3]])
     assert torch.equal(alone[0], got[3])
 
 

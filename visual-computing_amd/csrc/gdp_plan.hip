@@ -72,13 +72,15 @@ struct Shape2 {
 // Tile menus.  Kind 0 = down: coarse rows per strip (a block stacks four strips; a
 // strip is StripGeo<nu>::kDownCols coarse columns wide); entries 4, 5 and 6 are the
 // streamed (wavefront-order) down kernel, k_sdown_stream, the last one compiled for
-// nu = 1 and 2 only.  Kind 1 = up: fine rows per strip (kUpCols fine columns wide).
-// Kind 2 = Krylov: fine points per block.
-const int kDownRows[7] = {1, 2, 4, 8, 8, 16, 32};
+// nu = 1 and 2 only; entries 7, 8 and 9 the block-tiled kernel, k_sdown_blk, whose block
+// owns 8, 16 or 32 coarse rows (four times the listed rows, like the others).  Kind 1 =
+// up: fine rows per strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per
+// block.
+const int kDownRows[10] = {1, 2, 4, 8, 8, 16, 32, 2, 4, 8};
 const int kUpRows[4] = {2, 4, 8, 16};
 const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
 constexpr int kMenu = 4;       // up-kernel menu (rows per strip)
-constexpr int kDownMenuN = 7;  // down-kernel menu
+constexpr int kDownMenuN = 10;  // down-kernel menu
 constexpr int kKryMenuN = 6;   // Krylov tile menu
 int menu_size(int kind) { return kind == 2 ? kKryMenuN : (kind == 0 ? kDownMenuN : kMenu); }
 // The 32-row streamed down entry exists for nu = 1 and 2; other sweep counts run the
@@ -287,8 +289,8 @@ bool has_mask(const gdp_plan* P) {
 }
 
 // The down-kernel menu entry of level l: the masked kernels come in the sweep-order form
-// only, so a plan with masked domains runs a streamed entry as the tallest one of that
-// form.
+// only, so a plan with masked domains runs a streamed or block-tiled entry as the
+// tallest one of that form.
 int down_menu_of(const gdp_plan* P, int l) {
   const int m = down_entry(P->down_menu[l], P->opt.pre_smoothing);
   return (m >= 4 && has_mask(P)) ? 3 : m;

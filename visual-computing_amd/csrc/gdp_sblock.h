@@ -1,0 +1,314 @@
+// Block-tiled down kernel: the pre-smoothing + residual + restriction of sdown_body
+// (gdp_strip.h) for one tall strip per block, its rows shared among the block's four
+// waves instead of recomputed by each.
+//
+// A block owns CB coarse rows x kDownCols coarse columns.  Its fine rows are those of a
+// strip kernel with CB coarse rows, gr0 = 2 I0 - kOff .. gr0 + FH - 1 (FH = 2 CB + 4 NU
+// + 3), rounded up to 4 RW rows: wave w holds rows w RW .. w RW + RW - 1 of them in
+// registers, lane l the column pair (2l, 2l+1) of each, as a strip wave does.  A
+// half-sweep reads the rows just above and below a wave's rows from the neighbouring
+// waves: each wave publishes its first and last row to a double-buffered LDS slot and
+// reads its neighbours' after one barrier.  The margin rows of the block (kOff above,
+// 2 NU + 1 below the owned rows, 4 NU + 3 per 2 CB owned) are the only rows computed
+// twice, against 4 NU + 3 per 2 CR for a strip of CR rows: at CB = 16 and NU = 2 the
+// block loads 1.34 rows per owned row, the 2-row strip 3.75.
+//
+// Every point is updated by the same expression as in sdown_body (explicit fused
+// multiply-adds, the same neighbour order), and the rows and columns the block writes
+// lie inside the region a whole-grid sweep agrees with, so the results are the same
+// bits as the strip kernels'.  The residual rows the restriction reads go through LDS,
+// where the Neumann row extension (r[-1] = r[0], r[m] = r[m-1] for even m) is applied
+// as they are read.
+//
+// Arithmetic: gradient-domain/src/gradient_domain/multigrid.py smoother :147-166,
+// residual :210, restriction 0.25 P^T r P :169-174.
+#pragma once
+#include "gdp_strip.h"
+
+namespace gdp {
+
+constexpr int kBlkWaves = kThreads / 64;  // 4
+
+template <typename T, int NU, int CB>
+struct BlkGeo {
+  static constexpr int kOff = StripGeo<NU>::kOff;
+  static constexpr int kFH = 2 * CB + (NU > 0 ? 4 * NU + 3 : 4);
+  // rows per wave, even (a row's colour is then its index's parity within the wave)
+  static constexpr int kRW = 2 * ((kFH + 2 * kBlkWaves - 1) / (2 * kBlkWaves));
+  static constexpr int kR0 = kOff - 1, kR1 = kOff + 2 * CB;  // residual rows (inclusive)
+  static constexpr int kRes = kR1 - kR0 + 1;
+  // LDS: the halo slots (2 buffers x 4 waves x 2 rows of 64 pairs) and the residual rows
+  static constexpr int kHaloBytes = 2 * kBlkWaves * 2 * 64 * int(sizeof(T)) * 2;
+  static constexpr int kLdsBytes = kHaloBytes + kRes * 64 * int(sizeof(T)) * 2;
+};
+
+// Exchange of wave edge rows: publish this wave's first and last row, return the row
+// above its first (the previous wave's last) and below its last (the next wave's first);
+// zero at the block's ends.
+template <typename V2>
+__device__ __attribute__((always_inline)) void blk_halo(V2* xh, int& hbuf, int w, int lane,
+                                                      const V2& top, const V2& bot, V2& above,
+                                                      V2& below) {
+  V2* h = xh + hbuf * (kBlkWaves * 2 * 64);
+  h[(w * 2 + 0) * 64 + lane] = top;
+  h[(w * 2 + 1) * 64 + lane] = bot;
+  __syncthreads();
+  above = w > 0 ? h[((w - 1) * 2 + 1) * 64 + lane] : V2{0, 0};
+  below = w < kBlkWaves - 1 ? h[((w + 1) * 2 + 0) * 64 + lane] : V2{0, 0};
+  hbuf ^= 1;
+}
+
+// Half-sweep S of the wave's rows (block rows a = w RW + r; the colour of x is
+// (r + S + FLIP) even, RW being even).  EDGE: rows may lie outside the grid or on its
+// first / last row.
+template <typename T, int RW, int S, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void blk_halfsweep(typename Pair<T>::type* u,
+                                                           const typename Pair<T>::type* hb,
+                                                           const LaneCols<T>& lc, int gw0, int m,
+                                                           bool neu,
+                                                           const typename Pair<T>::type& above,
+                                                           const typename Pair<T>::type& below) {
+  using V2 = typename Pair<T>::type;
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    int vc = 2;
+    if (EDGE) {
+      const int gi = gw0 + r;
+      if (gi < 0 || gi >= m) continue;
+      vc = row_count(gi, m, neu);
+    }
+    const V2 up = r > 0 ? u[r - 1] : above;
+    const V2 dn = r < RW - 1 ? u[r + 1] : below;
+    if (((r + S + FLIP) & 1) == 0) {
+      const T nb = ((up.x + dn.x) + dpp_shr1(u[r].y)) + u[r].y;
+      const T w = EDGE ? lc.wx(vc) : lc.ix2;
+      u[r].x = fmad(w, nb, -hb[r].x);
+    } else {
+      const T nb = ((up.y + dn.y) + u[r].x) + dpp_shl1(u[r].x);
+      const T w = EDGE ? lc.wy(vc) : lc.iy2;
+      u[r].y = fmad(w, nb, -hb[r].y);
+    }
+  }
+}
+
+template <typename T, int RW, int S, int NS, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void blk_sweeps(typename Pair<T>::type* u,
+                                                        const typename Pair<T>::type* hb,
+                                                        const LaneCols<T>& lc, int gw0, int m,
+                                                        bool neu, typename Pair<T>::type* xh,
+                                                        int& hbuf, int w, int lane) {
+  using V2 = typename Pair<T>::type;
+  if constexpr (NS > 0) {
+    V2 above, below;
+    blk_halo(xh, hbuf, w, lane, u[0], u[RW - 1], above, below);
+    blk_halfsweep<T, RW, S, FLIP, EDGE>(u, hb, lc, gw0, m, neu, above, below);
+    blk_sweeps<T, RW, S + 1, NS - 1, FLIP, EDGE>(u, hb, lc, gw0, m, neu, xh, hbuf, w, lane);
+  }
+}
+
+// The body after the loads: hb holds the wave's rows of the right-hand side (zero
+// outside the grid).
+template <typename T, int NU, int CB, int FLIP, bool EDGE>
+__device__ __attribute__((always_inline)) void sdown_blk_body(const VTile& t, int w, int lane,
+                                                            typename Pair<T>::type* hb,
+                                                            unsigned char* lds) {
+  using V2 = typename Pair<T>::type;
+  using B = BlkGeo<T, NU, CB>;
+  constexpr int OFF = B::kOff, RW = B::kRW, R0 = B::kR0, R1 = B::kR1;
+  constexpr int CC = StripGeo<NU>::kDownCols;
+  const int m = t.m, n = t.n;
+  const int I0 = t.i0;
+  const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const int a0 = w * RW;       // the wave's first block row
+  const int gw0 = gr0 + a0;    // and its grid row
+  const bool neu = t.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(gc0, lane, n, neu);
+  const T h2 = T(t.h2);
+  V2* xh = reinterpret_cast<V2*>(lds);
+  V2* rbuf = reinterpret_cast<V2*>(lds + B::kHaloBytes);
+  int hbuf = 0;
+
+  // hb' = h^2 b / diag and the zero-guess half-sweep of colour 0, u = -hb' there
+  V2 u[RW];
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int gi = gw0 + r;
+    int vc = 2;
+    bool rin = true;
+    if (EDGE) {
+      rin = gi >= 0 && gi < m;
+      vc = row_count(gi, m, neu);
+    }
+    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+    hb[r].x = (rin && lc.inx) ? wx * h2 * hb[r].x : T(0);
+    hb[r].y = (rin && lc.iny) ? wy * h2 * hb[r].y : T(0);
+    if (NU > 0 && ((r + FLIP) & 1) == 0) {
+      u[r].x = -hb[r].x;
+      u[r].y = T(0);
+    } else if (NU > 0) {
+      u[r].x = T(0);
+      u[r].y = -hb[r].y;
+    } else {
+      u[r] = V2{T(0), T(0)};
+    }
+  }
+  if constexpr (NU > 0)
+    blk_sweeps<T, RW, 1, 2 * NU - 1, FLIP, EDGE>(u, hb, lc, gw0, m, neu, xh, hbuf, w, lane);
+
+  // pre-smoothed iterate of the owned fine points (the grid's last block row / strip
+  // column owns every remaining fine row / column)
+  {
+    const int fr0 = 2 * I0, fc0 = 2 * t.j0;
+    const int fr1 = (I0 + CB >= t.cm) ? m : min(m, fr0 + 2 * CB);
+    const int fc1 = (t.j0 + CC >= t.cn) ? n : min(n, fc0 + 2 * CC);
+    GDP_GLOBAL T* v = (GDP_GLOBAL T*)t.v;
+    const bool wx = lc.gjx >= fc0 && lc.gjx < fc1;
+    const bool wy = lc.gjx + 1 >= fc0 && lc.gjx + 1 < fc1;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int a = a0 + r;
+      const int gi = gw0 + r;
+      if (a < OFF || a >= OFF + 2 * CB + 2 || gi >= fr1) continue;
+      GDP_GLOBAL T* row = v + (size_t)gi * t.pitch + lc.gjx;
+      if (wx && wy) {
+        *(GDP_GLOBAL V2*)row = u[r];
+      } else if (wx) {
+        row[0] = u[r].x;
+      }
+    }
+  }
+
+  // scaled residual of the rows the restriction reads (R0 .. R1), with the Neumann
+  // column extension, to LDS
+  {
+    V2 above, below;
+    blk_halo(xh, hbuf, w, lane, u[0], u[RW - 1], above, below);
+    const bool colext = neu && (gc0 < 0 || gc0 + 128 > n);
+    const bool gl = lc.gjx + 1 == -1;
+    const bool gr = lc.gjx == n && (n & 1) == 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int a = a0 + r;
+      if (a < R0 || a > R1) continue;
+      int vc = 2;
+      bool rin = true;
+      if (EDGE) {
+        const int gi = gw0 + r;
+        rin = gi >= 0 && gi < m;
+        vc = row_count(gi, m, neu);
+      }
+      const V2 up = r > 0 ? u[r - 1] : above;
+      const V2 dn = r < RW - 1 ? u[r + 1] : below;
+      const T cx = T(vc) + lc.hx, cy = T(vc) + lc.hy;
+      const T nx = ((up.x + dn.x) + dpp_shr1(u[r].y)) + u[r].y;
+      const T ny = ((up.y + dn.y) + u[r].x) + dpp_shl1(u[r].x);
+      const T rx = fmad(cx, hb[r].x + u[r].x, -nx);
+      const T ry = fmad(cy, hb[r].y + u[r].y, -ny);
+      V2 res;
+      res.x = (rin && lc.inx) ? rx : T(0);
+      res.y = (rin && lc.iny) ? ry : T(0);
+      if (colext) {
+        const T sx = dpp_shl1(res.x);
+        const T sy = dpp_shr1(res.y);
+        if (gl) res.y = sx;
+        if (gr) res.x = sy;
+      }
+      rbuf[(a - R0) * 64 + lane] = res;
+    }
+  }
+  __syncthreads();
+
+  // restriction of the wave's coarse rows (CB / 4 of them) to the coarse right-hand side
+  GDP_GLOBAL T* cb = (GDP_GLOBAL T*)t.cb;
+  const T s = T(0.25) * T(t.inv_h2);
+  const int jc = lane - OFF / 2;
+  const int J = t.j0 + jc;
+  const bool wj = jc >= 0 && jc < CC && J < t.cn;
+  // residual row a (R0 .. R1) as the restriction reads it: Neumann rows past the grid
+  // take the edge row's value (r[-1] = r[0]; r[m] = r[m-1] for even m).  The rows a wave
+  // restricts from are not its own register rows, so this test does not follow EDGE.
+  auto res_row = [&](int a) {
+    if (neu) {
+      const int gi = gr0 + a;
+      if (gi == -1 && a < R1) a += 1;
+      else if (gi == m && (m & 1) == 0 && a > R0) a -= 1;
+    }
+    return rbuf[(a - R0) * 64 + lane];
+  };
+  constexpr int CW = CB / kBlkWaves;
+  static_assert(CB % kBlkWaves == 0, "coarse rows per block must split evenly over the waves");
+#pragma unroll
+  for (int k = 0; k < CW; ++k) {
+    const int i = w * CW + k;
+    const int I = I0 + i;
+    T c;
+    if (neu) {
+      c = restrict_neumann<T>(res_row(2 * i + OFF - 1), res_row(2 * i + OFF),
+                              res_row(2 * i + OFF + 1), res_row(2 * i + OFF + 2), s);
+    } else {
+      c = restrict_dirichlet<T>(res_row(2 * i + OFF), res_row(2 * i + OFF + 1),
+                                res_row(2 * i + OFF + 2), s);
+    }
+    if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
+  }
+}
+
+template <typename T, int NU, int CB>
+__global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict__ tiles,
+                                                        ProblemState* __restrict__ st,
+                                                        const double2* __restrict__ upart,
+                                                        int check_conv) {
+  using V2 = typename Pair<T>::type;
+  using B = BlkGeo<T, NU, CB>;
+  constexpr int OFF = B::kOff, RW = B::kRW;
+  __shared__ __align__(16) unsigned char lds[B::kLdsBytes];
+  const VTile t = tiles[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int gr0 = 2 * t.i0 - OFF, gc0 = 2 * t.j0 - OFF;
+  const int gw0 = gr0 + w * RW;
+  const int m = t.m, n = t.n;
+
+  // every load of the wave's rows is issued before anything waits on one: branch-free,
+  // from clamped addresses inside the padded rows (gjx is even), zeroed outside the grid
+  V2 hb[RW];
+  {
+    const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const int gjx = gc0 + 2 * lane;
+    const bool lin = gjx >= 0 && gjx < n;
+    const int jc = lin ? gjx : 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int gi = gw0 + r;
+      const bool in = lin && gi >= 0 && gi < m;
+      const V2 v = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jc);
+      hb[r] = in ? v : V2{T(0), T(0)};
+    }
+  }
+  // a compiler-only fence: it keeps the loads above from being sunk below the state load
+  // of the finished-problem test
+  asm volatile("" ::: "memory");
+  // every wave of the block takes the same exits (block-uniform tests), so the barriers
+  // below see all four waves
+  if (st[t.problem].done) return;
+  if (check_conv && t.level == 0) {
+    __shared__ double shc[2 * kThreads / 64];
+    __shared__ double2 bcc;
+    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
+                                  shc, &bcc))
+      return;
+  }
+  const bool edge = gw0 < 1 || gw0 + RW > m - 1;
+  const int flip = (gr0 + gc0) & 1;
+  if (edge) {
+    if (flip) sdown_blk_body<T, NU, CB, 1, true>(t, w, lane, hb, lds);
+    else sdown_blk_body<T, NU, CB, 0, true>(t, w, lane, hb, lds);
+  } else {
+    if (flip) sdown_blk_body<T, NU, CB, 1, false>(t, w, lane, hb, lds);
+    else sdown_blk_body<T, NU, CB, 0, false>(t, w, lane, hb, lds);
+  }
+}
+
+}  // namespace gdp

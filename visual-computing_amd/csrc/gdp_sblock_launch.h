@@ -1,0 +1,29 @@
+// Host dispatch of the block-tiled down kernel: (coarse rows per block, nu) -> template
+// instance.  Included by one translation unit per element type.
+#pragma once
+#include "gdp_internal.h"
+#include "gdp_sblock.h"
+
+namespace gdp {
+
+template <typename T, int CB>
+void down_blk_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWNB(NU) \
+  klaunch(k_sdown_blk<T, NU, CB>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv)
+  switch (nu) {
+    case 0: GDP_DOWNB(0); break;
+    case 1: GDP_DOWNB(1); break;
+    case 2: GDP_DOWNB(2); break;
+    default: GDP_DOWNB(3); break;
+  }
+#undef GDP_DOWNB
+}
+
+template <typename T>
+void launch_down_blk(int cb, int nu, int blocks, hipStream_t s, const DownArgs& a) {
+  if (cb == 8) down_blk_nu<T, 8>(nu, blocks, s, a);
+  else if (cb == 16) down_blk_nu<T, 16>(nu, blocks, s, a);
+  else down_blk_nu<T, 32>(nu, blocks, s, a);
+}
+
+}  // namespace gdp

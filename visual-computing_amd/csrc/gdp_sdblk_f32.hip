@@ -1,0 +1,4 @@
+// Block-tiled down kernels (k_sdown_blk, gdp_sblock.h), element type float.
+#include "gdp_sblock_launch.h"
+
+template void gdp::launch_down_blk<float>(int, int, int, hipStream_t, const gdp::DownArgs&);

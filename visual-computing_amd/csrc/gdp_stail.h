@@ -46,6 +46,16 @@ struct RowsAt {
   static constexpr int value = (RD0 >> D) > 0 ? (RD0 >> D) : 1;
 };
 
+// What the level code reads of a tail level's descriptor.  The sizes of every level are
+// loaded at kernel entry in one memory round trip (read level by level from global
+// memory, each level's first access would wait for a miss of its own under the
+// recursion); the spacing doubles from level to level, so h^2 and 1/h^2 of level D are
+// the first level's times 4^D and 4^-D, exactly.
+struct STailGeo {
+  int m, n;
+  double h2, inv_h2;
+};
+
 struct STailCtx {
   unsigned char* xb;   // level exchange rows (kXbRow elements of T per row)
   unsigned char* xh;   // halo slots [2][16 waves][2 rows][64 lanes] pairs
@@ -53,11 +63,26 @@ struct STailCtx {
   double* tabT;        // exact coarsest: Cn transposed
   double* w0;          // exact coarsest work arrays
   double* w1;
-  const LevelDesc* __restrict__ LV;  // the tail's levels in global memory (read through
-                                     // scalar loads: every field is wave-uniform)
+  int gmn[kSTailMaxLevels];  // level sizes m | n << 16 (scalar registers; sides <= 128)
+  double h2_0, inv_h2_0;                         // spacing of the first tail level
   int nlev, w, lane, pre, post, csweeps, order, hbuf;
   bool exact;
 };
+
+// What the final stores of the kernel use of the first tail level and of the Krylov
+// vectors, written to LDS at entry.
+struct STailEnd {
+  void* u;
+  double* kr;
+  double* kq;
+  int pitch;
+};
+
+template <int D>
+__device__ __attribute__((always_inline)) STailGeo st_geo(const STailCtx& c) {
+  constexpr double s = double(1ull << (2 * D));
+  return STailGeo{c.gmn[D] & 0xffff, c.gmn[D] >> 16, c.h2_0 * s, c.inv_h2_0 / s};
+}
 
 // Halo exchange: publish this wave's first and last rows, return the row above the
 // wave's first row and the row below its last (zero at the ends of the block).
@@ -170,7 +195,7 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
 // Coarsest level: u from the right-hand side b (plain) of this wave's rows; the
 // result is published to the exchange buffer.
 template <typename T, int RD, bool NEU>
-__device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const LevelDesc& L, typename Pair<T>::type* b) {
+__device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const STailGeo& L, typename Pair<T>::type* b) {
   using V2 = typename Pair<T>::type;
   using D2 = typename Pair<double>::type;
   const int m = L.m, n = L.n;
@@ -264,8 +289,8 @@ __device__ inline Quad<double> ld_quad(const double* p) {
 // (exchange columns 4l .. 4l+3) and the next lane's first two.  Every quad of the
 // wave's rows is loaded before any arithmetic, so the LDS latency is paid once.
 template <typename T, int RC, bool NEU>
-__device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const LevelDesc& F,
-                                                         const LevelDesc& C,
+__device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, const STailGeo& F,
+                                                         const STailGeo& C,
                                                          typename Pair<T>::type* cb) {
   constexpr int NT = NEU ? 4 : 3;  // vertical taps
   const T* R = reinterpret_cast<const T*>(c.xb);
@@ -337,8 +362,8 @@ __device__ __attribute__((always_inline)) void st_restrict(const STailCtx& c, co
 // for Dirichlet, edge-replicated for Neumann).  Fine lane l interpolates from coarse
 // columns l-1, l, l+1 (exchange columns l .. l+2).
 template <typename T, int RD, bool NEU>
-__device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const LevelDesc& F,
-                                                        const LevelDesc& C,
+__device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, const STailGeo& F,
+                                                        const STailGeo& C,
                                                         typename Pair<T>::type* u,
                                                         const LaneCols<T>& lc) {
   const T* E = reinterpret_cast<const T*>(c.xb) + c.lane;
@@ -439,12 +464,12 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
   using V2 = typename Pair<T>::type;
   constexpr int RD = RowsAt<STailRows<T>::kRows0, D>::value;
   constexpr int RC = RowsAt<STailRows<T>::kRows0, D + 1>::value;
-  const LevelDesc& L = c.LV[D];
+  const STailGeo L = st_geo<D>(c);
   if (D == c.nlev - 1) {
     st_coarsest<T, RD, NEU>(c, L, b);
     return;
   }
-  const LevelDesc& C = c.LV[D + 1];
+  const STailGeo C = st_geo<(D + 1 < kSTailMaxLevels ? D + 1 : D)>(c);
   const int m = L.m;
   LaneCols<T> lc;
   lc.init(0, c.lane, L.n, NEU);
@@ -558,8 +583,15 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   c.xb = smem;
   c.xh = smem + kSTailXbBytes;
   c.tab = reinterpret_cast<double*>(smem + kSTailXbBytes + kSTailXhBytes);
-  c.LV = levels + D.first;
+  const LevelDesc* __restrict__ LV = levels + D.first;
   c.nlev = D.nlev;
+#pragma unroll
+  for (int k = 0; k < kSTailMaxLevels; ++k) {
+    const LevelDesc& Lk = LV[min(k, c.nlev - 1)];
+    c.gmn[k] = Lk.m | (Lk.n << 16);
+  }
+  c.h2_0 = LV[0].h2;
+  c.inv_h2_0 = LV[0].inv_h2;
   c.w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   c.lane = threadIdx.x & 63;
   c.pre = pre;
@@ -569,8 +601,18 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   c.hbuf = 0;
   const double* __restrict__ dct = D.dct;
   c.exact = dct != nullptr;
-  const LevelDesc& L0 = c.LV[0];
-  const LevelDesc& Lc = c.LV[c.nlev - 1];
+  // the finest tail level's layout, and what the final stores use of it and of the
+  // Krylov vectors (the dots of a problem whose finest level is in the tail), read at
+  // entry and kept in LDS for the end
+  const LevelDesc& L0 = LV[0];
+  const int m0 = L0.m, n0 = L0.n;
+  const bool neu0 = L0.bc == kNeumann;
+  const bool dots = DOTS && D.krylov >= 0;
+  __shared__ STailEnd se;
+  if (threadIdx.x == 0)
+    se = STailEnd{L0.u, dots ? kds[D.krylov].r : nullptr, dots ? kds[D.krylov].q : nullptr,
+                  L0.pitch};
+  const LevelDesc& Lc = LV[c.nlev - 1];
   const int cm = Lc.m, cn = Lc.n;
   const int ntab = c.exact ? cm * cm + cn * cn + cm * cn : 0;
   c.tabT = c.tab + ntab;
@@ -585,8 +627,8 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
     // rows) before any is waited on, and zeroed outside the level
     const GDP_GLOBAL T* g = (const GDP_GLOBAL T*)L0.b;
     const int gj = 2 * c.lane;
-    const int m0 = L0.m, n0 = L0.n, pitch0 = L0.pitch;
     const int jc = gj < n0 ? gj : 0;
+    const int pitch0 = L0.pitch;
 #pragma unroll
     for (int r = 0; r < R0; ++r) {
       const int gi = c.w * R0 + r;
@@ -626,17 +668,17 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   }
   // (the first barrier inside the level recursion orders the table stores)
 
-  if (L0.bc == kNeumann) st_level<T, 0, true>(c, b);
+  if (neu0) st_level<T, 0, true>(c, b);
   else st_level<T, 0, false>(c, b);
 
   // the first tail level's iterate, from the exchange buffer, to global memory
-  GDP_GLOBAL T* g = (GDP_GLOBAL T*)L0.u;
-  const bool dots = DOTS && D.krylov >= 0;
-  const GDP_GLOBAL double* kr = dots ? (const GDP_GLOBAL double*)kds[D.krylov].r : nullptr;
-  const GDP_GLOBAL double* kq = dots ? (const GDP_GLOBAL double*)kds[D.krylov].q : nullptr;
+  const STailEnd e = se;
+  GDP_GLOBAL T* g = (GDP_GLOBAL T*)e.u;
+  const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)e.kr;
+  const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)e.kq;
+  const int pitch0 = e.pitch;
   double zr = 0.0, zq = 0.0;
   const int gj = 2 * c.lane;
-  const int m0 = L0.m, n0 = L0.n, pitch0 = L0.pitch;
   // the FP64 residual and q of the rows, loaded together (branch-free, clamped addresses
   // inside the padded rows) before any is used
   using D2 = typename Pair<double>::type;

@@ -66,7 +66,10 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
     case 1: down_nu<T, 2>(nu, blocks, s, a); break;
     case 2: down_nu<T, 4>(nu, blocks, s, a); break;
     case 3: down_nu<T, 8>(nu, blocks, s, a); break;
-    default: launch_down_stream<T>(menu == 4 ? 8 : (menu == 5 ? 16 : 32), nu, blocks, s, a); break;
+    case 4: case 5: case 6:
+      launch_down_stream<T>(menu == 4 ? 8 : (menu == 5 ? 16 : 32), nu, blocks, s, a);
+      break;
+    default: launch_down_blk<T>(menu == 7 ? 8 : (menu == 8 ? 16 : 32), nu, blocks, s, a); break;
   }
 }
 

@@ -27,6 +27,8 @@ SOURCES = [
     "gdp_sdstream_f64.hip",
     "gdp_sdstream32_f32.hip",
     "gdp_sdstream32_f64.hip",
+    "gdp_sdblk_f32.hip",
+    "gdp_sdblk_f64.hip",
     "gdp_stail_f32.hip",
     "gdp_stail_f64.hip",
     "gdp_sdown_f32.hip",

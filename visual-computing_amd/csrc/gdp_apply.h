@@ -1,6 +1,12 @@
 // The PCG direction update and operator apply of one iteration (solvers.py:149-151:
 // d = z + beta d, q = A d, <d, q>) in strip form: no LDS and no barrier.
 //
+// XUPD: the preceding k_update left its x += alpha d out (ProblemState.xpend) because
+// the strip-tail launch could not absorb it (a plan with about as many coarse tails as
+// CUs).  The apply reads that old direction anyway, so it adds alpha d to x of the points
+// it owns: 16 B per point instead of the 24 B of a separate pass.  A finished problem
+// still gets its pending update before the apply leaves it.
+//
 // A wave owns a strip of kApplyRows output rows and 128 loaded columns: lane l holds the
 // column pair (j0 - 1 + 2 l, j0 + 2 l) of each of its rows plus the rows above and below,
 // so vertical neighbours are in the lane and horizontal ones come from DPP row shifts.
@@ -17,7 +23,7 @@ constexpr int kApplyRows = 4;    // output rows per strip
 constexpr int kApplyCols = 126;  // output columns per strip
 constexpr int kApplyBlockRows = (kThreads / 64) * kApplyRows;  // one strip per wave
 
-template <typename T, bool MASK>
+template <typename T, bool MASK, bool XUPD = false>
 __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ tiles,
                                                       ProblemState* __restrict__ st,
                                                       const double2* __restrict__ zpart,
@@ -55,6 +61,14 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
     zv[a] = *(const GDP_GLOBAL TP*)(z + o);
     dv[a] = *(const GDP_GLOBAL D2*)(dold + o);
   }
+  // XUPD: x of the output rows, loaded with the rest
+  GDP_GLOBAL double* xv = (GDP_GLOBAL double*)t.x;
+  D2 xo[XUPD ? R : 1];
+  if constexpr (XUPD) {
+#pragma unroll
+    for (int a = 1; a <= R; ++a)
+      xo[a - 1] = *(const GDP_GLOBAL D2*)(xv + (size_t)min(r0 - 1 + a, m - 1) * pitch + jc);
+  }
   uint64_t mb = 0;
   if constexpr (MASK) mb = load_mask_bits<FH>(t.mask, r0 - 1, gj, m, n, true);
   auto unknown = [&](int a, int y) {
@@ -67,6 +81,28 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
   // load)
   const double2 z2 = reduce_slots<kThreads>(
       zpart, jacobi ? SlotRange{t.kbeg, t.kcnt} : SlotRange{t.zbeg, t.zcnt}, sh, &bc);
+  const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;
+  const bool outy = yin && gj + 1 < t.j0 + kApplyCols;
+  if constexpr (XUPD) {
+    // the pending x += alpha d of the previous iteration (solvers.py:151), d = dold
+    if (st[p].xpend) {
+      const double al = st[p].alpha;
+#pragma unroll
+      for (int a = 1; a <= R; ++a) {
+        const int gi = r0 - 1 + a;
+        if (gi >= m) break;
+        const size_t o = (size_t)gi * pitch + gj;
+        const double nx = __builtin_fma(al, dv[a].x, xo[a - 1].x);
+        const double ny = __builtin_fma(al, dv[a].y, xo[a - 1].y);
+        if (outx && outy) {
+          *(GDP_GLOBAL D2*)(xv + o) = D2{nx, ny};
+        } else {
+          if (outx) xv[o] = nx;
+          if (outy) xv[o + 1] = ny;
+        }
+      }
+    }
+  }
   if (st[p].done) return;
   double beta;
   if (st[p].iter == 0 || st[p].restart) beta = 0.0;
@@ -87,8 +123,6 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
                             : 0.0;
   }
   const bool neu = t.bc == kNeumann;
-  const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;
-  const bool outy = yin && gj + 1 < t.j0 + kApplyCols;
   double acc = 0.0;
 #pragma unroll
   for (int a = 1; a <= R; ++a) {

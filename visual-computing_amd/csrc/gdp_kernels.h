@@ -376,8 +376,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
 //   first V-cycle kernel: convergence test from the ||r||^2 partials
 // With defer_x the update leaves x += alpha d out (and says so in xpend) for the extra
 // blocks of the same iteration's tail launch (st_xupdate in gdp_stail.h), which run on
-// the CUs that the tail's one block per problem leaves idle; d stays valid until the
-// next iteration's apply writes the other direction buffer.
+// the CUs that the tail's one block per problem leaves idle, or, in a plan whose tails
+// leave too few CUs idle, for the next iteration's apply (k_apply_s<..., XUPD>), which
+// reads that direction anyway; d stays valid until the next iteration's apply writes
+// the other direction buffer.
 // Each kernel reads only state fields written by earlier kernels and its lead block
 // writes only fields that no block of the same kernel reads.
 
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
   for (int it = 0; it < NO; ++it) {
     if (!il[it]) continue;
     const double rn = rl[it] - alpha * ql[it];
-    if constexpr (!DEFER_X) x[ol[it]] = xl[it] + alpha * dl[it];
+    if constexpr (!DEFER_X) x[ol[it]] = __builtin_fma(alpha, dl[it], xl[it]);
     r[ol[it]] = rn;
     b0[ol[it]] = T(-rn);
     acc += rn * rn;

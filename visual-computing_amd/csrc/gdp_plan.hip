@@ -1224,27 +1224,46 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
 
 // One PCG iteration: direction and operator, update, preconditioner.  last: the
 // preconditioner is left out and only the update's convergence test runs (the host runs
-// the preconditioner before continuing if a problem is still open).
+// the preconditioner before continuing if a problem is still open).  next_apply: the next
+// kernel sequence on the stream starts with another iteration's apply (no true-residual
+// check in between).
 template <typename T>
-int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool last) {
+int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool last,
+                       bool next_apply) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
+  // Where the x half of this iteration's update goes: to the strip-tail launch of its
+  // V-cycle when the tails leave enough CUs idle (defer_x_update), else to the next
+  // iteration's apply when one follows before any true-residual check; k_update does it
+  // itself after the last update of a batch, for Jacobi and when no apply follows.
+  const bool tail_mode = defer_x_update(P);
+  const bool apply_mode = !jacobi && !tail_mode;  // this apply adds a pending x update
+  const bool tail_x = !jacobi && !last && tail_mode;
+  const bool apply_x = apply_mode && !last && next_apply;
   // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
   {
-    ProfScope ps(P, s, "apply", (24.0 + sizeof(T)) * N);
+    ProfScope ps(P, s, "apply", (24.0 + sizeof(T) + (apply_mode ? 16.0 : 0.0)) * N);
     const KTile* at = P->at<KTile>(P->kapply.off);
-    if (has_mask(P))
-      klaunch(k_apply_s<T, true>, dim3(P->kapply.count), dim3(kThreads), 0, s, at, P->d_state(), P->d_zpart(),
-                                                              P->d_apart(), din, jacobi ? 1 : 0);
-    else
-      klaunch(k_apply_s<T, false>, dim3(P->kapply.count), dim3(kThreads), 0, s, at, P->d_state(), P->d_zpart(),
-                                                               P->d_apart(), din, jacobi ? 1 : 0);
+    const int nt = P->kapply.count, jc = jacobi ? 1 : 0;
+    if (has_mask(P)) {
+      if (apply_mode)
+        klaunch(k_apply_s<T, true, true>, dim3(nt), dim3(kThreads), 0, s, at, P->d_state(),
+                P->d_zpart(), P->d_apart(), din, jc);
+      else
+        klaunch(k_apply_s<T, true, false>, dim3(nt), dim3(kThreads), 0, s, at, P->d_state(),
+                P->d_zpart(), P->d_apart(), din, jc);
+    } else {
+      if (apply_mode)
+        klaunch(k_apply_s<T, false, true>, dim3(nt), dim3(kThreads), 0, s, at, P->d_state(),
+                P->d_zpart(), P->d_apart(), din, jc);
+      else
+        klaunch(k_apply_s<T, false, false>, dim3(nt), dim3(kThreads), 0, s, at, P->d_state(),
+                P->d_zpart(), P->d_apart(), din, jc);
+    }
     GDP_CHECK_LAUNCH();
   }
-  // the x half of the update goes to the strip-tail launch of this iteration's V-cycle,
-  // when there is one (not after the last update of a batch, not for Jacobi)
-  const bool defer_x = !jacobi && !last && defer_x_update(P);
+  const bool defer_x = tail_x || apply_x;
   {
     ProfScope ps(P, s, "update", ((defer_x ? 24.0 : 48.0) + sizeof(T)) * N);
     if (defer_x) {
@@ -1265,17 +1284,22 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
     return GDP_OK;
   }
   if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
-  return enqueue_vcycle_t<T>(P, s, true, true, defer_x);
+  return enqueue_vcycle_t<T>(P, s, true, true, tail_x);
 }
 
-int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false) {
+// Two PCG iterations.  more: another pair follows in the same launch sequence, so the
+// second iteration's update may leave its x half to that pair's first apply; a pair
+// followed by the true-residual check leaves x complete.
+int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false,
+                     bool more = false) {
   int rc;
+  const bool more2 = more && !last;
   if (P->fp32) {
-    if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi, false))) return rc;
-    return enqueue_pcg_iter_t<float>(P, s, 1, jacobi, last);
+    if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi, false, true))) return rc;
+    return enqueue_pcg_iter_t<float>(P, s, 1, jacobi, last, more2);
   }
-  if ((rc = enqueue_pcg_iter_t<double>(P, s, 0, jacobi, false))) return rc;
-  return enqueue_pcg_iter_t<double>(P, s, 1, jacobi, last);
+  if ((rc = enqueue_pcg_iter_t<double>(P, s, 0, jacobi, false, true))) return rc;
+  return enqueue_pcg_iter_t<double>(P, s, 1, jacobi, last, more2);
 }
 
 struct ScalarArgs {
@@ -1508,7 +1532,7 @@ int launch_head(gdp_plan* P, hipStream_t s, bool jacobi, int pairs) {
     int rc;
     if ((rc = enqueue_start(P, cs, jacobi))) return rc;
     for (int k = 0; k < pairs; ++k)
-      if ((rc = enqueue_pcg_pair(P, cs, jacobi, k == pairs - 1))) return rc;
+      if ((rc = enqueue_pcg_pair(P, cs, jacobi, k == pairs - 1, k < pairs - 1))) return rc;
     return enqueue_check(P, cs, true);
   });
 }

@@ -546,9 +546,10 @@ __device__ __attribute__((always_inline)) void st_xupdate(const KTile* __restric
 #pragma unroll
     for (int u = 0; u < kXItems; ++u) {
       if (ok1[u]) {
-        *(GDP_GLOBAL D2*)(xp[u] + o[u]) = D2{xv[u].x + al[u] * dv[u].x, xv[u].y + al[u] * dv[u].y};
+        *(GDP_GLOBAL D2*)(xp[u] + o[u]) =
+            D2{__builtin_fma(al[u], dv[u].x, xv[u].x), __builtin_fma(al[u], dv[u].y, xv[u].y)};
       } else if (ok[u]) {
-        xp[u][o[u]] = xv[u].x + al[u] * dv[u].x;
+        xp[u][o[u]] = __builtin_fma(al[u], dv[u].x, xv[u].x);
       }
     }
   }

@@ -234,9 +234,9 @@ def composite_leg(steps: int) -> dict:
 
 
 def _family(base: str) -> str | None:
-    """The level-kernel family of a dispatch: the sweep-order and streamed down kernels are
-    one family (they run the same levels), ranked together by level area."""
-    if "k_sdown<" in base or "k_sdown_stream<" in base:
+    """The level-kernel family of a dispatch: the sweep-order, streamed and block-tiled down
+    kernels are one family (they run the same levels), ranked together by level area."""
+    if "k_sdown<" in base or "k_sdown_stream<" in base or "k_sdown_blk<" in base:
         return "k_sdown<"
     for key in ("k_sup<", "k_down<", "k_up<"):
         if key in base:
@@ -245,12 +245,14 @@ def _family(base: str) -> str | None:
 
 
 def _level_area(name: str, grid: int) -> int:
-    """Grid size x rows per strip (the third template argument of the strip kernels):
-    proportional to the level's area whatever tile shape the level was given."""
+    """Grid size x rows per strip (the third template argument of the strip kernels; the
+    block-tiled down kernel's is rows per block, four strips' worth): proportional to the
+    level's area whatever tile shape the level was given."""
     base = name.split("(")[0]
     if _family(base) in ("k_sdown<", "k_sup<"):
         args = base[base.index("<") + 1:].split(",")
-        return grid * int(args[2].strip(" >"))
+        rows = int(args[2].strip(" >"))
+        return grid * rows // 4 if "k_sdown_blk<" in base else grid * rows
     return grid
 
 
@@ -268,8 +270,11 @@ def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
         dots = kind == "up" and len(args) > 3 and args[3] == "true"
         lv = sorted(grids[fam], reverse=True).index(_level_area(name, grid))
         return f"{kind} L{lv}" + ("+dots" if dots else "")
+    if "k_update<" in base:
+        # k_update<T, TH, TW, defer_x>: the x half left to the tail launch or the next apply
+        return "update (x deferred)" if base.rstrip(" >").endswith("true") else "update"
     for key, kind in (("k_stail<", "tail"), ("k_tail<", "tail"),
-                      ("k_apply_s<", "apply"), ("k_update<", "update"), ("k_init<", "init"),
+                      ("k_apply_s<", "apply"), ("k_init<", "init"),
                       ("k_residual<", "residual"), ("k_finalize<", "finalize"),
                       ("k_init_stats<", "init_stats")):
         if key in base:
@@ -305,9 +310,9 @@ def reduce_traffic(fetch_csv: str, write_csv: str) -> dict:
             if fam is not None:
                 grids[fam].add(_level_area(r["Kernel_Name"], int(r["Grid_Size"])))
         # grouped by instantiation and launch shape: a kind with two of them (the tail
-        # with and without the deferred x update, k_update with and without its x half)
-        # averages over both, as its algorithmic bytes per launch do; the early-exit
-        # filter then compares each dispatch only with its own variant
+        # with and without the deferred x update) averages over both, as its algorithmic
+        # bytes per launch do; the early-exit filter then compares each dispatch only with
+        # its own variant
         vals = collections.defaultdict(lambda: collections.defaultdict(list))
         for r in rows:
             kind = kernel_kind(r["Kernel_Name"], int(r["Grid_Size"]), grids)
@@ -445,6 +450,13 @@ def batch_leg(steps: int, tune: bool = True) -> dict:
             "share": round(v["ms"] / total_ms, 3)}
         for k, v in sorted(prof.items(), key=lambda kv: -kv[1]["ms"])[:12]
     }
+    # the last update of a batch of iterations runs after planes that converged at fewer
+    # iterations have finished (they exit at once), so its algorithmic bytes overstate
+    # what it moves: no rate for it
+    if "update" in kernels:
+        kernels["update"].update(GBps=None, frac=None,
+                                 note="last update of a batch of iterations: planes that "
+                                      "finished earlier exit at once")
     # HBM bytes per launch measured by the FETCH_SIZE / WRITE_SIZE passes of this workload
     tfile = ROOT / "profiles" / "traffic_batch64_rgb_mixed.json"
     if tfile.exists():

@@ -55,14 +55,16 @@ def test_traffic_filters_early_exits_per_launch_shape(tmp_path):
     assert k["tail"]["fetch_bytes"] == round((5 * 100.0 + 1.0) / 6 * 1024 * 2)
     assert k["tail"]["write_bytes"] == round((5 * 50.0 + 0.5) / 6 * 1024)
     # the early-exited update is dropped
-    assert k["update"]["fetch_bytes"] == round(80.0 * 1024 * 2)
-    assert k["update"]["hbm_bytes_per_launch"] == round(80.0 * 1024 * 2 + 40.0 * 1024)
+    upd_k = k["update (x deferred)"]
+    assert upd_k["fetch_bytes"] == round(80.0 * 1024 * 2)
+    assert upd_k["hbm_bytes_per_launch"] == round(80.0 * 1024 * 2 + 40.0 * 1024)
 
 
 def test_traffic_keeps_the_two_update_variants_apart(tmp_path):
     """k_update with its x half deferred moves about half the bytes of the full update
-    on the same grid: each variant is filtered against its own maximum, so the deferred
-    launches are not taken for early exits, and the mean covers both."""
+    on the same grid: the two are kinds of their own (as the plan profiler names them),
+    each filtered against its own maximum, so the deferred launches are not taken for
+    early exits."""
     bench = _bench()
     copy_kib = bench.COPY_BYTES / 1024
     full = "void gdp::k_update<float, 16, 64, false>(...)"
@@ -79,8 +81,10 @@ def test_traffic_keeps_the_two_update_variants_apart(tmp_path):
     _write(fcsv, "FETCH_SIZE", fetch_rows)
     _write(wcsv, "WRITE_SIZE", write_rows)
     k = bench.reduce_traffic(str(fcsv), str(wcsv))["kernels"]
-    assert k["update"]["fetch_bytes"] == round((100.0 + 4 * 45.0) / 5 * 1024 * 2)
-    assert k["update"]["write_bytes"] == round((50.0 + 4 * 22.5) / 5 * 1024)
+    assert k["update"]["fetch_bytes"] == round(100.0 * 1024 * 2)
+    assert k["update"]["write_bytes"] == round(50.0 * 1024)
+    assert k["update (x deferred)"]["fetch_bytes"] == round(45.0 * 1024 * 2)
+    assert k["update (x deferred)"]["write_bytes"] == round(22.5 * 1024)
 
 
 def test_streamed_and_sweep_order_down_kernels_rank_as_one_family():
@@ -90,8 +94,9 @@ def test_streamed_and_sweep_order_down_kernels_rank_as_one_family():
     names = {
         ("void gdp::k_sdown_stream<float, 2, 32>(...)", 4096): None,  # area 4096 * 32
         ("void gdp::k_sdown<float, 2, 4, false>(...)", 8192): None,   # area 8192 * 4
+        ("void gdp::k_sdown_blk<float, 2, 8>(...)", 4096): None,      # area 4096 * 8 / 4
         ("void gdp::k_sdown<float, 2, 1, false>(...)", 4096): None,   # area 4096 * 1
     }
     grids = {"k_sdown<": {bench._level_area(n, g) for n, g in names}}
     kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
-    assert kinds == ["down L0", "down L1", "down L2"]
+    assert kinds == ["down L0", "down L1", "down L2", "down L3"]

@@ -415,7 +415,11 @@ def test_autotuned_plan_solves_the_same_problem(gd):
     plan = gd.Plan([shape] * 3, boundary="neumann", options=opts)
     choice = gd.autotune(plan, use_cache=False)
     assert set(k for k, _ in choice) <= {0, 1, 2}
-    assert all(0 <= v < (7 if k == 0 else 4 if k == 1 else 6) for (k, _), v in choice.items())
+    from gdpoisson.autotune import menu_size
+
+    sizes = {k: menu_size(k) for k in (0, 1, 2)}
+    assert sizes == {0: 10, 1: 4, 2: 6}
+    assert all(0 <= v < sizes[k] for (k, _), v in choice.items())
     got, rep = plan.mgcg(bs)
     for c in range(3):
         assert rep[c].converged and rep[c].relative_residual <= 1e-8

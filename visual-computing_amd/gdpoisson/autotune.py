@@ -84,7 +84,8 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
         # down kernel's convergence test reduces the update's ||r||^2 slots (one slot
         # per block of the producer)
         if kind == 2:
-            return sum(t for k, t in times.items() if k in ("apply", "update", "down L0"))
+            return sum(t for k, t in times.items()
+                       if k in ("apply", "update", "update (x deferred)", "down L0"))
         pre = "down" if kind == 0 else "up"
         t = sum(v for k, v in times.items() if re.fullmatch(rf"{pre} L{level}(\+dots)?", k))
         if kind == 1 and level == 0:

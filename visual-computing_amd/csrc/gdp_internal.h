@@ -80,11 +80,26 @@ struct TailArgs {
   const KTile* xt = nullptr;
   int nxt = 0, xth = 0, xtw = 0;
 };
+// Which strip-tail kernel a launch runs: one whose code is fixed to tail problems of
+// nlev levels with the given boundary (neu 1 / 0) and coarsest method (exact 1 / 0), when
+// every strip-tail problem of the plan is of that kind and kStailSpecMin <= nlev <=
+// kStailSpecMax, else the general kernel (nlev 0), which reads them per problem.
+struct STailSpec {
+  int nlev = 0, neu = -1, exact = -1;
+};
+constexpr int kStailSpecMin = 2, kStailSpecMax = 4;
 template <typename T>
-void launch_stail(bool dots, int blocks, int smem, hipStream_t s, const TailArgs& a);
-// Allow smem bytes of dynamic LDS for the strip-tail kernels of element type T.
+void launch_stail(const STailSpec& sp, bool dots, int blocks, int smem, hipStream_t s,
+                  const TailArgs& a);
+// Allow smem bytes of dynamic LDS for every strip-tail kernel of element type T.
 template <typename T>
 hipError_t stail_set_smem(int smem);
+// The fixed-kind kernels of one boundary (gdp_stail_spec_*.hip).
+template <typename T, int NEU>
+void launch_stail_spec(int nlev, int exact, bool dots, int blocks, int smem, hipStream_t s,
+                       const TailArgs& a);
+template <typename T, int NEU>
+hipError_t stail_spec_set_smem(int smem);
 // Dynamic LDS bytes of a strip tail whose coarsest level is cm x cn.
 int stail_lds_bytes(int cm, int cn, bool exact);
 // Largest first level a strip tail takes: rows / columns.

@@ -152,6 +152,7 @@ struct gdp_plan {
   int tail_smem = 0;   // dynamic LDS of the LDS tail (k_tail)
   int stail_smem = 0;  // dynamic LDS of the register-strip tail (k_stail)
   int n_stail = 0;     // strip-tail problems: the first n_stail entries of the tail table
+  STailSpec stail_spec;  // the strip-tail kernel fixed to its problems' kind, when they agree
   int n_cu = 0;        // compute units of the bound device (blocks for deferred x updates)
   int max_slots = 1;
   size_t ws_bytes = 0;
@@ -548,6 +549,25 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
     P->probs.push_back(pi);
   }
   P->total_levels = lvl_total;
+  {
+    // the strip-tail problems share a level count, boundary and coarsest method: their
+    // launches run the kernel fixed to that kind (same arithmetic, less code to fetch)
+    STailSpec sp;
+    bool first_st = true, same = true;
+    for (const ProbInfo& pi : P->probs) {
+      if (!pi.stail) continue;
+      const int nl = int(pi.lv.size()) - pi.tail_first;
+      const int ne = pi.p.boundary == GDP_NEUMANN ? 1 : 0, ex = pi.exact ? 1 : 0;
+      if (first_st) {
+        sp = STailSpec{nl, ne, ex};
+        first_st = false;
+      } else if (sp.nlev != nl || sp.neu != ne || sp.exact != ex) {
+        same = false;
+      }
+    }
+    if (!first_st && same && sp.nlev >= kStailSpecMin && sp.nlev <= kStailSpecMax)
+      P->stail_spec = sp;
+  }
 
   // ---- workspace layout
   size_t off = 0;
@@ -1174,7 +1194,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
         a.xth = ks.h;
         a.xtw = ks.w;
       }
-      launch_stail<T>(dots, ns, P->stail_smem, s, a);
+      launch_stail<T>(P->stail_spec, dots, ns, P->stail_smem, s, a);
       GDP_CHECK_LAUNCH();
     }
     if (no) {

@@ -193,14 +193,15 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
 }
 
 // Coarsest level: u from the right-hand side b (plain) of this wave's rows; the
-// result is published to the exchange buffer.
-template <typename T, int RD, bool NEU>
+// result is published to the exchange buffer.  EXACT: 1 the transform solve, 0 the FP64
+// sweeps, -1 whichever the problem has (c.exact).
+template <typename T, int RD, bool NEU, int EXACT>
 __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const STailGeo& L, typename Pair<T>::type* b) {
   using V2 = typename Pair<T>::type;
   using D2 = typename Pair<double>::type;
   const int m = L.m, n = L.n;
   V2 u[RD];
-  if (c.exact) {
+  if (EXACT == 1 || (EXACT < 0 && c.exact)) {
     const int gj = 2 * c.lane;
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
@@ -442,31 +443,36 @@ __device__ __attribute__((always_inline)) void st_prolong(const STailCtx& c, con
   __syncthreads();  // every wave has read the exchange buffer
 }
 
-template <typename T, int D, bool NEU>
+// The code of a tail kernel for problems with NLEV tail levels (0: any, the count read
+// per problem) and the EXACT coarsest method (-1: any).  A kernel fixed to one count and
+// method holds one copy of the coarsest solve and no level past it: about a sixth of the
+// instructions of the general kernel, which the tail's few CUs fetch cold.
+template <typename T, int D, bool NEU, int NLEV, int EXACT>
 __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b);
 
-template <typename T, int D, bool NEU>
+template <typename T, int D, bool NEU, int NLEV, int EXACT>
 struct STailLevel {
   static __device__ __attribute__((always_inline)) void run(STailCtx& c, typename Pair<T>::type* b) {
-    st_level<T, D, NEU>(c, b);
+    if constexpr (D < (NLEV > 0 ? NLEV : kSTailMaxLevels)) st_level<T, D, NEU, NLEV, EXACT>(c, b);
   }
-};
-template <typename T, bool NEU>
-struct STailLevel<T, kSTailMaxLevels, NEU> {
-  static __device__ void run(STailCtx&, typename Pair<T>::type*) {}
 };
 
 // Level D of the tail: b holds the plain right-hand side of this wave's rows.  On
 // return the level's iterate is published in the exchange buffer.  NEU: the problem's
 // boundary condition (Neumann), fixed for every level of the tail.
-template <typename T, int D, bool NEU>
+template <typename T, int D, bool NEU, int NLEV, int EXACT>
 __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pair<T>::type* b) {
   using V2 = typename Pair<T>::type;
   constexpr int RD = RowsAt<STailRows<T>::kRows0, D>::value;
   constexpr int RC = RowsAt<STailRows<T>::kRows0, D + 1>::value;
   const STailGeo L = st_geo<D>(c);
-  if (D == c.nlev - 1) {
-    st_coarsest<T, RD, NEU>(c, L, b);
+  if constexpr (NLEV > 0) {
+    if constexpr (D == NLEV - 1) {
+      st_coarsest<T, RD, NEU, EXACT>(c, L, b);
+      return;
+    }
+  } else if (D == c.nlev - 1) {
+    st_coarsest<T, RD, NEU, EXACT>(c, L, b);
     return;
   }
   const STailGeo C = st_geo<(D + 1 < kSTailMaxLevels ? D + 1 : D)>(c);
@@ -491,7 +497,7 @@ __device__ __attribute__((always_inline)) void st_level(STailCtx& c, typename Pa
   st_residual<T, RD, NEU>(c, u, hb, lc, m, L.n);
   V2 cb[RC];
   st_restrict<T, RC, NEU>(c, L, C, cb);
-  STailLevel<T, D + 1, NEU>::run(c, cb);
+  STailLevel<T, D + 1, NEU, NLEV, EXACT>::run(c, cb);
   {
     // recompute the lane's column data rather than keep it live across the recursion
     int lane = c.lane;
@@ -555,7 +561,9 @@ __device__ __attribute__((always_inline)) void st_xupdate(const KTile* __restric
   }
 }
 
-template <typename T, bool DOTS>
+// NLEV, NEU (1 Neumann, 0 Dirichlet, -1 either), EXACT: the kernel's code fixed to
+// tail problems of that kind (0 / -1: read per problem), see STailLevel.
+template <typename T, bool DOTS, int NLEV = 0, int NEU = -1, int EXACT = -1>
 __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restrict__ tails,
                                                         const LevelDesc* __restrict__ levels,
                                                         ProblemState* __restrict__ st,
@@ -586,8 +594,9 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   c.tab = reinterpret_cast<double*>(smem + kSTailXbBytes + kSTailXhBytes);
   const LevelDesc* __restrict__ LV = levels + D.first;
   c.nlev = D.nlev;
+  constexpr int kLev = NLEV > 0 ? NLEV : kSTailMaxLevels;  // levels the code has
 #pragma unroll
-  for (int k = 0; k < kSTailMaxLevels; ++k) {
+  for (int k = 0; k < kLev; ++k) {
     const LevelDesc& Lk = LV[min(k, c.nlev - 1)];
     c.gmn[k] = Lk.m | (Lk.n << 16);
   }
@@ -669,8 +678,12 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   }
   // (the first barrier inside the level recursion orders the table stores)
 
-  if (neu0) st_level<T, 0, true>(c, b);
-  else st_level<T, 0, false>(c, b);
+  if constexpr (NEU >= 0) {
+    st_level<T, 0, NEU == 1, NLEV, EXACT>(c, b);
+  } else {
+    if (neu0) st_level<T, 0, true, NLEV, EXACT>(c, b);
+    else st_level<T, 0, false, NLEV, EXACT>(c, b);
+  }
 
   // the first tail level's iterate, from the exchange buffer, to global memory
   const STailEnd e = se;

@@ -1,8 +1,18 @@
-// Register-strip coarse-tail kernels, element type float.
+// Register-strip coarse-tail kernels, element type float: the general kernel and the
+// dispatch of every tail launch.
 #include "gdp_stail_launch.h"
 
-template void gdp::launch_stail<float>(bool, int, int, hipStream_t, const gdp::TailArgs&);
-template hipError_t gdp::stail_set_smem<float>(int);
+namespace gdp {
+template <>
+void launch_stail<float>(const STailSpec& sp, bool dots, int blocks, int smem, hipStream_t s,
+                        const TailArgs& a) {
+  stail_launch<float>(sp, dots, blocks, smem, s, a);
+}
+template <>
+hipError_t stail_set_smem<float>(int smem) {
+  return stail_smem<float>(smem);
+}
+}  // namespace gdp
 
 int gdp::stail_lds_bytes(int cm, int cn, bool exact) {
   // Cm, Cn, 1/lambda, Cn transposed, and the two work arrays

@@ -1,0 +1,15 @@
+// Register-strip coarse-tail kernels fixed to Neumann tails of 2 to 4 levels (either
+// coarsest method), element type double.
+#include "gdp_stail_launch.h"
+
+namespace gdp {
+template <>
+void launch_stail_spec<double, 1>(int nlev, int exact, bool dots, int blocks, int smem,
+                                 hipStream_t s, const TailArgs& a) {
+  stail_spec_launch<double, 1>(nlev, exact, dots, blocks, smem, s, a);
+}
+template <>
+hipError_t stail_spec_set_smem<double, 1>(int smem) {
+  return stail_spec_smem<double, 1>(smem);
+}
+}  // namespace gdp

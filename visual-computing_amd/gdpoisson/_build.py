@@ -29,6 +29,10 @@ SOURCES = [
     "gdp_sdstream32_f64.hip",
     "gdp_sdblk_f32.hip",
     "gdp_sdblk_f64.hip",
+    "gdp_stail_n_f32.hip",
+    "gdp_stail_n_f64.hip",
+    "gdp_stail_d_f32.hip",
+    "gdp_stail_d_f64.hip",
     "gdp_stail_f32.hip",
     "gdp_stail_f64.hip",
     "gdp_sdown_f32.hip",

@@ -171,3 +171,15 @@ def test_batches_hold_one_image_shape():
         reqs, realtime=False)
     assert all(c.error is None for c in done)
     assert dev.batches == [[0, 2, 3], [1, 4]]
+
+
+def test_plan_capacity_buckets():
+    """Pooled plans come in capacities 1, 2, 3, 4, 6, 8, 12, 16, 24, ...: a batch's plane
+    count is rounded up to the next one, so at most a third of a plan's planes idle."""
+    from gdpoisson.serving import plan_capacity
+
+    assert [plan_capacity(n) for n in range(1, 14)] == [1, 2, 3, 4, 6, 6, 8, 8, 12, 12, 12,
+                                                        12, 16]
+    for n in range(1, 200):
+        c = plan_capacity(n)
+        assert c >= n and (c - n) * 3 <= c

@@ -18,10 +18,12 @@ rectangle route unless the request asks for ``domain="mask"``) or tone-mapping j
   that shape (one launch per kernel for all of them), then each request's back-end
   kernels; the only host waits are the worker's own (its stream and each plan's single
   end-of-solve read);
-* the plans stay resident: each worker keeps its bound plans (level pyramid, Krylov
-  vectors, tile tables and the hipGraphs of their PCG iterations) in a pool keyed by
-  the planes' shapes, within a byte cap, so a batch of shapes seen before reuses the
-  workspace and replays its captured graphs instead of building and capturing anew;
+* the plans stay resident: bound plans (level pyramid, Krylov vectors, tile tables and
+  the hipGraphs of their PCG iterations) wait between batches in a pool shared by the
+  workers, keyed by plane shape and a capacity of planes (the batch's plane count rounded
+  up to 1, 2, 3, 4, 6, 8, 12, ...; the planes a batch leaves free hold a zero right-hand
+  side and finish at set-up), within a byte cap, so a batch of a shape seen before reuses
+  a workspace and replays its captured graphs instead of building and capturing anew;
 * batches on different workers overlap on the device, and requests that arrive while
   others run are admitted as soon as a worker and budget free up.
 
@@ -96,6 +98,18 @@ def request_pixels(req: Request) -> int:
 _WS_CACHE: dict = {}
 
 
+def plan_capacity(planes: int) -> int:
+    """Planes of a pooled plan for a batch of ``planes``: the next of 1, 2, 3, 4, 6, 8,
+    12, 16, 24, ... (powers of two and three times powers of two), so batches of nearby
+    sizes share one resident plan and at most a third of its planes idle."""
+    c = 1
+    while True:
+        for v in (c, 3 * c // 2):
+            if v >= planes:
+                return max(v, 1)
+        c *= 2
+
+
 def _plan_bytes_per_point(boundary: str, precision: str) -> float:
     """Workspace bytes per unknown of a one-problem plan (measured on a 512 x 512 plan:
     the workspace is the level pyramid plus Krylov vectors, linear in the points)."""
@@ -133,13 +147,40 @@ def request_bytes(req: Request, precision: str = "mixed") -> int:
     return int(8 * h * w * (3 + 1 + 4 / 3 + 1 + 1 + 1) + per * h * w + (1 << 20))
 
 
+class _Pooled:
+    """A bound plan of ``capacity`` planes of one shape (key: its pool key, None for a
+    plan built for one batch).  ``solve`` runs MG-PCG on up to ``capacity`` planes: the
+    rest get a zero right-hand side, which the solver finishes at set-up (their outputs,
+    zeros, go to one shared scratch plane that nothing reads)."""
+
+    def __init__(self, key, plan, capacity: int):
+        self.key, self.plan, self.capacity = key, plan, capacity
+        self._pad_b = self._pad_u = None
+
+    def solve(self, planes: list):
+        import torch
+
+        n = len(planes)
+        if n > self.capacity:
+            raise ValueError(f"{n} planes for a plan of {self.capacity}")
+        if n == self.capacity:
+            return self.plan.mgcg(planes)
+        if self._pad_b is None:
+            self._pad_b = torch.zeros_like(planes[0])
+            self._pad_u = torch.empty_like(planes[0])
+        outs = [torch.empty_like(p) for p in planes] + [self._pad_u] * (self.capacity - n)
+        us, rs = self.plan.mgcg(list(planes) + [self._pad_b] * (self.capacity - n), outs)
+        return us[:n], rs[:n]
+
+
 class Server:
     """Continuous-batching request server on one GPU (see the module docstring).
 
     ``budget_bytes``: device bytes the server may hold (default: half of the free HBM);
     ``plan_pool_bytes`` of it (default an eighth) is kept for the resident plans, the rest
     is the admission budget of the requests.  ``slots``: workers, one HIP stream each.
-    ``max_batch``: problems (channels) per batched plan.  ``method``: "mgcg_mixed" or
+    ``max_batch``: problems (channels) per batched plan.  The resident plans of all
+    workers form one pool (a plan is used by one worker at a time).  ``method``: "mgcg_mixed" or
     "mgcg".  ``tonemap_method``: "dct" integrates each tone map exactly by the device DCT,
     as the reference's tone_map does (sides whose prime factors are not all 2, 3, 5 join
     the batched plan), "mg" puts every tone map in the batched MG-PCG plan.  ``graphs``:
@@ -195,14 +236,21 @@ class Server:
         self.plan_hits = 0     # batches served by a resident plan
         self.plan_builds = 0   # plans built (and, when pooled, kept)
         self.unpooled_builds = 0  # of which built for one batch (mask route, no pool)
-        self.pool_peak = 0     # largest resident-plan bytes of one worker
+        self.pool_peak = 0     # largest bytes of resident plans (in use or waiting)
+        # resident plans waiting for a batch: key -> [_Pooled, ...] (least recently
+        # returned first), and the bytes of every pooled plan, in use or waiting
+        self._pool: OrderedDict = OrderedDict()
+        self._pool_used = 0
         self._workers = None
 
     def close(self) -> None:
-        """Stop the workers; their streams and resident plans are released."""
+        """Stop the workers; their streams and the resident plans are released."""
         if self._workers is not None:
             self._workers.shutdown(wait=True)
             self._workers = None
+        with self._lock:
+            self._pool.clear()
+            self._pool_used = 0
 
     # ------------------------------------------------------------ scheduling
     def _bytes(self, req: Request) -> int:
@@ -353,10 +401,10 @@ class Server:
         return None
 
     def _plan(self, shapes: list, boundary: str, masks):
-        """A bound plan for these planes: from this worker's pool when it holds one for
-        the same shapes (its workspace and captured graphs are reused), else built, and
-        kept in the pool (least recently used plans are dropped past the worker's share of
-        ``plan_pool_bytes``).  Mask-route plans are built per batch."""
+        """A bound plan for these planes, as a _Pooled (its ``solve`` pads the planes to
+        the plan's capacity): a resident one of the same plane shape and capacity when one
+        is waiting in the pool (its workspace and captured graphs are reused), else built.
+        Hand it back with :meth:`_release`.  Mask-route plans are built per batch."""
         from .plan import Plan
 
         tiles = self._tiles_of(boundary, shapes)
@@ -364,31 +412,46 @@ class Server:
             with self._lock:
                 self.plan_builds += 1
                 self.unpooled_builds += 1
-            return Plan(shapes, boundary, 1.0, self._plan_options(boundary, False),
+            plan = Plan(shapes, boundary, 1.0, self._plan_options(boundary, False),
                         device=self.device, tiles=tiles, masks=masks)
-        pool = getattr(self._local, "plans", None)
-        if pool is None:
-            pool = self._local.plans = OrderedDict()
-            self._local.pool_used = 0
-        key = (boundary, tuple(shapes))
-        plan = pool.get(key)
-        if plan is not None:
-            pool.move_to_end(key)
-            with self._lock:
+            return _Pooled(None, plan, len(shapes))
+        shape = tuple(shapes[0])
+        if any(tuple(sh) != shape for sh in shapes):
+            raise ValueError("a pooled plan holds planes of one shape")
+        cap = plan_capacity(len(shapes))
+        key = (boundary, shape, cap)
+        with self._lock:
+            waiting = self._pool.get(key)
+            if waiting:
+                pooled = waiting.pop()
+                if not waiting:
+                    del self._pool[key]
                 self.plan_hits += 1
-            return plan
-        plan = Plan(shapes, boundary, 1.0, self._plan_options(boundary, self.graphs),
+                return pooled
+        plan = Plan([shape] * cap, boundary, 1.0, self._plan_options(boundary, self.graphs),
                     device=self.device, tiles=tiles)
-        pool[key] = plan
-        self._local.pool_used += plan.workspace_bytes
-        share = self.pool_bytes // self.slots
-        while self._local.pool_used > share and len(pool) > 1:
-            _, old = pool.popitem(last=False)  # its last solve has finished (synchronous)
-            self._local.pool_used -= old.workspace_bytes
         with self._lock:
             self.plan_builds += 1
-            self.pool_peak = max(self.pool_peak, self._local.pool_used)
-        return plan
+            self._pool_used += plan.workspace_bytes
+            self.pool_peak = max(self.pool_peak, self._pool_used)
+        return _Pooled(key, plan, cap)
+
+    def _release(self, pooled: "_Pooled") -> None:
+        """Return a pooled plan after its batch (its last solve has finished: the solve is
+        synchronous); plans waiting longest are dropped past ``plan_pool_bytes``."""
+        if pooled.key is None:
+            return
+        with self._lock:
+            self._pool.setdefault(pooled.key, []).insert(0, pooled)
+            self._pool.move_to_end(pooled.key)
+            while self._pool_used > self.pool_bytes and self._pool:
+                key, waiting = next(iter(self._pool.items()))
+                old = waiting.pop()  # the longest waiting of the least recent key
+                if not waiting:
+                    del self._pool[key]
+                self._pool_used -= old.plan.workspace_bytes
+                if old is pooled:
+                    break
 
     def tune(self, requests: Sequence[Request], solves: int = 2) -> dict:
         """Autotune the tile shapes of every plane shape these requests solve by MG-PCG (a
@@ -500,11 +563,13 @@ class Server:
                 gm = [masks[k] for k in idx]
                 go = [owner[k] for k in idx]
                 try:
-                    plan = self._plan([tuple(planes[k].shape) for k in idx],
-                                      BOUNDARY_OF[batch[0].kind],
-                                      gm if any(m is not None for m in gm) else None)
-                    us, rs = plan.mgcg([planes[k] for k in idx])
-                    del plan
+                    pooled = self._plan([tuple(planes[k].shape) for k in idx],
+                                        BOUNDARY_OF[batch[0].kind],
+                                        gm if any(m is not None for m in gm) else None)
+                    try:
+                        us, rs = pooled.solve([planes[k] for k in idx])
+                    finally:
+                        self._release(pooled)
                 except Exception as e:  # noqa: BLE001 -- the plan's requests fail alone
                     msg = f"{type(e).__name__}: {e}"
                     for i in set(go):

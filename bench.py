@@ -235,9 +235,12 @@ def composite_leg(steps: int) -> dict:
 
 def _family(base: str) -> str | None:
     """The level-kernel family of a dispatch: the sweep-order, streamed and block-tiled down
-    kernels are one family (they run the same levels), ranked together by level area."""
+    kernels are one family (they run the same levels), ranked together by level area, and
+    so are the strip and block-tiled up kernels."""
     if "k_sdown<" in base or "k_sdown_stream<" in base or "k_sdown_blk<" in base:
         return "k_sdown<"
+    if "k_sup_blk<" in base:
+        return "k_sup<"
     for key in ("k_sup<", "k_down<", "k_up<"):
         if key in base:
             return key
@@ -246,13 +249,13 @@ def _family(base: str) -> str | None:
 
 def _level_area(name: str, grid: int) -> int:
     """Grid size x rows per strip (the third template argument of the strip kernels; the
-    block-tiled down kernel's is rows per block, four strips' worth): proportional to the
+    block-tiled kernels' is rows per block, four strips' worth): proportional to the
     level's area whatever tile shape the level was given."""
     base = name.split("(")[0]
     if _family(base) in ("k_sdown<", "k_sup<"):
         args = base[base.index("<") + 1:].split(",")
         rows = int(args[2].strip(" >"))
-        return grid * rows // 4 if "k_sdown_blk<" in base else grid * rows
+        return grid * rows // 4 if "_blk<" in base else grid * rows
     return grid
 
 

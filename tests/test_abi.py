@@ -126,7 +126,7 @@ def test_python_errors_map_status_codes(L):
 
 def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
-    for kind, size in ((0, 10), (1, 4), (2, 6)):
+    for kind, size in ((0, 10), (1, 6), (2, 6)):
         for idx in range(size):
             for nu in range(4):
                 assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
@@ -150,7 +150,11 @@ def test_tile_menu(L):
         for nu in range(4):
             L.lib().gdp_tile_menu(0, idx, nu, ctypes.byref(th), ctypes.byref(tw))
             assert th.value == rows
-    for bad in ((3, 0, 1), (0, 10, 1), (1, 4, 1), (2, 6, 1), (0, 0, 4)):
+    # up entries 4, 5: block-tiled up kernels of 16 and 32 fine rows per block
+    for idx, rows in ((4, 16), (5, 32)):
+        L.lib().gdp_tile_menu(1, idx, 2, ctypes.byref(th), ctypes.byref(tw))
+        assert th.value == rows
+    for bad in ((3, 0, 1), (0, 10, 1), (1, 6, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
 
 

@@ -100,3 +100,17 @@ def test_streamed_and_sweep_order_down_kernels_rank_as_one_family():
     grids = {"k_sdown<": {bench._level_area(n, g) for n, g in names}}
     kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
     assert kinds == ["down L0", "down L1", "down L2", "down L3"]
+
+
+def test_strip_and_block_tiled_up_kernels_rank_as_one_family():
+    """The block-tiled up kernel's rows argument counts four strips: a level it runs ranks
+    among the strip kernel's levels by area, and its dots flag is read like theirs."""
+    bench = _bench()
+    names = [
+        ("void gdp::k_sup_blk<float, 2, 16, true>(...)", 16384),   # area 16384 * 16 / 4
+        ("void gdp::k_sup<float, 2, 4, false, false>(...)", 4096),  # area 4096 * 4
+        ("void gdp::k_sup_blk<float, 2, 32, false>(...)", 1024),   # area 1024 * 32 / 4
+    ]
+    grids = {"k_sup<": {bench._level_area(n, g) for n, g in names}}
+    kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
+    assert kinds == ["up L0+dots", "up L1", "up L2"]

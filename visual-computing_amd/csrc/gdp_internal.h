@@ -57,9 +57,12 @@ template <typename T>
 void launch_down_stream(int crs, int nu, int blocks, hipStream_t s, const DownArgs& a);
 template <typename T>
 void launch_down_stream32(int nu, int blocks, hipStream_t s, const DownArgs& a);
-// The block-tiled down kernel (k_sdown_blk), cb = 8, 16 or 32 coarse rows per block.
+// The block-tiled down kernel (k_sdown_blk), cb = 8, 16 or 32 coarse rows per block, and
+// up kernel (k_sup_blk), tb = 16 or 32 fine rows per block.
 template <typename T>
 void launch_down_blk(int cb, int nu, int blocks, hipStream_t s, const DownArgs& a);
+template <typename T>
+void launch_up_blk(int tb, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a);
 
 // The register-strip coarse tail (gdp_stail_*.hip): one 512-thread block per problem.
 constexpr int kStailBlockThreads = 512;  // threads of a strip-tail block

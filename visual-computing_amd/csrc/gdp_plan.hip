@@ -74,12 +74,13 @@ struct Shape2 {
 // streamed (wavefront-order) down kernel, k_sdown_stream, the last one compiled for
 // nu = 1 and 2 only; entries 7, 8 and 9 the block-tiled kernel, k_sdown_blk, whose block
 // owns 8, 16 or 32 coarse rows (four times the listed rows, like the others).  Kind 1 =
-// up: fine rows per strip (kUpCols fine columns wide).  Kind 2 = Krylov: fine points per
+// up: fine rows per strip (kUpCols fine columns wide); entries 4 and 5 the block-tiled
+// k_sup_blk, whose block owns 16 or 32 fine rows.  Kind 2 = Krylov: fine points per
 // block.
 const int kDownRows[10] = {1, 2, 4, 8, 8, 16, 32, 2, 4, 8};
-const int kUpRows[4] = {2, 4, 8, 16};
+const int kUpRows[6] = {2, 4, 8, 16, 4, 8};
 const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
-constexpr int kMenu = 4;       // up-kernel menu (rows per strip)
+constexpr int kMenu = 6;       // up-kernel menu (rows per strip)
 constexpr int kDownMenuN = 10;  // down-kernel menu
 constexpr int kKryMenuN = 6;   // Krylov tile menu
 int menu_size(int kind) { return kind == 2 ? kKryMenuN : (kind == 0 ? kDownMenuN : kMenu); }
@@ -294,6 +295,11 @@ bool has_mask(const gdp_plan* P) {
 // tallest one of that form.
 int down_menu_of(const gdp_plan* P, int l) {
   const int m = down_entry(P->down_menu[l], P->opt.pre_smoothing);
+  return (m >= 4 && has_mask(P)) ? 3 : m;
+}
+// The up-kernel menu entry of level l: likewise the masked up kernels are strips only.
+int up_menu_of(const gdp_plan* P, int l) {
+  const int m = P->up_menu[l];
   return (m >= 4 && has_mask(P)) ? 3 : m;
 }
 
@@ -937,7 +943,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   for (int l = 0; l < P->depth; ++l) {
     std::vector<VTile> vd, vu;
     const Shape2 sd = menu_shape(0, down_menu_of(P, l), P->opt.pre_smoothing);
-    const Shape2 su = menu_shape(1, P->up_menu[l], P->opt.post_smoothing);
+    const Shape2 su = menu_shape(1, up_menu_of(P, l), P->opt.post_smoothing);
     for (int p = 0; p < NP; ++p) {
       const ProbInfo& pi = P->probs[p];
       if (pi.tail_first <= l) continue;
@@ -1219,7 +1225,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
     ProfScope ps(P, s, nm, (3.25 * sizeof(T) + (dl ? 16.0 : 0.0)) * level_points(P, l));
     UpArgs a{P->at<VTile>(P->up[l].off), P->d_state(), P->d_zpart(), order,
              has_mask(P) ? 1 : 0};
-    launch_up<T>(P->up_menu[l], o.post_smoothing, dl, P->up[l].count, s, a);
+    launch_up<T>(up_menu_of(P, l), o.post_smoothing, dl, P->up[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
   return GDP_OK;

@@ -1,4 +1,4 @@
-// Block-tiled down kernel: the pre-smoothing + residual + restriction of sdown_body
+// Block-tiled level kernels.  Down: the pre-smoothing + residual + restriction of sdown_body
 // (gdp_strip.h) for one tall strip per block, its rows shared among the block's four
 // waves instead of recomputed by each.
 //
@@ -309,6 +309,187 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
     if (flip) sdown_blk_body<T, NU, CB, 1, false>(t, w, lane, hb, lds);
     else sdown_blk_body<T, NU, CB, 0, false>(t, w, lane, hb, lds);
   }
+}
+
+// ------------------------------------------------------------------ up
+// Block-tiled up kernel: the prolongation of the coarse correction + post-smoothing of
+// sup_body (gdp_strip.h) for a block owning TB fine rows x kUpCols fine columns, its
+// TB + 4 NU rows (rounded up to 4 RW, RW even) shared among the four waves like the down
+// kernel's: each wave interpolates its own rows from the coarse rows it reads, and the
+// 2 NU half-sweeps exchange the waves' edge rows through LDS.  The margins (2 NU rows
+// above and below the owned rows) are the only rows computed twice: 1.5x at TB = 16 and
+// NU = 2, against 3x for a strip of 4 rows.  Same expressions, same bits.  DOTS: the
+// <r, z>, <q, z> partials of the owned points (finest level of a PCG iteration).
+template <typename T, int NU, int TB>
+struct UpBlkGeo {
+  static constexpr int kH = StripGeo<NU>::kHalo;  // 2 NU
+  static constexpr int kFH = TB + 2 * kH;
+  static constexpr int kRW = 2 * ((kFH + 2 * kBlkWaves - 1) / (2 * kBlkWaves));
+  static constexpr int kNE = kRW / 2 + 2;  // coarse rows a wave interpolates from
+  static constexpr int kLdsBytes = 2 * kBlkWaves * 2 * 64 * int(sizeof(T)) * 2;
+};
+
+template <typename T, int NU, int TB, int FLIP, bool EDGE, bool DOTS>
+__device__ __attribute__((always_inline)) void sup_blk_body(const VTile& t, int w, int lane,
+                                                          typename Pair<T>::type* u,
+                                                          typename Pair<T>::type* hb,
+                                                          const T* E, double& zr, double& zq,
+                                                          unsigned char* lds) {
+  using V2 = typename Pair<T>::type;
+  using B = UpBlkGeo<T, NU, TB>;
+  using G = StripGeo<NU>;
+  constexpr int H = B::kH, RW = B::kRW, TC = G::kUpCols;
+  const int m = t.m, n = t.n;
+  const int gr0 = t.i0 - H, gc0 = t.j0 - G::kColHalo;
+  const int a0 = w * RW, gw0 = gr0 + a0;
+  const bool neu = t.bc == kNeumann;
+  LaneCols<T> lc;
+  lc.init(gc0, lane, n, neu);
+  const T h2 = T(t.h2);
+  V2* xh = reinterpret_cast<V2*>(lds);
+  int hbuf = 0;
+
+  // u = v + P e (zero outside the grid); hb' = h^2 b / diag
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int gi = gw0 + r;
+    int vc = 2;
+    bool rin = true;
+    if (EDGE) {
+      rin = gi >= 0 && gi < m;
+      vc = row_count(gi, m, neu);
+    }
+    T cx, cy;
+    if (neu) {
+      const int P = r / 2 + 1;
+      const int O = (r & 1) ? P + 1 : P - 1;
+      // explicit fused multiply-adds: the same bits in every kernel that interpolates
+      const T tv = fmad(T(0.25), E[O], T(0.75) * E[P]);
+      cx = fmad(T(0.25), dpp_shr1(tv), T(0.75) * tv);
+      cy = fmad(T(0.25), dpp_shl1(tv), T(0.75) * tv);
+    } else {
+      const T tv = (r & 1) ? E[(r + 1) / 2] : T(0.5) * (E[r / 2] + E[r / 2 + 1]);
+      cx = T(0.5) * (dpp_shr1(tv) + tv);
+      cy = tv;
+    }
+    u[r].x = (rin && lc.inx) ? u[r].x + cx : T(0);
+    u[r].y = (rin && lc.iny) ? u[r].y + cy : T(0);
+    const T wx = EDGE ? lc.wx(vc) : lc.ix2;
+    const T wy = EDGE ? lc.wy(vc) : lc.iy2;
+    hb[r].x = (rin && lc.inx) ? wx * h2 * hb[r].x : T(0);
+    hb[r].y = (rin && lc.iny) ? wy * h2 * hb[r].y : T(0);
+  }
+  blk_sweeps<T, RW, 0, 2 * NU, FLIP, EDGE>(u, hb, lc, gw0, m, neu, xh, hbuf, w, lane);
+
+  GDP_GLOBAL T* uo = (GDP_GLOBAL T*)t.u;
+  const int fc1 = min(n, t.j0 + TC);
+  const bool wx = lc.gjx >= t.j0 && lc.gjx < fc1;
+  const bool wy = lc.gjx + 1 >= t.j0 && lc.gjx + 1 < fc1;
+  const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
+  const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
+  // DOTS: the FP64 residual and q of the wave's rows, loaded together (branch-free,
+  // clamped addresses inside the padded rows) before any is used
+  using D2 = typename Pair<double>::type;
+  D2 rq[DOTS ? 2 * RW : 1];
+  if constexpr (DOTS) {
+    const int jc = (lc.gjx >= 0 && lc.gjx < n) ? lc.gjx : 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const size_t o = (size_t)min(max(gw0 + r, 0), m - 1) * t.pitch + jc;
+      rq[2 * r] = *(const GDP_GLOBAL D2*)(kr + o);
+      rq[2 * r + 1] = *(const GDP_GLOBAL D2*)(kq + o);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RW; ++r) {
+    const int a = a0 + r;
+    const int gi = gw0 + r;
+    if (a < H || a >= H + TB || gi >= m) continue;
+    const size_t o = (size_t)gi * t.pitch + lc.gjx;
+    if (wx && wy) {
+      *(GDP_GLOBAL V2*)(uo + o) = u[r];
+    } else if (wx) {
+      uo[o] = u[r].x;
+    }
+    if constexpr (DOTS) {
+      const D2 r2 = rq[2 * r], q2 = rq[2 * r + 1];
+      const double rx = wx ? r2.x : 0.0, ry = (wx && wy) ? r2.y : 0.0;
+      const double qx = wx ? q2.x : 0.0, qy = (wx && wy) ? q2.y : 0.0;
+      zr += double(u[r].x) * rx + double(u[r].y) * ry;
+      zq += double(u[r].x) * qx + double(u[r].y) * qy;
+    }
+  }
+}
+
+template <typename T, int NU, int TB, bool DOTS>
+__global__ __launch_bounds__(kThreads) void k_sup_blk(const VTile* __restrict__ tiles,
+                                                      const ProblemState* __restrict__ st,
+                                                      double2* __restrict__ zpart, int order) {
+  using V2 = typename Pair<T>::type;
+  using B = UpBlkGeo<T, NU, TB>;
+  using G = StripGeo<NU>;
+  constexpr int H = B::kH, RW = B::kRW, NE = B::kNE;
+  __shared__ double sh[2 * kThreads / 64];
+  __shared__ __align__(16) unsigned char lds[B::kLdsBytes];
+  const VTile t = tiles[blockIdx.x];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int gr0 = t.i0 - H, gc0 = t.j0 - G::kColHalo;
+  const int gw0 = gr0 + w * RW;
+  const int m = t.m, n = t.n;
+  const bool neu = t.bc == kNeumann;
+
+  // loads: v and b of the wave's rows, and the coarse correction rows they interpolate
+  // from (edge-replicated for Neumann, zero outside for Dirichlet), all branch-free from
+  // clamped addresses before any is waited on
+  V2 u[RW], hb[RW];
+  T E[NE];
+  {
+    const GDP_GLOBAL T* vv = (const GDP_GLOBAL T*)t.v;
+    const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const GDP_GLOBAL T* e = (const GDP_GLOBAL T*)t.cu;
+    const int gjx = gc0 + 2 * lane;
+    const bool lin = gjx >= 0 && gjx < n;
+    const int cr0 = gw0 / 2 - 1;
+    int K = gc0 / 2 + lane;
+    bool kin = K >= 0 && K < t.cn;
+    if (neu) {
+      K = min(max(K, 0), t.cn - 1);
+      kin = true;
+    }
+    const int Kc = min(max(K, 0), t.cn - 1);
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      const int I = cr0 + k;
+      const bool in = neu || (kin && I >= 0 && I < t.cm);
+      const T ev = e[(size_t)min(max(I, 0), t.cm - 1) * t.cpitch + Kc];
+      E[k] = in ? ev : T(0);
+    }
+    const int jc = lin ? gjx : 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int gi = gw0 + r;
+      const bool in = lin && gi >= 0 && gi < m;
+      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jc;
+      const V2 uv = *(const GDP_GLOBAL V2*)(vv + o);
+      const V2 bv = *(const GDP_GLOBAL V2*)(b + o);
+      u[r] = in ? uv : V2{T(0), T(0)};
+      hb[r] = in ? bv : V2{T(0), T(0)};
+    }
+  }
+  // every wave takes the same exit (a block-uniform test), so the barriers see all four
+  if (st[t.problem].done) return;
+  double zr = 0.0, zq = 0.0;
+  const bool edge = gw0 < 1 || gw0 + RW > m - 1;
+  const int flip = ((gr0 + gc0) + order) & 1;
+  if (edge) {
+    if (flip) sup_blk_body<T, NU, TB, 1, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+    else sup_blk_body<T, NU, TB, 0, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+  } else {
+    if (flip) sup_blk_body<T, NU, TB, 1, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+    else sup_blk_body<T, NU, TB, 0, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+  }
+  if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
 }
 
 }  // namespace gdp

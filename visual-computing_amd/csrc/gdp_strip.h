@@ -745,9 +745,10 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
     if (neu) {
       const int P = a / 2 + 1;
       const int O = (a & 1) ? P + 1 : P - 1;
-      const T tv = T(0.75) * E[P] + T(0.25) * E[O];
-      cx = T(0.75) * tv + T(0.25) * dpp_shr1(tv);
-      cy = T(0.75) * tv + T(0.25) * dpp_shl1(tv);
+      // explicit fused multiply-adds: the same bits in every kernel that interpolates
+      const T tv = fmad(T(0.25), E[O], T(0.75) * E[P]);
+      cx = fmad(T(0.25), dpp_shr1(tv), T(0.75) * tv);
+      cy = fmad(T(0.25), dpp_shl1(tv), T(0.75) * tv);
     } else {
       const T tv = (a & 1) ? E[(a + 1) / 2] : T(0.5) * (E[a / 2] + E[a / 2 + 1]);
       cx = T(0.5) * (dpp_shr1(tv) + tv);

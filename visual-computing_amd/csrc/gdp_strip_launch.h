@@ -92,7 +92,8 @@ void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
     case 0: up_nu<T, 2, DOTS>(nu, blocks, s, a); break;
     case 1: up_nu<T, 4, DOTS>(nu, blocks, s, a); break;
     case 2: up_nu<T, 8, DOTS>(nu, blocks, s, a); break;
-    default: up_nu<T, 16, DOTS>(nu, blocks, s, a); break;
+    case 3: up_nu<T, 16, DOTS>(nu, blocks, s, a); break;
+    default: launch_up_blk<T>(menu == 4 ? 16 : 32, nu, DOTS, blocks, s, a); break;
   }
 }
 

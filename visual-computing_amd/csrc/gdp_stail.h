@@ -61,6 +61,7 @@ struct STailCtx {
   unsigned char* xh;   // halo slots [2][16 waves][2 rows][64 lanes] pairs
   double* tab;         // exact coarsest: Cm, Cn, 1/lambda
   double* tabT;        // exact coarsest: Cn transposed
+  double* tabTm;       // exact coarsest: Cm transposed
   double* w0;          // exact coarsest work arrays
   double* w1;
   int gmn[kSTailMaxLevels];  // level sizes m | n << 16 (scalar registers; sides <= 128)
@@ -192,6 +193,72 @@ __device__ __attribute__((always_inline)) void st_publish(const STailCtx& c, con
   __syncthreads();
 }
 
+// tab[i] where ok, zero elsewhere (the address clamped to the table's first element).
+__device__ __attribute__((always_inline)) double lds_or_zero(const double* tab, int i, bool ok) {
+  const double v = tab[ok ? i : 0];
+  return ok ? v : 0.0;
+}
+
+// The exact coarsest solve of a level of at most 16 x 16 points by wave 0 on the FP64
+// matrix cores, published to the exchange buffer (the caller synchronises).  It computes
+// what lds_dct_solve computes, U = Cm^T ((Cm W Cn^T) ./ lambda) Cn with W = h^2 b in w0,
+// every matrix zero-padded to 16 x 16.  v_mfma_f64_16x16x4f64 takes A[i][k] from lane
+// 16 k + i and B[k][j] from lane 16 k + j, and leaves D[g + 4 v][j] in item v of lane
+// 16 g + j; so a result X serves the next product as the left operand X^T with k-block v
+// taken from item v, and each stage is X^T M for the stage's table M: W^T Cm^T =
+// (Cm W)^T, then (Cm W) Cn^T, then (Cm^T Y)^T for the scaled Y, then Cm^T Y Cn.
+template <typename T, bool NEU>
+__device__ __attribute__((always_inline)) void st_coarsest_mfma(const STailCtx& c, int m, int n) {
+  typedef double D4 __attribute__((ext_vector_type(4)));
+  const int g = c.lane >> 4, cc = c.lane & 15;
+  const double* Cm = c.tab;
+  const double* Cn = c.tab + m * m;
+  const double* il = Cn + n * n;
+  // every operand of the four stages, loaded before the first product
+  double a0[4], b1[4], b2[4], b3[4], b4[4], sc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int K = g + 4 * k;
+    a0[k] = lds_or_zero(c.w0, K * n + cc, K < m && cc < n);        // W[K][cc]
+    b1[k] = lds_or_zero(c.tabTm, K * m + cc, K < m && cc < m);     // Cm^T[K][cc]
+    b2[k] = lds_or_zero(c.tabT, K * n + cc, K < n && cc < n);      // Cn^T[K][cc]
+    b3[k] = lds_or_zero(Cm, K * m + cc, K < m && cc < m);          // Cm[K][cc]
+    b4[k] = lds_or_zero(Cn, K * n + cc, K < n && cc < n);          // Cn[K][cc]
+    sc[k] = lds_or_zero(il, K * n + cc, K < m && cc < n);          // 1/lambda[K][cc]
+  }
+  D4 x = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) x = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[k], b1[k], x, 0, 0, 0);
+  D4 y = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k], b2[k], y, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y[k] *= sc[k];
+  x = D4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) x = __builtin_amdgcn_mfma_f64_16x16x4f64(y[k], b3[k], x, 0, 0, 0);
+  y = D4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) y = __builtin_amdgcn_mfma_f64_16x16x4f64(x[k], b4[k], y, 0, 0, 0);
+  // publish rows g + 4 v (exchange column j + 1 holds column j, zero past the level)
+  // with the ghost columns -1 and n: the edge values for Neumann, zero for Dirichlet.
+  // Lane j = n (n < 16) writes ghost n from lane n - 1 of its group; lane 15 writes it
+  // when n = 16.
+  T* xb = reinterpret_cast<T*>(c.xb);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = g + 4 * v;
+    const T val = T(y[v]);
+    const T left = dpp_shr1(val);
+    if (row < m) {
+      T* xr = xb + row * kXbRow;
+      xr[cc + 1] = cc < n ? val : ((NEU && cc == n) ? left : T(0));
+      if (cc == 15 && n == 16) xr[17] = NEU ? val : T(0);
+      if (cc == 0) xr[0] = NEU ? val : T(0);
+    }
+  }
+}
+
 // Coarsest level: u from the right-hand side b (plain) of this wave's rows; the
 // result is published to the exchange buffer.  EXACT: 1 the transform solve, 0 the FP64
 // sweeps, -1 whichever the problem has (c.exact).
@@ -212,6 +279,11 @@ __device__ __attribute__((always_inline)) void st_coarsest(STailCtx& c, const ST
       }
     }
     __syncthreads();
+    if (m <= 16 && n <= 16) {
+      if (c.w == 0) st_coarsest_mfma<T, NEU>(c, m, n);
+      __syncthreads();
+      return;
+    }
     lds_dct_solve(c.w0, c.w1, m, n, c.tab, kSTailThreads, c.tabT);
 #pragma unroll
     for (int r = 0; r < RD; ++r) {
@@ -626,7 +698,8 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
   const int cm = Lc.m, cn = Lc.n;
   const int ntab = c.exact ? cm * cm + cn * cn + cm * cn : 0;
   c.tabT = c.tab + ntab;
-  c.w0 = c.tabT + (c.exact ? cn * cn : 0);
+  c.tabTm = c.tabT + (c.exact ? cn * cn : 0);
+  c.w0 = c.tabTm + (c.exact ? cm * cm : 0);
   c.w1 = c.w0 + cm * cn;
 
   // right-hand side of the first tail level and the first elements of the transform
@@ -660,11 +733,14 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
       check_converged<kSTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
     return;
   if (c.exact) {
-    // Cm, Cn, 1/lambda as they are, and Cn once more transposed
+    // Cm, Cn, 1/lambda as they are, and Cn and Cm once more transposed
     const int cn0 = cm * cm, cn1 = cm * cm + cn * cn;
     auto put = [&](int o, double v) {
       c.tab[o] = v;
-      if (o >= cn0 && o < cn1) {
+      if (o < cn0) {
+        const int row = o / cm;
+        c.tabTm[(o - row * cm) * cm + row] = v;
+      } else if (o < cn1) {
         const int e = o - cn0, row = e / cn;
         c.tabT[(e - row * cn) * cn + row] = v;
       }

@@ -15,8 +15,8 @@ hipError_t stail_set_smem<float>(int smem) {
 }  // namespace gdp
 
 int gdp::stail_lds_bytes(int cm, int cn, bool exact) {
-  // Cm, Cn, 1/lambda, Cn transposed, and the two work arrays
-  const int tables = exact ? (cm * cm + 2 * cn * cn + cm * cn) + 2 * cm * cn : 0;
+  // Cm, Cn, 1/lambda, Cn and Cm transposed, and the two work arrays
+  const int tables = exact ? (2 * cm * cm + 2 * cn * cn + cm * cn) + 2 * cm * cn : 0;
   return kSTailXbBytes + kSTailXhBytes + 8 * tables;
 }
 int gdp::stail_max_rows(bool fp32) {

@@ -126,7 +126,7 @@ def test_python_errors_map_status_codes(L):
 
 def test_tile_menu(L):
     th, tw = ctypes.c_int(), ctypes.c_int()
-    for kind, size in ((0, 10), (1, 6), (2, 6)):
+    for kind, size in ((0, 16), (1, 6), (2, 6)):
         for idx in range(size):
             for nu in range(4):
                 assert L.lib().gdp_tile_menu(kind, idx, nu, ctypes.byref(th),
@@ -154,8 +154,27 @@ def test_tile_menu(L):
     for idx, rows in ((4, 16), (5, 32)):
         L.lib().gdp_tile_menu(1, idx, 2, ctypes.byref(th), ctypes.byref(tw))
         assert th.value == rows
-    for bad in ((3, 0, 1), (0, 10, 1), (1, 6, 1), (2, 6, 1), (0, 0, 4)):
+    # entries 10 .. 15: the streamed strips of entries 4, 5, 6 and the blocks of entries
+    # 7, 8, 9 carrying the PCG update (the same tiles)
+    th2, tw2 = ctypes.c_int(), ctypes.c_int()
+    for idx in range(10, 16):
+        for nu in range(4):
+            L.lib().gdp_tile_menu(0, idx, nu, ctypes.byref(th), ctypes.byref(tw))
+            L.lib().gdp_tile_menu(0, idx - 6, nu, ctypes.byref(th2), ctypes.byref(tw2))
+            assert (th.value, tw.value) == (th2.value, tw2.value)
+    for bad in ((3, 0, 1), (0, 16, 1), (1, 6, 1), (2, 6, 1), (0, 0, 4)):
         assert L.lib().gdp_tile_menu(*bad, ctypes.byref(th), ctypes.byref(tw)) == L.GDP_EINVAL
+
+
+def test_fused_update_entries_are_finest_level_only(L):
+    """The down entries that carry the PCG update are refused on the coarser levels."""
+    rc, plan = make_plan(L, [(256, 256)], L.NEUMANN)
+    assert rc == 0
+    assert L.lib().gdp_plan_set_tile(plan, 0, 0, 11) == 0
+    assert L.lib().gdp_plan_set_tile(plan, 0, 1, 11) == L.GDP_EINVAL
+    assert b"finest" in L.lib().gdp_last_error()
+    assert L.lib().gdp_plan_set_tile(plan, 0, 1, 5) == 0
+    assert L.lib().gdp_plan_destroy(plan) == 0
 
 
 def test_planner_problem_flags(L):

@@ -114,3 +114,17 @@ def test_strip_and_block_tiled_up_kernels_rank_as_one_family():
     grids = {"k_sup<": {bench._level_area(n, g) for n, g in names}}
     kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
     assert kinds == ["up L0+dots", "up L1", "up L2"]
+
+
+def test_fused_update_down_kernel_is_the_finest_level_with_the_update():
+    """The streamed kernel carrying the PCG update (its fourth template argument) ranks
+    with the plain one of the same level and gets the plan profiler's kind name."""
+    bench = _bench()
+    names = [
+        ("void gdp::k_sdown_stream<float, 2, 16, true>(...)", 8192),   # area 8192 * 16
+        ("void gdp::k_sdown_stream<float, 2, 16, false>(...)", 8192),  # the same level
+        ("void gdp::k_sdown_blk<float, 2, 8>(...)", 4096),            # area 4096 * 8 / 4
+    ]
+    grids = {"k_sdown<": {bench._level_area(n, g) for n, g in names}}
+    kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
+    assert kinds == ["down L0+update", "down L0", "down L1"]

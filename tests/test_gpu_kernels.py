@@ -422,6 +422,71 @@ def test_streamed_down_kernel_in_batched_solves(gd, precision, down):
     assert torch.equal(alone[0], got[3])
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+@pytest.mark.parametrize("fused", [10, 11, 12, 13, 14, 15])
+@pytest.mark.parametrize("shapes", [[(1024, 1024)] * 3, [(300, 260)],
+                                    [(300, 260), (1030, 250), (257, 255)]])
+def test_fused_update_down_kernel_gives_the_same_solve(gd, shapes, fused, bc, precision):
+    """Down entries 10 .. 15 carry the PCG update in the finest down kernel (the update's
+    right-hand side formed from r and q, its convergence test in the second level's
+    kernel, r stored by the finest up kernel): the same iterations and the same solution
+    bits as the streamed strips and blocks of entries 4 .. 9 with k_update, alone and
+    batched; the profile shows the fused kernel ran."""
+    import torch
+
+    bs = []
+    for k, s in enumerate(shapes):
+        if bc == "neumann":
+            b, _ = G.neumann_problem(s, seed=4700 + k)
+        else:
+            b = np.random.default_rng(4800 + k).standard_normal(s)
+        bs.append(torch.from_numpy(b).cuda())
+    opts = gd.PlanOptions(precision=precision)
+    outs, reps = {}, {}
+    for menu in (fused - 6, fused):
+        plan = gd.Plan(shapes, boundary=bc, options=opts, tiles={(0, 0): menu})
+        outs[menu], reps[menu] = plan.mgcg(bs)
+        plan.profile(True)
+        plan.mgcg(bs)
+        kinds = set(plan.profile_report())
+        plan.profile(False)
+        assert ("down L0+update" in kinds) == (menu == fused), kinds
+        assert ("update (x deferred)" in kinds) == (menu != fused), kinds
+    for k in range(len(shapes)):
+        a, f = reps[fused - 6][k], reps[fused][k]
+        assert f.converged and f.relative_residual <= 1e-8
+        assert a.iterations == f.iterations and a.relative_residual == f.relative_residual
+        assert torch.equal(outs[fused - 6][k], outs[fused][k]), (k, shapes[k])
+    if len(shapes) > 1:
+        alone, _ = gd.Plan([shapes[1]], boundary=bc, options=opts,
+                           tiles={(0, 0): fused}).mgcg([bs[1]])
+        assert torch.equal(alone[0], outs[fused][1])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_update_entry_without_two_tiled_levels(gd, precision):
+    """A plan with a problem whose second level is already in the coarse tail keeps
+    k_update (the fused kernel needs the second level's kernel for the convergence test):
+    a fused entry then solves as the plain streamed strips."""
+    import torch
+
+    shapes = [(300, 260), (60, 131)]  # 30 x 66 is in the tail at both precisions
+    bs = [torch.from_numpy(G.neumann_problem(s, seed=4900 + k)[0]).cuda()
+          for k, s in enumerate(shapes)]
+    opts = gd.PlanOptions(precision=precision)
+    ref, _ = gd.Plan(shapes, boundary="neumann", options=opts, tiles={(0, 0): 5}).mgcg(bs)
+    plan = gd.Plan(shapes, boundary="neumann", options=opts, tiles={(0, 0): 11})
+    got, reps = plan.mgcg(bs)
+    plan.profile(True)
+    plan.mgcg(bs)
+    assert "down L0+update" not in plan.profile_report()
+    plan.profile(False)
+    assert all(r.converged for r in reps)
+    for k in range(len(shapes)):
+        assert torch.equal(ref[k], got[k])
+
+
 @pytest.mark.parametrize("kry", range(6))
 def test_every_krylov_tile_solves_like_the_oracle(gd, kry):
     """Each compiled Krylov tile shape (apply, update, set-up and check kernels) gives the
@@ -454,7 +519,7 @@ def test_autotuned_plan_solves_the_same_problem(gd):
     from gdpoisson.autotune import menu_size
 
     sizes = {k: menu_size(k) for k in (0, 1, 2)}
-    assert sizes == {0: 10, 1: 6, 2: 6}
+    assert sizes == {0: 16, 1: 6, 2: 6}
     assert all(0 <= v < sizes[k] for (k, _), v in choice.items())
     got, rep = plan.mgcg(bs)
     for c in range(3):

@@ -36,9 +36,15 @@ inline void klaunch(void (*k)(KArgs...), dim3 grid, dim3 block, unsigned smem, h
 struct DownArgs {
   const VTile* tiles;
   ProblemState* st;
-  const double2* upart;
+  // ||r||^2 partials: read by the convergence test, written by the fused update
+  double2* upart;
+  // the level whose kernel tests convergence, plus one (0: no test)
   int check_conv;
   int masked;  // finest level of a plan with masked domains (strip height menu 0)
+  // the fused update-down kernel: <d, q> partials (for alpha) and the direction buffer
+  // whose x half the update leaves to a later kernel
+  const double2* apart = nullptr;
+  int dcur = 0;
 };
 struct UpArgs {
   const VTile* tiles;
@@ -46,6 +52,8 @@ struct UpArgs {
   double2* zpart;
   int order;
   int masked;
+  // finest level after the fused update-down kernel: r -= alpha q of the owned points
+  int rupd = 0;
 };
 template <typename T>
 void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a);
@@ -57,10 +65,25 @@ template <typename T>
 void launch_down_stream(int crs, int nu, int blocks, hipStream_t s, const DownArgs& a);
 template <typename T>
 void launch_down_stream32(int nu, int blocks, hipStream_t s, const DownArgs& a);
+// The streamed down kernel of the finest level that also carries the PCG update
+// (k_sdown_stream with UPD): CRS = 8, 16 or 32 coarse rows (32 for nu = 1 and 2 only),
+// one translation unit each (gdp_sdfuse*.hip).
+template <typename T, int CRS>
+void launch_down_fused_crs(int nu, int blocks, hipStream_t s, const DownArgs& a);
+template <typename T>
+inline void launch_down_fused(int crs, int nu, int blocks, hipStream_t s, const DownArgs& a) {
+  if (crs == 8) launch_down_fused_crs<T, 8>(nu, blocks, s, a);
+  else if (crs == 16 || (nu != 1 && nu != 2)) launch_down_fused_crs<T, 16>(nu, blocks, s, a);
+  else launch_down_fused_crs<T, 32>(nu, blocks, s, a);
+}
 // The block-tiled down kernel (k_sdown_blk), cb = 8, 16 or 32 coarse rows per block, and
 // up kernel (k_sup_blk), tb = 16 or 32 fine rows per block.
 template <typename T>
 void launch_down_blk(int cb, int nu, int blocks, hipStream_t s, const DownArgs& a);
+// The block-tiled down kernel of the finest level carrying the PCG update (k_sdown_blk
+// with UPD), cb = 8, 16 or 32.
+template <typename T>
+void launch_down_blk_fused(int cb, int nu, int blocks, hipStream_t s, const DownArgs& a);
 template <typename T>
 void launch_up_blk(int tb, int nu, bool dots, int blocks, hipStream_t s, const UpArgs& a);
 

@@ -440,7 +440,8 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
 #pragma unroll
   for (int it = 0; it < NO; ++it) {
     if (!il[it]) continue;
-    const double rn = rl[it] - alpha * ql[it];
+    // one rounding, as the fused update-down and finest up kernels form it
+    const double rn = __builtin_fma(-alpha, ql[it], rl[it]);
     if constexpr (!DEFER_X) x[ol[it]] = __builtin_fma(alpha, dl[it], xl[it]);
     r[ol[it]] = rn;
     b0[ol[it]] = T(-rn);

@@ -76,17 +76,24 @@ struct Shape2 {
 // owns 8, 16 or 32 coarse rows (four times the listed rows, like the others).  Kind 1 =
 // up: fine rows per strip (kUpCols fine columns wide); entries 4 and 5 the block-tiled
 // k_sup_blk, whose block owns 16 or 32 fine rows.  Kind 2 = Krylov: fine points per
-// block.
-const int kDownRows[10] = {1, 2, 4, 8, 8, 16, 32, 2, 4, 8};
+// block.  Down entries 10 .. 15 (finest level only) also carry the PCG update of an
+// iteration, in place of k_update (the finest up kernel then stores r): 10, 11, 12 the
+// streamed strips of 8, 16 and 32 coarse rows (k_sdown_stream with UPD), 13, 14, 15 the
+// block-tiled kernel's blocks of 8, 16 and 32 coarse rows (k_sdown_blk with UPD).
+const int kDownRows[16] = {1, 2, 4, 8, 8, 16, 32, 2, 4, 8, 8, 16, 32, 2, 4, 8};
+constexpr int kFusedDown = 10;     // first fused update-down entry
+constexpr int kFusedDownBlk = 13;  // first block-tiled one
 const int kUpRows[6] = {2, 4, 8, 16, 4, 8};
 const Shape2 kKryMenu[6] = {{8, 64}, {16, 64}, {32, 64}, {8, 128}, {16, 128}, {32, 128}};
 constexpr int kMenu = 6;       // up-kernel menu (rows per strip)
-constexpr int kDownMenuN = 10;  // down-kernel menu
+constexpr int kDownMenuN = 16;  // down-kernel menu
 constexpr int kKryMenuN = 6;   // Krylov tile menu
 int menu_size(int kind) { return kind == 2 ? kKryMenuN : (kind == 0 ? kDownMenuN : kMenu); }
 // The 32-row streamed down entry exists for nu = 1 and 2; other sweep counts run the
 // 16-row one.
-int down_entry(int idx, int nu) { return (idx == 6 && nu != 1 && nu != 2) ? 5 : idx; }
+int down_entry(int idx, int nu) {
+  return ((idx == 6 || idx == 12) && nu != 1 && nu != 2) ? idx - 1 : idx;
+}
 constexpr int kStrips = kThreads / 64;  // strips per block
 int down_cols(int nu) { return (125 - 4 * nu) / 2; }
 int up_cols(int nu) { return 124 - 4 * nu; }
@@ -296,6 +303,15 @@ bool has_mask(const gdp_plan* P) {
 int down_menu_of(const gdp_plan* P, int l) {
   const int m = down_entry(P->down_menu[l], P->opt.pre_smoothing);
   return (m >= 4 && has_mask(P)) ? 3 : m;
+}
+// The finest level's down kernel carries the PCG update (fused update-down entries)
+// when its entry asks for it, no domain is masked and every problem has at least two
+// tiled levels (the update's convergence test runs in the second level's kernel).
+bool fused_update(const gdp_plan* P) {
+  if (P->depth < 2 || down_menu_of(P, 0) < kFusedDown) return false;
+  for (const ProbInfo& pi : P->probs)
+    if (pi.tail_first < 2) return false;
+  return true;
 }
 // The up-kernel menu entry of level l: likewise the masked up kernels are strips only.
 int up_menu_of(const gdp_plan* P, int l) {
@@ -615,19 +631,22 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   // tile tables (worst case over the menus)
   int slots = 1;
   size_t tiles = 0;
-  int kry_slots = 0, up0_slots = 0, apply_slots = 0;
+  int kry_slots = 0, up0_slots = 0, apply_slots = 0, down0_slots = 0;
   for (auto& pi : P->probs) {
     kry_slots += max_tiles(2, pi.lv[0].m, pi.lv[0].n, 0);
     apply_slots += apply_tiles(pi.lv[0].m, pi.lv[0].n);
-    if (pi.tail_first > 0)
+    if (pi.tail_first > 0) {
       up0_slots += max_tiles(1, pi.lv[0].m, pi.lv[0].n, o.post_smoothing);
+      // the fused update-down kernel's ||r||^2 partials: one slot per finest down tile
+      down0_slots += max_tiles(0, pi.lv[1].m, pi.lv[1].n, o.pre_smoothing);
+    }
     for (int l = 0; l < pi.tail_first; ++l) {
       tiles += max_tiles(0, pi.lv[l + 1].m, pi.lv[l + 1].n, o.pre_smoothing);
       tiles += max_tiles(1, pi.lv[l].m, pi.lv[l].n, o.post_smoothing);
     }
   }
   tiles += kry_slots;
-  slots = std::max({slots, kry_slots, apply_slots, up0_slots + P_});
+  slots = std::max({slots, kry_slots, apply_slots, up0_slots + P_, down0_slots});
   P->max_slots = slots;
   P->off_apart = take(sizeof(double2) * slots);
   P->off_upart = take(sizeof(double2) * slots);
@@ -689,6 +708,11 @@ int gdp_plan_set_tile(gdp_plan* P, int kind, int level, int menu_index) {
   if (!P || kind < 0 || kind > 2 || menu_index < 0 || menu_index >= menu_size(kind) ||
       (kind < 2 && (level < 0 || level >= kMaxDepth))) {
     set_error("gdp_plan_set_tile: bad argument");
+    return GDP_EINVAL;
+  }
+  if (kind == 0 && level > 0 && menu_index >= kFusedDown) {
+    set_error("gdp_plan_set_tile: down entries %d and up carry the PCG update: finest level only",
+              kFusedDown);
     return GDP_EINVAL;
   }
   if (kind == 0) P->down_menu[level] = menu_index;
@@ -876,7 +900,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     return GDP_OK;
   };
   // Krylov slot ranges (fixed by the Krylov tile shape)
-  std::vector<SlotRange> rk(NP), ru(NP), ra(NP);
+  std::vector<SlotRange> rk(NP), ru(NP), ra(NP), rd(NP);
   {
     const Shape2 s = kKryMenu[P->kry_menu];
     int c = 0, ca = 0;
@@ -920,8 +944,13 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     t.j0 = j0;
     t.slot = slot;
     t.lead = lead;
-    t.rbeg = rk[p].begin;
-    t.rcnt = rk[p].count;
+    // the convergence test of an update: level 0 after k_update (its Krylov-tile
+    // partials), level 1 after the fused update-down kernel (that kernel's tiles)
+    const SlotRange rt = l == 1 ? rd[p] : rk[p];
+    t.rbeg = rt.begin;
+    t.rcnt = rt.count;
+    t.abeg = ra[p].begin;
+    t.acnt = ra[p].count;
     if (pi.mask) t.mask = l == 0 ? pi.mask : reinterpret_cast<const uint8_t*>(P->ws + f.off_mask);
     return t;
   };
@@ -953,6 +982,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
       for (int i = 0; i < c.m; i += sd.h)
         for (int j = 0; j < c.n; j += sd.w)
           vd.push_back(vtile(p, l, i, j, int(vd.size()), vd.size() == d0));
+      if (l == 0) rd[p] = SlotRange{int(d0), int(vd.size() - d0)};
       if (l == 0) ru[p].begin = int(vu.size());
       for (int i = 0; i < f.m; i += su.h)
         for (int j = 0; j < f.n; j += su.w) vu.push_back(vtile(p, l, i, j, int(vu.size()), 0));
@@ -1158,19 +1188,36 @@ bool defer_x_update(const gdp_plan* P) {
 }
 
 // xupd: the previous k_update deferred its x half (defer_x); the strip-tail launch adds
-// it in blocks of its own.
+// it in blocks of its own.  fused (fused_update plans, PCG iterations that defer x): the
+// finest down kernel carries the iteration's update (in place of k_update, dcur its
+// direction buffer), the second level's kernel tests its convergence and the finest up
+// kernel stores its r.
 template <typename T>
 int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
-                     bool xupd = false) {
+                     bool xupd = false, bool fused = false, int dcur = 0) {
   const gdp_options& o = P->opt;
   for (int l = 0; l < P->depth; ++l) {
     if (!P->down[l].count) continue;
     char nm[32];
-    snprintf(nm, sizeof nm, "down L%d", l);
-    ProfScope ps(P, s, nm, (2.25 * sizeof(T)) * level_points(P, l));
-    DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), check_conv ? 1 : 0,
+    const bool fl = fused && l == 0;
+    snprintf(nm, sizeof nm, fl ? "down L%d+update" : "down L%d", l);
+    ProfScope ps(P, s, nm, ((fl ? 16.0 : 0.0) + 2.25 * sizeof(T)) * level_points(P, l));
+    const int cc = !check_conv ? 0 : (fused ? 2 : 1);
+    DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), cc,
                has_mask(P) ? 1 : 0};
-    launch_down<T>(down_menu_of(P, l), o.pre_smoothing, P->down[l].count, s, a);
+    if (fl) {
+      a.check_conv = 0;
+      a.apart = P->d_apart();
+      a.dcur = dcur;
+      const int e = down_menu_of(P, 0);
+      if (e >= kFusedDownBlk)
+        launch_down_blk_fused<T>(8 << (e - kFusedDownBlk), o.pre_smoothing, P->down[l].count, s,
+                                 a);
+      else
+        launch_down_fused<T>(8 << (e - kFusedDown), o.pre_smoothing, P->down[l].count, s, a);
+    } else {
+      launch_down<T>(down_menu_of(P, l), o.pre_smoothing, P->down[l].count, s, a);
+    }
     GDP_CHECK_LAUNCH();
   }
   // Neumann problems use the symmetric cycle (post-smoothing colour 1 then 0); a plan
@@ -1221,10 +1268,13 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
     if (!P->up[l].count) continue;
     char nm[32];
     const bool dl = dots && l == 0;
+    const bool ru = dl && fused;  // + the r store the fused update left (8 B a point)
     snprintf(nm, sizeof nm, dl ? "up L%d+dots" : "up L%d", l);
-    ProfScope ps(P, s, nm, (3.25 * sizeof(T) + (dl ? 16.0 : 0.0)) * level_points(P, l));
+    ProfScope ps(P, s, nm,
+                 (3.25 * sizeof(T) + (dl ? 16.0 : 0.0) + (ru ? 8.0 : 0.0)) * level_points(P, l));
     UpArgs a{P->at<VTile>(P->up[l].off), P->d_state(), P->d_zpart(), order,
              has_mask(P) ? 1 : 0};
+    a.rupd = ru ? 1 : 0;
     launch_up<T>(up_menu_of(P, l), o.post_smoothing, dl, P->up[l].count, s, a);
     GDP_CHECK_LAUNCH();
   }
@@ -1295,6 +1345,8 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
   }
   const bool defer_x = tail_x || apply_x;
   if (deferred) *deferred = apply_x;
+  // the update in the finest down kernel of the V-cycle that follows it
+  if (defer_x && fused_update(P)) return enqueue_vcycle_t<T>(P, s, true, true, tail_x, true, din ^ 1);
   {
     ProfScope ps(P, s, defer_x ? "update (x deferred)" : "update",
                  ((defer_x ? 24.0 : 48.0) + sizeof(T)) * N);

@@ -255,12 +255,21 @@ __device__ __attribute__((always_inline)) void sdown_blk_body(const VTile& t, in
   }
 }
 
-template <typename T, int NU, int CB>
+// UPD: the finest level of a PCG iteration whose update this kernel carries, as the
+// streamed kernel's UPD form does (gdp_strip.h): alpha from the <d, q> partials, the lead
+// block's state record of the update, the right-hand side -rn of the wave's rows from the
+// FP64 r and q rows (rn = r - alpha q, one rounding), b0 = -rn stored at the owned points
+// and their rn^2 summed into the block's slot of upart; r is stored by the finest up
+// kernel (the block's margin rows are other blocks' owned rows).
+template <typename T, int NU, int CB, bool UPD = false>
 __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict__ tiles,
                                                         ProblemState* __restrict__ st,
-                                                        const double2* __restrict__ upart,
-                                                        int check_conv) {
+                                                        double2* __restrict__ upart,
+                                                        int check_conv,
+                                                        const double2* __restrict__ apart,
+                                                        int dcur) {
   using V2 = typename Pair<T>::type;
+  using D2 = typename Pair<double>::type;
   using B = BlkGeo<T, NU, CB>;
   constexpr int OFF = B::kOff, RW = B::kRW;
   __shared__ __align__(16) unsigned char lds[B::kLdsBytes];
@@ -270,20 +279,30 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
   const int gr0 = 2 * t.i0 - OFF, gc0 = 2 * t.j0 - OFF;
   const int gw0 = gr0 + w * RW;
   const int m = t.m, n = t.n;
+  const int gjx = gc0 + 2 * lane;
+  const bool lin = gjx >= 0 && gjx < n;
+  const int jl = lin ? gjx : 0;
 
   // every load of the wave's rows is issued before anything waits on one: branch-free,
   // from clamped addresses inside the padded rows (gjx is even), zeroed outside the grid
   V2 hb[RW];
-  {
+  D2 rr[UPD ? RW : 1], qq[UPD ? RW : 1];
+  if constexpr (UPD) {
+    const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
+    const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const size_t o = (size_t)min(max(gw0 + r, 0), m - 1) * t.pitch + jl;
+      rr[r] = *(const GDP_GLOBAL D2*)(kr + o);
+      qq[r] = *(const GDP_GLOBAL D2*)(kq + o);
+    }
+  } else {
     const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
-    const int gjx = gc0 + 2 * lane;
-    const bool lin = gjx >= 0 && gjx < n;
-    const int jc = lin ? gjx : 0;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
       const int gi = gw0 + r;
       const bool in = lin && gi >= 0 && gi < m;
-      const V2 v = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jc);
+      const V2 v = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jl);
       hb[r] = in ? v : V2{T(0), T(0)};
     }
   }
@@ -292,13 +311,64 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
   asm volatile("" ::: "memory");
   // every wave of the block takes the same exits (block-uniform tests), so the barriers
   // below see all four waves
-  if (st[t.problem].done) return;
-  if (check_conv && t.level == 0) {
-    __shared__ double shc[2 * kThreads / 64];
-    __shared__ double2 bcc;
-    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
-                                  shc, &bcc))
+  __shared__ double shc[2 * kThreads / 64];
+  __shared__ double2 bcc;
+  double acc = 0.0;
+  if constexpr (UPD) {
+    // alpha as k_update forms it; the finished-problem test after the reduction's barrier
+    const int p = t.problem;
+    const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, shc, &bcc);
+    if (st[p].done) {
+      if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
       return;
+    }
+    const double delta = st[p].delta;
+    const double alpha = delta / a2.x;
+    if (t.lead && threadIdx.x == 0) {
+      st[p].alpha = alpha;
+      st[p].dq = a2.x;
+      st[p].delta_old = delta;
+      st[p].iter += 1;
+      st[p].restart = 0;
+      st[p].xpend = dcur + 1;  // the x half goes to the tail launch or the next apply
+    }
+    // the updated residual of the wave's rows: b0 = -rn and rn^2 at the owned points
+    // (rows and columns as the pre-smoothed iterate's), -rn into the strip rows
+    const int fr0 = 2 * t.i0, fc0 = 2 * t.j0;
+    constexpr int CC = StripGeo<NU>::kDownCols;
+    const int fr1 = (t.i0 + CB >= t.cm) ? m : min(m, fr0 + 2 * CB);
+    const int fc1 = (t.j0 + CC >= t.cn) ? n : min(n, fc0 + 2 * CC);
+    const bool vx = gjx >= fc0 && gjx < fc1;
+    const bool vy = gjx + 1 >= fc0 && gjx + 1 < fc1;
+    GDP_GLOBAL T* b0 = (GDP_GLOBAL T*)t.b;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int a = w * RW + r;
+      const int gi = gw0 + r;
+      D2 rn;
+      rn.x = __builtin_fma(-alpha, qq[r].x, rr[r].x);
+      rn.y = __builtin_fma(-alpha, qq[r].y, rr[r].y);
+      const V2 nb = V2{T(-rn.x), T(-rn.y)};
+      if (a >= OFF && a < OFF + 2 * CB + 2 && gi < fr1) {
+        GDP_GLOBAL T* row = b0 + (size_t)gi * t.pitch + gjx;
+        if (vx && vy) {
+          *(GDP_GLOBAL V2*)row = nb;
+          acc = __builtin_fma(rn.x, rn.x, acc);
+          acc = __builtin_fma(rn.y, rn.y, acc);
+        } else if (vx) {
+          row[0] = nb.x;
+          acc = __builtin_fma(rn.x, rn.x, acc);
+        }
+      }
+      hb[r] = (lin && gi >= 0 && gi < m) ? nb : V2{T(0), T(0)};
+    }
+  } else {
+    if (st[t.problem].done) return;
+    if (check_conv == t.level + 1) {
+      if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt},
+                                    t.lead != 0, shc, &bcc))
+        return;
+    }
   }
   const bool edge = gw0 < 1 || gw0 + RW > m - 1;
   const int flip = (gr0 + gc0) & 1;
@@ -309,6 +379,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
     if (flip) sdown_blk_body<T, NU, CB, 1, false>(t, w, lane, hb, lds);
     else sdown_blk_body<T, NU, CB, 0, false>(t, w, lane, hb, lds);
   }
+  if constexpr (UPD) write_partial<kThreads>(acc, 0.0, upart, t.slot, shc);
 }
 
 // ------------------------------------------------------------------ up
@@ -334,7 +405,8 @@ __device__ __attribute__((always_inline)) void sup_blk_body(const VTile& t, int 
                                                           typename Pair<T>::type* u,
                                                           typename Pair<T>::type* hb,
                                                           const T* E, double& zr, double& zq,
-                                                          unsigned char* lds) {
+                                                          unsigned char* lds, int rupd,
+                                                          double alpha) {
   using V2 = typename Pair<T>::type;
   using B = UpBlkGeo<T, NU, TB>;
   using G = StripGeo<NU>;
@@ -412,7 +484,19 @@ __device__ __attribute__((always_inline)) void sup_blk_body(const VTile& t, int 
       uo[o] = u[r].x;
     }
     if constexpr (DOTS) {
-      const D2 r2 = rq[2 * r], q2 = rq[2 * r + 1];
+      // rupd: the residual update left by the fused update-down kernel (as sup_body)
+      D2 r2 = rq[2 * r];
+      const D2 q2 = rq[2 * r + 1];
+      if (rupd) {
+        r2.x = __builtin_fma(-alpha, q2.x, r2.x);
+        r2.y = __builtin_fma(-alpha, q2.y, r2.y);
+        GDP_GLOBAL double* rw = (GDP_GLOBAL double*)t.kr;
+        if (wx && wy) {
+          *(GDP_GLOBAL D2*)(rw + o) = r2;
+        } else if (wx) {
+          rw[o] = r2.x;
+        }
+      }
       const double rx = wx ? r2.x : 0.0, ry = (wx && wy) ? r2.y : 0.0;
       const double qx = wx ? q2.x : 0.0, qy = (wx && wy) ? q2.y : 0.0;
       zr += double(u[r].x) * rx + double(u[r].y) * ry;
@@ -424,7 +508,8 @@ __device__ __attribute__((always_inline)) void sup_blk_body(const VTile& t, int 
 template <typename T, int NU, int TB, bool DOTS>
 __global__ __launch_bounds__(kThreads) void k_sup_blk(const VTile* __restrict__ tiles,
                                                       const ProblemState* __restrict__ st,
-                                                      double2* __restrict__ zpart, int order) {
+                                                      double2* __restrict__ zpart, int order,
+                                                      int rupd) {
   using V2 = typename Pair<T>::type;
   using B = UpBlkGeo<T, NU, TB>;
   using G = StripGeo<NU>;
@@ -479,15 +564,16 @@ __global__ __launch_bounds__(kThreads) void k_sup_blk(const VTile* __restrict__ 
   }
   // every wave takes the same exit (a block-uniform test), so the barriers see all four
   if (st[t.problem].done) return;
+  const double alpha = (DOTS && rupd) ? st[t.problem].alpha : 0.0;
   double zr = 0.0, zq = 0.0;
   const bool edge = gw0 < 1 || gw0 + RW > m - 1;
   const int flip = ((gr0 + gc0) + order) & 1;
   if (edge) {
-    if (flip) sup_blk_body<T, NU, TB, 1, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
-    else sup_blk_body<T, NU, TB, 0, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+    if (flip) sup_blk_body<T, NU, TB, 1, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds, rupd, alpha);
+    else sup_blk_body<T, NU, TB, 0, true, DOTS>(t, w, lane, u, hb, E, zr, zq, lds, rupd, alpha);
   } else {
-    if (flip) sup_blk_body<T, NU, TB, 1, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
-    else sup_blk_body<T, NU, TB, 0, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds);
+    if (flip) sup_blk_body<T, NU, TB, 1, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds, rupd, alpha);
+    else sup_blk_body<T, NU, TB, 0, false, DOTS>(t, w, lane, u, hb, E, zr, zq, lds, rupd, alpha);
   }
   if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);
 }

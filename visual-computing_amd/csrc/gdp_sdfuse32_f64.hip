@@ -1,0 +1,16 @@
+// Streamed down-strip kernels of the finest level carrying the PCG update (the fused
+// update-down kernel), 32 coarse rows per strip, element type double, for nu = 1 and 2 (the
+// planner runs other sweep counts with 16-row strips).
+#include "gdp_strip_launch.h"
+
+namespace gdp {
+template <>
+void launch_down_fused_crs<double, 32>(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+  if (nu == 1)
+    klaunch(k_sdown_stream<double, 1, 32, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st,
+            a.upart, a.check_conv, a.apart, a.dcur);
+  else
+    klaunch(k_sdown_stream<double, 2, 32, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st,
+            a.upart, a.check_conv, a.apart, a.dcur);
+}
+}  // namespace gdp

@@ -7,9 +7,9 @@ template <>
 void launch_down_stream32<float>(int nu, int blocks, hipStream_t s, const DownArgs& a) {
   if (nu == 1)
     klaunch(k_sdown_stream<float, 1, 32>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st,
-            a.upart, a.check_conv);
+            a.upart, a.check_conv, a.apart, a.dcur);
   else
     klaunch(k_sdown_stream<float, 2, 32>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st,
-            a.upart, a.check_conv);
+            a.upart, a.check_conv, a.apart, a.dcur);
 }
 }  // namespace gdp

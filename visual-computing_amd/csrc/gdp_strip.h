@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
   // of the finished-problem test (which would add its round trip before them)
   asm volatile("" ::: "memory");
   if (st[t.problem].done) return;
-  if (check_conv && t.level == 0) {
+  if (check_conv == t.level + 1) {
     // convergence test of the preceding PCG update (the loads above are in flight)
     __shared__ double shc[2 * kThreads / 64];
     __shared__ double2 bcc;
@@ -490,6 +490,8 @@ __global__ __launch_bounds__(kThreads) void k_sdown(const VTile* __restrict__ ti
 // at a time (plus the prefetched loads), so a strip can own CRS = 8 .. 32 coarse rows with
 // few registers: the rows recomputed at its ends are 4 NU + 3 per 2 CRS owned.
 constexpr int kPrefetch = 8;
+// rows in flight of the fused update-down kernel (two FP64 pairs a row)
+constexpr int kPrefetchUpd = 4;
 
 // f(integral_constant<int, I>) for I = B .. E - 1, expanded at compile time, so the row
 // index of every step is a constant and the row arrays stay in registers however many
@@ -502,12 +504,22 @@ __device__ __attribute__((always_inline)) void static_for(F&& f) {
   }
 }
 
-template <typename T, int NU, int CRS, int FLIP, bool EDGE>
-__device__ __attribute__((always_inline)) void sdown_stream_body(const VTile& t, int I0,
-                                                                 int lane,
-                                                                 typename Pair<T>::type* raw) {
+// UPD (finest level of a PCG iteration, the fused update): the right-hand side is the
+// updated residual's negative, formed row by row from the FP64 r and q rows (rr, qq,
+// prefetched like raw) as rn = r - alpha q, the same fused multiply-add k_update uses;
+// the kernel stores the FP32/FP64 right-hand side b0 = -rn of its owned points and adds
+// their rn^2 to acc.  r itself is left to the finest up kernel, which owns each point
+// alone (a strip's halo rows are other strips' owned rows, so a store here would race
+// with their loads).
+template <typename T, int NU, int CRS, int FLIP, bool EDGE, bool UPD = false>
+__device__ __attribute__((always_inline)) void sdown_stream_body(
+    const VTile& t, int I0, int lane, typename Pair<T>::type* raw,
+    typename Pair<double>::type* rr = nullptr, typename Pair<double>::type* qq = nullptr,
+    double alpha = 0.0, double* acc = nullptr) {
   using V2 = typename Pair<T>::type;
+  using D2 = typename Pair<double>::type;
   using G = StripGeo<NU>;
+  constexpr int PF = UPD ? kPrefetchUpd : kPrefetch;
   constexpr int OFF = G::kOff, CC = G::kDownCols;
   constexpr int FH = 2 * CRS + (NU > 0 ? 4 * NU + 3 : 4);
   constexpr int K = NU > 0 ? 2 * NU - 1 : 0;  // half-sweeps after the zero-guess one
@@ -539,15 +551,46 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(const VTile& t,
   const bool gl = lc.gjx + 1 == -1;
   const bool gr = lc.gjx == n && (n & 1) == 0;
 
+  const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
+  const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
+  GDP_GLOBAL T* b0 = (GDP_GLOBAL T*)t.b;
+
   V2 u[FH], hb[FH];
   static_for<0, FH>([&](auto step) __attribute__((always_inline)) {
     constexpr int st = decltype(step)::value;
-    // prefetch the right-hand side kPrefetch rows ahead (clamped address, zero outside)
-    if (st + kPrefetch < FH) {
-      const int a = st + kPrefetch;
+    // prefetch the right-hand side PF rows ahead (clamped address, zero outside)
+    if (st + PF < FH) {
+      const int a = st + PF;
       const int gi = gr0 + a;
-      const V2 q = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jload);
-      raw[a] = (lin && gi >= 0 && gi < m) ? q : V2{T(0), T(0)};
+      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jload;
+      if constexpr (UPD) {
+        rr[a] = *(const GDP_GLOBAL D2*)(kr + o);
+        qq[a] = *(const GDP_GLOBAL D2*)(kq + o);
+      } else {
+        const V2 q = *(const GDP_GLOBAL V2*)(b + o);
+        raw[a] = (lin && gi >= 0 && gi < m) ? q : V2{T(0), T(0)};
+      }
+    }
+    // UPD: the updated residual of row st; its owned points store b0 = -rn and count rn^2
+    if constexpr (UPD) {
+      const int a = st;
+      const int gi = gr0 + a;
+      D2 rn;
+      rn.x = __builtin_fma(-alpha, qq[a].x, rr[a].x);
+      rn.y = __builtin_fma(-alpha, qq[a].y, rr[a].y);
+      const V2 nb = V2{T(-rn.x), T(-rn.y)};
+      if (a >= OFF && a < OFF + 2 * CRS + 2 && gi < fr1) {
+        GDP_GLOBAL T* row = b0 + (size_t)gi * t.pitch + lc.gjx;
+        if (vx && vy) {
+          *(GDP_GLOBAL V2*)row = nb;
+          *acc = __builtin_fma(rn.x, rn.x, *acc);
+          *acc = __builtin_fma(rn.y, rn.y, *acc);
+        } else if (vx) {
+          row[0] = nb.x;
+          *acc = __builtin_fma(rn.x, rn.x, *acc);
+        }
+      }
+      raw[a] = (lin && gi >= 0 && gi < m) ? nb : V2{T(0), T(0)};
     }
     // row st: scaled right-hand side and the zero-guess half-sweep (colour 0)
     {
@@ -656,12 +699,20 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(const VTile& t,
   });
 }
 
-template <typename T, int NU, int CRS>
+// UPD: the finest level of a PCG iteration whose update this kernel carries (in place of
+// k_update<T, TH, TW, true>): alpha from the <d, q> partials (the same slots and order as
+// k_update, so the same bits), the lead block's state record of the update, the right-hand
+// side -rn row by row, and the rn^2 partial of each block at its slot in upart; the
+// convergence test of that update falls to the next level's kernel.
+template <typename T, int NU, int CRS, bool UPD = false>
 __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restrict__ tiles,
                                                            ProblemState* __restrict__ st,
-                                                           const double2* __restrict__ upart,
-                                                           int check_conv) {
+                                                           double2* __restrict__ upart,
+                                                           int check_conv,
+                                                           const double2* __restrict__ apart,
+                                                           int dcur) {
   using V2 = typename Pair<T>::type;
+  using D2 = typename Pair<double>::type;
   using G = StripGeo<NU>;
   constexpr int OFF = G::kOff;
   constexpr int FH = 2 * CRS + (NU > 0 ? 4 * NU + 3 : 4);
@@ -672,40 +723,85 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
   const bool work = I0 < t.cm;
   const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
   const int m = t.m, n = t.n;
-  // the first kPrefetch rows are requested before the state tests (branch-free, clamped
+  // the first rows are requested before the state tests (branch-free, clamped
   // addresses, zero outside the grid); the body requests the rest as it goes
   V2 raw[FH];
+  D2 rr[UPD ? FH : 1], qq[UPD ? FH : 1];
   {
     const GDP_GLOBAL T* b = (const GDP_GLOBAL T*)t.b;
+    const GDP_GLOBAL double* kr = (const GDP_GLOBAL double*)t.kr;
+    const GDP_GLOBAL double* kq = (const GDP_GLOBAL double*)t.kq;
     const int gjx = gc0 + 2 * lane;
     const bool lin = gjx >= 0 && gjx < n;
     const int jc = lin ? gjx : 0;
+    constexpr int PF = UPD ? kPrefetchUpd : kPrefetch;
 #pragma unroll
-    for (int a = 0; a < kPrefetch && a < FH; ++a) {
+    for (int a = 0; a < PF && a < FH; ++a) {
       const int gi = gr0 + a;
-      const bool in = work && lin && gi >= 0 && gi < m;
-      const V2 q = *(const GDP_GLOBAL V2*)(b + (size_t)min(max(gi, 0), m - 1) * t.pitch + jc);
-      raw[a] = in ? q : V2{T(0), T(0)};
+      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jc;
+      if constexpr (UPD) {
+        rr[a] = *(const GDP_GLOBAL D2*)(kr + o);
+        qq[a] = *(const GDP_GLOBAL D2*)(kq + o);
+      } else {
+        const bool in = work && lin && gi >= 0 && gi < m;
+        const V2 q = *(const GDP_GLOBAL V2*)(b + o);
+        raw[a] = in ? q : V2{T(0), T(0)};
+      }
     }
   }
   asm volatile("" ::: "memory");
-  if (st[t.problem].done) return;
-  if (check_conv && t.level == 0) {
-    __shared__ double shc[2 * kThreads / 64];
-    __shared__ double2 bcc;
-    if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt}, t.lead != 0,
-                                  shc, &bcc))
-      return;
-  }
-  if (!work) return;
   const bool edge = gr0 < 1 || gr0 + FH > m - 1;
   const int flip = (gr0 + gc0) & 1;
-  if (edge) {
-    if (flip) sdown_stream_body<T, NU, CRS, 1, true>(t, I0, lane, raw);
-    else sdown_stream_body<T, NU, CRS, 0, true>(t, I0, lane, raw);
+  if constexpr (UPD) {
+    // alpha as k_update forms it; the finished-problem test after the reduction's barrier
+    __shared__ double sh[2 * kThreads / 64];
+    __shared__ double2 bc;
+    const int p = t.problem;
+    const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, sh, &bc);
+    if (st[p].done) {
+      if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
+      return;
+    }
+    const double delta = st[p].delta;
+    const double alpha = delta / a2.x;
+    if (t.lead && threadIdx.x == 0) {
+      st[p].alpha = alpha;
+      st[p].dq = a2.x;
+      st[p].delta_old = delta;
+      st[p].iter += 1;
+      st[p].restart = 0;
+      st[p].xpend = dcur + 1;  // the x half goes to the tail launch or the next apply
+    }
+    double acc = 0.0;
+    if (work) {
+      if (edge) {
+        if (flip) sdown_stream_body<T, NU, CRS, 1, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+        else sdown_stream_body<T, NU, CRS, 0, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+      } else {
+        if (flip) sdown_stream_body<T, NU, CRS, 1, false, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+        else sdown_stream_body<T, NU, CRS, 0, false, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+      }
+    }
+    // every wave joins the block sum (the waves past the grid with zero)
+    write_partial<kThreads>(acc, 0.0, upart, t.slot, sh);
+    return;
   } else {
-    if (flip) sdown_stream_body<T, NU, CRS, 1, false>(t, I0, lane, raw);
-    else sdown_stream_body<T, NU, CRS, 0, false>(t, I0, lane, raw);
+    if (st[t.problem].done) return;
+    if (check_conv == t.level + 1) {
+      __shared__ double shc[2 * kThreads / 64];
+      __shared__ double2 bcc;
+      if (check_converged<kThreads>(st, t.problem, upart, SlotRange{t.rbeg, t.rcnt},
+                                    t.lead != 0, shc, &bcc))
+        return;
+    }
+    if (!work) return;
+    if (edge) {
+      if (flip) sdown_stream_body<T, NU, CRS, 1, true>(t, I0, lane, raw);
+      else sdown_stream_body<T, NU, CRS, 0, true>(t, I0, lane, raw);
+    } else {
+      if (flip) sdown_stream_body<T, NU, CRS, 1, false>(t, I0, lane, raw);
+      else sdown_stream_body<T, NU, CRS, 0, false>(t, I0, lane, raw);
+    }
   }
 }
 
@@ -714,10 +810,14 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
 // kUpCols fine columns starting at (i0, j0) and loads a margin of 2 NU rows and
 // 2 NU + 2 columns around them.  With DOTS (finest level of a PCG iteration) the block also sums
 // <r, z> and <q, z> over its points.
+//
+// rupd (DOTS only): the iteration's update ran in the fused update-down kernel, which left
+// r to this kernel: each owned point's residual becomes r - alpha q (k_update's fused
+// multiply-add) before the dot products use it, and is stored.
 template <typename T, int NU, int TR, int FLIP, bool EDGE, bool DOTS, bool MASK = false>
 __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, int lane, typename Pair<T>::type* u,
                          typename Pair<T>::type* hb, const T* E, double& zr, double& zq,
-                         uint64_t mb = 0) {
+                         uint64_t mb = 0, int rupd = 0, double alpha = 0.0) {
   using V2 = typename Pair<T>::type;
   using G = StripGeo<NU>;
   constexpr int H = G::kHalo, TC = G::kUpCols;
@@ -801,7 +901,18 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
       uo[o] = u[a].x;
     }
     if constexpr (DOTS) {
-      const D2 r2 = rq[2 * (a - H)], q2 = rq[2 * (a - H) + 1];
+      D2 r2 = rq[2 * (a - H)];
+      const D2 q2 = rq[2 * (a - H) + 1];
+      if (rupd) {
+        r2.x = __builtin_fma(-alpha, q2.x, r2.x);
+        r2.y = __builtin_fma(-alpha, q2.y, r2.y);
+        GDP_GLOBAL double* rw = (GDP_GLOBAL double*)t.kr;
+        if (wx && wy) {
+          *(GDP_GLOBAL D2*)(rw + o) = r2;
+        } else if (wx) {
+          rw[o] = r2.x;
+        }
+      }
       const double rx = wx ? r2.x : 0.0, ry = (wx && wy) ? r2.y : 0.0;
       const double qx = wx ? q2.x : 0.0, qy = (wx && wy) ? q2.y : 0.0;
       zr += double(u[a].x) * rx + double(u[a].y) * ry;
@@ -813,7 +924,8 @@ __device__ __attribute__((always_inline)) void sup_body(const VTile& t, int i0, 
 template <typename T, int NU, int TR, bool DOTS, bool MASK = false>
 __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tiles,
                                                   const ProblemState* __restrict__ st,
-                                                  double2* __restrict__ zpart, int order) {
+                                                  double2* __restrict__ zpart, int order,
+                                                  int rupd) {
   using V2 = typename Pair<T>::type;
   using G = StripGeo<NU>;
   constexpr int H = G::kHalo;
@@ -872,6 +984,7 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
     }
   }
   if (st[t.problem].done) return;
+  const double alpha = (DOTS && rupd) ? st[t.problem].alpha : 0.0;
   double zr = 0.0, zq = 0.0;
   if (work) {
     const bool edge = gr0 < 1 || gr0 + FH > m - 1;
@@ -883,11 +996,11 @@ __global__ __launch_bounds__(kThreads) void k_sup(const VTile* __restrict__ tile
       if (flip) sup_body<T, NU, TR, 1, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
       else sup_body<T, NU, TR, 0, true, DOTS, true>(t, i0, lane, u, hb, E, zr, zq, mb);
     } else if (edge) {
-      if (flip) sup_body<T, NU, TR, 1, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
-      else sup_body<T, NU, TR, 0, true, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+      if (flip) sup_body<T, NU, TR, 1, true, DOTS>(t, i0, lane, u, hb, E, zr, zq, 0, rupd, alpha);
+      else sup_body<T, NU, TR, 0, true, DOTS>(t, i0, lane, u, hb, E, zr, zq, 0, rupd, alpha);
     } else {
-      if (flip) sup_body<T, NU, TR, 1, false, DOTS>(t, i0, lane, u, hb, E, zr, zq);
-      else sup_body<T, NU, TR, 0, false, DOTS>(t, i0, lane, u, hb, E, zr, zq);
+      if (flip) sup_body<T, NU, TR, 1, false, DOTS>(t, i0, lane, u, hb, E, zr, zq, 0, rupd, alpha);
+      else sup_body<T, NU, TR, 0, false, DOTS>(t, i0, lane, u, hb, E, zr, zq, 0, rupd, alpha);
     }
   }
   if (DOTS) write_partial<kThreads>(zr, zq, zpart, t.slot, sh);

@@ -39,7 +39,7 @@ void down_masked(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 template <typename T, int CRS>
 void down_stream_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
 #define GDP_DOWNS(NU) \
-  klaunch(k_sdown_stream<T, NU, CRS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv)
+  klaunch(k_sdown_stream<T, NU, CRS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv, a.apart, a.dcur)
   switch (nu) {
     case 0: GDP_DOWNS(0); break;
     case 1: GDP_DOWNS(1); break;
@@ -47,6 +47,20 @@ void down_stream_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
     default: GDP_DOWNS(3); break;
   }
 #undef GDP_DOWNS
+}
+
+// The same strips carrying the PCG update of the finest level (gdp_sdfuse*.hip).
+template <typename T, int CRS>
+void down_fused_nu(int nu, int blocks, hipStream_t s, const DownArgs& a) {
+#define GDP_DOWNF(NU) \
+  klaunch(k_sdown_stream<T, NU, CRS, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.upart, a.check_conv, a.apart, a.dcur)
+  switch (nu) {
+    case 0: GDP_DOWNF(0); break;
+    case 1: GDP_DOWNF(1); break;
+    case 2: GDP_DOWNF(2); break;
+    default: GDP_DOWNF(3); break;
+  }
+#undef GDP_DOWNF
 }
 
 template <typename T>
@@ -69,14 +83,23 @@ void launch_down(int menu, int nu, int blocks, hipStream_t s, const DownArgs& a)
     case 4: case 5: case 6:
       launch_down_stream<T>(menu == 4 ? 8 : (menu == 5 ? 16 : 32), nu, blocks, s, a);
       break;
-    default: launch_down_blk<T>(menu == 7 ? 8 : (menu == 8 ? 16 : 32), nu, blocks, s, a); break;
+    case 7: case 8: case 9:
+      launch_down_blk<T>(menu == 7 ? 8 : (menu == 8 ? 16 : 32), nu, blocks, s, a);
+      break;
+    // entries 10 .. 15: the streamed strips and the blocks, which on the finest level of a
+    // PCG iteration also carry its update (launch_down_fused, launch_down_blk_fused); here,
+    // where no update is pending, plain
+    case 10: case 11: case 12:
+      launch_down_stream<T>(menu == 10 ? 8 : (menu == 11 ? 16 : 32), nu, blocks, s, a);
+      break;
+    default: launch_down_blk<T>(menu == 13 ? 8 : (menu == 14 ? 16 : 32), nu, blocks, s, a); break;
   }
 }
 
 template <typename T, int TR, bool DOTS>
 void up_nu(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU) \
-  klaunch(k_sup<T, NU, TR, DOTS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order)
+  klaunch(k_sup<T, NU, TR, DOTS>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order, a.rupd)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;
@@ -100,7 +123,7 @@ void launch_up_d(int menu, int nu, int blocks, hipStream_t s, const UpArgs& a) {
 template <typename T, int TR, bool DOTS>
 void up_masked(int nu, int blocks, hipStream_t s, const UpArgs& a) {
 #define GDP_UP(NU) \
-  klaunch(k_sup<T, NU, TR, DOTS, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order)
+  klaunch(k_sup<T, NU, TR, DOTS, true>, dim3(blocks), dim3(kThreads), 0, s, a.tiles, a.st, a.zpart, a.order, a.rupd)
   switch (nu) {
     case 0: GDP_UP(0); break;
     case 1: GDP_UP(1); break;

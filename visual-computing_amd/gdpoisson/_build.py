@@ -23,6 +23,14 @@ LIB = PKG / "libgdpoisson.so"
 
 SOURCES = [
     # the longest compiles first, so they start before the pool fills
+    "gdp_sdblkfuse_f32.hip",
+    "gdp_sdblkfuse_f64.hip",
+    "gdp_sdfuse16_f32.hip",
+    "gdp_sdfuse16_f64.hip",
+    "gdp_sdfuse8_f32.hip",
+    "gdp_sdfuse8_f64.hip",
+    "gdp_sdfuse32_f32.hip",
+    "gdp_sdfuse32_f64.hip",
     "gdp_sdstream_f32.hip",
     "gdp_sdstream_f64.hip",
     "gdp_sdstream32_f32.hip",

@@ -36,6 +36,8 @@ def menu_size(kind: int) -> int:
 
 
 def _kernel_times(plan: Plan, b: Sequence[torch.Tensor], solves: int) -> dict[str, float]:
+    """Mean time per launch of every kernel kind (ms); the time per solve of each kind
+    under the key ("solve", kind)."""
     out = [torch.empty_like(x) for x in b]
     plan.mgcg(b, out)  # settle the new tables
     plan.profile(True)
@@ -43,7 +45,9 @@ def _kernel_times(plan: Plan, b: Sequence[torch.Tensor], solves: int) -> dict[st
         plan.mgcg(b, out)
     rep = plan.profile_report()
     plan.profile(False)
-    return {k: v["ms"] / max(v["launches"], 1) for k, v in rep.items()}
+    times: dict = {k: v["ms"] / max(v["launches"], 1) for k, v in rep.items()}
+    times.update({("solve", k): v["ms"] / solves for k, v in rep.items()})
+    return times
 
 
 def _synthetic_rhs(plan: Plan, seed: int = 1234) -> list[torch.Tensor]:
@@ -75,7 +79,8 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
         return dict(_CACHE[key])
     b = list(b) if b is not None else _synthetic_rhs(plan)
     base = _kernel_times(plan, b, solves)
-    levels = sorted({int(mt.group(1)) for k in base for mt in [re.match(r"down L(\d+)", k)] if mt})
+    levels = sorted({int(mt.group(1)) for k in base if isinstance(k, str)
+                     for mt in [re.match(r"down L(\d+)", k)] if mt})
     choice: dict = {}
 
     def cost(times: dict, kind: int, level: int) -> float:
@@ -86,8 +91,17 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
         if kind == 2:
             return sum(t for k, t in times.items()
                        if k in ("apply", "update", "update (x deferred)", "down L0"))
+        if kind == 0 and level == 0:
+            # the fused update-down entries replace the update of most iterations, move
+            # its convergence test to the second level's kernel and its r store to the
+            # finest up kernel: compare the time per solve of all of these
+            return sum(v for k, v in times.items()
+                       if isinstance(k, tuple) and k[1] in (
+                           "down L0", "down L0+update", "update", "update (x deferred)",
+                           "down L1", "up L0+dots"))
         pre = "down" if kind == 0 else "up"
-        t = sum(v for k, v in times.items() if re.fullmatch(rf"{pre} L{level}(\+dots)?", k))
+        t = sum(v for k, v in times.items()
+                if isinstance(k, str) and re.fullmatch(rf"{pre} L{level}(\+dots)?", k))
         if kind == 1 and level == 0:
             t += times.get("apply", 0.0)
         return t
@@ -95,7 +109,10 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
     for kind, level in [(k, l) for l in levels for k in (0, 1)] + [(2, 0)]:
         best_idx, best_t = None, float("inf")
         for idx in range(menu_size(kind)):
-            plan.set_tiles({(kind, level): idx})
+            try:
+                plan.set_tiles({(kind, level): idx})
+            except ValueError:
+                continue  # an entry this level cannot take (fused update: finest only)
             t = cost(_kernel_times(plan, b, solves), kind, level)
             if t < best_t:
                 best_idx, best_t = idx, t

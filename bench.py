@@ -268,11 +268,12 @@ def kernel_kind(name: str, grid: int, grids: dict) -> str | None:
     fam = _family(base)
     if fam is not None:
         kind = "down" if fam in ("k_sdown<", "k_down<") else "up"
-        # k_sup<T, nu, rows, dots, masked>; k_sdown_stream<T, nu, rows, upd> (upd: the
-        # finest level carrying the PCG update)
+        # k_sup<T, nu, rows, dots, masked>; k_sdown_stream<T, nu, rows, upd> and
+        # k_sdown_blk<T, nu, rows, upd> (upd: the finest level carrying the PCG update)
         args = [a.strip(" >") for a in base[base.index("<") + 1:].split(",")]
         dots = kind == "up" and len(args) > 3 and args[3] == "true"
-        upd = "k_sdown_stream<" in base and len(args) > 3 and args[3] == "true"
+        upd = (("k_sdown_stream<" in base or "k_sdown_blk<" in base) and len(args) > 3
+               and args[3] == "true")
         lv = sorted(grids[fam], reverse=True).index(_level_area(name, grid))
         return f"{kind} L{lv}" + ("+dots" if dots else "") + ("+update" if upd else "")
     if "k_update<" in base:

@@ -123,8 +123,9 @@ def test_fused_update_down_kernel_is_the_finest_level_with_the_update():
     names = [
         ("void gdp::k_sdown_stream<float, 2, 16, true>(...)", 8192),   # area 8192 * 16
         ("void gdp::k_sdown_stream<float, 2, 16, false>(...)", 8192),  # the same level
-        ("void gdp::k_sdown_blk<float, 2, 8>(...)", 4096),            # area 4096 * 8 / 4
+        ("void gdp::k_sdown_blk<float, 2, 8, true>(...)", 32768),     # area 32768 * 8 / 4
+        ("void gdp::k_sdown_blk<float, 2, 8, false>(...)", 4096),     # area 4096 * 8 / 4
     ]
     grids = {"k_sdown<": {bench._level_area(n, g) for n, g in names}}
     kinds = [bench.kernel_kind(n, g, grids) for n, g in names]
-    assert kinds == ["down L0+update", "down L0", "down L1"]
+    assert kinds == ["down L0+update", "down L0", "down L1+update", "down L2"]

@@ -429,8 +429,8 @@ def test_streamed_down_kernel_in_batched_solves(gd, precision, down):
                                     [(300, 260), (1030, 250), (257, 255)]])
 def test_fused_update_down_kernel_gives_the_same_solve(gd, shapes, fused, bc, precision):
     """Down entries 10 .. 15 carry the PCG update in the finest down kernel (the update's
-    right-hand side formed from r and q, its convergence test in the second level's
-    kernel, r stored by the finest up kernel): the same iterations and the same solution
+    right-hand side formed from r and q, its convergence test at the tail kernels' entry,
+    r stored by the finest up kernel): the same iterations and the same solution
     bits as the streamed strips and blocks of entries 4 .. 9 with k_update, alone and
     batched; the profile shows the fused kernel ran."""
     import torch
@@ -466,12 +466,12 @@ def test_fused_update_down_kernel_gives_the_same_solve(gd, shapes, fused, bc, pr
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_fused_update_entry_without_two_tiled_levels(gd, precision):
-    """A plan with a problem whose second level is already in the coarse tail keeps
-    k_update (the fused kernel needs the second level's kernel for the convergence test):
-    a fused entry then solves as the plain streamed strips."""
+    """A plan with a problem whose finest level is in the coarse tail keeps k_update
+    (that problem has no finest down kernel to carry it): a fused entry then solves as
+    the plain streamed strips."""
     import torch
 
-    shapes = [(300, 260), (60, 131)]  # 30 x 66 is in the tail at both precisions
+    shapes = [(300, 260), (30, 100)]  # in the tail from its finest level at both precisions
     bs = [torch.from_numpy(G.neumann_problem(s, seed=4900 + k)[0]).cuda()
           for k, s in enumerate(shapes)]
     opts = gd.PlanOptions(precision=precision)

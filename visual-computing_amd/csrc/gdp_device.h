@@ -85,9 +85,7 @@ struct VTile {
   int32_t cm, cn, cpitch, level;
   int32_t problem, i0, j0, slot;  // slot: this tile's partial-sum slot (finest up kernel)
   int32_t lead;                   // the one tile per problem that writes its state
-  int32_t rbeg, rcnt;             // ||r||^2 partials to test convergence from (the level
-                                  // that tests it: 0 after k_update, 1 after the fused
-                                  // update-down kernel)
+  int32_t rbeg, rcnt;             // ||r||^2 partials to test convergence from (level 0)
   int32_t pad;
   const uint8_t* mask;            // finest level of a masked domain: m x n bytes, nonzero =
                                   // unknown (the rest held at zero); else null

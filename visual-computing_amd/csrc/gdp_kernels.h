@@ -260,8 +260,10 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
   const int p = D.problem;
   if (st[p].done) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // a problem whose finest level is in the tail gets its convergence test here
-  if (check_conv && D.krylov >= 0 &&
+  // the convergence test of the update: here for a problem whose finest level is in the
+  // tail (check_conv 1, k_update's partials), and for every problem after the fused
+  // update-down kernel (check_conv 2, that kernel's partials)
+  if ((check_conv == 2 || (check_conv == 1 && D.krylov >= 0)) &&
       check_converged<kTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
     return;
 

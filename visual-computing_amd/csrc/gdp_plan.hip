@@ -100,6 +100,9 @@ int up_cols(int nu) { return 124 - 4 * nu; }
 constexpr int kMaxDepth = 32;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+// Bytes of the problem states rounded up to 64: the offset of the check's flag behind them,
+// on the device and in the pinned host block alike.
+size_t state_span(int count) { return (sizeof(ProblemState) * size_t(count) + 63) & ~size_t(63); }
 int round_up(int x, int a) { return (x + a - 1) / a * a; }
 int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -166,7 +169,8 @@ struct gdp_plan {
   size_t ws_bytes = 0;
   size_t off_levels = 0, off_kry = 0, off_tails = 0, off_state = 0;
   size_t off_apart = 0, off_upart = 0, off_zpart = 0, off_spart = 0;
-  size_t off_ranges_kry = 0, off_ranges_up0 = 0, off_io = 0, off_init = 0;
+  size_t off_ranges_kry = 0, off_ranges_up0 = 0, off_ranges_down0 = 0, off_io = 0,
+         off_init = 0;
   size_t off_hist = 0, off_flag = 0;
   size_t off_tiles = 0;     // start of the tile tables region
   size_t tiles_bytes = 0;   // reserved bytes for the tile tables
@@ -216,6 +220,8 @@ struct gdp_plan {
   double2* d_spart() const { return at<double2>(off_spart); }  // set-up / residual partials
   SlotRange* d_rkry() const { return at<SlotRange>(off_ranges_kry); }
   SlotRange* d_rup0() const { return at<SlotRange>(off_ranges_up0); }
+  // ||r||^2 partials of the fused update-down kernel, per problem
+  SlotRange* d_rdown0() const { return at<SlotRange>(off_ranges_down0); }
   IoDesc* d_io() const { return at<IoDesc>(off_io); }
   const double** d_init() const { return at<const double*>(off_init); }
   double* d_hist() const { return at<double>(off_hist); }
@@ -305,12 +311,12 @@ int down_menu_of(const gdp_plan* P, int l) {
   return (m >= 4 && has_mask(P)) ? 3 : m;
 }
 // The finest level's down kernel carries the PCG update (fused update-down entries)
-// when its entry asks for it, no domain is masked and every problem has at least two
-// tiled levels (the update's convergence test runs in the second level's kernel).
+// when its entry asks for it, no domain is masked and every problem's finest level is a
+// tiled one (a problem whose finest level is in the tail needs k_update).
 bool fused_update(const gdp_plan* P) {
-  if (P->depth < 2 || down_menu_of(P, 0) < kFusedDown) return false;
+  if (P->depth < 1 || down_menu_of(P, 0) < kFusedDown) return false;
   for (const ProbInfo& pi : P->probs)
-    if (pi.tail_first < 2) return false;
+    if (pi.tail_first < 1) return false;
   return true;
 }
 // The up-kernel menu entry of level l: likewise the masked up kernels are strips only.
@@ -621,13 +627,16 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   P->off_levels = take(sizeof(LevelDesc) * lvl_total);
   P->off_kry = take(sizeof(KrylovDesc) * P_);
   P->off_tails = take(sizeof(TailDesc) * P_);
-  P->off_state = take(sizeof(ProblemState) * P_);
+  // the state, then (at the 64-byte boundary the host's pinned copy also uses) the
+  // check's flag: one device-to-host copy reads both
+  P->off_state = take(state_span(P_) + 64);
+  P->off_flag = P->off_state + state_span(P_);
   P->off_ranges_kry = take(sizeof(SlotRange) * P_);
   P->off_ranges_up0 = take(sizeof(SlotRange) * P_);
+  P->off_ranges_down0 = take(sizeof(SlotRange) * P_);
   P->off_io = take(sizeof(IoDesc) * P_);
   P->off_init = take(sizeof(double*) * P_);
   P->off_hist = take(sizeof(double) * P_ * (size_t(o.max_iterations) + 2));
-  P->off_flag = take(sizeof(int) * 4);
   // tile tables (worst case over the menus)
   int slots = 1;
   size_t tiles = 0;
@@ -762,7 +771,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   if (!P->h_block) {
     auto a64 = [](size_t x) { return (x + 63) & ~size_t(63); };
     const size_t b_io = a64(sizeof(IoDesc) * NP), b_init = a64(sizeof(double*) * NP);
-    const size_t b_state = a64(sizeof(ProblemState) * NP);
+    const size_t b_state = state_span(NP);  // the device layout's offset of the flag
     P->h_block_bytes = b_io + b_init + b_state + 64;
     GDP_HIP(pinned_get(P->h_block_bytes, &P->h_block));
     char* hb = static_cast<char*>(P->h_block);
@@ -944,11 +953,8 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
     t.j0 = j0;
     t.slot = slot;
     t.lead = lead;
-    // the convergence test of an update: level 0 after k_update (its Krylov-tile
-    // partials), level 1 after the fused update-down kernel (that kernel's tiles)
-    const SlotRange rt = l == 1 ? rd[p] : rk[p];
-    t.rbeg = rt.begin;
-    t.rcnt = rt.count;
+    t.rbeg = rk[p].begin;
+    t.rcnt = rk[p].count;
     t.abeg = ra[p].begin;
     t.acnt = ra[p].count;
     if (pi.mask) t.mask = l == 0 ? pi.mask : reinterpret_cast<const uint8_t*>(P->ws + f.off_mask);
@@ -1080,6 +1086,7 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   }
   GDP_HIP(h2d(P->d_rkry(), rk.data(), sizeof(SlotRange) * NP));
   GDP_HIP(h2d(P->d_rup0(), ru.data(), sizeof(SlotRange) * NP));
+  GDP_HIP(h2d(P->d_rdown0(), rd.data(), sizeof(SlotRange) * NP));
   GDP_HIP(hipMemsetAsync(P->d_state(), 0, sizeof(ProblemState) * NP, P->cap));
   for (size_t o : {P->off_apart, P->off_upart, P->off_zpart, P->off_spart})
     GDP_HIP(hipMemsetAsync(P->ws + o, 0, sizeof(double2) * P->max_slots, P->cap));
@@ -1190,8 +1197,9 @@ bool defer_x_update(const gdp_plan* P) {
 // xupd: the previous k_update deferred its x half (defer_x); the strip-tail launch adds
 // it in blocks of its own.  fused (fused_update plans, PCG iterations that defer x): the
 // finest down kernel carries the iteration's update (in place of k_update, dcur its
-// direction buffer), the second level's kernel tests its convergence and the finest up
-// kernel stores its r.
+// direction buffer), the tail kernels test its convergence at their entry (from that
+// kernel's partials, with their own loads in flight) and the finest up kernel stores its
+// r.
 template <typename T>
 int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
                      bool xupd = false, bool fused = false, int dcur = 0) {
@@ -1202,7 +1210,7 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
     const bool fl = fused && l == 0;
     snprintf(nm, sizeof nm, fl ? "down L%d+update" : "down L%d", l);
     ProfScope ps(P, s, nm, ((fl ? 16.0 : 0.0) + 2.25 * sizeof(T)) * level_points(P, l));
-    const int cc = !check_conv ? 0 : (fused ? 2 : 1);
+    const int cc = check_conv && !fused ? 1 : 0;
     DownArgs a{P->at<VTile>(P->down[l].off), P->d_state(), P->d_upart(), cc,
                has_mask(P) ? 1 : 0};
     if (fl) {
@@ -1229,11 +1237,14 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
   {
     ProfScope ps(P, s, "tail",
                  2.0 * sizeof(T) * tail_pts + (xupd ? 24.0 * krylov_points(P) : 0.0));
-    const int cc = check_conv ? 1 : 0;
+    // 1: the problems whose finest level is in the tail test the update's convergence
+    // (k_update's partials); 2: every problem does (the fused update-down kernel's)
+    const int cc = !check_conv ? 0 : (fused ? 2 : 1);
+    const SlotRange* rk = fused ? P->d_rdown0() : P->d_rkry();
     const int ns = P->n_stail, no = P->count() - P->n_stail;
     if (ns) {
       TailArgs a{P->d_tails(), P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
-                 P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
+                 rk, P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
                  o.post_smoothing, P->csweeps, order};
       if (xupd) {
         // one block on each CU the tail blocks leave free, at most one per kXItems items
@@ -1254,12 +1265,12 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
       if (dots)
         klaunch(k_tail<T, true>, dim3(no), dim3(kTailThreads), P->tail_smem, s,
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
-            P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
+            rk, P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
             P->csweeps, order);
       else
         klaunch(k_tail<T, false>, dim3(no), dim3(kTailThreads), P->tail_smem, s,
             P->d_tails() + ns, P->d_levels(), P->d_state(), P->d_kry(), P->d_upart(),
-            P->d_rkry(), P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
+            rk, P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing, o.post_smoothing,
             P->csweeps, order);
       GDP_CHECK_LAUNCH();
     }
@@ -1518,10 +1529,12 @@ int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double
 }
 
 int read_state(gdp_plan* P, hipStream_t s, bool with_flag = false) {
-  GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
+  // with the flag: one copy of the state and the flag behind it (the same offset on both
+  // sides), not two
+  GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(),
+                         with_flag ? state_span(P->count()) + sizeof(int)
+                                   : sizeof(ProblemState) * P->count(),
                          hipMemcpyDeviceToHost, s));
-  if (with_flag)
-    GDP_HIP(hipMemcpyAsync(P->h_flag, P->d_flag(), sizeof(int), hipMemcpyDeviceToHost, s));
   GDP_HIP(hipStreamSynchronize(s));
   if (P->prof) prof_collect(P);
   return GDP_OK;

@@ -728,8 +728,10 @@ __global__ __launch_bounds__(kSTailThreads) void k_stail(const TailDesc* __restr
     tv[k] = o < ntab ? dct[o] : 0.0;
   }
   if (st[p].done) return;
-  // a problem whose finest level is in the tail gets its convergence test here
-  if (check_conv && D.krylov >= 0 &&
+  // the convergence test of the update: here for a problem whose finest level is in the
+  // tail (check_conv 1, k_update's partials), and for every problem after the fused
+  // update-down kernel (check_conv 2, that kernel's partials)
+  if ((check_conv == 2 || (check_conv == 1 && D.krylov >= 0)) &&
       check_converged<kSTailThreads>(st, p, upart, rk[p], true, sh, &bcc))
     return;
   if (c.exact) {

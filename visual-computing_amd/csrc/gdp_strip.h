@@ -703,7 +703,7 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
 // k_update<T, TH, TW, true>): alpha from the <d, q> partials (the same slots and order as
 // k_update, so the same bits), the lead block's state record of the update, the right-hand
 // side -rn row by row, and the rn^2 partial of each block at its slot in upart; the
-// convergence test of that update falls to the next level's kernel.
+// convergence test of that update falls to the tail kernels' entry.
 template <typename T, int NU, int CRS, bool UPD = false>
 __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restrict__ tiles,
                                                            ProblemState* __restrict__ st,

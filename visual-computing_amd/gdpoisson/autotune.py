@@ -93,12 +93,12 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
                        if k in ("apply", "update", "update (x deferred)", "down L0"))
         if kind == 0 and level == 0:
             # the fused update-down entries replace the update of most iterations, move
-            # its convergence test to the second level's kernel and its r store to the
-            # finest up kernel: compare the time per solve of all of these
+            # its convergence test to the tail kernels and its r store to the finest up
+            # kernel: compare the time per solve of all of these
             return sum(v for k, v in times.items()
                        if isinstance(k, tuple) and k[1] in (
                            "down L0", "down L0+update", "update", "update (x deferred)",
-                           "down L1", "up L0+dots"))
+                           "tail", "up L0+dots"))
         pre = "down" if kind == 0 else "up"
         t = sum(v for k, v in times.items()
                 if isinstance(k, str) and re.fullmatch(rf"{pre} L{level}(\+dots)?", k))

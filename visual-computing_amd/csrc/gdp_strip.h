@@ -511,7 +511,16 @@ __device__ __attribute__((always_inline)) void static_for(F&& f) {
 // their rn^2 to acc.  r itself is left to the finest up kernel, which owns each point
 // alone (a strip's halo rows are other strips' owned rows, so a store here would race
 // with their loads).
-template <typename T, int NU, int CRS, int FLIP, bool EDGE, bool UPD = false>
+//
+// REV (strips that touch neither grid edge): the rows are visited bottom to top, the same
+// schedule mirrored (half-sweep s at row t + s as row t arrives, the residual formed one
+// row below the final one, a coarse row restricted once its first residual row is
+// ready).  Every point sees the values it sees top to bottom, so the results are the same
+// bits.  Alternate strips run mirrored so that two neighbouring strips load the halo rows
+// they share at about the same time (both at their start, or both at their end), while
+// those rows are still in L2; in one direction for all, one strip loads them first and the
+// other last.
+template <typename T, int NU, int CRS, int FLIP, bool EDGE, bool UPD = false, bool REV = false>
 __device__ __attribute__((always_inline)) void sdown_stream_body(
     const VTile& t, int I0, int lane, typename Pair<T>::type* raw,
     typename Pair<double>::type* rr = nullptr, typename Pair<double>::type* qq = nullptr,
@@ -524,6 +533,9 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
   constexpr int FH = 2 * CRS + (NU > 0 ? 4 * NU + 3 : 4);
   constexpr int K = NU > 0 ? 2 * NU - 1 : 0;  // half-sweeps after the zero-guess one
   constexpr int R0 = OFF - 1, R1 = OFF + 2 * CRS;
+  static_assert(!(REV && EDGE), "mirrored strips touch no grid edge");
+  // the row of step s of the schedule
+  auto ro = [](int s) constexpr { return REV ? FH - 1 - s : s; };
   const int m = t.m, n = t.n;
   const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
   const bool neu = t.bc == kNeumann;
@@ -556,11 +568,27 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
   GDP_GLOBAL T* b0 = (GDP_GLOBAL T*)t.b;
 
   V2 u[FH], hb[FH];
+  if constexpr (REV) {
+    // the first PF rows of the mirrored order (the kernel requested the others' early)
+#pragma unroll
+    for (int k = 0; k < PF && k < FH; ++k) {
+      const int a = ro(k);
+      const int gi = gr0 + a;
+      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jload;
+      if constexpr (UPD) {
+        rr[a] = *(const GDP_GLOBAL D2*)(kr + o);
+        qq[a] = *(const GDP_GLOBAL D2*)(kq + o);
+      } else {
+        const V2 q = *(const GDP_GLOBAL V2*)(b + o);
+        raw[a] = (lin && gi >= 0 && gi < m) ? q : V2{T(0), T(0)};
+      }
+    }
+  }
   static_for<0, FH>([&](auto step) __attribute__((always_inline)) {
     constexpr int st = decltype(step)::value;
     // prefetch the right-hand side PF rows ahead (clamped address, zero outside)
     if (st + PF < FH) {
-      const int a = st + PF;
+      const int a = ro(st + PF);
       const int gi = gr0 + a;
       const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jload;
       if constexpr (UPD) {
@@ -573,7 +601,7 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
     }
     // UPD: the updated residual of row st; its owned points store b0 = -rn and count rn^2
     if constexpr (UPD) {
-      const int a = st;
+      const int a = ro(st);
       const int gi = gr0 + a;
       D2 rn;
       rn.x = __builtin_fma(-alpha, qq[a].x, rr[a].x);
@@ -594,7 +622,7 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
     }
     // row st: scaled right-hand side and the zero-guess half-sweep (colour 0)
     {
-      const int a = st;
+      const int a = ro(st);
       const int gi = gr0 + a;
       int vc = 2;
       bool rin = true;
@@ -616,10 +644,10 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
         u[a] = V2{T(0), T(0)};
       }
     }
-    // half-sweep s at row st - s
+    // half-sweep s at row st - s (mirrored: st + s)
 #pragma unroll
     for (int s = 1; s <= K; ++s) {
-      const int a = st - s;
+      const int a = ro(st - s);
       if (a < 1 || a > FH - 2) continue;
       int vc = 2;
       if (EDGE) {
@@ -639,7 +667,7 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
     }
     // row st - K is final: the pre-smoothed iterate of the owned rows
     {
-      const int a = st - K;
+      const int a = ro(st - K);
       if (a >= OFF && a < OFF + 2 * CRS + 2) {
         const int gi = gr0 + a;
         if (gi < fr1) {
@@ -655,7 +683,7 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
     // scaled residual of the row above it, in place of hb', then the extensions past the
     // grid (Neumann: columns, then rows) and the restriction of a completed coarse row
     {
-      const int a = st - K - 1;
+      const int a = ro(st - K - 1);
       if (a >= R0 && a <= R1) {
         const int gi = gr0 + a;
         int vc = 2;
@@ -681,18 +709,35 @@ __device__ __attribute__((always_inline)) void sdown_stream_body(
           if (a > R0 && gi - 1 == -1) hb[a - 1] = hb[a];
           if (gi == m && (m & 1) == 0 && a > R0) hb[a] = hb[a - 1];
         }
-        if (a >= OFF + 2 && ((a - OFF - 2) & 1) == 0) {
-          const int i = (a - OFF - 2) / 2;
-          const int I = I0 + i;
-          T c;
-          if (neu) {
-            c = restrict_neumann<T>(hb[2 * i + OFF - 1], hb[2 * i + OFF], hb[2 * i + OFF + 1],
-                                    hb[2 * i + OFF + 2], sres);
-          } else {
-            c = restrict_dirichlet<T>(hb[2 * i + OFF], hb[2 * i + OFF + 1],
-                                      hb[2 * i + OFF + 2], sres);
+        if constexpr (!REV) {
+          if (a >= OFF + 2 && ((a - OFF - 2) & 1) == 0) {
+            const int i = (a - OFF - 2) / 2;
+            const int I = I0 + i;
+            T c;
+            if (neu) {
+              c = restrict_neumann<T>(hb[2 * i + OFF - 1], hb[2 * i + OFF],
+                                      hb[2 * i + OFF + 1], hb[2 * i + OFF + 2], sres);
+            } else {
+              c = restrict_dirichlet<T>(hb[2 * i + OFF], hb[2 * i + OFF + 1],
+                                        hb[2 * i + OFF + 2], sres);
+            }
+            if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
           }
-          if (I < t.cm && wj) cb[(size_t)I * t.cpitch + J] = c;
+        } else {
+          // mirrored: coarse row i once its first residual row is ready, 2 i + OFF - 1
+          // (Neumann) or 2 i + OFF (Dirichlet)
+          const int in = (a - OFF + 1) / 2, id = (a - OFF) / 2;
+          if (neu) {
+            if (((a - OFF + 1) & 1) == 0 && in < CRS) {
+              const T c = restrict_neumann<T>(hb[2 * in + OFF - 1], hb[2 * in + OFF],
+                                              hb[2 * in + OFF + 1], hb[2 * in + OFF + 2], sres);
+              if (I0 + in < t.cm && wj) cb[(size_t)(I0 + in) * t.cpitch + J] = c;
+            }
+          } else if (a >= OFF && ((a - OFF) & 1) == 0 && id < CRS) {
+            const T c = restrict_dirichlet<T>(hb[2 * id + OFF], hb[2 * id + OFF + 1],
+                                              hb[2 * id + OFF + 2], sres);
+            if (I0 + id < t.cm && wj) cb[(size_t)(I0 + id) * t.cpitch + J] = c;
+          }
         }
       }
     }
@@ -723,8 +768,12 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
   const bool work = I0 < t.cm;
   const int gr0 = 2 * I0 - OFF, gc0 = 2 * t.j0 - OFF;
   const int m = t.m, n = t.n;
-  // the first rows are requested before the state tests (branch-free, clamped
-  // addresses, zero outside the grid); the body requests the rest as it goes
+  const bool edge = gr0 < 1 || gr0 + FH > m - 1;
+  // odd strips that touch no grid edge visit their rows bottom to top (wave-uniform)
+  const bool rev = !edge && (w & 1);
+  // the first rows of the strip's order are requested before the state tests
+  // (branch-free, clamped addresses, zero outside the grid); the body requests the rest
+  // as it goes
   V2 raw[FH];
   D2 rr[UPD ? FH : 1], qq[UPD ? FH : 1];
   {
@@ -735,22 +784,24 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
     const bool lin = gjx >= 0 && gjx < n;
     const int jc = lin ? gjx : 0;
     constexpr int PF = UPD ? kPrefetchUpd : kPrefetch;
+    // (a mirrored strip requests its first rows, the last ones, in its body)
+    if (!rev) {
 #pragma unroll
-    for (int a = 0; a < PF && a < FH; ++a) {
-      const int gi = gr0 + a;
-      const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jc;
-      if constexpr (UPD) {
-        rr[a] = *(const GDP_GLOBAL D2*)(kr + o);
-        qq[a] = *(const GDP_GLOBAL D2*)(kq + o);
-      } else {
-        const bool in = work && lin && gi >= 0 && gi < m;
-        const V2 q = *(const GDP_GLOBAL V2*)(b + o);
-        raw[a] = in ? q : V2{T(0), T(0)};
+      for (int a = 0; a < PF && a < FH; ++a) {
+        const int gi = gr0 + a;
+        const size_t o = (size_t)min(max(gi, 0), m - 1) * t.pitch + jc;
+        if constexpr (UPD) {
+          rr[a] = *(const GDP_GLOBAL D2*)(kr + o);
+          qq[a] = *(const GDP_GLOBAL D2*)(kq + o);
+        } else {
+          const bool in = work && lin && gi >= 0 && gi < m;
+          const V2 q = *(const GDP_GLOBAL V2*)(b + o);
+          raw[a] = in ? q : V2{T(0), T(0)};
+        }
       }
     }
   }
   asm volatile("" ::: "memory");
-  const bool edge = gr0 < 1 || gr0 + FH > m - 1;
   const int flip = (gr0 + gc0) & 1;
   if constexpr (UPD) {
     // alpha as k_update forms it; the finished-problem test after the reduction's barrier
@@ -777,6 +828,9 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
       if (edge) {
         if (flip) sdown_stream_body<T, NU, CRS, 1, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
         else sdown_stream_body<T, NU, CRS, 0, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+      } else if (rev) {
+        if (flip) sdown_stream_body<T, NU, CRS, 1, false, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
+        else sdown_stream_body<T, NU, CRS, 0, false, true, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
       } else {
         if (flip) sdown_stream_body<T, NU, CRS, 1, false, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
         else sdown_stream_body<T, NU, CRS, 0, false, true>(t, I0, lane, raw, rr, qq, alpha, &acc);
@@ -798,6 +852,9 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
     if (edge) {
       if (flip) sdown_stream_body<T, NU, CRS, 1, true>(t, I0, lane, raw);
       else sdown_stream_body<T, NU, CRS, 0, true>(t, I0, lane, raw);
+    } else if (rev) {
+      if (flip) sdown_stream_body<T, NU, CRS, 1, false, false, true>(t, I0, lane, raw);
+      else sdown_stream_body<T, NU, CRS, 0, false, false, true>(t, I0, lane, raw);
     } else {
       if (flip) sdown_stream_body<T, NU, CRS, 1, false>(t, I0, lane, raw);
       else sdown_stream_body<T, NU, CRS, 0, false>(t, I0, lane, raw);

@@ -251,7 +251,12 @@ int gdp_plan_destroy(gdp_plan* plan);
 int gdp_plan_levels(const gdp_plan* plan, int problem, int* levels);
 int gdp_plan_level_shape(const gdp_plan* plan, int problem, int level, int* m, int* n);
 /* Tile shapes: kind 0 = down (coarse tile), 1 = up (fine tile), 2 = Krylov (fine tile).
-   Must be called before gdp_plan_bind; index into the compiled menu (see gdp_tile_menu). */
+   Must be called before gdp_plan_bind; index into the compiled menu (see gdp_tile_menu).
+   Down entries 10 .. 15 are finest-level only (GDP_EINVAL on a coarser level): the
+   streamed (10 .. 12) and block-tiled (13 .. 15) down kernels that also carry the PCG
+   update of an iteration in place of a separate update kernel; a plan with a masked
+   domain, or with a problem in the coarse tail from its finest level, runs them as the
+   plain kernels of the same shape. */
 int gdp_plan_set_tile(gdp_plan* plan, int kind, int level, int menu_index);
 /* Masked domain of a Dirichlet problem (solve_masked, poisson.py:154-244): `mask` is a
    device array of rows x cols bytes, nonzero = unknown; the other points are held at zero

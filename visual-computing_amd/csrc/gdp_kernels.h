@@ -507,14 +507,22 @@ __global__ __launch_bounds__(kThreads) void k_init_stats(const KTile* __restrict
   const KTile t = tiles[blockIdx.x];
   __shared__ double sh[2 * kThreads / 64];
   const double* __restrict__ b = io[t.problem].b;
-  double acc = 0.0;
+  // branch-free loads (clamped in-bounds addresses, the points outside not used), all
+  // issued before the first wait; summed in the same order as one at a time
+  constexpr int NI = (TH * TW) / kThreads;
+  double v[NI];
+  bool in[NI];
 #pragma unroll
-  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
-    acc += (gi < t.m && gj < t.n) ? b[(size_t)gi * t.n + gj] : 0.0;
+    in[it] = gi < t.m && gj < t.n;
+    v[it] = b[(size_t)min(gi, t.m - 1) * t.n + min(gj, t.n - 1)];
   }
+  double acc = 0.0;
+#pragma unroll
+  for (int it = 0; it < NI; ++it) acc += in[it] ? v[it] : 0.0;
   write_partial<kThreads>(acc, 0.0, spart, t.slot, sh);
 }
 
@@ -529,19 +537,29 @@ __global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ til
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
   __shared__ double sh[2 * kThreads / 64];
-  const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
   const double* __restrict__ x0 = initial ? initial[p] : nullptr;
   T* __restrict__ b0 = static_cast<T*>(t.b0);
+  // branch-free loads of b (clamped in-bounds addresses), all issued before the first
+  // wait and before the state load of the mean
+  constexpr int NI = (TH * TW) / kThreads;
+  double bv[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    const int a = k / TW, c = k - a * TW;
+    bv[it] = b[(size_t)min(t.i0 + a, t.m - 1) * t.n + min(t.j0 + c, t.n - 1)];
+  }
+  const double mean = st[p].bmean;
   double acc = 0.0;
 #pragma unroll
-  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= t.m || gj >= t.n) continue;
     const size_t o = (size_t)gi * t.pitch + gj;
-    const double bb = b[(size_t)gi * t.n + gj] - mean;
+    const double bb = bv[it] - mean;
     acc += bb * bb;
     if (mode == 0) {
       // d and q need no clearing: the first direction is z itself (beta = 0) and q is
@@ -700,18 +718,28 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const KTile* __restrict__
                                                        int mean_fix, int only_checked) {
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
+  // x first (branch-free, clamped in-bounds addresses), the state test after
+  constexpr int NI = (TH * TW) / kThreads;
+  double xv[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) {
+    const int k = threadIdx.x + it * kThreads;
+    const int a = k / TW, c = k - a * TW;
+    xv[it] = t.x[(size_t)min(t.i0 + a, t.m - 1) * t.pitch + min(t.j0 + c, t.n - 1)];
+  }
+  asm volatile("" ::: "memory");  // keeps the loads above ahead of the state load
   if (only_checked && !st[p].checked) return;
   const double mean = (mean_fix && t.bc == kNeumann)
                           ? st[p].sumx / (double(t.m) * double(t.n))
                           : 0.0;
   double* __restrict__ u = io[p].u;
 #pragma unroll
-  for (int it = 0; it < (TH * TW) / kThreads; ++it) {
+  for (int it = 0; it < NI; ++it) {
     const int k = threadIdx.x + it * kThreads;
     const int a = k / TW, c = k - a * TW;
     const int gi = t.i0 + a, gj = t.j0 + c;
     if (gi >= t.m || gj >= t.n) continue;
-    u[(size_t)gi * t.n + gj] = t.x[(size_t)gi * t.pitch + gj] - mean;
+    u[(size_t)gi * t.n + gj] = xv[it] - mean;
   }
 }
 

@@ -163,3 +163,31 @@ def test_masked_solve_of_a_single_pixel(gd, method, where):
     assert len(reps) == 1 and reps[0].converged
     assert np.abs(got - want).max() < 1e-9
     assert np.array_equal(got[~mask], img[~mask])
+
+
+@pytest.mark.parametrize("entry", [11, 14])
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_fused_update_entries_fall_back_on_masked_plans(gd, entry, precision):
+    """A masked plan runs the finest level with the masked strip kernels and a separate
+    update whatever down entry it is given: the down entries that carry the update
+    (streamed 11, block-tiled 14) give the bits of the tallest masked strips (entry 3)."""
+    import torch
+
+    shape = (300, 260)
+    mask = torch.from_numpy(G.blob_mask(shape, seed=77, sigma=8.0).astype(np.uint8)).cuda()
+    rng = np.random.default_rng(78)
+    # a masked right-hand side is zero at the fixed points (their values live in it)
+    bs = [torch.from_numpy(rng.standard_normal(shape)).cuda() * mask for _ in range(2)]
+    opts = gd.PlanOptions(precision=precision)
+    outs = {}
+    for menu in (3, entry):
+        plan = gd.Plan([shape] * 2, boundary="dirichlet", options=opts,
+                       tiles={(0, 0): menu}, masks=[mask, mask])
+        outs[menu], reps = plan.mgcg(bs)
+        assert all(r.converged for r in reps)
+        plan.profile(True)
+        plan.mgcg(bs)
+        assert "down L0+update" not in plan.profile_report()
+        plan.profile(False)
+    for k in range(2):
+        assert torch.equal(outs[3][k], outs[entry][k])

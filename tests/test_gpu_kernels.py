@@ -465,6 +465,34 @@ def test_fused_update_down_kernel_gives_the_same_solve(gd, shapes, fused, bc, pr
 
 
 @pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("nu", [1, 3])
+@pytest.mark.parametrize("fused", [10, 12, 13, 15])
+@pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
+def test_fused_update_down_kernel_at_other_sweep_counts(gd, bc, fused, nu, precision):
+    """The fused update-down entries at nu = 1 and 3 (entry 12's 32-row strips run as
+    16-row strips at nu = 3): the bits of the plain entries with k_update."""
+    import torch
+
+    shapes = [(300, 260), (513, 190)]
+    bs = []
+    for k, s in enumerate(shapes):
+        if bc == "neumann":
+            b, _ = G.neumann_problem(s, seed=5100 + k)
+        else:
+            b = np.random.default_rng(5200 + k).standard_normal(s)
+        bs.append(torch.from_numpy(b).cuda())
+    opts = gd.PlanOptions(precision=precision, pre_smoothing=nu, post_smoothing=nu)
+    outs, reps = {}, {}
+    for menu in (fused - 6, fused):
+        outs[menu], reps[menu] = gd.Plan(shapes, boundary=bc, options=opts,
+                                         tiles={(0, 0): menu}).mgcg(bs)
+    for k in range(len(shapes)):
+        assert reps[fused][k].converged
+        assert reps[fused - 6][k].iterations == reps[fused][k].iterations
+        assert torch.equal(outs[fused - 6][k], outs[fused][k]), (k, shapes[k])
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
 def test_fused_update_entry_without_two_tiled_levels(gd, precision):
     """A plan with a problem whose finest level is in the coarse tail keeps k_update
     (that problem has no finest down kernel to carry it): a fused entry then solves as

@@ -306,6 +306,29 @@ def test_more_problems_than_free_cus_match_each_alone(gd, count, low, high):
         assert np.abs(together[k].cpu().numpy() - exact[k]).max() < 1e-6
 
 
+@pytest.mark.parametrize("precision,flexible", [("fp64", None), ("fp64", True), ("mixed", False)])
+def test_apply_carried_x_update_at_other_options(gd, precision, flexible):
+    """The x update carried by the applies (a batch whose Krylov points the CUs left free
+    by the tails cannot absorb), which add it to x every other iteration and hold it back
+    otherwise, in FP64 (Fletcher-Reeves and flexible) and in mixed Fletcher-Reeves PCG:
+    problems that finish after an odd and after an even number of iterations get bit for
+    bit their solve alone (where the tail launch adds x every iteration)."""
+    import torch
+
+    shapes = G.batch_shapes(count=200, seed=4400, low=200, high=300)
+    bs = [torch.from_numpy(G.neumann_problem(s, seed=4500 + k)[0]).cuda()
+          for k, s in enumerate(shapes)]
+    opts = gd.PlanOptions(precision=precision, flexible=flexible)
+    together, reps = gd.Plan(shapes, boundary="neumann", options=opts).mgcg(bs)
+    assert all(r.converged and r.relative_residual <= 1e-8 for r in reps)
+    its = {r.iterations for r in reps}
+    picks = [next(k for k, r in enumerate(reps) if r.iterations == i) for i in sorted(its)]
+    for k in picks + [len(shapes) - 1]:
+        alone, rep1 = gd.Plan([shapes[k]], boundary="neumann", options=opts).mgcg([bs[k]])
+        assert reps[k].iterations == rep1[0].iterations
+        assert torch.equal(together[k], alone[0]), (k, shapes[k])
+
+
 @pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
 @pytest.mark.parametrize("menu", [0, 1, 2, 3])
 def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):

@@ -3,9 +3,14 @@
 //
 // XUPD: the preceding k_update left its x += alpha d out (ProblemState.xpend) because
 // the strip-tail launch could not absorb it (a plan with about as many coarse tails as
-// CUs).  The apply reads that old direction anyway, so it adds alpha d to x of the points
-// it owns: 16 B per point instead of the 24 B of a separate pass.  A finished problem
-// still gets its pending update before the apply leaves it.
+// CUs).  The apply reads that old direction anyway, and pays x's traffic every other
+// iteration only: it holds a single pending update back (xhold; the next update keeps
+// that direction, which the apply after it would overwrite, and either adds it itself
+// or leaves both to that apply, xdouble), and adds a double one to x of the points it
+// owns, first xalpha times the held direction, read from the buffer the apply is about
+// to overwrite, then alpha times the old direction: 12 B per point and iteration on
+// average, instead of the 24 B of a separate pass, with the bits of one x += alpha d per
+// iteration.  A finished problem gets everything pending before the apply leaves it.
 //
 // A wave owns a strip of kApplyRows output rows and 128 loaded columns: lane l holds the
 // column pair (j0 - 1 + 2 l, j0 + 2 l) of each of its rows plus the rows above and below,
@@ -61,13 +66,30 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
     zv[a] = *(const GDP_GLOBAL TP*)(z + o);
     dv[a] = *(const GDP_GLOBAL D2*)(dold + o);
   }
-  // XUPD: x of the output rows, loaded with the rest
+  // XUPD: x of the output rows, and for a double update the held direction (in dnew's
+  // buffer, read before this thread overwrites it), loaded with the rest when x is due
   GDP_GLOBAL double* xv = (GDP_GLOBAL double*)t.x;
-  D2 xo[XUPD ? R : 1];
+  D2 xo[XUPD ? R : 1], dh[XUPD ? R : 1];
+  int xdue = 0;  // 0: x is left alone, 1: x += alpha dold, 2: the held direction first
   if constexpr (XUPD) {
+    const int xp = st[p].xpend, xd = st[p].xdouble, fin = st[p].done;
+    xdue = !xp ? 0 : xd ? 2 : fin ? 1 : 0;
+    if (xdue) {
 #pragma unroll
-    for (int a = 1; a <= R; ++a)
-      xo[a - 1] = *(const GDP_GLOBAL D2*)(xv + (size_t)min(r0 - 1 + a, m - 1) * pitch + jc);
+      for (int a = 1; a <= R; ++a) {
+        const size_t o = (size_t)min(r0 - 1 + a, m - 1) * pitch + jc;
+        xo[a - 1] = *(const GDP_GLOBAL D2*)(xv + o);
+        if (xdue == 2) dh[a - 1] = *(const GDP_GLOBAL D2*)(dnew + o);
+      }
+    }
+    // the lead records whether this apply holds a single pending update back; no block
+    // of this launch reads xhold, and xalpha only for a double update, which is never
+    // held
+    if (t.lead && threadIdx.x == 0) {
+      const int hold = xp && !xd && !fin;
+      st[p].xhold = hold;
+      if (hold) st[p].xalpha = st[p].alpha;
+    }
   }
   uint64_t mb = 0;
   if constexpr (MASK) mb = load_mask_bits<FH>(t.mask, r0 - 1, gj, m, n, true);
@@ -84,16 +106,23 @@ __global__ __launch_bounds__(kThreads) void k_apply_s(const KTile* __restrict__ 
   const bool outx = xin && (gj >= t.j0 || gj == 0) && gj < t.j0 + kApplyCols;
   const bool outy = yin && gj + 1 < t.j0 + kApplyCols;
   if constexpr (XUPD) {
-    // the pending x += alpha d of the previous iteration (solvers.py:151), d = dold
-    if (st[p].xpend) {
+    // the pending x += alpha d of the previous iteration (solvers.py:151), d = dold, after
+    // the held one of the iteration before it
+    if (xdue) {
       const double al = st[p].alpha;
+      const double ah = xdue == 2 ? st[p].xalpha : 0.0;
 #pragma unroll
       for (int a = 1; a <= R; ++a) {
         const int gi = r0 - 1 + a;
         if (gi >= m) break;
         const size_t o = (size_t)gi * pitch + gj;
-        const double nx = __builtin_fma(al, dv[a].x, xo[a - 1].x);
-        const double ny = __builtin_fma(al, dv[a].y, xo[a - 1].y);
+        double nx = xo[a - 1].x, ny = xo[a - 1].y;
+        if (xdue == 2) {
+          nx = __builtin_fma(ah, dh[a - 1].x, nx);
+          ny = __builtin_fma(ah, dh[a - 1].y, ny);
+        }
+        nx = __builtin_fma(al, dv[a].x, nx);
+        ny = __builtin_fma(al, dv[a].y, ny);
         if (outx && outy) {
           *(GDP_GLOBAL D2*)(xv + o) = D2{nx, ny};
         } else {

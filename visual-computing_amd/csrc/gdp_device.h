@@ -59,6 +59,7 @@ struct ProblemState {
   double bmean;      // mean of b (Neumann projection)
   double true_rr;    // ||rhs - (-A) x||^2 recomputed
   double sumx;       // sum of x (Neumann zero-mean output)
+  double xalpha;     // alpha of the direction whose x update an apply held back (xhold)
   int32_t iter;
   int32_t converged; // recursive residual met the target
   int32_t done;      // converged or out of iterations: every kernel skips this problem
@@ -67,7 +68,12 @@ struct ProblemState {
   int32_t flexible;
   int32_t checked;   // done, true residual checked, output written (PCG)
   int32_t xpend;     // 1 + the direction buffer whose alpha d the last k_update left out
-                     // of x for the tail launch of the same iteration to add; 0: none
+                     // of x for the tail launch of the same iteration or the next apply
+                     // to add; 0: none
+  int32_t xhold;     // the last apply left a pending alpha d in x's debt (xalpha, and the
+                     // direction buffer the update after it does not write)
+  int32_t xdouble;   // the pending x update covers two directions: xalpha times the
+                     // buffer other than xpend's, then alpha times xpend's
 };
 
 // One workgroup's work item of a V-cycle kernel: everything the block needs about its

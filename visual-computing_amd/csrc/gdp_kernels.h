@@ -381,9 +381,13 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(const TailDesc* __restric
 // the CUs that the tail's one block per problem leaves idle, or, in a plan whose tails
 // leave too few CUs idle, for the next iteration's apply (k_apply_s<..., XUPD>), which
 // reads that direction anyway; d stays valid until the next iteration's apply writes
-// the other direction buffer.
+// the other direction buffer.  That apply adds it to x every other iteration only and
+// holds it back otherwise (xhold, xalpha): the update after it then reads the held
+// direction from the buffer it does not write, and adds both to x itself or, deferred
+// again, leaves both to the next apply (xdouble).
 // Each kernel reads only state fields written by earlier kernels and its lead block
-// writes only fields that no block of the same kernel reads.
+// writes only fields that no block of the same kernel reads (the apply's lead writes
+// xalpha only for a problem whose blocks do not read it in that launch).
 
 template <typename T, int TH, int TW, bool DEFER_X>
 __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ tiles,
@@ -400,8 +404,12 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
   GDP_GLOBAL double* r = (GDP_GLOBAL double*)t.r;
   const GDP_GLOBAL double* q = (const GDP_GLOBAL double*)t.q;
   const GDP_GLOBAL double* d = (const GDP_GLOBAL double*)(dcur ? t.d1 : t.d0);
+  // an x update the apply held back: xalpha times the other direction buffer, added
+  // before this iteration's (k_apply_s in gdp_apply.h)
+  const GDP_GLOBAL double* dh = (const GDP_GLOBAL double*)(dcur ? t.d0 : t.d1);
+  const bool held = !DEFER_X && st[p].xhold;
   constexpr int NO = (TH * TW) / kThreads;
-  double xl[NO], dl[NO], rl[NO], ql[NO];
+  double xl[NO], dl[NO], rl[NO], ql[NO], hl[NO];
   size_t ol[NO];
   bool il[NO];
   // branch-free loads (clamped in-bounds addresses; the points outside are not used), so
@@ -420,12 +428,16 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
     rl[it] = r[ol[it]];
     ql[it] = q[ol[it]];
   }
+  if (held) {
+#pragma unroll
+    for (int it = 0; it < NO; ++it) hl[it] = dh[ol[it]];
+  }
   // the finished-problem test comes after the reduction's barrier, which keeps the
   // loads above in front of it (a test before it lets the compiler sink them below the
   // state load)
   const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, sh, &bc);
   if (st[p].done) {
-    if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
+    if (t.lead && threadIdx.x == 0) st[p].xpend = st[p].xdouble = 0;
     return;
   }
   const double delta = st[p].delta;
@@ -437,14 +449,18 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
     st[p].iter += 1;
     st[p].restart = 0;
     st[p].xpend = DEFER_X ? dcur + 1 : 0;
+    // a deferred update after an apply that held one back leaves both to the next apply
+    st[p].xdouble = DEFER_X ? st[p].xhold : 0;
   }
+  const double ah = held ? st[p].xalpha : 0.0;
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < NO; ++it) {
     if (!il[it]) continue;
     // one rounding, as the fused update-down and finest up kernels form it
     const double rn = __builtin_fma(-alpha, ql[it], rl[it]);
-    if constexpr (!DEFER_X) x[ol[it]] = __builtin_fma(alpha, dl[it], xl[it]);
+    if constexpr (!DEFER_X)
+      x[ol[it]] = __builtin_fma(alpha, dl[it], held ? __builtin_fma(ah, hl[it], xl[it]) : xl[it]);
     r[ol[it]] = rn;
     b0[ol[it]] = T(-rn);
     acc += rn * rn;
@@ -670,8 +686,8 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
     S.done = S.converged || max_iter <= 0;
     S.restart = 0;
     S.checked = 0;
-    S.xpend = 0;
-    S.alpha = S.beta = S.delta = S.delta_old = 0.0;
+    S.xpend = S.xhold = S.xdouble = 0;
+    S.alpha = S.beta = S.delta = S.delta_old = S.xalpha = 0.0;
     S.rr = tot.x;
     S.max_iter = max_iter;
     S.flexible = flexible;

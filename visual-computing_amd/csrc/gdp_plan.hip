@@ -1313,12 +1313,13 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
 // preconditioner is left out and only the update's convergence test runs (the host runs
 // the preconditioner before continuing if a problem is still open).  next_apply: the next
 // kernel sequence on the stream starts with another iteration's apply (no true-residual
-// check in between).  x_pending: the preceding update left its x half to this apply (only
-// the profiler's byte count uses it; the kernel reads the state).  Returns through
-// *deferred whether this iteration's update leaves its x half to the next apply.
+// check in between).  x_pending: the x updates the preceding update left to this apply,
+// 0, 1 (which the apply holds back for the next update) or 2 (which it adds to x); only
+// the profiler's byte counts use it, the kernels read the state.  Returns through
+// *deferred the same for the next apply.
 template <typename T>
 int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool last,
-                       bool next_apply, bool x_pending, bool* deferred) {
+                       bool next_apply, int x_pending, int* deferred) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
@@ -1332,8 +1333,8 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
   const bool apply_x = apply_mode && !last && next_apply;
   // the Jacobi preconditioner publishes its <r, z> partials with the Krylov tiles
   {
-    // with the x update: x read (always) and written (when an update is pending)
-    const double xb = apply_mode ? (x_pending ? 16.0 : 8.0) : 0.0;
+    // with a double x update: x read and written, the held direction read
+    const double xb = apply_mode && x_pending == 2 ? 24.0 : 0.0;
     ProfScope ps(P, s, "apply", (24.0 + sizeof(T) + xb) * N);
     const KTile* at = P->at<KTile>(P->kapply.off);
     const int nt = P->kapply.count, jc = jacobi ? 1 : 0;
@@ -1355,12 +1356,14 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
     GDP_CHECK_LAUNCH();
   }
   const bool defer_x = tail_x || apply_x;
-  if (deferred) *deferred = apply_x;
+  // an update after an apply that held one back: both to the next apply, or both to x
+  const bool held = apply_mode && x_pending == 1;
+  if (deferred) *deferred = apply_x ? (held ? 2 : 1) : 0;
   // the update in the finest down kernel of the V-cycle that follows it
   if (defer_x && fused_update(P)) return enqueue_vcycle_t<T>(P, s, true, true, tail_x, true, din ^ 1);
   {
     ProfScope ps(P, s, defer_x ? "update (x deferred)" : "update",
-                 ((defer_x ? 24.0 : 48.0) + sizeof(T)) * N);
+                 ((defer_x ? 24.0 : held ? 56.0 : 48.0) + sizeof(T)) * N);
     if (defer_x) {
       GDP_KRY_DISPATCH(P->kry_menu,
                        (klaunch(k_update<T, TH, TW, true>, dim3(nb), dim3(kThreads), 0, s, t,
@@ -1384,14 +1387,14 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
 
 // Two PCG iterations.  more: another pair follows in the same launch sequence, so the
 // second iteration's update may leave its x half to that pair's first apply; a pair
-// followed by the true-residual check leaves x complete.  *pending: in, whether the
-// preceding update left its x half to this pair's first apply; out, the same for the
-// next pair.
+// followed by the true-residual check leaves x complete.  *pending: in, the x updates
+// the preceding update left to this pair's first apply (enqueue_pcg_iter_t); out, the
+// same for the next pair.
 int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false,
-                     bool more = false, bool* pending = nullptr) {
+                     bool more = false, int* pending = nullptr) {
   int rc;
   const bool more2 = more && !last;
-  bool in = pending && *pending, mid = false, out = false;
+  int in = pending ? *pending : 0, mid = 0, out = 0;
   if (P->fp32) {
     if ((rc = enqueue_pcg_iter_t<float>(P, s, 0, jacobi, false, true, in, &mid))) return rc;
     rc = enqueue_pcg_iter_t<float>(P, s, 1, jacobi, last, more2, mid, &out);
@@ -1634,7 +1637,7 @@ int launch_head(gdp_plan* P, hipStream_t s, bool jacobi, int pairs) {
   return launch_captured(P, s, v[at].second, [&](hipStream_t cs) {
     int rc;
     if ((rc = enqueue_start(P, cs, jacobi))) return rc;
-    bool pending = false;
+    int pending = 0;
     for (int k = 0; k < pairs; ++k)
       if ((rc = enqueue_pcg_pair(P, cs, jacobi, k == pairs - 1, k < pairs - 1, &pending)))
         return rc;

@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
     const int p = t.problem;
     const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, shc, &bcc);
     if (st[p].done) {
-      if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
+      if (t.lead && threadIdx.x == 0) st[p].xpend = st[p].xdouble = 0;
       return;
     }
     const double delta = st[p].delta;
@@ -331,6 +331,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown_blk(const VTile* __restrict_
       st[p].iter += 1;
       st[p].restart = 0;
       st[p].xpend = dcur + 1;  // the x half goes to the tail launch or the next apply
+      st[p].xdouble = st[p].xhold;  // with the one that apply held back, if it did
     }
     // the updated residual of the wave's rows: b0 = -rn and rn^2 at the owned points
     // (rows and columns as the pre-smoothed iterate's), -rn into the strip rows

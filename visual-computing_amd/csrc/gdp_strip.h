@@ -810,7 +810,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
     const int p = t.problem;
     const double2 a2 = reduce_slots<kThreads>(apart, SlotRange{t.abeg, t.acnt}, sh, &bc);
     if (st[p].done) {
-      if (t.lead && threadIdx.x == 0) st[p].xpend = 0;
+      if (t.lead && threadIdx.x == 0) st[p].xpend = st[p].xdouble = 0;
       return;
     }
     const double delta = st[p].delta;
@@ -822,6 +822,7 @@ __global__ __launch_bounds__(kThreads) void k_sdown_stream(const VTile* __restri
       st[p].iter += 1;
       st[p].restart = 0;
       st[p].xpend = dcur + 1;  // the x half goes to the tail launch or the next apply
+      st[p].xdouble = st[p].xhold;  // with the one that apply held back, if it did
     }
     double acc = 0.0;
     if (work) {

@@ -329,6 +329,36 @@ def test_apply_carried_x_update_at_other_options(gd, precision, flexible):
         assert torch.equal(together[k], alone[0]), (k, shapes[k])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_residual_replacement_below_the_rounding_floor(gd, precision):
+    """Tolerances near and below the FP64 rounding floor, where the recursive residual can
+    meet the target while the true one does not: the device check then reopens the problem
+    from its true residual (at most 4 times per problem) and afterwards accepts it
+    unconverged.  Every report carries the true residual of the solution it returns (the
+    oracle's b - A u), converged says whether that met the tolerance, and each problem
+    gets the same iterations and bits batched and alone."""
+    import torch
+
+    shapes = [(130, 70), (64, 64), (257, 129)]
+    bs = [G.neumann_problem(s, seed=77 + k)[0] for k, s in enumerate(shapes)]
+    for tol in (1e-13, 1e-15):
+        opts = gd.PlanOptions(precision=precision, tolerance=tol, max_iterations=80)
+        u, reps = gd.Plan(shapes, boundary="neumann", options=opts).mgcg(
+            [torch.from_numpy(b).cuda() for b in bs])
+        for k, b in enumerate(bs):
+            true = OS.relative_residual(u[k].cpu().numpy(), b, 1.0, bc=OS.NEUMANN)
+            # near the floor the two residual computations differ by their own rounding
+            assert reps[k].relative_residual == pytest.approx(true, rel=0.05, abs=2e-15)
+            assert reps[k].converged == (reps[k].relative_residual <= tol)
+            assert reps[k].iterations <= 80
+        assert any(not r.converged for r in reps) or tol > 1e-14
+        for k, b in enumerate(bs):
+            alone, rep1 = gd.Plan([shapes[k]], boundary="neumann", options=opts).mgcg(
+                [torch.from_numpy(b).cuda()])
+            assert rep1[0].iterations == reps[k].iterations, (tol, k)
+            assert torch.equal(alone[0], u[k]), (tol, k)
+
+
 @pytest.mark.parametrize("bc", ["dirichlet", "neumann"])
 @pytest.mark.parametrize("menu", [0, 1, 2, 3])
 def test_every_tile_menu_entry_matches_oracle(gd, bc, menu):

@@ -74,7 +74,9 @@ struct ProblemState {
                      // direction buffer the update after it does not write)
   int32_t xdouble;   // the pending x update covers two directions: xalpha times the
                      // buffer other than xpend's, then alpha times xpend's
+  int32_t restarts;  // residual replacements so far (at most kMaxRestarts)
 };
+constexpr int kMaxRestarts = 4;
 
 // One workgroup's work item of a V-cycle kernel: everything the block needs about its
 // level and the next coarser one, so a block starts with a single descriptor load.

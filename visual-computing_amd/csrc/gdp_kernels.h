@@ -468,17 +468,6 @@ __global__ __launch_bounds__(kThreads) void k_update(const KTile* __restrict__ t
   write_partial<kThreads>(acc, 0.0, upart, t.slot, sh);
 }
 
-// Convergence test of an update whose preconditioner is not run (the last update of the
-// first batch of a solve, where the test usually ends the solve): one block per problem.
-__global__ __launch_bounds__(kThreads) void k_check_conv(ProblemState* __restrict__ st,
-                                                         const double2* __restrict__ upart,
-                                                         const SlotRange* __restrict__ rk) {
-  const int p = blockIdx.x;
-  if (st[p].done) return;
-  __shared__ double sh[2 * kThreads / 64];
-  __shared__ double2 bc;
-  check_converged<kThreads>(st, p, upart, rk[p], true, sh, &bc);
-}
 
 // Jacobi preconditioner of the reference "cg" method (solvers.py:114-117):
 // z = r / (4 / h^2) and its <r, z> partials (consumed by k_apply_s with beta = delta'/delta).
@@ -599,18 +588,37 @@ __device__ inline bool skip_problem(const ProblemState& S, int respect) {
 // r_true = (b - mean) - L x (FP64): partial sums of ||r_true||^2 and of x; the V-cycle
 // RHS becomes r_true and, in mode 0 (PCG), r = -r_true so an unconverged problem can
 // restart from it.
+// check_conv (with respect_done 2, after an update whose preconditioner is not run): the
+// update's convergence test first, in every block of a problem that was open, from the
+// ||r||^2 partials (slot ranges rk); the lead block records it, and the problems it
+// finishes get their true residual in the same launch.  Whether a block tests is decided
+// once for the block from thread 0's read of the state: another block's lead may set
+// done meanwhile, to the value this block's own test gives.
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__ tiles,
                                                        const IoDesc* __restrict__ io,
-                                                       const ProblemState* __restrict__ st,
+                                                       ProblemState* __restrict__ st,
                                                        double2* __restrict__ spart, int mode,
-                                                       int respect_done) {
+                                                       int respect_done,
+                                                       const double2* __restrict__ upart,
+                                                       const SlotRange* __restrict__ rk,
+                                                       int check_conv) {
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
-  if (skip_problem(st[p], respect_done)) return;
   constexpr int RH = TH + 2, RW = TW + 2;
   __shared__ double sx[RH * RW];
   __shared__ double sh[2 * kThreads / 64];
+  if (check_conv) {
+    __shared__ double2 bc;
+    __shared__ int flags;
+    if (threadIdx.x == 0) flags = (st[p].done ? 1 : 0) | (st[p].checked ? 2 : 0);
+    __syncthreads();
+    const int f = flags;
+    if (f & 2) return;
+    if (!(f & 1) && !check_converged<kThreads>(st, p, upart, rk[p], t.lead, sh, &bc)) return;
+  } else if (skip_problem(st[p], respect_done)) {
+    return;
+  }
   const int m = t.m, n = t.n, pitch = t.pitch;
   const double mean = st[p].bmean;
   const double* __restrict__ b = io[p].b;
@@ -659,7 +667,12 @@ __global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__
 //   kind 0 (after k_init_stats): bmean
 //   kind 1 (after k_init): ||b||^2, target, fresh iteration state
 //   kind 2 (after k_residual): true ||r||^2, sum(x); MG modes also record the history
-//          and test rel < tolerance (multigrid.py:250-254)
+//          and test rel < tolerance (multigrid.py:250-254); the check of a PCG batch
+//          (reopen 1) also takes the restart decision: a problem whose recursive
+//          residual converged but whose true residual did not is reopened with a fresh
+//          Krylov sequence (residual replacement) if it has iterations and restarts
+//          (kMaxRestarts per problem, so a problem's course does not depend on its batch)
+//          left; every other one is marked checked
 __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restrict__ kds,
                                                      ProblemState* __restrict__ st,
                                                      const double2* __restrict__ spart,
@@ -667,7 +680,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
                                                      int mode, int project, double tol,
                                                      int max_iter, int flexible, int zero_done,
                                                      int respect_done, double* history,
-                                                     int hist_stride) {
+                                                     int hist_stride, int reopen) {
   const int p = blockIdx.x;
   __shared__ double sh[2 * kThreads / 64];
   __shared__ double2 bc;
@@ -686,7 +699,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
     S.done = S.converged || max_iter <= 0;
     S.restart = 0;
     S.checked = 0;
-    S.xpend = S.xhold = S.xdouble = 0;
+    S.xpend = S.xhold = S.xdouble = S.restarts = 0;
     S.alpha = S.beta = S.delta = S.delta_old = S.xalpha = 0.0;
     S.rr = tot.x;
     S.max_iter = max_iter;
@@ -694,6 +707,19 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
   } else {
     S.true_rr = tot.x;
     S.sumx = tot.y;
+    if (reopen) {
+      // a done, unchecked problem (respect_done 2)
+      const bool bad = tot.x > S.target2 && S.bnorm2 > 0.0;
+      if (bad && S.iter < S.max_iter && S.restarts < kMaxRestarts) {
+        S.converged = 0;
+        S.done = 0;
+        S.restart = 1;
+        S.restarts += 1;
+      } else {
+        if (bad) S.converged = 0;
+        S.checked = 1;
+      }
+    }
     if (mode >= 1) {
       const double rel = S.bnorm2 > 0.0 ? sqrt(tot.x / S.bnorm2) : 0.0;
       if (mode == 2) S.iter += 1;
@@ -757,33 +783,6 @@ __global__ __launch_bounds__(kThreads) void k_finalize(const KTile* __restrict__
     if (gi >= t.m || gj >= t.n) continue;
     u[(size_t)gi * t.n + gj] = xv[it] - mean;
   }
-}
-
-// Restart bookkeeping after the FP64 true-residual check, for the problems that are
-// done and not yet checked: one whose recursive residual converged but whose true
-// residual did not is reopened with a fresh Krylov sequence (residual replacement) if
-// it has iterations left and allow is set; every other one is marked checked.  One
-// block; *reopened = number of problems reopened (a plain store, no atomics).
-__global__ __launch_bounds__(kThreads) void k_reopen(ProblemState* __restrict__ st, int count,
-                                                     int allow, int* __restrict__ reopened) {
-  __shared__ double sh[kThreads / 64];
-  int mine = 0;
-  for (int p = threadIdx.x; p < count; p += kThreads) {
-    ProblemState& S = st[p];
-    if (!S.done || S.checked) continue;
-    const bool bad = S.true_rr > S.target2 && S.bnorm2 > 0.0;
-    if (bad && allow && S.iter < S.max_iter) {
-      S.converged = 0;
-      S.done = 0;
-      S.restart = 1;
-      ++mine;
-    } else {
-      if (bad) S.converged = 0;
-      S.checked = 1;
-    }
-  }
-  const double tot = block_sum<kThreads>(double(mine), sh);
-  if (threadIdx.x == 0) *reopened = int(tot);
 }
 
 }  // namespace gdp

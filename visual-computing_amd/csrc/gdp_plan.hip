@@ -171,7 +171,7 @@ struct gdp_plan {
   size_t off_apart = 0, off_upart = 0, off_zpart = 0, off_spart = 0;
   size_t off_ranges_kry = 0, off_ranges_up0 = 0, off_ranges_down0 = 0, off_io = 0,
          off_init = 0;
-  size_t off_hist = 0, off_flag = 0;
+  size_t off_hist = 0;
   size_t off_tiles = 0;     // start of the tile tables region
   size_t tiles_bytes = 0;   // reserved bytes for the tile tables
   Table down[kMaxDepth], up[kMaxDepth], kry, kapply;
@@ -180,8 +180,7 @@ struct gdp_plan {
   IoDesc* h_io = nullptr;
   const double** h_init = nullptr;
   ProblemState* h_state = nullptr;
-  int* h_flag = nullptr;
-  void* h_block = nullptr;     // pinned block holding h_io, h_init, h_state, h_flag
+  void* h_block = nullptr;     // pinned block holding h_io, h_init, h_state
   size_t h_block_bytes = 0;
   hipStream_t cap = nullptr;
   hipEvent_t io_ev = nullptr;  // the last upload of h_io / h_init has been consumed
@@ -225,7 +224,6 @@ struct gdp_plan {
   IoDesc* d_io() const { return at<IoDesc>(off_io); }
   const double** d_init() const { return at<const double*>(off_init); }
   double* d_hist() const { return at<double>(off_hist); }
-  int* d_flag() const { return at<int>(off_flag); }
   int count() const { return int(probs.size()); }
 };
 
@@ -627,10 +625,7 @@ int gdp_plan_create(const gdp_problem* problems, int count, const gdp_options* o
   P->off_levels = take(sizeof(LevelDesc) * lvl_total);
   P->off_kry = take(sizeof(KrylovDesc) * P_);
   P->off_tails = take(sizeof(TailDesc) * P_);
-  // the state, then (at the 64-byte boundary the host's pinned copy also uses) the
-  // check's flag: one device-to-host copy reads both
-  P->off_state = take(state_span(P_) + 64);
-  P->off_flag = P->off_state + state_span(P_);
+  P->off_state = take(state_span(P_));
   P->off_ranges_kry = take(sizeof(SlotRange) * P_);
   P->off_ranges_up0 = take(sizeof(SlotRange) * P_);
   P->off_ranges_down0 = take(sizeof(SlotRange) * P_);
@@ -771,14 +766,12 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   if (!P->h_block) {
     auto a64 = [](size_t x) { return (x + 63) & ~size_t(63); };
     const size_t b_io = a64(sizeof(IoDesc) * NP), b_init = a64(sizeof(double*) * NP);
-    const size_t b_state = state_span(NP);  // the device layout's offset of the flag
-    P->h_block_bytes = b_io + b_init + b_state + 64;
+    P->h_block_bytes = b_io + b_init + state_span(NP);
     GDP_HIP(pinned_get(P->h_block_bytes, &P->h_block));
     char* hb = static_cast<char*>(P->h_block);
     P->h_io = reinterpret_cast<IoDesc*>(hb);
     P->h_init = reinterpret_cast<const double**>(hb + b_io);
     P->h_state = reinterpret_cast<ProblemState*>(hb + b_io + b_init);
-    P->h_flag = reinterpret_cast<int*>(hb + b_io + b_init + b_state);
   }
   if (!P->cap) GDP_HIP(hipStreamCreateWithFlags(&P->cap, hipStreamNonBlocking));
   if (!P->io_ev) GDP_HIP(hipEventCreateWithFlags(&P->io_ev, hipEventDisableTiming));
@@ -1090,7 +1083,6 @@ int gdp_plan_bind(gdp_plan* P, void* workspace, size_t bytes) {
   GDP_HIP(hipMemsetAsync(P->d_state(), 0, sizeof(ProblemState) * NP, P->cap));
   for (size_t o : {P->off_apart, P->off_upart, P->off_zpart, P->off_spart})
     GDP_HIP(hipMemsetAsync(P->ws + o, 0, sizeof(double2) * P->max_slots, P->cap));
-  GDP_HIP(hipMemsetAsync(P->d_flag(), 0, sizeof(int) * 4, P->cap));
 
   // the tail kernels may use more than the default dynamic LDS
   if (P->n_stail) GDP_HIP(raise_smem(P->fp32 ? 0 : 1, P->stail_smem));
@@ -1310,8 +1302,9 @@ int enqueue_jacobi_t(gdp_plan* P, hipStream_t s, bool check_conv) {
 }
 
 // One PCG iteration: direction and operator, update, preconditioner.  last: the
-// preconditioner is left out and only the update's convergence test runs (the host runs
-// the preconditioner before continuing if a problem is still open).  next_apply: the next
+// preconditioner is left out, and the update's convergence test runs in the check that
+// follows (the host runs the preconditioner before continuing if a problem is still
+// open).  next_apply: the next
 // kernel sequence on the stream starts with another iteration's apply (no true-residual
 // check in between).  x_pending: the x updates the preceding update left to this apply,
 // 0, 1 (which the apply holds back for the next update) or 2 (which it adds to x); only
@@ -1375,12 +1368,7 @@ int enqueue_pcg_iter_t(gdp_plan* P, hipStream_t s, int din, bool jacobi, bool la
     }
     GDP_CHECK_LAUNCH();
   }
-  if (last) {
-    ProfScope ps(P, s, "conv test", 16.0 * P->kry.count);
-    klaunch(k_check_conv, dim3(P->count()), dim3(kThreads), 0, s, P->d_state(), P->d_upart(), P->d_rkry());
-    GDP_CHECK_LAUNCH();
-    return GDP_OK;
-  }
+  if (last) return GDP_OK;  // the check that follows tests its convergence
   if (jacobi) return enqueue_jacobi_t<T>(P, s, true);
   return enqueue_vcycle_t<T>(P, s, true, true, tail_x);
 }
@@ -1411,6 +1399,7 @@ struct ScalarArgs {
   double tol = 0.0;
   int max_iter = 0, zero_done = 0, respect_done = 0;
   bool hist = false;
+  int reopen = 0;  // kind 2 of a PCG check: the restart decision
 };
 
 int enqueue_scalar(gdp_plan* P, hipStream_t s, const ScalarArgs& a) {
@@ -1419,7 +1408,7 @@ int enqueue_scalar(gdp_plan* P, hipStream_t s, const ScalarArgs& a) {
   klaunch(k_scalar, dim3(P->count()), dim3(kThreads), 0, s,
       P->d_kry(), P->d_state(), P->d_spart(), P->d_rkry(), a.kind, a.mode, a.project, a.tol,
       a.max_iter, P->opt.flexible, a.zero_done, a.respect_done, a.hist ? P->d_hist() : nullptr,
-      P->opt.max_iterations + 2);
+      P->opt.max_iterations + 2, a.reopen);
   GDP_CHECK_LAUNCH();
   return GDP_OK;
 }
@@ -1457,14 +1446,16 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
 }
 
 template <typename T>
-int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, bool hist) {
+int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, bool hist,
+                       bool reopen, bool check_conv) {
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   {
     ProfScope ps(P, s, "residual", (24.0 + sizeof(T)) * krylov_points(P));
     GDP_KRY_DISPATCH(P->kry_menu,
                      (klaunch(k_residual<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
-                         t, P->d_io(), P->d_state(), P->d_spart(), mode, respect_done)));
+                         t, P->d_io(), P->d_state(), P->d_spart(), mode, respect_done,
+                         P->d_upart(), P->d_rkry(), check_conv ? 1 : 0)));
     GDP_CHECK_LAUNCH();
   }
   ScalarArgs a;
@@ -1473,6 +1464,7 @@ int enqueue_residual_t(gdp_plan* P, hipStream_t s, int mode, int respect_done, b
   a.tol = P->opt.tolerance;
   a.respect_done = respect_done;
   a.hist = hist;
+  a.reopen = reopen ? 1 : 0;
   return enqueue_scalar(P, s, a);
 }
 
@@ -1531,12 +1523,8 @@ int stage_io(gdp_plan* P, const double* const* b, double* const* u, const double
   return GDP_OK;
 }
 
-int read_state(gdp_plan* P, hipStream_t s, bool with_flag = false) {
-  // with the flag: one copy of the state and the flag behind it (the same offset on both
-  // sides), not two
-  GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(),
-                         with_flag ? state_span(P->count()) + sizeof(int)
-                                   : sizeof(ProblemState) * P->count(),
+int read_state(gdp_plan* P, hipStream_t s) {
+  GDP_HIP(hipMemcpyAsync(P->h_state, P->d_state(), sizeof(ProblemState) * P->count(),
                          hipMemcpyDeviceToHost, s));
   GDP_HIP(hipStreamSynchronize(s));
   if (P->prof) prof_collect(P);
@@ -1547,6 +1535,15 @@ bool all_checked(const gdp_plan* P) {
   for (int p = 0; p < P->count(); ++p)
     if (!P->h_state[p].checked) return false;
   return true;
+}
+
+// Problems the last check reopened: a reopened problem is open with restart set, which
+// every update clears, and a check follows a pair of updates.
+int reopened(const gdp_plan* P) {
+  int k = 0;
+  for (int p = 0; p < P->count(); ++p)
+    k += !P->h_state[p].done && P->h_state[p].restart;
+  return k;
 }
 
 bool all_done(const gdp_plan* P) {
@@ -1604,13 +1601,10 @@ int enqueue_start(gdp_plan* P, hipStream_t s, bool jacobi) {
 }
 
 // The device-side end of a batch of pairs: FP64 true residual of the problems that
-// stopped, reopen (residual replacement, if allowed) or accept, and the output of the
-// accepted ones.
-int enqueue_check(gdp_plan* P, hipStream_t s, bool allow_reopen) {
+// stopped, reopen (residual replacement) or accept, and the output of the accepted ones.
+int enqueue_check(gdp_plan* P, hipStream_t s, bool check_conv = false) {
   int rc;
-  if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false))) return rc;
-  klaunch(k_reopen, dim3(1), dim3(kThreads), 0, s, P->d_state(), P->count(), allow_reopen ? 1 : 0, P->d_flag());
-  GDP_CHECK_LAUNCH();
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 0, 2, false, true, check_conv))) return rc;
   return enqueue_finalize(P, s, 1, 1);
 }
 
@@ -1660,12 +1654,10 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
   // solve.  Otherwise it continues one pair at a time with the same check.
   const int first_pairs = std::max(1, (P->last_iters + 1) / 2);
   if ((rc = launch_head(P, s, jacobi, first_pairs))) return rc;
-  if ((rc = read_state(P, s, true))) return rc;
-  int restarts = 0;
+  if ((rc = read_state(P, s))) return rc;
   bool precondition = true;  // the first batch left its last preconditioner out
   while (!all_checked(P)) {
-    if (P->h_flag[0] > 0) ++restarts;
-    if (P->h_flag[0] > 0 || precondition) {
+    if (reopened(P) > 0 || precondition) {
       // z = M r of the open problems: those the first batch left before their
       // preconditioner, and reopened ones, which restart from their true residual (no
       // convergence test: the recursive residual they stopped on is stale)
@@ -1675,8 +1667,8 @@ int run_pcg(gdp_plan* P, const double* const* b, double* const* u, gdp_report* r
     }
     precondition = false;
     if ((rc = launch_pair(P, s, jacobi))) return rc;
-    if ((rc = enqueue_check(P, s, restarts < 4))) return rc;
-    if ((rc = read_state(P, s, true))) return rc;
+    if ((rc = enqueue_check(P, s))) return rc;
+    if ((rc = read_state(P, s))) return rc;
   }
   int mx = 0;
   for (int p = 0; p < P->count(); ++p) mx = std::max(mx, P->h_state[p].iter);
@@ -1737,7 +1729,7 @@ int gdp_vcycle(gdp_plan* P, const double* const* u_in, const double* const* b,
   int rc;
   if ((rc = stage_io(P, b, u_out, u_in, s))) return rc;
   if ((rc = GDP_T(enqueue_init_t)(P, s, 1, 0, 0.0, 1, true, 0))) return rc;
-  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 0, false))) return rc;
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 0, false, false, false))) return rc;
   if ((rc = enqueue_vcycle(P, s, false, false))) return rc;
   if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
   return enqueue_finalize(P, s, 0);
@@ -1755,14 +1747,14 @@ int gdp_mg_solve(gdp_plan* P, const double* const* b, const double* const* initi
   if ((rc = stage_io(P, b, u, initial, s))) return rc;
   if ((rc = GDP_T(enqueue_init_t)(P, s, 1, 1, 0.0, o.max_iterations, initial != nullptr, 1)))
     return rc;
-  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 1, true))) return rc;
+  if ((rc = GDP_T(enqueue_residual_t)(P, s, 1, 1, true, false, false))) return rc;
   int cycles = 0;
   int batch = std::max(1, P->last_iters);
   for (;;) {
     for (int k = 0; k < batch; ++k) {
       if ((rc = enqueue_vcycle(P, s, false, false))) return rc;
       if ((rc = GDP_T(enqueue_correct_t)(P, s))) return rc;
-      if ((rc = GDP_T(enqueue_residual_t)(P, s, 2, 1, true))) return rc;
+      if ((rc = GDP_T(enqueue_residual_t)(P, s, 2, 1, true, false, false))) return rc;
     }
     cycles += batch;
     batch = 1;

@@ -533,20 +533,26 @@ __global__ __launch_bounds__(kThreads) void k_init_stats(const KTile* __restrict
 
 // Set-up pass 2.  mode 0 (PCG): r = -(b - mean), x = d = q = 0, V-cycle RHS b - mean.
 // mode 1 (MG iteration): x = initial guess (or 0).  Partial sums of (b - mean)^2.
+// The mean (project: of a Neumann problem, else 0) comes from k_init_stats' partial sums
+// (bpart, slot ranges rk), reduced by every block of the problem; the lead records it.
 template <typename T, int TH, int TW>
 __global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ tiles,
                                                    const IoDesc* __restrict__ io,
                                                    const double* const* __restrict__ initial,
-                                                   const ProblemState* __restrict__ st,
-                                                   double2* __restrict__ spart, int mode) {
+                                                   ProblemState* __restrict__ st,
+                                                   double2* __restrict__ spart, int mode,
+                                                   const double2* __restrict__ bpart,
+                                                   const SlotRange* __restrict__ rk,
+                                                   int project) {
   const KTile t = tiles[blockIdx.x];
   const int p = t.problem;
   __shared__ double sh[2 * kThreads / 64];
+  __shared__ double2 bc;
   const double* __restrict__ b = io[p].b;
   const double* __restrict__ x0 = initial ? initial[p] : nullptr;
   T* __restrict__ b0 = static_cast<T*>(t.b0);
   // branch-free loads of b (clamped in-bounds addresses), all issued before the first
-  // wait and before the state load of the mean
+  // wait, the reduction of the mean's partial sums
   constexpr int NI = (TH * TW) / kThreads;
   double bv[NI];
 #pragma unroll
@@ -555,7 +561,10 @@ __global__ __launch_bounds__(kThreads) void k_init(const KTile* __restrict__ til
     const int a = k / TW, c = k - a * TW;
     bv[it] = b[(size_t)min(t.i0 + a, t.m - 1) * t.n + min(t.j0 + c, t.n - 1)];
   }
-  const double mean = st[p].bmean;
+  const double2 bs = reduce_slots<kThreads>(bpart, rk[p], sh, &bc);
+  const double mean =
+      (project && t.bc == kNeumann) ? bs.x / (double(t.m) * double(t.n)) : 0.0;
+  if (t.lead && threadIdx.x == 0) st[p].bmean = mean;
   double acc = 0.0;
 #pragma unroll
   for (int it = 0; it < NI; ++it) {
@@ -664,7 +673,6 @@ __global__ __launch_bounds__(kThreads) void k_residual(const KTile* __restrict__
 }
 
 // One block per problem: reduce a set-up / residual pass and update the state.
-//   kind 0 (after k_init_stats): bmean
 //   kind 1 (after k_init): ||b||^2, target, fresh iteration state
 //   kind 2 (after k_residual): true ||r||^2, sum(x); MG modes also record the history
 //          and test rel < tolerance (multigrid.py:250-254); the check of a PCG batch
@@ -677,7 +685,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
                                                      ProblemState* __restrict__ st,
                                                      const double2* __restrict__ spart,
                                                      const SlotRange* __restrict__ rk, int kind,
-                                                     int mode, int project, double tol,
+                                                     int mode, double tol,
                                                      int max_iter, int flexible, int zero_done,
                                                      int respect_done, double* history,
                                                      int hist_stride, int reopen) {
@@ -689,9 +697,7 @@ __global__ __launch_bounds__(kThreads) void k_scalar(const KrylovDesc* __restric
   if (threadIdx.x != 0) return;
   ProblemState& S = st[p];
   const KrylovDesc& K = kds[p];
-  if (kind == 0) {
-    S.bmean = (project && K.bc == kNeumann) ? tot.x / (double(K.m) * double(K.n)) : 0.0;
-  } else if (kind == 1) {
+  if (kind == 1) {
     S.bnorm2 = tot.x;
     S.target2 = tol * tol * tot.x;
     S.iter = 0;

@@ -1395,7 +1395,7 @@ int enqueue_pcg_pair(gdp_plan* P, hipStream_t s, bool jacobi, bool last = false,
 }
 
 struct ScalarArgs {
-  int kind = 0, mode = 0, project = 0;
+  int kind = 1, mode = 0;
   double tol = 0.0;
   int max_iter = 0, zero_done = 0, respect_done = 0;
   bool hist = false;
@@ -1403,10 +1403,9 @@ struct ScalarArgs {
 };
 
 int enqueue_scalar(gdp_plan* P, hipStream_t s, const ScalarArgs& a) {
-  static const char* names[3] = {"scalar stats", "scalar init", "scalar residual"};
-  ProfScope ps(P, s, names[a.kind], 16.0 * P->kry.count);
+  ProfScope ps(P, s, a.kind == 1 ? "scalar init" : "scalar residual", 16.0 * P->kry.count);
   klaunch(k_scalar, dim3(P->count()), dim3(kThreads), 0, s,
-      P->d_kry(), P->d_state(), P->d_spart(), P->d_rkry(), a.kind, a.mode, a.project, a.tol,
+      P->d_kry(), P->d_state(), P->d_spart(), P->d_rkry(), a.kind, a.mode, a.tol,
       a.max_iter, P->opt.flexible, a.zero_done, a.respect_done, a.hist ? P->d_hist() : nullptr,
       P->opt.max_iterations + 2, a.reopen);
   GDP_CHECK_LAUNCH();
@@ -1419,25 +1418,24 @@ int enqueue_init_t(gdp_plan* P, hipStream_t s, int mode, int project, double tol
   const int nb = P->kry.count;
   const KTile* t = P->at<KTile>(P->kry.off);
   const double N = krylov_points(P);
-  int rc;
   {
+    // the partial sums of b go to the apply's partial-sum slots, which no kernel reads
+    // before the first apply writes them, so that k_init can reduce them while it writes
+    // its own to spart
     ProfScope ps(P, s, "init_stats", 8.0 * N);
     GDP_KRY_DISPATCH(P->kry_menu, (klaunch(k_init_stats<TH, TW>, dim3(nb), dim3(kThreads), 0, s,
-                                      t, P->d_io(), P->d_spart())));
+                                      t, P->d_io(), P->d_apart())));
     GDP_CHECK_LAUNCH();
   }
-  ScalarArgs a;
-  a.kind = 0;
-  a.project = project;
-  if ((rc = enqueue_scalar(P, s, a))) return rc;
   {
     ProfScope ps(P, s, "init", (mode == 0 ? 24.0 + sizeof(T) : 16.0) * N);
     GDP_KRY_DISPATCH(P->kry_menu,
                      (klaunch(k_init<T, TH, TW>, dim3(nb), dim3(kThreads), 0, s,
                          t, P->d_io(), with_init ? P->d_init() : nullptr, P->d_state(),
-                         P->d_spart(), mode)));
+                         P->d_spart(), mode, P->d_apart(), P->d_rkry(), project)));
     GDP_CHECK_LAUNCH();
   }
+  ScalarArgs a;
   a.kind = 1;
   a.tol = tol;
   a.max_iter = max_it;

@@ -136,6 +136,33 @@ def test_masked_and_unmasked_problems_in_one_plan(gd, precision):
             assert (us[k] - ua).abs().max().item() <= 1e-6 * scale
 
 
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_masked_batch_with_the_apply_carried_x_update(gd, precision):
+    """200 masked Dirichlet problems in one plan: too many Krylov points for the CUs the
+    tails leave free, so the masked applies carry the x update (every other iteration,
+    two directions at a time), where a problem alone has it added by its tail launch.
+    Sampled problems are bit for bit their solve alone, and converge."""
+    import torch
+
+    from gdpoisson.plan import Plan, PlanOptions
+
+    shapes = G.batch_shapes(count=200, seed=4600, low=200, high=300)
+    rng = np.random.default_rng(47)
+    masks, bs = [], []
+    for k, s in enumerate(shapes):
+        mk = G.blob_mask(s, seed=4700 + k, sigma=6.0) if k % 2 else _disc(s, min(s) / 3)
+        masks.append(torch.from_numpy(mk.astype(np.uint8)).cuda())
+        bs.append(torch.from_numpy(rng.standard_normal(s) * mk).cuda())
+    opts = PlanOptions(precision=precision)
+    us, reps = Plan(shapes, "dirichlet", 1.0, opts, masks=masks).mgcg(bs)
+    assert all(r.converged and r.relative_residual <= opts.tolerance for r in reps)
+    for k in (0, 1, 101, len(shapes) - 1):
+        (ua,), (ra,) = Plan([shapes[k]], "dirichlet", 1.0, opts, masks=[masks[k]]).mgcg([bs[k]])
+        assert ra.iterations == reps[k].iterations, k
+        assert torch.equal(us[k], ua), k
+        assert torch.count_nonzero(us[k][masks[k] == 0]).item() == 0
+
+
 def test_masked_solve_of_an_empty_selection_keeps_the_image(gd):
     """No selected pixel off the border (poisson.py:183-184): the image comes back, no
     reports."""

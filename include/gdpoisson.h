@@ -280,7 +280,11 @@ int gdp_mg_solve(gdp_plan* plan, const double* const* b, const double* const* in
                  double* const* u, gdp_report* reports, double* residual_history,
                  void* stream);
 /* MG-preconditioned CG on -A u = -b; Neumann problems are solved for the zero-mean
-   compatible part of b and return the zero-mean solution.  Synchronises. */
+   compatible part of b and return the zero-mean solution.  A problem whose recursive
+   residual meets the tolerance while its FP64 true residual does not restarts from the
+   true residual, at most 4 times (per problem: a problem's iterations and result do not
+   depend on the other problems of the plan); reports carry the true residual.
+   Synchronises. */
 int gdp_mgcg_solve(gdp_plan* plan, const double* const* b, double* const* u,
                    gdp_report* reports, void* stream);
 /* Jacobi-preconditioned CG (the reference's "cg" method).  Synchronises. */

@@ -1180,6 +1180,12 @@ void prof_collect(gdp_plan* P) {
 // more, the x update would outlast the tail on too few CUs (a batch of about as many
 // problems as CUs), and the whole-chip k_update does it instead.
 constexpr double kXBytesPerFreeCu = 1048576.0;
+// The x-update blocks of a tail launch each take about this much of the x stream: fewer
+// blocks than free CUs where the stream is small.  The stream's rate saturates well
+// below one block per free CU, and its traffic slows the tail's own latency-bound
+// chain: at 1024x1024 RGB (75 MB) 128-152 blocks made the tail launch 1.1-1.8 us shorter
+// than 253, and 108 or 64 blocks made it 2 or 13 us longer.
+constexpr double kXBytesPerBlock = 524288.0;
 bool defer_x_update(const gdp_plan* P) {
   const int free_cu = P->n_cu - P->n_stail;
   if (P->n_stail <= 0 || free_cu <= 0) return false;
@@ -1239,12 +1245,14 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
                  rk, P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
                  o.post_smoothing, P->csweeps, order};
       if (xupd) {
-        // one block on each CU the tail blocks leave free, at most one per kXItems items
+        // one block per kXBytesPerBlock of the x stream, on CUs the tail blocks leave
+        // free, at most one per kXItems items
         const Shape2 ks = kKryMenu[P->kry_menu];
         const int items =
             P->kry.count * ((ks.h * ks.w / 2 + kStailBlockThreads - 1) / kStailBlockThreads);
-        a.xblocks =
-            std::max(1, std::min(P->n_cu - ns, (items + kStailXItems - 1) / kStailXItems));
+        const int by_bytes = int(24.0 * krylov_points(P) / kXBytesPerBlock) + 1;
+        a.xblocks = std::max(1, std::min(std::min(P->n_cu - ns, by_bytes),
+                                         (items + kStailXItems - 1) / kStailXItems));
         a.xt = P->at<KTile>(P->kry.off);
         a.nxt = P->kry.count;
         a.xth = ks.h;

@@ -87,10 +87,11 @@ def autotune(plan: Plan, b: Sequence[torch.Tensor] | None = None, solves: int = 
         # a tile choice also costs the kernels that reduce its partial sums: every block
         # of the apply reduces the finest up kernel's <r,z>, <q,z> slots, and the first
         # down kernel's convergence test reduces the update's ||r||^2 slots (one slot
-        # per block of the producer)
+        # per block of the producer); the x-update blocks of the tail launch walk the
+        # Krylov tiles
         if kind == 2:
             return sum(t for k, t in times.items()
-                       if k in ("apply", "update", "update (x deferred)", "down L0"))
+                       if k in ("apply", "update", "update (x deferred)", "down L0", "tail"))
         if kind == 0 and level == 0:
             # the fused update-down entries replace the update of most iterations, move
             # its convergence test to the tail kernels and its r store to the finest up

@@ -1180,11 +1180,13 @@ void prof_collect(gdp_plan* P) {
 // more, the x update would outlast the tail on too few CUs (a batch of about as many
 // problems as CUs), and the whole-chip k_update does it instead.
 constexpr double kXBytesPerFreeCu = 1048576.0;
-// The x-update blocks of a tail launch each take about this much of the x stream: fewer
-// blocks than free CUs where the stream is small.  The stream's rate saturates well
-// below one block per free CU, and its traffic slows the tail's own latency-bound
-// chain: at 1024x1024 RGB (75 MB) 128-152 blocks made the tail launch 1.1-1.8 us shorter
-// than 253, and 108 or 64 blocks made it 2 or 13 us longer.
+// The x-update blocks of an FP32 tail launch each take about this much of the x stream:
+// fewer blocks than free CUs where the stream is small.  The stream's rate saturates
+// well below one block per free CU, and its traffic slows the tail's own latency-bound
+// chain: at 1024x1024 RGB mixed (75 MB) 128-152 blocks made the tail launch 1.1-1.8 us
+// shorter than 253, and 108 or 64 blocks made it 2 or 13 us longer.  An FP64 plan's
+// tail starts one level coarser and is done sooner, so the stream has to be: there 144
+// blocks made the launch 3 us longer than one per free CU, which it keeps.
 constexpr double kXBytesPerBlock = 524288.0;
 bool defer_x_update(const gdp_plan* P) {
   const int free_cu = P->n_cu - P->n_stail;
@@ -1245,12 +1247,13 @@ int enqueue_vcycle_t(gdp_plan* P, hipStream_t s, bool dots, bool check_conv,
                  rk, P->d_zpart(), P->d_rup0(), cc, o.pre_smoothing,
                  o.post_smoothing, P->csweeps, order};
       if (xupd) {
-        // one block per kXBytesPerBlock of the x stream, on CUs the tail blocks leave
-        // free, at most one per kXItems items
+        // one block per kXBytesPerBlock of the x stream (FP32 tails), on CUs the tail
+        // blocks leave free, at most one per kXItems items
         const Shape2 ks = kKryMenu[P->kry_menu];
         const int items =
             P->kry.count * ((ks.h * ks.w / 2 + kStailBlockThreads - 1) / kStailBlockThreads);
-        const int by_bytes = int(24.0 * krylov_points(P) / kXBytesPerBlock) + 1;
+        const int by_bytes =
+            P->fp32 ? int(24.0 * krylov_points(P) / kXBytesPerBlock) + 1 : P->n_cu;
         a.xblocks = std::max(1, std::min(std::min(P->n_cu - ns, by_bytes),
                                          (items + kStailXItems - 1) / kStailXItems));
         a.xt = P->at<KTile>(P->kry.off);
